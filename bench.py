@@ -1,0 +1,312 @@
+#!/usr/bin/env python3
+"""Benchmark of the GAS engine on one MI355X at RMAT scale 24.
+
+Workload (one step): a plain SSSP run of the reference's vertex program
+(sssp.cu:69-111: all vertices active first, Bellman-Ford-style min-gather)
+over RMAT scale 24 (16,777,216 vertices, edge factor 16, a/b/c/d =
+.57/.19/.19/.05, seed 6, self-loops and duplicate arcs removed, uint32
+weights uniform on [1, 64]) from the vertex of maximum out-degree.  The graph,
+weights and vertex data are resident in HBM before timing; a step restores
+the vertex data on the device, runs the engine loop to an empty frontier and
+leaves the result in HBM.
+
+value = GTEPS = sum of out-degrees of reached vertices / step time (the
+reference's TEPS, sssp.cu:195-213).  roofline = the merge-path gather kernel
+(k_gather_tiles): algorithmic bytes 12 B per gathered in-edge + 16 B per
+active vertex, divided by its measured duration (hipEvents on the engine's
+stream).  cpu_baseline = the CPU oracle (restated refgas.h engine, one
+thread) on the same graph, which also checks the GPU result bit for bit.
+Extra fields report plain BFS, CC and PageRank at the same scale.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "GTEPS + HBM GB/s fraction at RMAT-24; batched vs plain aggregate TEPS, p99 lat"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def run_loop(eng, all_active_n=None, source=None):
+    """The reference app loop on an engine whose graph is bound."""
+    eng.resetVertexData()
+    if all_active_n is not None:
+        eng.setActive(0, all_active_n)
+    else:
+        eng.setActive(source, source + 1)
+    it = 0
+    eng.setIterationCount(0)
+    while eng.countActive():
+        eng.gather()
+        eng.apply()
+        eng.scatterActivate()
+        eng.nextIter()
+        it += 1
+        eng.setIterationCount(it)
+    return it
+
+
+def timed(fn, steps, sync):
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    sync()
+    return (time.perf_counter() - t0) / steps
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--seed", type=int, default=6)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
+              "the multi-GPU sharding is a design, not a runnable path", file=sys.stderr)
+        return 2
+
+    import graphreduce_amd as gr
+
+    info = gr.device_info()
+    log(f"device: {info}")
+    n = 1 << args.scale
+    t = time.time()
+    g = gr.Graph.rmat(args.scale, 16, seed=args.seed)
+    gr.synchronize()
+    log(f"rmat{args.scale}: V={g.n_vertices} E={g.n_edges} built on device in {time.time() - t:.2f}s")
+    src, src_deg = g.max_out_degree_vertex()
+    off_out, dsts = g.csr()
+    outdeg = np.diff(off_out).astype(np.int64)
+    w_dev = g.hash_weights_device(seed=args.seed)
+
+    # ---------------------------------------------------------- SSSP (value)
+    vd = np.full(n, gr.SSSP_INF, np.int32)
+    vd[src] = 0
+    eng = gr.GASEngine(gr.SSSP)
+    eng.bindGraph(g, vd, w_dev)
+    step = lambda: run_loop(eng, all_active_n=n)  # noqa: E731
+    trips = 0
+    for _ in range(args.warmup):
+        trips = step()
+    dist = eng.readVertexData()
+    reached = dist < gr.SSSP_INF
+    e_reached = int(outdeg[reached].sum())
+    sec = timed(step, args.steps, gr.synchronize)
+    gteps = e_reached / sec / 1e9
+    log(f"SSSP: trips={trips} reached={int(reached.sum())} E_reached={e_reached} "
+        f"{sec * 1e3:.3f} ms/step {gteps:.2f} GTEPS")
+
+    # kernel timing pass: one more step with hipEvents around every phase
+    eng.resetStats()
+    eng.enableTiming(True)
+    step()
+    st = eng.stats()
+    eng.enableTiming(False)
+    eng.resetStats()
+    gather_bytes = 12 * st["gathered_edges"] + 16 * st["active_vertices"]
+    gather_s = st["gather_ms"] / 1e3
+    achieved = gather_bytes / gather_s / 1e9 if gather_s > 0 else 0.0
+    phases = {k: round(st[k + "_ms"], 4) for k in ("gather", "apply", "scatter", "next")}
+    launches = {k: st[k + "_launches"] for k in ("gather", "apply", "scatter", "next")}
+    step_alg_gbs = gather_bytes / sec / 1e9
+    log(f"  phases ms {phases} launches {launches}; gather {achieved:.1f} GB/s algorithmic")
+
+    prof_traffic = None
+    tfile = ROOT / "profiles" / "gather_tiles_sssp_rmat24_traffic.json"
+    if tfile.exists():
+        try:
+            prof_traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
+        except Exception:
+            prof_traffic = None
+
+    result = {
+        "metric": METRIC,
+        "value": round(gteps, 3),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sec * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (device RMAT generator, counter-based hash streams)",
+        "config": {
+            "workload": f"plain SSSP (sssp.cu loop, all-active start) on RMAT-{args.scale}, "
+                        f"edge factor 16, a/b/c/d=.57/.19/.19/.05, seed {args.seed}, "
+                        "dedup + no self-loops, weights U[1,64], source = max out-degree",
+            "n_vertices": g.n_vertices,
+            "n_edges": int(g.n_edges),
+            "source": src,
+            "source_out_degree": src_deg,
+            "trips": trips,
+            "edges_reached": e_reached,
+        },
+        "roofline": {
+            "kernel": "k_gather_tiles<SSSPProgram>",
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": prof_traffic,
+            "bytes_per_launch": round(gather_bytes / max(1, launches["gather"])),
+            "avg_launch_ms": round(st["gather_ms"] / max(1, launches["gather"]), 5),
+            "launches": launches["gather"],
+            "algorithmic_bytes": "12 B per gathered in-edge (source id, weight, source "
+                                 "distance) + 16 B per active vertex",
+        },
+        "phase_ms_per_step": phases,
+        "step_algorithmic_gbs": round(step_alg_gbs, 1),
+    }
+
+    extra = {}
+    if not args.no_extras:
+        try:
+            extra.update(run_extras(gr, g, args, src, outdeg))
+        except Exception as exc:  # extras never hide the headline
+            extra["error"] = repr(exc)
+    result["extra"] = extra
+
+    # ---------------------------------------------------------- CPU baseline
+    if not args.no_cpu_baseline:
+        try:
+            result["cpu_baseline"] = cpu_baseline(gr, g, args, src, off_out, dsts, dist, e_reached)
+        except Exception as exc:
+            result["cpu_baseline"] = {"error": repr(exc)}
+    eng.close()
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def run_extras(gr, g, args, src, outdeg):
+    out = {}
+    n = g.n_vertices
+    # plain BFS (bfs.cu loop)
+    vd = np.zeros(n, gr.PROGRAMS[gr.BFS]["vertex"])
+    vd["depth"] = -1
+    eng = gr.GASEngine(gr.BFS)
+    eng.bindGraph(g, vd)
+    step = lambda: run_loop(eng, source=src)  # noqa: E731
+    trips = step()
+    depth = eng.readVertexData()["depth"]
+    e_r = int(outdeg[depth >= 0].sum())
+    sec = timed(step, max(3, args.steps), gr.synchronize)
+    eng.resetStats()
+    eng.enableTiming(True)
+    step()
+    st = eng.stats()
+    eng.close()
+    out["bfs"] = dict(gteps=round(e_r / sec / 1e9, 3), ms=round(sec * 1e3, 4), trips=trips,
+                      edges_reached=e_r,
+                      alg_gbs=round((4 * e_r + 16 * n) / sec / 1e9, 1),
+                      phase_ms={k: round(st[k + "_ms"], 4) for k in ("apply", "scatter", "next")},
+                      pull_iterations=st["pull_iterations"])
+    log(f"BFS: {out['bfs']}")
+    # plain PageRank (pagerank.cu loop: activation-driven)
+    vd = np.zeros(n, gr.PROGRAMS[gr.PAGERANK]["vertex"])
+    vd["rank"] = np.float32(0.15)
+    vd["numOutEdges"] = outdeg.astype(np.int32)
+    eng = gr.GASEngine(gr.PAGERANK)
+    eng.bindGraph(g, vd)
+    step = lambda: run_loop(eng, all_active_n=n)  # noqa: E731
+    trips = step()
+    eng.resetStats()
+    eng.enableTiming(True)
+    t0 = time.perf_counter()
+    step()
+    gr.synchronize()
+    sec = time.perf_counter() - t0
+    st = eng.stats()
+    eng.close()
+    ein = st["gathered_edges"]
+    out["pagerank"] = dict(gteps=round(ein / sec / 1e9, 3), ms=round(sec * 1e3, 3), trips=trips,
+                           gathered_edges=ein,
+                           gather_alg_gbs=round((8 * ein + 16 * st["active_vertices"])
+                                                / (st["gather_ms"] / 1e3) / 1e9, 1))
+    log(f"PR: {out['pagerank']}")
+    # plain CC on the symmetrised graph
+    gs = gr.Graph.rmat(args.scale, 16, seed=args.seed, symmetrize=True)
+    vd = np.arange(n, dtype=np.int32)
+    eng = gr.GASEngine(gr.CC)
+    eng.bindGraph(gs, vd)
+    step = lambda: run_loop(eng, all_active_n=n)  # noqa: E731
+    trips = step()
+    eng.resetStats()
+    eng.enableTiming(True)
+    t0 = time.perf_counter()
+    step()
+    gr.synchronize()
+    sec = time.perf_counter() - t0
+    st = eng.stats()
+    eng.close()
+    sym_edges = int(gs.n_edges)
+    gs.close()
+    ein = st["gathered_edges"]
+    out["cc"] = dict(gteps=round(ein / sec / 1e9, 3), ms=round(sec * 1e3, 3), trips=trips,
+                     n_edges=sym_edges, gathered_edges=ein,
+                     gather_alg_gbs=round((8 * ein + 16 * st["active_vertices"])
+                                          / (st["gather_ms"] / 1e3) / 1e9, 1))
+    log(f"CC: {out['cc']}")
+    return out
+
+
+def cpu_baseline(gr, g, args, src, off_out, dsts, dist_gpu, e_reached):
+    import oracle
+
+    off_in, srcs = g.csc()
+    _, csc_eid = g.perm()
+    w = gr.hash_weights(g.n_edges, seed=args.seed)
+    w_csc = w[csc_eid]
+    log("cpu baseline: oracle SSSP (1 thread) on the same graph ...")
+    dist, trips, st = oracle.sssp_csr(g.n_vertices, off_out, dsts, off_in, srcs, w_csc, src)
+    sec = st["seconds"]
+    exact = bool(np.array_equal(dist, dist_gpu))
+    log(f"cpu baseline: {sec:.2f}s trips={trips} bit-exact={exact}")
+    return {
+        "value": round(e_reached / sec / 1e9, 5),
+        "unit": "GTEPS",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"one full SSSP run on the same RMAT-{args.scale} graph (loop only, "
+                  f"{sec:.2f} s); CPU {cpu_model()}, nproc {os.cpu_count()}",
+        "seconds": round(sec, 3),
+        "gpu_result_bit_exact": exact,
+    }
+
+
+if __name__ == "__main__":
+    sys.exit(main())

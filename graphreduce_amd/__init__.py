@@ -1,0 +1,548 @@
+"""graphreduce_amd: Gather-Apply-Scatter graph engine for one AMD MI355X.
+
+Python mirror of the reference's engine interface (``GASEngineGPU<Program>``,
+gpugas.h:1025-1809) over the C ABI of ``lib/libgraphreduce_amd.so``.  The
+engine methods keep the reference's names and argument meaning
+(``setGraph``, ``setActive``, ``countActive``, ``gather``, ``apply``,
+``scatterActivate``, ``nextIter``, ``getResults``, ``run``).  Errors raise
+:class:`GRError` (the reference aborts, gpugas.h:158-165).
+
+There is no CPU fallback: every call goes to the HIP library, and importing
+the package fails loudly when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+__all__ = [
+    "GRError", "lib", "Graph", "GASEngine", "PROGRAMS", "BFS", "SSSP", "PAGERANK", "CC",
+    "rmat_edges", "grid_edges", "hash_weights", "device_count", "device_info",
+    "run_bfs", "run_sssp", "run_cc", "run_pagerank", "SSSP_INF",
+]
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "lib" / "libgraphreduce_amd.so"
+
+# Program ids (include/graphreduce_amd/gr_abi.h)
+BFS, SSSP, PAGERANK, CC = 1, 2, 3, 4
+
+# Vertex/edge data layouts of the reference programs (bfs.cu:33-38,
+# sssp.cu:28-29, pagerank.cu:32-39, connected_component.cu:28-29).
+PROGRAMS = {
+    BFS: dict(name="BFS", vertex=np.dtype([("depth", "<i4")]), edge=None),
+    SSSP: dict(name="SSSP", vertex=np.dtype("<i4"), edge=np.dtype("<i4")),
+    PAGERANK: dict(name="PageRank",
+                   vertex=np.dtype([("rank", "<f4"), ("numOutEdges", "<i4")]), edge=None),
+    CC: dict(name="CC", vertex=np.dtype("<i4"), edge=None),
+}
+
+# SSSP::gatherZero = INT_MAX - maxLength (sssp.cu:32-33): the "unreached" distance.
+SSSP_INF = 2**31 - 1 - 100000
+
+RMAT_ABCD = (0.57, 0.19, 0.19)
+
+
+class GRError(RuntimeError):
+    """A call into libgraphreduce_amd failed (carries the status code)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[gr error {code}] {msg}")
+        self.code = code
+
+
+class _Stats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int64),
+        ("active_vertices", C.c_int64),
+        ("gathered_edges", C.c_int64),
+        ("changed_vertices", C.c_int64),
+        ("pushed_edges", C.c_int64),
+        ("pull_iterations", C.c_int64),
+        ("gather_launches", C.c_int64),
+        ("gather_ms", C.c_double),
+        ("scatter_launches", C.c_int64),
+        ("scatter_ms", C.c_double),
+        ("apply_launches", C.c_int64),
+        ("apply_ms", C.c_double),
+        ("next_launches", C.c_int64),
+        ("next_ms", C.c_double),
+    ]
+
+
+_P = C.c_void_p
+_I32P = C.POINTER(C.c_int32)
+_I64P = C.POINTER(C.c_int64)
+
+# name -> (restype, argtypes); every entry point of gr_abi.h
+_SIGS = {
+    "gr_last_error": (C.c_char_p, []),
+    "gr_version": (C.c_char_p, []),
+    "gr_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "gr_device_info": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_int), C.POINTER(C.c_size_t)]),
+    "gr_device_synchronize": (C.c_int, []),
+    "gr_device_alloc": (C.c_int, [C.c_size_t, C.POINTER(_P)]),
+    "gr_device_free": (C.c_int, [_P]),
+    "gr_memcpy_h2d": (C.c_int, [_P, _P, C.c_size_t]),
+    "gr_memcpy_d2h": (C.c_int, [_P, _P, C.c_size_t]),
+    "gr_host_rmat_edges": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                      C.c_uint64, C.c_int, _I32P, _I32P, C.c_int64, _I64P]),
+    "gr_host_grid_edges": (C.c_int, [C.c_int, C.c_double, C.c_uint64, _I32P, _I32P,
+                                      C.c_int64, _I64P]),
+    "gr_host_hash_weights": (C.c_int, [C.c_uint64, C.c_int64, C.c_int32, C.c_int, _I32P]),
+    "gr_graph_from_edges": (C.c_int, [C.c_int32, C.c_int32, _I32P, _I32P, C.POINTER(_P)]),
+    "gr_graph_rmat": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                 C.c_uint64, C.c_int, C.POINTER(_P)]),
+    "gr_graph_grid": (C.c_int, [C.c_int, C.c_double, C.c_uint64, C.POINTER(_P)]),
+    "gr_graph_destroy": (C.c_int, [_P]),
+    "gr_graph_info": (C.c_int, [_P, _I32P, _I64P]),
+    "gr_graph_copy_csr": (C.c_int, [_P, _I32P, _I32P]),
+    "gr_graph_copy_csc": (C.c_int, [_P, _I32P, _I32P]),
+    "gr_graph_copy_edges": (C.c_int, [_P, _I32P, _I32P]),
+    "gr_graph_copy_perm": (C.c_int, [_P, _I32P, _I32P]),
+    "gr_graph_max_out_degree_vertex": (C.c_int, [_P, _I32P, _I32P]),
+    "gr_graph_hash_weights_device": (C.c_int, [_P, C.c_uint64, C.c_int32, C.c_int,
+                                                C.POINTER(_P)]),
+    "gr_engine_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "gr_engine_destroy": (C.c_int, [_P]),
+    "gr_engine_set_graph": (C.c_int, [_P, C.c_int32, _P, C.c_size_t, C.c_int32, _P, C.c_size_t,
+                                       _I32P, _I32P]),
+    "gr_engine_bind_graph": (C.c_int, [_P, _P, _P, C.c_size_t, _P, C.c_size_t, C.c_int]),
+    "gr_engine_set_active": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "gr_engine_count_active": (C.c_int, [_P, _I32P]),
+    "gr_engine_gather": (C.c_int, [_P, C.c_int]),
+    "gr_engine_apply": (C.c_int, [_P]),
+    "gr_engine_scatter_activate": (C.c_int, [_P, C.c_int]),
+    "gr_engine_next_iter": (C.c_int, [_P, _I32P]),
+    "gr_engine_get_results": (C.c_int, [_P]),
+    "gr_engine_run": (C.c_int, [_P, _I32P]),
+    "gr_engine_set_iteration_count": (C.c_int, [_P, C.c_int32]),
+    "gr_engine_get_iteration_count": (C.c_int, [_P, _I32P]),
+    "gr_engine_read_vertex_data": (C.c_int, [_P, _P, C.c_size_t]),
+    "gr_engine_reset_vertex_data": (C.c_int, [_P]),
+    "gr_engine_set_direction": (C.c_int, [_P, C.c_int, C.c_double]),
+    "gr_engine_enable_timing": (C.c_int, [_P, C.c_int]),
+    "gr_engine_get_stats": (C.c_int, [_P, C.POINTER(_Stats)]),
+    "gr_engine_reset_stats": (C.c_int, [_P]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The loaded native library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        path = os.environ.get("GRAPHREDUCE_AMD_LIB", str(LIB_PATH))
+        if not Path(path).exists():
+            raise ImportError(
+                f"graphreduce_amd native library not found at {path}; "
+                "run `python -c 'import __graft_entry__; __graft_entry__.build()'`")
+        L = C.CDLL(path)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise GRError(rc, lib().gr_last_error().decode(errors="replace"))
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _p32(a: np.ndarray):
+    return a.ctypes.data_as(_I32P)
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    _check(lib().gr_device_count(C.byref(n)))
+    return n.value
+
+
+def device_info() -> dict:
+    name = C.create_string_buffer(256)
+    cus = C.c_int(0)
+    mem = C.c_size_t(0)
+    _check(lib().gr_device_info(name, 256, C.byref(cus), C.byref(mem)))
+    return dict(name=name.value.decode(), compute_units=cus.value, total_bytes=mem.value)
+
+
+def synchronize() -> None:
+    _check(lib().gr_device_synchronize())
+
+
+# ------------------------------------------------------------------ inputs
+
+def rmat_edges(scale: int, edge_factor: int = 16, seed: int = 1, abc=RMAT_ABCD,
+               symmetrize: bool = False):
+    """Host RMAT edge list (self-loops and duplicates removed, sorted by (src, dst))."""
+    a, b, c = abc
+    m = (edge_factor << scale) * (2 if symmetrize else 1)
+    srcs = np.empty(m, np.int32)
+    dsts = np.empty(m, np.int32)
+    n = C.c_int64(0)
+    _check(lib().gr_host_rmat_edges(scale, edge_factor, a, b, c, seed, int(symmetrize),
+                                    _p32(srcs), _p32(dsts), m, C.byref(n)))
+    return srcs[: n.value].copy(), dsts[: n.value].copy()
+
+
+def grid_edges(side: int, delete_frac: float = 0.01, seed: int = 5):
+    """Host 4-neighbour grid arcs (both directions of every kept edge)."""
+    n = C.c_int64(0)
+    _check(lib().gr_host_grid_edges(side, delete_frac, seed, None, None, 0, C.byref(n)))
+    srcs = np.empty(n.value, np.int32)
+    dsts = np.empty(n.value, np.int32)
+    _check(lib().gr_host_grid_edges(side, delete_frac, seed, _p32(srcs), _p32(dsts), n.value,
+                                    C.byref(n)))
+    return srcs, dsts
+
+
+def hash_weights(n: int, seed: int, lo: int = 1, span_log2: int = 6) -> np.ndarray:
+    """Uniform integer weights on [lo, lo + 2**span_log2) per edge-list position."""
+    w = np.empty(n, np.int32)
+    _check(lib().gr_host_hash_weights(seed, n, lo, span_log2, _p32(w)))
+    return w
+
+
+# ------------------------------------------------------------------- graph
+
+class Graph:
+    """Device-resident CSR + CSC graph shared by engines."""
+
+    def __init__(self, handle):
+        self._h = handle
+        nv = C.c_int32(0)
+        ne = C.c_int64(0)
+        _check(lib().gr_graph_info(self._h, C.byref(nv), C.byref(ne)))
+        self.n_vertices = nv.value
+        self.n_edges = ne.value
+
+    @classmethod
+    def from_edges(cls, n_vertices: int, srcs, dsts) -> "Graph":
+        s, d = _i32(srcs), _i32(dsts)
+        h = _P()
+        _check(lib().gr_graph_from_edges(n_vertices, len(s), _p32(s), _p32(d), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def rmat(cls, scale: int, edge_factor: int = 16, seed: int = 1, abc=RMAT_ABCD,
+             symmetrize: bool = False) -> "Graph":
+        a, b, c = abc
+        h = _P()
+        _check(lib().gr_graph_rmat(scale, edge_factor, a, b, c, seed, int(symmetrize), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def grid(cls, side: int, delete_frac: float = 0.01, seed: int = 5) -> "Graph":
+        h = _P()
+        _check(lib().gr_graph_grid(side, delete_frac, seed, C.byref(h)))
+        return cls(h)
+
+    def close(self) -> None:
+        if self._h:
+            lib().gr_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def csr(self):
+        off = np.empty(self.n_vertices + 1, np.int32)
+        adj = np.empty(self.n_edges, np.int32)
+        _check(lib().gr_graph_copy_csr(self._h, _p32(off), _p32(adj)))
+        return off, adj
+
+    def csc(self):
+        off = np.empty(self.n_vertices + 1, np.int32)
+        adj = np.empty(self.n_edges, np.int32)
+        _check(lib().gr_graph_copy_csc(self._h, _p32(off), _p32(adj)))
+        return off, adj
+
+    def edges(self):
+        s = np.empty(self.n_edges, np.int32)
+        d = np.empty(self.n_edges, np.int32)
+        _check(lib().gr_graph_copy_edges(self._h, _p32(s), _p32(d)))
+        return s, d
+
+    def perm(self):
+        """(csr_eid, csc_eid): edge-list index of each CSR / CSC position."""
+        a = np.empty(self.n_edges, np.int32)
+        b = np.empty(self.n_edges, np.int32)
+        _check(lib().gr_graph_copy_perm(self._h, _p32(a), _p32(b)))
+        return a, b
+
+    def max_out_degree_vertex(self):
+        v = C.c_int32(0)
+        deg = C.c_int32(0)
+        _check(lib().gr_graph_max_out_degree_vertex(self._h, C.byref(v), C.byref(deg)))
+        return v.value, deg.value
+
+    def hash_weights_device(self, seed: int, lo: int = 1, span_log2: int = 6) -> "DeviceArray":
+        p = _P()
+        _check(lib().gr_graph_hash_weights_device(self._h, seed, lo, span_log2, C.byref(p)))
+        return DeviceArray(p, self.n_edges * 4)
+
+
+class DeviceArray:
+    """A raw device allocation owned by Python."""
+
+    def __init__(self, ptr, nbytes: int):
+        self.ptr = ptr
+        self.nbytes = nbytes
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().gr_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ engine
+
+class GASEngine:
+    """GASEngineGPU<Program> (gpugas.h:1025) for one of the compiled programs."""
+
+    def __init__(self, program: int):
+        if program not in PROGRAMS:
+            raise GRError(5, f"unknown program id {program}")
+        self.program = program
+        self.vertex_dtype = PROGRAMS[program]["vertex"]
+        self.edge_dtype = PROGRAMS[program]["edge"]
+        h = _P()
+        _check(lib().gr_engine_create(program, C.byref(h)))
+        self._h = h
+        self._vdata = None
+        self._graph = None
+
+    def close(self) -> None:
+        if self._h:
+            lib().gr_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _vertex_array(self, vertex_data) -> np.ndarray:
+        vd = np.ascontiguousarray(vertex_data)
+        if vd.dtype != self.vertex_dtype:
+            vd = np.ascontiguousarray(vd.astype(self.vertex_dtype))
+        return vd
+
+    def _edge_ptr(self, edge_data):
+        if self.edge_dtype is None:
+            return None, 0, None
+        if edge_data is None:
+            raise GRError(1, f"{PROGRAMS[self.program]['name']} needs edge data")
+        ed = np.ascontiguousarray(edge_data, dtype=self.edge_dtype)
+        return ed.ctypes.data_as(_P), ed.dtype.itemsize, ed
+
+    # gpugas.h:1314-1319
+    def setGraph(self, nVertices: int, vertexData, nEdges: int, edgeData, srcs, dsts) -> None:
+        vd = self._vertex_array(vertexData)
+        s, d = _i32(srcs), _i32(dsts)
+        if len(s) != nEdges or len(d) != nEdges or len(vd) != nVertices:
+            raise GRError(1, "setGraph: array lengths do not match nVertices/nEdges")
+        ep, esz, keep = self._edge_ptr(edgeData)
+        _check(lib().gr_engine_set_graph(self._h, nVertices, vd.ctypes.data_as(_P),
+                                         vd.dtype.itemsize, nEdges, ep, esz, _p32(s), _p32(d)))
+        self._vdata = vd
+        self._graph = None
+
+    def bindGraph(self, graph: Graph, vertexData, edgeData=None) -> None:
+        """setGraph with a prebuilt device graph; edgeData may be a DeviceArray."""
+        vd = self._vertex_array(vertexData)
+        if len(vd) != graph.n_vertices:
+            raise GRError(1, "bindGraph: vertex data length != graph vertices")
+        if isinstance(edgeData, DeviceArray):
+            ep, esz, on_dev = edgeData.ptr, 4, 1
+        else:
+            ep, esz, _keep = self._edge_ptr(edgeData)
+            on_dev = 0
+        _check(lib().gr_engine_bind_graph(self._h, graph.handle, vd.ctypes.data_as(_P),
+                                          vd.dtype.itemsize, ep, esz, on_dev))
+        self._vdata = vd
+        self._graph = graph
+
+    @property
+    def vertexData(self) -> np.ndarray:
+        """The host array bound by setGraph (updated by getResults)."""
+        return self._vdata
+
+    def setActive(self, start: int, end: int) -> None:
+        _check(lib().gr_engine_set_active(self._h, start, end))
+
+    def countActive(self) -> int:
+        n = C.c_int32(0)
+        _check(lib().gr_engine_count_active(self._h, C.byref(n)))
+        return n.value
+
+    def gather(self, haveGather: bool = True) -> None:
+        _check(lib().gr_engine_gather(self._h, int(haveGather)))
+
+    def apply(self) -> None:
+        _check(lib().gr_engine_apply(self._h))
+
+    def scatterActivate(self, haveScatter: bool = True) -> None:
+        _check(lib().gr_engine_scatter_activate(self._h, int(haveScatter)))
+
+    def nextIter(self) -> int:
+        n = C.c_int32(0)
+        _check(lib().gr_engine_next_iter(self._h, C.byref(n)))
+        return n.value
+
+    def getResults(self) -> None:
+        _check(lib().gr_engine_get_results(self._h))
+
+    def run(self) -> int:
+        it = C.c_int32(0)
+        _check(lib().gr_engine_run(self._h, C.byref(it)))
+        return it.value
+
+    def setIterationCount(self, iteration: int) -> None:
+        _check(lib().gr_engine_set_iteration_count(self._h, iteration))
+
+    def readVertexData(self) -> np.ndarray:
+        out = np.empty_like(self._vdata)
+        _check(lib().gr_engine_read_vertex_data(self._h, out.ctypes.data_as(_P),
+                                                out.dtype.itemsize))
+        return out
+
+    def resetVertexData(self) -> None:
+        """Device-side restore of the vertex data given at setGraph/bindGraph."""
+        _check(lib().gr_engine_reset_vertex_data(self._h))
+
+    def setDirection(self, mode: int = 0, alpha: float = 15.0) -> None:
+        """0 = automatic push/pull, 1 = push only, 2 = pull only."""
+        _check(lib().gr_engine_set_direction(self._h, mode, alpha))
+
+    def enableTiming(self, on: bool = True) -> None:
+        _check(lib().gr_engine_enable_timing(self._h, int(on)))
+
+    def stats(self) -> dict:
+        s = _Stats()
+        _check(lib().gr_engine_get_stats(self._h, C.byref(s)))
+        return {name: getattr(s, name) for name, _ in _Stats._fields_}
+
+    def resetStats(self) -> None:
+        _check(lib().gr_engine_reset_stats(self._h))
+
+
+# ----------------------------------------------------- reference app loops
+
+def _graph_args(graph_or_edges):
+    if isinstance(graph_or_edges, Graph):
+        return graph_or_edges, None
+    n, s, d = graph_or_edges
+    return None, (n, _i32(s), _i32(d))
+
+
+def _bind(eng: GASEngine, graph_or_edges, vd, ed=None):
+    g, el = _graph_args(graph_or_edges)
+    if g is not None:
+        eng.bindGraph(g, vd, ed)
+    else:
+        n, s, d = el
+        eng.setGraph(n, vd, len(s), ed, s, d)
+
+
+def run_bfs(graph_or_edges, source: int, direction: int = 0):
+    """bfs.cu:93-134 loop; returns (depth int32[V], loop trips)."""
+    n = graph_or_edges.n_vertices if isinstance(graph_or_edges, Graph) else graph_or_edges[0]
+    vd = np.zeros(n, PROGRAMS[BFS]["vertex"])
+    vd["depth"] = -1
+    eng = GASEngine(BFS)
+    eng.setDirection(direction)
+    _bind(eng, graph_or_edges, vd)
+    eng.setActive(source, source + 1)
+    iteration = 0
+    eng.setIterationCount(iteration)
+    while eng.countActive():
+        eng.gather(False)
+        eng.apply()
+        eng.scatterActivate(False)
+        eng.nextIter()
+        iteration += 1
+        eng.setIterationCount(iteration)
+    eng.getResults()
+    out = eng.vertexData["depth"].copy()
+    eng.close()
+    return out, iteration
+
+
+def run_sssp(graph_or_edges, weights, source: int, direction: int = 0):
+    """sssp.cu:69-111 loop (all vertices active first); returns (dist, trips)."""
+    n = graph_or_edges.n_vertices if isinstance(graph_or_edges, Graph) else graph_or_edges[0]
+    vd = np.full(n, SSSP_INF, np.int32)
+    vd[source] = 0
+    eng = GASEngine(SSSP)
+    eng.setDirection(direction)
+    _bind(eng, graph_or_edges, vd, weights)
+    eng.setActive(0, n)
+    it = 0
+    while eng.countActive():
+        eng.gather()
+        eng.apply()
+        eng.scatterActivate()
+        eng.nextIter()
+        it += 1
+    eng.getResults()
+    out = eng.vertexData.copy()
+    eng.close()
+    return out, it
+
+
+def run_cc(graph_or_edges, direction: int = 0):
+    """connected_component.cu:66-81 (label = own id, all active, run())."""
+    n = graph_or_edges.n_vertices if isinstance(graph_or_edges, Graph) else graph_or_edges[0]
+    vd = np.arange(n, dtype=np.int32)
+    eng = GASEngine(CC)
+    eng.setDirection(direction)
+    _bind(eng, graph_or_edges, vd)
+    eng.setActive(0, n)
+    it = eng.run()
+    eng.getResults()
+    out = eng.vertexData.copy()
+    eng.close()
+    return out, it
+
+
+def run_pagerank(graph_or_edges, out_degree, direction: int = 0):
+    """pagerank.cu:85-101 (rank = 0.15, all active, run()); returns (rank f32, trips)."""
+    n = graph_or_edges.n_vertices if isinstance(graph_or_edges, Graph) else graph_or_edges[0]
+    vd = np.zeros(n, PROGRAMS[PAGERANK]["vertex"])
+    vd["rank"] = np.float32(0.15)
+    vd["numOutEdges"] = out_degree
+    eng = GASEngine(PAGERANK)
+    eng.setDirection(direction)
+    _bind(eng, graph_or_edges, vd)
+    eng.setActive(0, n)
+    it = eng.run()
+    eng.getResults()
+    out = eng.vertexData["rank"].copy()
+    eng.close()
+    return out, it

@@ -1,0 +1,101 @@
+// Device-side building blocks shared by the HIP sources: error checks, wave64
+// primitives, and the device graph object.
+#ifndef GRAPHREDUCE_AMD_GR_DEVICE_H
+#define GRAPHREDUCE_AMD_GR_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "gr_internal.h"
+
+#define GR_HIP(expr)                                                               \
+  do {                                                                             \
+    hipError_t _e = (expr);                                                        \
+    if (_e != hipSuccess)                                                          \
+      return ::gr::set_errorf(GR_ERR_HIP, "%s:%d: %s failed: %s", __FILE__, __LINE__, \
+                              #expr, hipGetErrorString(_e));                       \
+  } while (0)
+
+#define GR_TRY(expr)            \
+  do {                          \
+    int _rc = (expr);           \
+    if (_rc != GR_OK) return _rc; \
+  } while (0)
+
+namespace gr {
+
+constexpr int kWave = 64;
+
+// ------------------------------------------------------------ wave64 helpers
+
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+// Number of set bits of `mask` in lanes below this one.
+__device__ __forceinline__ int lanes_below(uint64_t mask) {
+  return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+template <class T>
+__device__ __forceinline__ T wave_incl_scan_add(T v) {
+  const int l = lane_id();
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    T o = __shfl_up(v, off, kWave);
+    if (l >= off) v += o;
+  }
+  return v;
+}
+
+template <class T>
+__device__ __forceinline__ T wave_reduce_add(T v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// Exclusive prefix sum over a workgroup of NT threads (NT a multiple of 64).
+// `scratch` must hold NT / 64 elements.  Returns the exclusive prefix and the
+// block total in *total.
+template <int NT, class T>
+__device__ __forceinline__ T block_excl_scan_add(T v, T* scratch, T* total) {
+  constexpr int NW = NT / kWave;
+  const int l = lane_id(), w = threadIdx.x / kWave;
+  T incl = wave_incl_scan_add(v);
+  if (l == kWave - 1) scratch[w] = incl;
+  __syncthreads();
+  T before = 0, sum = 0;
+#pragma unroll
+  for (int i = 0; i < NW; ++i) {
+    T s = scratch[i];
+    if (i < w) before += s;
+    sum += s;
+  }
+  __syncthreads();
+  *total = sum;
+  return before + incl - v;
+}
+
+// ----------------------------------------------------------- device graph
+
+}  // namespace gr
+
+struct gr_graph {
+  int32_t nV = 0;
+  int64_t nE = 0;
+  int32_t* off_out = nullptr;  // CSR offsets [V+1]
+  int32_t* dsts = nullptr;     // CSR targets [E]
+  int32_t* off_in = nullptr;   // CSC offsets [V+1]
+  int32_t* srcs = nullptr;     // CSC sources [E]
+  int32_t* csr_eid = nullptr;  // CSR position -> edge-list index (nullptr: identity)
+  int32_t* csc_eid = nullptr;  // CSC position -> edge-list index
+  std::vector<int32_t> h_off_out, h_off_in;  // host copies of the offsets
+};
+
+namespace gr {
+int graph_free(gr_graph* g);
+}
+
+#endif

@@ -1,0 +1,472 @@
+// Device graph construction (CSR + CSC in HBM), device generators, and the
+// runtime utilities of the C ABI.
+//
+// The reference builds CSR/CSC on the host with an unstable std::sort over the
+// edge list (util.cuh:48-121, called from gpugas.h:1346-1373) and then streams
+// shards to the GPU on every phase.  Here the whole graph is built once on the
+// device with LSD radix sorts of packed (row, column) keys; the sort is stable,
+// so parallel arcs keep their edge-list order and the layout is deterministic.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstring>
+
+#include "gr_device.h"
+#include "gr_hash.h"
+
+namespace gr {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline int bits_for(int64_t n) {  // bits needed for values in [0, n)
+  int b = 1;
+  while (b < 62 && ((int64_t)1 << b) < n) ++b;
+  return b;
+}
+
+inline unsigned grid_for(int64_t n, int block = kBlock) {
+  int64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > (1 << 30)) g = 1 << 30;
+  return (unsigned)g;
+}
+
+__global__ void k_pack_keys(const int32_t* __restrict__ rows, const int32_t* __restrict__ cols,
+                            int64_t n, int shift, uint64_t* __restrict__ keys,
+                            int32_t* __restrict__ vals) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  keys[i] = ((uint64_t)(uint32_t)rows[i] << shift) | (uint32_t)cols[i];
+  vals[i] = (int32_t)i;
+}
+
+__global__ void k_iota(int32_t* __restrict__ v, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = (int32_t)i;
+}
+
+__global__ void k_swap_keys(const uint64_t* __restrict__ in, int64_t n, int shift,
+                            uint64_t* __restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t mask = (1ULL << shift) - 1;
+  uint64_t k = in[i];
+  out[i] = ((k & mask) << shift) | (k >> shift);
+}
+
+// Column part of sorted keys, and the row offsets: offsets[v] = first position
+// whose row is >= v.  Positions fill the gaps before their own row; the tail
+// (rows after the last key) is filled by k_offsets_tail.
+__global__ void k_split_sorted(const uint64_t* __restrict__ keys, int64_t n, int shift,
+                               int32_t* __restrict__ cols, int32_t* __restrict__ offsets) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t mask = (1ULL << shift) - 1;
+  uint64_t k = keys[i];
+  int32_t r = (int32_t)(k >> shift);
+  cols[i] = (int32_t)(k & mask);
+  int32_t pr = (i == 0) ? -1 : (int32_t)(keys[i - 1] >> shift);
+  for (int32_t v = pr + 1; v <= r; ++v) offsets[v] = (int32_t)i;
+}
+
+__global__ void k_offsets_tail(const uint64_t* __restrict__ keys, int64_t n, int shift,
+                               int32_t nV, int32_t* __restrict__ offsets) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v > nV) return;
+  int32_t last = (n == 0) ? -1 : (int32_t)(keys[n - 1] >> shift);
+  if (v > last) offsets[v] = (int32_t)n;
+}
+
+__global__ void k_rmat_keys(uint64_t base, int64_t m, int scale, RmatThresholds t, int sym,
+                            uint64_t* __restrict__ keys) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= m) return;
+  uint32_t s, d;
+  rmat_edge(base, (uint64_t)k, scale, t, &s, &d);
+  keys[k] = ((uint64_t)s << scale) | d;
+  if (sym) keys[m + k] = ((uint64_t)d << scale) | s;
+}
+
+__global__ void k_mark_unique(const uint64_t* __restrict__ keys, int64_t n, int shift,
+                              uint8_t* __restrict__ keep) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t mask = (1ULL << shift) - 1;
+  uint64_t k = keys[i];
+  bool self = (k >> shift) == (k & mask);
+  bool dup = (i > 0) && keys[i - 1] == k;
+  keep[i] = (!self && !dup) ? 1 : 0;
+}
+
+__global__ void k_hash_weights(uint64_t base, int64_t n, uint32_t lo, int span_log2,
+                               int32_t* __restrict__ w) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) w[i] = (int32_t)hash_weight(base, (uint64_t)i, lo, span_log2);
+}
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  int alloc(int64_t n) {
+    if (n <= 0) n = 1;
+    hipError_t e = hipMalloc((void**)&p, sizeof(T) * (size_t)n);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return set_errorf(GR_ERR_NOMEM, "hipMalloc(%lld bytes) failed: %s",
+                        (long long)(sizeof(T) * n), hipGetErrorString(e));
+    }
+    return GR_OK;
+  }
+  T* release() {
+    T* r = p;
+    p = nullptr;
+    return r;
+  }
+};
+
+// Sorts (keys, vals) by the low `bits` bits of the keys; results in *_out.
+int radix_sort_pairs(uint64_t* keys_in, uint64_t* keys_out, int32_t* vals_in,
+                     int32_t* vals_out, int64_t n, int bits) {
+  if (n == 0) return GR_OK;
+  size_t temp = 0;
+  GR_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, temp, keys_in, keys_out, vals_in,
+                                            vals_out, (int)n, 0, bits));
+  DevBuf<uint8_t> t;
+  GR_TRY(t.alloc((int64_t)temp));
+  GR_HIP(hipcub::DeviceRadixSort::SortPairs(t.p, temp, keys_in, keys_out, vals_in, vals_out,
+                                            (int)n, 0, bits));
+  return GR_OK;
+}
+
+// One side (CSR or CSC) from keys row << shift | col (unsorted) with values.
+// On return: *off [V+1], *adj [E], *eid [E] are fresh device arrays.
+int build_side(int32_t nV, int64_t n, int shift, uint64_t* keys, int32_t* vals,
+               int32_t** off, int32_t** adj, int32_t** eid) {
+  DevBuf<uint64_t> ksorted;
+  DevBuf<int32_t> vsorted, o, a;
+  GR_TRY(ksorted.alloc(n));
+  GR_TRY(vsorted.alloc(n));
+  GR_TRY(o.alloc((int64_t)nV + 1));
+  GR_TRY(a.alloc(n));
+  GR_TRY(radix_sort_pairs(keys, ksorted.p, vals, vsorted.p, n, shift + bits_for(nV)));
+  if (n > 0) {
+    k_split_sorted<<<grid_for(n), kBlock>>>(ksorted.p, n, shift, a.p, o.p);
+    GR_HIP(hipGetLastError());
+  }
+  k_offsets_tail<<<grid_for((int64_t)nV + 1), kBlock>>>(ksorted.p, n, shift, nV, o.p);
+  GR_HIP(hipGetLastError());
+  *off = o.release();
+  *adj = a.release();
+  *eid = vsorted.release();
+  return GR_OK;
+}
+
+int finish_graph(gr_graph* g) {
+  g->h_off_out.resize((size_t)g->nV + 1);
+  g->h_off_in.resize((size_t)g->nV + 1);
+  GR_HIP(hipMemcpy(g->h_off_out.data(), g->off_out, sizeof(int32_t) * (g->nV + 1),
+                   hipMemcpyDeviceToHost));
+  GR_HIP(hipMemcpy(g->h_off_in.data(), g->off_in, sizeof(int32_t) * (g->nV + 1),
+                   hipMemcpyDeviceToHost));
+  return GR_OK;
+}
+
+}  // namespace
+
+int graph_free(gr_graph* g) {
+  if (!g) return GR_OK;
+  for (int32_t* p : {g->off_out, g->dsts, g->off_in, g->srcs, g->csr_eid, g->csc_eid})
+    if (p) (void)hipFree(p);
+  delete g;
+  return GR_OK;
+}
+
+int graph_from_device_edges(int32_t nV, int64_t nE, const int32_t* d_src,
+                            const int32_t* d_dst, gr_graph** out) {
+  const int shift = bits_for(nV);
+  gr_graph* g = new gr_graph();
+  g->nV = nV;
+  g->nE = nE;
+  int rc = GR_OK;
+  {
+    DevBuf<uint64_t> keys;
+    DevBuf<int32_t> vals;
+    if ((rc = keys.alloc(nE)) || (rc = vals.alloc(nE))) goto fail;
+    if (nE > 0) {
+      k_pack_keys<<<grid_for(nE), kBlock>>>(d_src, d_dst, nE, shift, keys.p, vals.p);
+      if (hipGetLastError() != hipSuccess) { rc = set_error(GR_ERR_HIP, "k_pack_keys"); goto fail; }
+    }
+    if ((rc = build_side(nV, nE, shift, keys.p, vals.p, &g->off_out, &g->dsts, &g->csr_eid)))
+      goto fail;
+    if (nE > 0) {
+      k_pack_keys<<<grid_for(nE), kBlock>>>(d_dst, d_src, nE, shift, keys.p, vals.p);
+      if (hipGetLastError() != hipSuccess) { rc = set_error(GR_ERR_HIP, "k_pack_keys"); goto fail; }
+    }
+    if ((rc = build_side(nV, nE, shift, keys.p, vals.p, &g->off_in, &g->srcs, &g->csc_eid)))
+      goto fail;
+  }
+  if ((rc = finish_graph(g))) goto fail;
+  *out = g;
+  return GR_OK;
+fail:
+  graph_free(g);
+  return rc;
+}
+
+// Graph from sorted, de-duplicated keys src << shift | dst (edge-list order =
+// CSR order, so csr_eid stays the identity).
+int graph_from_sorted_keys(int32_t nV, int64_t nE, int shift, uint64_t* d_keys,
+                           gr_graph** out) {
+  gr_graph* g = new gr_graph();
+  g->nV = nV;
+  g->nE = nE;
+  int rc = GR_OK;
+  {
+    DevBuf<int32_t> off, adj;
+    if ((rc = off.alloc((int64_t)nV + 1)) || (rc = adj.alloc(nE))) goto fail;
+    if (nE > 0) {
+      k_split_sorted<<<grid_for(nE), kBlock>>>(d_keys, nE, shift, adj.p, off.p);
+    }
+    k_offsets_tail<<<grid_for((int64_t)nV + 1), kBlock>>>(d_keys, nE, shift, nV, off.p);
+    if (hipGetLastError() != hipSuccess) { rc = set_error(GR_ERR_HIP, "csr split"); goto fail; }
+    g->off_out = off.release();
+    g->dsts = adj.release();
+    DevBuf<uint64_t> swapped;
+    DevBuf<int32_t> vals;
+    if ((rc = swapped.alloc(nE)) || (rc = vals.alloc(nE))) goto fail;
+    if (nE > 0) {
+      k_swap_keys<<<grid_for(nE), kBlock>>>(d_keys, nE, shift, swapped.p);
+      k_iota<<<grid_for(nE), kBlock>>>(vals.p, nE);
+      if (hipGetLastError() != hipSuccess) { rc = set_error(GR_ERR_HIP, "csc keys"); goto fail; }
+    }
+    if ((rc = build_side(nV, nE, shift, swapped.p, vals.p, &g->off_in, &g->srcs, &g->csc_eid)))
+      goto fail;
+  }
+  if ((rc = finish_graph(g))) goto fail;
+  *out = g;
+  return GR_OK;
+fail:
+  graph_free(g);
+  return rc;
+}
+
+}  // namespace gr
+
+using namespace gr;
+
+// ------------------------------------------------------------------ runtime
+
+extern "C" int gr_device_count(int* count) {
+  if (!count) return set_error(GR_ERR_INVALID, "count is NULL");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  *count = n;
+  return GR_OK;
+}
+
+extern "C" int gr_device_info(char* name, size_t name_len, int* cu_count, size_t* total_bytes) {
+  hipDeviceProp_t p;
+  GR_HIP(hipGetDeviceProperties(&p, 0));
+  if (name && name_len) {
+    snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
+  }
+  if (cu_count) *cu_count = p.multiProcessorCount;
+  if (total_bytes) *total_bytes = p.totalGlobalMem;
+  return GR_OK;
+}
+
+extern "C" int gr_device_synchronize(void) {
+  GR_HIP(hipDeviceSynchronize());
+  return GR_OK;
+}
+
+extern "C" int gr_device_alloc(size_t bytes, void** device_ptr) {
+  if (!device_ptr) return set_error(GR_ERR_INVALID, "device_ptr is NULL");
+  hipError_t e = hipMalloc(device_ptr, bytes ? bytes : 1);
+  if (e != hipSuccess) return set_errorf(GR_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  return GR_OK;
+}
+
+extern "C" int gr_device_free(void* device_ptr) {
+  if (device_ptr) GR_HIP(hipFree(device_ptr));
+  return GR_OK;
+}
+
+extern "C" int gr_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+  if (bytes) GR_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return GR_OK;
+}
+
+extern "C" int gr_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+  if (bytes) GR_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return GR_OK;
+}
+
+// ------------------------------------------------------------------- graph
+
+extern "C" int gr_graph_from_edges(int32_t n_vertices, int32_t n_edges, const int32_t* srcs,
+                                   const int32_t* dsts, gr_graph** out) {
+  if (!out || n_vertices < 1 || n_edges < 0 || (n_edges > 0 && (!srcs || !dsts)))
+    return set_error(GR_ERR_INVALID, "gr_graph_from_edges: bad arguments");
+  for (int32_t i = 0; i < n_edges; ++i) {
+    if ((uint32_t)srcs[i] >= (uint32_t)n_vertices || (uint32_t)dsts[i] >= (uint32_t)n_vertices)
+      return set_errorf(GR_ERR_INVALID, "gr_graph_from_edges: edge %d (%d -> %d) out of range",
+                        i, srcs[i], dsts[i]);
+  }
+  DevBuf<int32_t> s, d;
+  GR_TRY(s.alloc(n_edges));
+  GR_TRY(d.alloc(n_edges));
+  if (n_edges) {
+    GR_HIP(hipMemcpy(s.p, srcs, sizeof(int32_t) * n_edges, hipMemcpyHostToDevice));
+    GR_HIP(hipMemcpy(d.p, dsts, sizeof(int32_t) * n_edges, hipMemcpyHostToDevice));
+  }
+  return graph_from_device_edges(n_vertices, n_edges, s.p, d.p, out);
+}
+
+extern "C" int gr_graph_rmat(int scale, int edge_factor, double a, double b, double c,
+                             uint64_t seed, int symmetrize, gr_graph** out) {
+  if (!out || scale < 1 || scale > 30 || edge_factor < 1 || a < 0 || b < 0 || c < 0 ||
+      a + b + c > 1.0)
+    return set_error(GR_ERR_INVALID, "gr_graph_rmat: bad arguments");
+  const int64_t m = (int64_t)edge_factor << scale;
+  const int64_t mm = symmetrize ? 2 * m : m;
+  if (mm >= ((int64_t)1 << 31)) return set_error(GR_ERR_INVALID, "gr_graph_rmat: too many edges");
+  const RmatThresholds t = rmat_thresholds(a, b, c);
+  const uint64_t base = stream_base(seed, kStreamRmat);
+  int64_t n = 0;
+  DevBuf<uint64_t> uniq;
+  {
+    DevBuf<uint64_t> raw, sorted;
+    GR_TRY(raw.alloc(mm));
+    GR_TRY(sorted.alloc(mm));
+    k_rmat_keys<<<grid_for(m), kBlock>>>(base, m, scale, t, symmetrize, raw.p);
+    GR_HIP(hipGetLastError());
+    size_t temp = 0;
+    GR_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, temp, raw.p, sorted.p, (int)mm, 0, 2 * scale));
+    {
+      DevBuf<uint8_t> tb;
+      GR_TRY(tb.alloc((int64_t)temp));
+      GR_HIP(hipcub::DeviceRadixSort::SortKeys(tb.p, temp, raw.p, sorted.p, (int)mm, 0, 2 * scale));
+    }
+    DevBuf<uint8_t> keep;
+    GR_TRY(keep.alloc(mm));
+    k_mark_unique<<<grid_for(mm), kBlock>>>(sorted.p, mm, scale, keep.p);
+    GR_HIP(hipGetLastError());
+    GR_TRY(uniq.alloc(mm));
+    DevBuf<int> d_n;
+    GR_TRY(d_n.alloc(1));
+    temp = 0;
+    GR_HIP(hipcub::DeviceSelect::Flagged(nullptr, temp, sorted.p, keep.p, uniq.p, d_n.p, (int)mm));
+    {
+      DevBuf<uint8_t> tb;
+      GR_TRY(tb.alloc((int64_t)temp));
+      GR_HIP(hipcub::DeviceSelect::Flagged(tb.p, temp, sorted.p, keep.p, uniq.p, d_n.p, (int)mm));
+    }
+    int hn = 0;
+    GR_HIP(hipMemcpy(&hn, d_n.p, sizeof(int), hipMemcpyDeviceToHost));
+    n = hn;
+  }
+  return graph_from_sorted_keys((int32_t)1 << scale, n, scale, uniq.p, out);
+}
+
+extern "C" int gr_graph_grid(int side, double delete_frac, uint64_t seed, gr_graph** out) {
+  if (!out || side < 2 || side > 32768 || delete_frac < 0 || delete_frac > 1)
+    return set_error(GR_ERR_INVALID, "gr_graph_grid: bad arguments");
+  int64_t n = host_grid_arcs(side, delete_frac, seed, nullptr, nullptr);
+  std::vector<int32_t> s((size_t)n), d((size_t)n);
+  host_grid_arcs(side, delete_frac, seed, s.data(), d.data());
+  return gr_graph_from_edges(side * side, (int32_t)n, s.data(), d.data(), out);
+}
+
+extern "C" int gr_graph_destroy(gr_graph* g) { return graph_free(g); }
+
+extern "C" int gr_graph_info(const gr_graph* g, int32_t* n_vertices, int64_t* n_edges) {
+  if (!g) return set_error(GR_ERR_INVALID, "graph is NULL");
+  if (n_vertices) *n_vertices = g->nV;
+  if (n_edges) *n_edges = g->nE;
+  return GR_OK;
+}
+
+extern "C" int gr_graph_copy_csr(const gr_graph* g, int32_t* offsets, int32_t* dsts) {
+  if (!g) return set_error(GR_ERR_INVALID, "graph is NULL");
+  if (offsets) memcpy(offsets, g->h_off_out.data(), sizeof(int32_t) * (g->nV + 1));
+  if (dsts && g->nE) GR_HIP(hipMemcpy(dsts, g->dsts, sizeof(int32_t) * g->nE, hipMemcpyDeviceToHost));
+  return GR_OK;
+}
+
+extern "C" int gr_graph_copy_csc(const gr_graph* g, int32_t* offsets, int32_t* srcs) {
+  if (!g) return set_error(GR_ERR_INVALID, "graph is NULL");
+  if (offsets) memcpy(offsets, g->h_off_in.data(), sizeof(int32_t) * (g->nV + 1));
+  if (srcs && g->nE) GR_HIP(hipMemcpy(srcs, g->srcs, sizeof(int32_t) * g->nE, hipMemcpyDeviceToHost));
+  return GR_OK;
+}
+
+extern "C" int gr_graph_copy_edges(const gr_graph* g, int32_t* srcs, int32_t* dsts) {
+  if (!g || !srcs || !dsts) return set_error(GR_ERR_INVALID, "bad arguments");
+  const int64_t n = g->nE;
+  std::vector<int32_t> adj((size_t)n), eid;
+  if (n) GR_HIP(hipMemcpy(adj.data(), g->dsts, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+  if (g->csr_eid) {
+    eid.resize((size_t)n);
+    if (n) GR_HIP(hipMemcpy(eid.data(), g->csr_eid, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+  }
+  for (int32_t v = 0; v < g->nV; ++v) {
+    for (int32_t p = g->h_off_out[v]; p < g->h_off_out[v + 1]; ++p) {
+      int32_t e = g->csr_eid ? eid[p] : p;
+      srcs[e] = v;
+      dsts[e] = adj[p];
+    }
+  }
+  return GR_OK;
+}
+
+extern "C" int gr_graph_copy_perm(const gr_graph* g, int32_t* csr_eid, int32_t* csc_eid) {
+  if (!g) return set_error(GR_ERR_INVALID, "graph is NULL");
+  const int64_t n = g->nE;
+  if (csr_eid) {
+    if (g->csr_eid) {
+      if (n) GR_HIP(hipMemcpy(csr_eid, g->csr_eid, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    } else {
+      for (int64_t i = 0; i < n; ++i) csr_eid[i] = (int32_t)i;
+    }
+  }
+  if (csc_eid && n)
+    GR_HIP(hipMemcpy(csc_eid, g->csc_eid, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+  return GR_OK;
+}
+
+extern "C" int gr_graph_max_out_degree_vertex(const gr_graph* g, int32_t* vertex, int32_t* degree) {
+  if (!g || !vertex) return set_error(GR_ERR_INVALID, "bad arguments");
+  int32_t best = -1, bestDeg = -1;
+  for (int32_t v = 0; v < g->nV; ++v) {
+    int32_t d = g->h_off_out[v + 1] - g->h_off_out[v];
+    if (d > bestDeg) {  // strict: first vertex wins ties (bfs.cu:187)
+      bestDeg = d;
+      best = v;
+    }
+  }
+  *vertex = best;
+  if (degree) *degree = bestDeg;
+  return GR_OK;
+}
+
+extern "C" int gr_graph_hash_weights_device(const gr_graph* g, uint64_t seed, int32_t lo,
+                                            int span_log2, int32_t** device_weights) {
+  if (!g || !device_weights || span_log2 < 1 || span_log2 > 31)
+    return set_error(GR_ERR_INVALID, "bad arguments");
+  DevBuf<int32_t> w;
+  GR_TRY(w.alloc(g->nE));
+  if (g->nE) {
+    k_hash_weights<<<grid_for(g->nE), kBlock>>>(stream_base(seed, kStreamWeight), g->nE,
+                                                (uint32_t)lo, span_log2, w.p);
+    GR_HIP(hipGetLastError());
+    GR_HIP(hipDeviceSynchronize());
+  }
+  *device_weights = w.release();
+  return GR_OK;
+}

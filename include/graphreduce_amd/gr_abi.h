@@ -1,0 +1,230 @@
+/*
+ * graphreduce_amd C ABI: the drop-in boundary of the Gather-Apply-Scatter
+ * engine on one AMD Instinct MI355X (gfx950).
+ *
+ * The reference drives its GPU engine through the duck-typed C++ class
+ * GASEngineGPU<Program> (gpugas.h:1025-1809) whose Program parameter carries
+ * the vertex program (bfs.cu:31-80, sssp.cu:25-66, pagerank.cu:27-73,
+ * connected_component.cu:25-63).  Templates cannot cross a C ABI, so each
+ * Program is instantiated inside the HIP library and selected here by a
+ * program id; the C++ header GASEngineGPU.h rebuilds the reference's class
+ * on top of these functions.
+ *
+ * Conventions
+ *  - Every function returns GR_OK (0) or a GR_ERR_* code; gr_last_error()
+ *    gives the message of the last failure on the calling thread.  (The
+ *    reference prints and abort()s, gpugas.h:158-165.)
+ *  - Vertex ids, offsets and edge counts are int32, as in the reference
+ *    (Int = int32_t, gpugas.h:1026).
+ *  - Host pointers unless the name or argument says "device".
+ *  - Calls on one engine are not re-entrant; separate engines are
+ *    independent and may share one gr_graph.
+ */
+#ifndef GRAPHREDUCE_AMD_GR_ABI_H
+#define GRAPHREDUCE_AMD_GR_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GR_OK 0
+#define GR_ERR_INVALID 1     /* bad argument or size mismatch */
+#define GR_ERR_HIP 2         /* HIP runtime failure */
+#define GR_ERR_NOMEM 3       /* device or host allocation failed */
+#define GR_ERR_STATE 4       /* call out of order (e.g. gather before setGraph) */
+#define GR_ERR_UNSUPPORTED 5 /* program/feature not available */
+#define GR_ERR_BUDGET 6      /* request exceeds the configured memory budget */
+
+/* Vertex programs compiled into the library (bfs.cu:31, sssp.cu:25,
+ * pagerank.cu:27, connected_component.cu:25). */
+#define GR_PROGRAM_BFS 1
+#define GR_PROGRAM_SSSP 2
+#define GR_PROGRAM_PAGERANK 3
+#define GR_PROGRAM_CC 4
+
+typedef struct gr_graph gr_graph;
+typedef struct gr_engine gr_engine;
+
+/* ---------------------------------------------------------------- runtime */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* gr_last_error(void);
+
+/* Library version string. */
+const char* gr_version(void);
+
+/* Number of visible HIP devices (0 on a machine without a GPU; never fails). */
+int gr_device_count(int* count);
+
+/* Name, compute-unit count and total HBM bytes of device 0. */
+int gr_device_info(char* name, size_t name_len, int* cu_count, size_t* total_bytes);
+
+/* hipDeviceSynchronize on the current device. */
+int gr_device_synchronize(void);
+
+/* Plain device buffers, for callers without their own allocator. */
+int gr_device_alloc(size_t bytes, void** device_ptr);
+int gr_device_free(void* device_ptr);
+int gr_memcpy_h2d(void* device_dst, const void* host_src, size_t bytes);
+int gr_memcpy_d2h(void* host_dst, const void* device_src, size_t bytes);
+
+/* ------------------------------------------------------- host generators */
+/* Deterministic synthetic inputs.  All draws are counter-based, so host and
+ * device generators give identical edge lists. */
+
+/* RMAT edge list with 2^scale vertices and edge_factor * 2^scale draws,
+ * quadrant probabilities a, b, c (d = 1 - a - b - c).  Self-loops and
+ * duplicate (src, dst) pairs are removed; the list comes back sorted by
+ * (src, dst).  With symmetrize != 0 every arc (u, v) also yields (v, u)
+ * before de-duplication.  srcs/dsts must hold capacity entries; *n_edges
+ * receives the count.  Pass srcs = dsts = NULL to query the count only. */
+int gr_host_rmat_edges(int scale, int edge_factor, double a, double b, double c,
+                       uint64_t seed, int symmetrize, int32_t* srcs, int32_t* dsts,
+                       int64_t capacity, int64_t* n_edges);
+
+/* side x side 4-neighbour grid (vertex id r*side + c), each undirected edge
+ * deleted with probability delete_frac (seeded), both arcs of every kept
+ * edge emitted, sorted by (src, dst). */
+int gr_host_grid_edges(int side, double delete_frac, uint64_t seed, int32_t* srcs,
+                       int32_t* dsts, int64_t capacity, int64_t* n_edges);
+
+/* weights[i] = lo + uniform draw on [0, 2^span_log2) for edge-list position i. */
+int gr_host_hash_weights(uint64_t seed, int64_t n, int32_t lo, int span_log2,
+                         int32_t* weights);
+
+/* ------------------------------------------------------------------ graph */
+/* A device-resident graph: CSC (gather side) and CSR (activation side) plus
+ * the permutations from each order back to edge-list order.  Replaces the
+ * host builders edgeListToCSR/edgeListToCSC (util.cuh:79-121) used by
+ * setGraph (gpugas.h:1346-1373); built on the device with radix sorts. */
+
+/* From a host edge list (any order, duplicates allowed). */
+int gr_graph_from_edges(int32_t n_vertices, int32_t n_edges, const int32_t* srcs,
+                        const int32_t* dsts, gr_graph** out);
+
+/* RMAT generated on the device; identical to gr_host_rmat_edges. */
+int gr_graph_rmat(int scale, int edge_factor, double a, double b, double c,
+                  uint64_t seed, int symmetrize, gr_graph** out);
+
+/* Grid generated on the host (gr_host_grid_edges) and uploaded. */
+int gr_graph_grid(int side, double delete_frac, uint64_t seed, gr_graph** out);
+
+int gr_graph_destroy(gr_graph* g);
+int gr_graph_info(const gr_graph* g, int32_t* n_vertices, int64_t* n_edges);
+
+/* Copies back to the host: CSR offsets[V+1] + dsts[E] (either may be NULL),
+ * CSC offsets[V+1] + srcs[E], and the edge list (srcs, dsts) in edge-list
+ * order. */
+int gr_graph_copy_csr(const gr_graph* g, int32_t* offsets, int32_t* dsts);
+int gr_graph_copy_csc(const gr_graph* g, int32_t* offsets, int32_t* srcs);
+int gr_graph_copy_edges(const gr_graph* g, int32_t* srcs, int32_t* dsts);
+
+/* Permutations to edge-list order: csr_eid[p] / csc_eid[p] = edge-list index
+ * of CSR / CSC position p (either may be NULL). */
+int gr_graph_copy_perm(const gr_graph* g, int32_t* csr_eid, int32_t* csc_eid);
+
+/* Vertex of maximum out-degree, lowest id on ties (bfs.cu:175-195). */
+int gr_graph_max_out_degree_vertex(const gr_graph* g, int32_t* vertex, int32_t* degree);
+
+/* Device weights for the edge list of g: lo + draw on [0, 2^span_log2) per
+ * edge-list position (same values as gr_host_hash_weights). */
+int gr_graph_hash_weights_device(const gr_graph* g, uint64_t seed, int32_t lo,
+                                 int span_log2, int32_t** device_weights);
+
+/* ----------------------------------------------------------------- engine */
+/* One GASEngineGPU<Program> (gpugas.h:1025).  Each call below replaces the
+ * member function cited beside it, with refgas.h's semantics
+ * (refgas.h:128-229): bulk-synchronous gather over CSC in-edges, apply over
+ * the active list, activation of CSR out-neighbours of vertices whose apply
+ * returned true, and an ascending active list for the next iteration. */
+
+int gr_engine_create(int program_id, gr_engine** out);
+int gr_engine_destroy(gr_engine* e);
+
+/* setGraph (gpugas.h:1314-1319): copies vertexData[n_vertices] (elements of
+ * vertex_elem_size bytes, which must equal the program's VertexData) and
+ * edge_data[n_edges] (edge-list order; NULL for programs with empty
+ * EdgeData) to the device and builds CSR + CSC.  The engine keeps
+ * vertex_data to write results back in gr_engine_get_results. */
+int gr_engine_set_graph(gr_engine* e, int32_t n_vertices, void* vertex_data,
+                        size_t vertex_elem_size, int32_t n_edges, const void* edge_data,
+                        size_t edge_elem_size, const int32_t* srcs, const int32_t* dsts);
+
+/* Same as setGraph with a prebuilt device graph (shared, not owned).
+ * edge_data is in the graph's edge-list order, on the host or (with
+ * edge_data_on_device != 0) on the device. */
+int gr_engine_bind_graph(gr_engine* e, gr_graph* g, void* vertex_data,
+                         size_t vertex_elem_size, const void* edge_data,
+                         size_t edge_elem_size, int edge_data_on_device);
+
+/* setActive (gpugas.h:1524): active = [start, end). */
+int gr_engine_set_active(gr_engine* e, int32_t start, int32_t end);
+/* countActive (gpugas.h:1548). */
+int gr_engine_count_active(gr_engine* e, int32_t* n_active);
+/* gather (gpugas.h:1553).  As in refgas.h:143 the flag never skips a
+ * non-trivial gather; programs whose gather is constant (BFS) skip it. */
+int gr_engine_gather(gr_engine* e, int have_gather);
+/* apply (gpugas.h:1621). */
+int gr_engine_apply(gr_engine* e);
+/* scatterActivate (gpugas.h:1650); Program::scatter runs when
+ * have_scatter != 0 (refgas.h:192-196). */
+int gr_engine_scatter_activate(gr_engine* e, int have_scatter);
+/* nextIter (gpugas.h:1761): builds the next active list; returns its size. */
+int gr_engine_next_iter(gr_engine* e, int32_t* n_active);
+/* getResults (gpugas.h:1778): copies device vertex data back into the
+ * caller's vertexData (the reference's version is compiled out). */
+int gr_engine_get_results(gr_engine* e);
+/* run (gpugas.h:1792): loop until no vertex is active. */
+int gr_engine_run(gr_engine* e, int32_t* iterations);
+
+/* Current iteration number handed to Program::apply (BFS level).  Replaces
+ * setIterationCount<true> / g_iterationCountGPU (bfs.cu:27-28, 83-90).  The
+ * engine sets it to 0 in setActive and adds 1 in every nextIter. */
+int gr_engine_set_iteration_count(gr_engine* e, int32_t iteration);
+int gr_engine_get_iteration_count(gr_engine* e, int32_t* iteration);
+
+/* Copies device vertex data into out (n_vertices elements) without touching
+ * the bound host array. */
+int gr_engine_read_vertex_data(gr_engine* e, void* out, size_t vertex_elem_size);
+
+/* Restores the device vertex data to the values given at setGraph/bindGraph
+ * with a device-to-device copy (repeated runs without a host round trip;
+ * the reference re-runs setGraph for that, bfs.cu:106-108). */
+int gr_engine_reset_vertex_data(gr_engine* e);
+
+/* Push/pull switch of scatterActivate: mode 0 = automatic, 1 = push only,
+ * 2 = pull only; alpha: pull is chosen when the out-edges of the changed
+ * vertices exceed n_edges / alpha. */
+int gr_engine_set_direction(gr_engine* e, int mode, double alpha);
+
+/* Work and timing counters, summed since the last reset. */
+typedef struct gr_engine_stats {
+  int64_t iterations;        /* nextIter calls with a non-empty active set */
+  int64_t active_vertices;   /* sum of #A over gathers */
+  int64_t gathered_edges;    /* sum of E_in(A) over gathers */
+  int64_t changed_vertices;  /* sum of #C (apply returned true) */
+  int64_t pushed_edges;      /* sum of E_out(C) */
+  int64_t pull_iterations;   /* scatterActivate calls that ran pull mode */
+  int64_t gather_launches;   /* gather tile-kernel launches timed */
+  double gather_ms;          /* summed duration of the gather tile kernel */
+  int64_t scatter_launches;  /* scatterActivate calls timed (push + pull kernels) */
+  double scatter_ms;
+  int64_t apply_launches;    /* apply kernel */
+  double apply_ms;
+  int64_t next_launches;     /* nextIter compaction kernels */
+  double next_ms;
+} gr_engine_stats;
+
+/* Enables hipEvent timing of the engine phases on the engine's stream. */
+int gr_engine_enable_timing(gr_engine* e, int on);
+int gr_engine_get_stats(gr_engine* e, gr_engine_stats* out);
+int gr_engine_reset_stats(gr_engine* e);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GRAPHREDUCE_AMD_GR_ABI_H */

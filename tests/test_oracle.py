@@ -1,0 +1,291 @@
+"""Pins of the CPU oracle (oracle/) against independent references.
+
+Each function of the oracle is checked against something that does not share
+its code: scipy's graph routines (BFS levels, Dijkstra, connected
+components), closed forms of PageRank on cycles and stars, an FP64 sparse
+mat-vec of the all-active PageRank recurrence, brute-force enumeration of the
+GAS loop on tiny graphs written separately below, and the committed golden
+fixtures of tests/golden/.
+"""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.csgraph as cg
+
+import graphreduce_amd as gr
+import oracle
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+INF = oracle.SSSP_INF
+
+
+def _adj(n, s, d, w=None):
+    data = np.ones(len(s)) if w is None else np.asarray(w, dtype=np.float64)
+    return sp.csr_matrix((data, (s, d)), shape=(n, n))
+
+
+def _bfs_levels(n, s, d, src):
+    lv = cg.shortest_path(_adj(n, s, d), unweighted=True, indices=src)
+    return np.where(np.isinf(lv), -1, lv).astype(np.int32)
+
+
+def _bfs_trips(n, s, depth):
+    """Loop trips of bfs.cu's loop: one per level, plus one more when a vertex of
+    the deepest level has an out-edge (its neighbours are re-activated and
+    every apply returns false)."""
+    mx = depth.max()
+    outdeg = np.bincount(s, minlength=n)
+    return int(mx + 1 + (1 if outdeg[depth == mx].sum() > 0 else 0))
+
+
+def _cc_min_labels(n, s, d):
+    nc, lbl = cg.connected_components(_adj(n, s, d), directed=False)
+    mins = np.full(nc, n, np.int64)
+    np.minimum.at(mins, lbl, np.arange(n))
+    return mins[lbl].astype(np.int32)
+
+
+@pytest.mark.parametrize("scale,seed", [(10, 1), (12, 1), (14, 3), (16, 1)])
+def test_bfs_vs_scipy(scale, seed):
+    n = 1 << scale
+    s, d = gr.rmat_edges(scale, 16, seed=seed)
+    src = int(np.argmax(np.bincount(s, minlength=n)))
+    depth, trips, st = oracle.bfs(n, s, d, src)
+    assert np.array_equal(depth, _bfs_levels(n, s, d, src))
+    assert trips == _bfs_trips(n, s, depth)
+    # E_reached of bfs.cu:219-238 equals the out-edges pushed by changed vertices
+    outdeg = np.bincount(s, minlength=n)
+    assert st["out_edges"] == outdeg[depth >= 0].sum()
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_bfs_random_graphs(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(1, 400))
+    m = int(rng.integers(0, 5 * n))
+    s = rng.integers(0, n, m).astype(np.int32)
+    d = rng.integers(0, n, m).astype(np.int32)
+    src = int(rng.integers(0, n))
+    depth, trips, _ = oracle.bfs(n, s, d, src)
+    assert np.array_equal(depth, _bfs_levels(n, s, d, src))
+    assert trips == _bfs_trips(n, s, depth)
+
+
+@pytest.mark.parametrize("scale,seed", [(10, 2), (14, 2), (16, 2)])
+def test_sssp_vs_dijkstra(scale, seed):
+    n = 1 << scale
+    s, d = gr.rmat_edges(scale, 16, seed=seed)
+    w = gr.hash_weights(len(s), seed=seed)
+    src = int(np.argmax(np.bincount(s, minlength=n)))
+    dist, _, _ = oracle.sssp(n, s, d, w, src)
+    ref = cg.dijkstra(_adj(n, s, d, w), indices=src)
+    ref = np.where(np.isinf(ref), INF, ref).astype(np.int64)
+    assert np.array_equal(dist, ref)
+
+
+def test_sssp_multigraph_takes_min_parallel_arc():
+    s = np.array([0, 0, 1, 1], np.int32)
+    d = np.array([1, 1, 2, 2], np.int32)
+    w = np.array([9, 4, 7, 1], np.int32)
+    dist, trips, _ = oracle.sssp(3, s, d, w, 0)
+    assert dist.tolist() == [0, 4, 5]
+
+
+@pytest.mark.parametrize("scale,seed", [(10, 4), (14, 4), (16, 4)])
+def test_cc_vs_scipy(scale, seed):
+    n = 1 << scale
+    s, d = gr.rmat_edges(scale, 16, seed=seed, symmetrize=True)
+    lab, _, _ = oracle.cc(n, s, d)
+    assert np.array_equal(lab, _cc_min_labels(n, s, d))
+
+
+def test_cc_grid_vs_scipy():
+    side = 64
+    s, d = gr.grid_edges(side, 0.05, seed=5)
+    n = side * side
+    lab, trips, _ = oracle.cc(n, s, d)
+    assert np.array_equal(lab, _cc_min_labels(n, s, d))
+    assert trips > side  # label propagation needs about one trip per hop
+
+
+# ---------------------------------------------------------------- PageRank
+
+def test_pagerank_cycle_closed_form():
+    """Directed n-cycle: after t trips every vertex holds 1 - 0.85^(t+1); trip k
+    changes ranks by 0.15 * 0.85^(k+1) and the loop ends after the first trip
+    whose change falls below 0.01."""
+    n = 50
+    s = np.arange(n, dtype=np.int32)
+    d = ((s + 1) % n).astype(np.int32)
+    rank, trips, _ = oracle.pagerank(n, s, d)
+    expected_trips = next(t for t in range(1, 100) if 0.15 * 0.85**t < 0.01)
+    assert trips == expected_trips
+    closed = 1.0 - 0.85 ** (trips + 1)
+    assert np.allclose(rank, closed, rtol=2e-6, atol=0)
+    assert np.all(rank == rank[0])
+
+
+def test_pagerank_star_and_sources():
+    k = 40
+    s = np.arange(1, k + 1, dtype=np.int32)
+    d = np.zeros(k, np.int32)
+    rank, trips, _ = oracle.pagerank(k + 1, s, d)
+    assert trips == 1
+    assert rank[1:].tolist() == [np.float32(0.15)] * k  # zero in-degree
+    assert abs(rank[0] - (0.15 + 0.85 * 0.15 * k)) <= 1e-5 * rank[0]
+
+
+def test_pagerank64_close_to_fp32_on_small_graphs():
+    n = 1 << 10
+    s, d = gr.rmat_edges(10, 16, seed=3)
+    r32, t32, _ = oracle.pagerank(n, s, d)
+    r64, t64, _ = oracle.pagerank64(n, s, d)
+    assert t32 == t64
+    assert np.max(np.abs(r32 - r64) / r64) < 1e-5
+
+
+def test_pagerank_fp64_all_active_vs_scipy():
+    n = 1 << 12
+    s, d = gr.rmat_edges(12, 16, seed=3)
+    off_out, _ = oracle.build_csr(n, s, d)
+    off_in, srcs = oracle.build_csr(n, d, s)
+    iters = 20
+    rank, l1 = oracle.pagerank_fp64_csr(n, off_out, off_in, srcs, iters)
+    outdeg = np.diff(off_out).astype(np.float64)
+    A = _adj(n, s, d).T.tocsr()  # A[v, u] = 1 for arc u -> v
+    r = np.full(n, 0.15)
+    l1_ref = []
+    for _ in range(iters):
+        c = np.divide(r, outdeg, out=np.zeros(n), where=outdeg > 0)
+        nr = 0.15 + 0.85 * (A @ c)
+        l1_ref.append(np.abs(nr - r).sum())
+        r = nr
+    assert np.allclose(rank, r, rtol=1e-12, atol=0)
+    assert np.allclose(l1, l1_ref, rtol=1e-9)
+
+
+# ------------------------------------------------ brute force, tiny graphs
+
+def _brute_gas(n, s, d, vd, gather_map, reduce_fn, zero, apply_fn, active, it0=0):
+    """Independent Python statement of the bulk-synchronous loop of
+    refgas.h:220-229 (separate gather and apply sweeps, flag-based
+    activation, ascending next active list)."""
+    it = it0
+    trips = 0
+    while active:
+        g = {}
+        for v in active:
+            acc = zero
+            for e in range(len(s)):
+                if d[e] == v:
+                    acc = reduce_fn(acc, gather_map(vd, int(s[e]), e))
+            g[v] = acc
+        changed = [v for v in active if apply_fn(vd, v, g[v], it)]
+        flags = set()
+        for u in changed:
+            for e in range(len(s)):
+                if s[e] == u:
+                    flags.add(int(d[e]))
+        active = sorted(flags)
+        it += 1
+        trips += 1
+    return trips
+
+
+@pytest.mark.parametrize("seed", range(25))
+def test_brute_force_tiny(seed):
+    rng = np.random.default_rng(1000 + seed)
+    n = int(rng.integers(1, 9))
+    m = int(rng.integers(0, 16))
+    s = rng.integers(0, n, m).astype(np.int32)
+    d = rng.integers(0, n, m).astype(np.int32)
+    w = rng.integers(1, 65, m).astype(np.int32)
+    src = int(rng.integers(0, n))
+
+    dep = [-1] * n
+
+    def bfs_apply(vd, v, g, it):
+        if vd[v] == -1:
+            vd[v] = it
+            return True
+        return False
+
+    trips = _brute_gas(n, s, d, dep, lambda vd, u, e: 0, lambda a, b: 0, 0, bfs_apply, [src])
+    got, t, _ = oracle.bfs(n, s, d, src)
+    assert got.tolist() == dep and t == trips
+
+    dist = [INF] * n
+    dist[src] = 0
+
+    def min_apply(vd, v, g, it):
+        ch = g < vd[v]
+        vd[v] = min(vd[v], g)
+        return ch
+
+    trips = _brute_gas(n, s, d, dist, lambda vd, u, e: vd[u] + int(w[e]), min, INF, min_apply,
+                       list(range(n)))
+    got, t, _ = oracle.sssp(n, s, d, w, src)
+    assert got.tolist() == dist and t == trips
+
+    lab = list(range(n))
+    trips = _brute_gas(n, s, d, lab, lambda vd, u, e: vd[u], min, 2**31 - 1, min_apply,
+                       list(range(n)))
+    got, t, _ = oracle.cc(n, s, d)
+    assert got.tolist() == lab and t == trips
+
+    outdeg = np.bincount(s, minlength=n)
+    rk = [np.float32(0.15)] * n
+
+    def pr_apply(vd, v, g, it):
+        nr = np.float32(np.float32(0.15) + np.float32(np.float32(1.0) - np.float32(0.15)) * g)
+        ret = abs(np.float32(nr - vd[v])) >= np.float32(0.01)
+        vd[v] = nr
+        return bool(ret)
+
+    trips = _brute_gas(n, s, d, rk, lambda vd, u, e: np.float32(vd[u] / np.float32(outdeg[u])),
+                       lambda a, b: np.float32(a + b), np.float32(0.0), pr_apply, list(range(n)))
+    got, t, _ = oracle.pagerank(n, s, d)
+    assert t == trips
+    assert np.array_equal(got, np.array(rk, np.float32))
+
+
+# ------------------------------------------------------------ golden files
+
+def test_golden_fixtures():
+    """Fixtures written by tests/golden/make_golden.py from scipy."""
+    meta = json.loads((GOLDEN / "golden.json").read_text())
+    data = np.load(GOLDEN / "golden.npz")
+    for case in meta["cases"]:
+        name, n = case["name"], case["n"]
+        s, d = data[name + "_src"], data[name + "_dst"]
+        if case["kind"] == "bfs":
+            got, trips, _ = oracle.bfs(n, s, d, case["source"])
+            assert np.array_equal(got, data[name + "_out"]), name
+            assert trips == case["trips"], name
+        elif case["kind"] == "sssp":
+            got, _, _ = oracle.sssp(n, s, d, data[name + "_w"], case["source"])
+            assert np.array_equal(got, data[name + "_out"]), name
+        elif case["kind"] == "cc":
+            got, _, _ = oracle.cc(n, s, d)
+            assert np.array_equal(got, data[name + "_out"]), name
+        elif case["kind"] == "pr64":
+            off_out, _ = oracle.build_csr(n, s, d)
+            off_in, srcs = oracle.build_csr(n, d, s)
+            got, _ = oracle.pagerank_fp64_csr(n, off_out, off_in, srcs, case["iterations"])
+            assert np.allclose(got, data[name + "_out"], rtol=1e-12), name
+
+
+# ---------------------------------------------------------------- builders
+
+def test_build_csr_vs_scipy():
+    rng = np.random.default_rng(3)
+    n = 300
+    s = rng.integers(0, n, 3000).astype(np.int32)
+    d = rng.integers(0, n, 3000).astype(np.int32)
+    off, adj = oracle.build_csr(n, s, d)
+    order = np.argsort(s, kind="stable")
+    assert np.array_equal(adj, d[order])
+    assert np.array_equal(off, np.concatenate([[0], np.cumsum(np.bincount(s, minlength=n))]))
