@@ -90,6 +90,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device copy (reference point for HBM counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -98,6 +100,11 @@ def main() -> int:
         return 2
 
     import graphreduce_amd as gr
+
+    if args.calibrate_copy:
+        gbs = gr.device_copy_gbs(1 << 30, repeats=3)
+        print(json.dumps({"copy_bytes": 1 << 30, "copy_read_plus_write_gbs": round(gbs, 1)}))
+        return 0
 
     info = gr.device_info()
     log(f"device: {info}")

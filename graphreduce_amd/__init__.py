@@ -21,7 +21,8 @@ import numpy as np
 __all__ = [
     "GRError", "lib", "Graph", "GASEngine", "PROGRAMS", "BFS", "SSSP", "PAGERANK", "CC",
     "rmat_edges", "grid_edges", "hash_weights", "device_count", "device_info",
-    "run_bfs", "run_sssp", "run_cc", "run_pagerank", "SSSP_INF",
+    "run_bfs", "run_sssp", "run_cc", "run_pagerank", "SSSP_INF", "pagerank_precise",
+    "msbfs", "mssp", "upload_i32",
 ]
 
 _PKG = Path(__file__).resolve().parent
@@ -73,6 +74,20 @@ class _Stats(C.Structure):
     ]
 
 
+class _PRResult(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int32),
+        ("residual", C.c_double),
+        ("ms", C.c_double),
+        ("gather_ms", C.c_double),
+        ("gathered_edges", C.c_int64),
+    ]
+
+
+class _BatchResult(C.Structure):
+    _fields_ = [("levels", C.c_int32), ("ms", C.c_double), ("grid", C.c_int32)]
+
+
 _P = C.c_void_p
 _I32P = C.POINTER(C.c_int32)
 _I64P = C.POINTER(C.c_int64)
@@ -88,6 +103,7 @@ _SIGS = {
     "gr_device_free": (C.c_int, [_P]),
     "gr_memcpy_h2d": (C.c_int, [_P, _P, C.c_size_t]),
     "gr_memcpy_d2h": (C.c_int, [_P, _P, C.c_size_t]),
+    "gr_device_copy": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(C.c_float)]),
     "gr_host_rmat_edges": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                       C.c_uint64, C.c_int, _I32P, _I32P, C.c_int64, _I64P]),
     "gr_host_grid_edges": (C.c_int, [C.c_int, C.c_double, C.c_uint64, _I32P, _I32P,
@@ -124,9 +140,16 @@ _SIGS = {
     "gr_engine_read_vertex_data": (C.c_int, [_P, _P, C.c_size_t]),
     "gr_engine_reset_vertex_data": (C.c_int, [_P]),
     "gr_engine_set_direction": (C.c_int, [_P, C.c_int, C.c_double]),
+    "gr_engine_set_gather_tile": (C.c_int, [_P, C.c_int]),
     "gr_engine_enable_timing": (C.c_int, [_P, C.c_int]),
     "gr_engine_get_stats": (C.c_int, [_P, C.POINTER(_Stats)]),
     "gr_engine_reset_stats": (C.c_int, [_P]),
+    "gr_msbfs": (C.c_int, [_P, _I32P, C.c_int, _P, C.c_int, C.c_double,
+                            C.POINTER(_BatchResult)]),
+    "gr_mssp": (C.c_int, [_P, _P, _I32P, C.c_int, _P, C.POINTER(_BatchResult)]),
+    "gr_pagerank_precise": (C.c_int, [_P, C.c_int, C.c_double, C.c_int,
+                                       C.POINTER(C.c_float), C.POINTER(C.c_double),
+                                       C.POINTER(_PRResult)]),
 }
 
 _lib = None
@@ -179,6 +202,23 @@ def device_info() -> dict:
 
 def synchronize() -> None:
     _check(lib().gr_device_synchronize())
+
+
+def device_copy_gbs(nbytes: int = 1 << 30, repeats: int = 5) -> float:
+    """Best read+write bandwidth (GB/s) of a plain device copy of nbytes."""
+    src, dst = C.c_void_p(), C.c_void_p()
+    _check(lib().gr_device_alloc(nbytes, C.byref(src)))
+    _check(lib().gr_device_alloc(nbytes, C.byref(dst)))
+    try:
+        best = 0.0
+        for _ in range(repeats):
+            ms = C.c_float(0)
+            _check(lib().gr_device_copy(dst, src, nbytes, C.byref(ms)))
+            best = max(best, 2 * nbytes / (ms.value * 1e-3) / 1e9)
+        return best
+    finally:
+        lib().gr_device_free(src)
+        lib().gr_device_free(dst)
 
 
 # ------------------------------------------------------------------ inputs
@@ -440,6 +480,10 @@ class GASEngine:
         """0 = automatic push/pull, 1 = push only, 2 = pull only."""
         _check(lib().gr_engine_set_direction(self._h, mode, alpha))
 
+    def setGatherTile(self, items_per_thread: int) -> None:
+        """Merge-path items per thread of the gather tiles (4, 8 or 16)."""
+        _check(lib().gr_engine_set_gather_tile(self._h, items_per_thread))
+
     def enableTiming(self, on: bool = True) -> None:
         _check(lib().gr_engine_enable_timing(self._h, int(on)))
 
@@ -546,3 +590,76 @@ def run_pagerank(graph_or_edges, out_degree, direction: int = 0):
     out = eng.vertexData["rank"].copy()
     eng.close()
     return out, it
+
+
+def pagerank_precise(graph: Graph, max_iters: int = 50, tol: float = 1e-7,
+                     compensated: bool = True):
+    """All-active PageRank (FP32, compensated sums optional, FP64 L1 residual).
+
+    Returns (ranks float32[V], info) with info = iterations, residual (last
+    step), residuals (per step), ms, gather_ms, gathered_edges."""
+    ranks = np.empty(graph.n_vertices, np.float32)
+    resid = np.zeros(max(1, max_iters), np.float64)
+    r = _PRResult()
+    _check(lib().gr_pagerank_precise(graph.handle, max_iters, tol, int(compensated),
+                                     ranks.ctypes.data_as(C.POINTER(C.c_float)),
+                                     resid.ctypes.data_as(C.POINTER(C.c_double)), C.byref(r)))
+    info = {name: getattr(r, name) for name, _ in _PRResult._fields_}
+    info["residuals"] = resid[: r.iterations].tolist()
+    return ranks, info
+
+
+def _batch_sources(sources):
+    src = _i32(sources)
+    if not 1 <= len(src) <= 64:
+        raise GRError(1, "batched runs take 1..64 sources")
+    return src
+
+
+def msbfs(graph: Graph, sources, direction: int = 0, alpha: float = 20.0):
+    """BFS from up to 64 sources in one persistent launch.
+
+    Returns (depth int32[len(sources), V], info)."""
+    src = _batch_sources(sources)
+    n = graph.n_vertices
+    nbytes = 4 * n * len(src)
+    d = C.c_void_p()
+    _check(lib().gr_device_alloc(nbytes, C.byref(d)))
+    try:
+        r = _BatchResult()
+        _check(lib().gr_msbfs(graph.handle, _p32(src), len(src), d, direction, alpha, C.byref(r)))
+        out = np.empty((len(src), n), np.int32)
+        _check(lib().gr_memcpy_d2h(out.ctypes.data, d, nbytes))
+    finally:
+        lib().gr_device_free(d)
+    return out, {name: getattr(r, name) for name, _ in _BatchResult._fields_}
+
+
+def mssp(graph: Graph, weights: "DeviceArray", sources):
+    """SSSP from up to 64 sources (64 lanes per wave, hipGraph-replayed rounds).
+
+    weights: DeviceArray of int32 per edge in the graph's edge-list order.
+    Returns (dist int32[len(sources), V], info)."""
+    src = _batch_sources(sources)
+    n = graph.n_vertices
+    nbytes = 4 * n * len(src)
+    d = C.c_void_p()
+    _check(lib().gr_device_alloc(nbytes, C.byref(d)))
+    try:
+        r = _BatchResult()
+        _check(lib().gr_mssp(graph.handle, weights.ptr, _p32(src), len(src), d, C.byref(r)))
+        out = np.empty((len(src), n), np.int32)
+        _check(lib().gr_memcpy_d2h(out.ctypes.data, d, nbytes))
+    finally:
+        lib().gr_device_free(d)
+    return out, {name: getattr(r, name) for name, _ in _BatchResult._fields_}
+
+
+def upload_i32(a) -> DeviceArray:
+    """Copies a host int32 array into a new DeviceArray."""
+    h = _i32(a)
+    d = C.c_void_p()
+    _check(lib().gr_device_alloc(max(1, h.nbytes), C.byref(d)))
+    if h.nbytes:
+        _check(lib().gr_memcpy_h2d(d, h.ctypes.data, h.nbytes))
+    return DeviceArray(d, h.nbytes)

@@ -45,6 +45,8 @@ SOURCES = [
     "gr_host_graph.cpp",
     "gr_graph.hip",
     "gr_engine.hip",
+    "gr_pagerank.hip",
+    "gr_batched.hip",
 ]
 
 APP_SOURCES = ["bfs.cpp", "sssp.cpp", "pagerank.cpp", "connected_component.cpp"]
@@ -104,7 +106,7 @@ def build_apps(verbose: bool = False) -> list[Path]:
         exe = BINDIR / Path(src).stem
         outs.append(exe)
         cmds.append([
-            cxx, "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}",
+            cxx, "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}", f"-I{ROOT / 'include' / 'graphreduce_amd'}",
             str(APPS / src), "-o", str(exe),
             f"-L{LIBDIR}", "-lgraphreduce_amd", "-Wl,-rpath,$ORIGIN/../lib",
         ])

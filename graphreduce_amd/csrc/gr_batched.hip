@@ -1,0 +1,476 @@
+// Batched multi-source traversals: up to 64 BFS or SSSP sources share one
+// pass over the graph, one bit (BFS) or one lane (SSSP) per source.
+//
+// Multi-source BFS (k_msbfs): ONE persistent cooperative launch runs every
+// level.  Per vertex three 64-bit masks live in HBM: visit (sources that
+// reached it), frontier (sources for which it is on the current level) and
+// next.  Each level is an expand phase and an update phase separated by grid
+// barriers:
+//   expand, push  every frontier vertex ORs its frontier mask into next[u] of
+//                 its CSR out-neighbours (skipping those that every one of its
+//                 sources already visited);
+//   expand, pull  every vertex with unvisited sources ORs the frontier masks of
+//                 its CSC in-neighbours, stopping once all of its unvisited
+//                 sources are covered;
+//   update        new = next & ~visit; visit |= new; frontier = new; the
+//                 level is written for every new (source, vertex) pair.
+// The direction is chosen per level from the frontier's out-edge count.
+// Depths equal those of independent plain runs (the BFS level of a vertex
+// does not depend on the schedule), which the tests check.
+//
+// Multi-source SSSP (k_mssp_round): Bellman-Ford rounds over 64 lanes, one
+// wave per destination vertex, lane s relaxing source s; only in-edges whose
+// source changed in the previous round are read (a source that did not
+// change cannot lower a distance it already lowered), so distances equal the
+// plain engine's fixed point.  Rounds are recorded once into a hipGraph and
+// replayed.
+#include <hip/hip_cooperative_groups.h>
+
+#include <vector>
+
+#include "gr_device.h"
+#include "gr_gather.h"
+
+namespace cg = cooperative_groups;
+
+namespace gr {
+
+constexpr int kBatchNT = 256;
+constexpr int kMsHeavy = 64;  // out-/in-degree above which a wave cooperates on a vertex
+
+struct MsCounters {
+  unsigned long long new_vertices[3];   // per level (ring of 3 slots)
+  unsigned long long frontier_edges[3]; // out-edges of the new frontier
+  int32_t levels;
+  int32_t error;
+};
+
+// OR-reduction of a 64-bit value over the wave.
+__device__ __forceinline__ uint64_t wave_or64(uint64_t v) {
+  uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    lo |= __shfl_xor(lo, off, kWave);
+    hi |= __shfl_xor(hi, off, kWave);
+  }
+  return ((uint64_t)hi << 32) | lo;
+}
+
+__global__ void k_ms_init(int nV, uint64_t* __restrict__ visit, uint64_t* __restrict__ frontier,
+                          uint64_t* __restrict__ next) {
+  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nV; v += gridDim.x * blockDim.x) {
+    visit[v] = 0;
+    frontier[v] = 0;
+    next[v] = 0;
+  }
+}
+
+__global__ void k_ms_sources(int ns, const int32_t* __restrict__ sources, int nV,
+                             uint64_t* __restrict__ visit, uint64_t* __restrict__ frontier,
+                             int32_t* __restrict__ depth, const int32_t* __restrict__ off_out,
+                             MsCounters* __restrict__ ctr) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  unsigned long long fe = 0;
+  for (int s = 0; s < ns; ++s) {
+    const int v = sources[s];
+    if (frontier[v] == 0) fe += (unsigned long long)(off_out[v + 1] - off_out[v]);
+    visit[v] |= 1ull << s;
+    frontier[v] |= 1ull << s;
+    depth[(size_t)s * nV + v] = 0;
+  }
+  for (int i = 0; i < 3; ++i) {
+    ctr->new_vertices[i] = 0;
+    ctr->frontier_edges[i] = 0;
+  }
+  ctr->new_vertices[0] = (unsigned long long)ns;
+  ctr->frontier_edges[0] = fe;
+  ctr->levels = 0;
+  ctr->error = 0;
+}
+
+__global__ __launch_bounds__(kBatchNT) void k_msbfs(
+    int nV, long long nE, int ns, int mode, double alpha, const int32_t* __restrict__ off_out,
+    const int32_t* __restrict__ dsts, const int32_t* __restrict__ off_in,
+    const int32_t* __restrict__ srcs, uint64_t* __restrict__ visit,
+    uint64_t* __restrict__ frontier, uint64_t* __restrict__ next, int32_t* __restrict__ depth,
+    MsCounters* __restrict__ ctr, int max_levels) {
+  cg::grid_group grid = cg::this_grid();
+  const uint64_t all = ns >= 64 ? ~0ull : ((1ull << ns) - 1);
+  const int lane = lane_id();
+  const int wpb = blockDim.x / kWave;
+  const int wave_id = blockIdx.x * wpb + threadIdx.x / kWave;
+  const int n_waves = gridDim.x * wpb;
+  const long long threshold = (long long)((double)nE / (alpha > 0 ? alpha : 1.0));
+  for (int level = 0; level < max_levels; ++level) {
+    const int slot = level % 3;
+    if (ctr->new_vertices[slot] == 0) break;  // uniform: read after a grid barrier
+    const bool pull = mode == 2 || (mode == 0 && (long long)ctr->frontier_edges[slot] > threshold);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      const int nx = (level + 1) % 3;
+      ctr->new_vertices[nx] = 0;
+      ctr->frontier_edges[nx] = 0;
+    }
+    // ---------------------------------------------------------- expand
+    for (int base = wave_id * kWave; base < nV; base += n_waves * kWave) {
+      const int v = base + lane;
+      const bool valid = v < nV;
+      if (!pull) {
+        const uint64_t f = valid ? frontier[v] : 0ull;
+        int b = 0, e = 0;
+        if (f) {
+          b = off_out[v];
+          e = off_out[v + 1];
+        }
+        const bool light = (e - b) <= kMsHeavy;
+        if (f && light) {
+          for (int p = b; p < e; ++p) {
+            const int u = dsts[p];
+            if (f & ~visit[u]) atomicOr((unsigned long long*)&next[u], (unsigned long long)f);
+          }
+        }
+        uint64_t heavy = __ballot(f && !light);
+        while (heavy) {
+          const int hl = __builtin_ctzll(heavy);
+          heavy &= heavy - 1;
+          const int hb = __shfl(b, hl, kWave), he = __shfl(e, hl, kWave);
+          const uint32_t flo = __shfl((uint32_t)f, hl, kWave);
+          const uint32_t fhi = __shfl((uint32_t)(f >> 32), hl, kWave);
+          const uint64_t hf = ((uint64_t)fhi << 32) | flo;
+          for (int p = hb + lane; p < he; p += kWave) {
+            const int u = dsts[p];
+            if (hf & ~visit[u]) atomicOr((unsigned long long*)&next[u], (unsigned long long)hf);
+          }
+        }
+      } else {
+        const uint64_t need = valid ? (all & ~visit[v]) : 0ull;
+        int b = 0, e = 0;
+        if (need) {
+          b = off_in[v];
+          e = off_in[v + 1];
+        }
+        const bool light = (e - b) <= kMsHeavy;
+        uint64_t acc = 0;
+        if (need && light) {
+          for (int p = b; p < e; ++p) {
+            acc |= frontier[srcs[p]];
+            if ((acc & need) == need) break;
+          }
+          next[v] = acc & need;
+        }
+        uint64_t heavy = __ballot(need && !light);
+        while (heavy) {
+          const int hl = __builtin_ctzll(heavy);
+          heavy &= heavy - 1;
+          const int hb = __shfl(b, hl, kWave), he = __shfl(e, hl, kWave);
+          const uint32_t nlo = __shfl((uint32_t)need, hl, kWave);
+          const uint32_t nhi = __shfl((uint32_t)(need >> 32), hl, kWave);
+          const uint64_t hn = ((uint64_t)nhi << 32) | nlo;
+          uint64_t hacc = 0;
+          for (int pb = hb; pb < he; pb += kWave) {  // wave-uniform trip count
+            const int p = pb + lane;
+            const uint64_t m = p < he ? frontier[srcs[p]] : 0ull;
+            hacc |= wave_or64(m);
+            if ((hacc & hn) == hn) break;
+          }
+          if (lane == hl) next[v] = hacc & hn;
+        }
+      }
+    }
+    grid.sync();
+    // ---------------------------------------------------------- update
+    const int slot_next = (level + 1) % 3;
+    for (int base = wave_id * kWave; base < nV; base += n_waves * kWave) {
+      const int v = base + lane;
+      uint64_t nw = 0;
+      int deg = 0;
+      if (v < nV) {
+        const uint64_t vis = visit[v];
+        nw = next[v] & ~vis & all;
+        frontier[v] = nw;
+        if (nw) {
+          next[v] = 0;
+          visit[v] = vis | nw;
+          deg = off_out[v + 1] - off_out[v];
+          uint64_t bits = nw;
+          while (bits) {
+            const int s = __builtin_ctzll(bits);
+            bits &= bits - 1;
+            depth[(size_t)s * nV + v] = level + 1;
+          }
+        } else if (!pull) {
+          next[v] = 0;
+        }
+      }
+      const uint64_t nb = __ballot(nw != 0);
+      unsigned long long fe = (unsigned long long)deg;
+#pragma unroll
+      for (int off = kWave / 2; off > 0; off >>= 1) fe += __shfl_xor(fe, off, kWave);
+      if (lane == 0 && nb) {
+        atomicAdd(&ctr->new_vertices[slot_next], (unsigned long long)__popcll(nb));
+        atomicAdd(&ctr->frontier_edges[slot_next], fe);
+      }
+    }
+    grid.sync();
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctr->levels = level + 1;
+  }
+}
+
+// ------------------------------------------------------------ batched SSSP
+
+// dist is vertex-major: dist[v * 64 + s].  One wave per destination vertex.
+__global__ __launch_bounds__(kBatchNT) void k_mssp_round(
+    int nV, const int32_t* __restrict__ off_in, const int32_t* __restrict__ srcs,
+    const int32_t* __restrict__ w_csc, int32_t* __restrict__ dist,
+    const uint64_t* __restrict__ chg_in, uint64_t* __restrict__ chg_out,
+    unsigned long long* __restrict__ changed_count) {
+  const int lane = lane_id();
+  const int wpb = blockDim.x / kWave;
+  const int n_waves = gridDim.x * wpb;
+  const uint64_t my_bit = 1ull << lane;
+  unsigned long long cnt = 0;
+  for (int v = blockIdx.x * wpb + threadIdx.x / kWave; v < nV; v += n_waves) {
+    const int b = off_in[v], e = off_in[v + 1];
+    int32_t best = dist[(size_t)v * kWave + lane];
+    const int32_t before = best;
+    for (int pb = b; pb < e; pb += kWave) {
+      // each lane fetches one in-edge; the wave then walks the changed ones
+      const int p = pb + lane;
+      int u = 0, wt = 0;
+      uint64_t cu = 0;
+      if (p < e) {
+        u = srcs[p];
+        cu = chg_in[u];
+        if (cu) wt = w_csc[p];
+      }
+      uint64_t todo = __ballot(cu != 0);
+      while (todo) {
+        const int k = __builtin_ctzll(todo);
+        todo &= todo - 1;
+        const int uu = __shfl(u, k, kWave);
+        const int ww = __shfl(wt, k, kWave);
+        const uint32_t clo = __shfl((uint32_t)cu, k, kWave);
+        const uint32_t chi = __shfl((uint32_t)(cu >> 32), k, kWave);
+        const uint64_t cm = ((uint64_t)chi << 32) | clo;
+        if (cm & my_bit) {
+          const int32_t du = dist[(size_t)uu * kWave + lane];
+          const int32_t cand = du + ww;
+          if (cand < best) best = cand;
+        }
+      }
+    }
+    const bool improved = best < before;
+    if (improved) dist[(size_t)v * kWave + lane] = best;
+    const uint64_t m = __ballot(improved);
+    if (lane == 0) {
+      chg_out[v] = m;
+      if (m) cnt += 1;
+    }
+  }
+  if (lane == 0 && cnt) atomicAdd(changed_count, cnt);
+}
+
+__global__ void k_mssp_init(int nV, int ns, const int32_t* __restrict__ sources,
+                            int32_t* __restrict__ dist, uint64_t* __restrict__ chg,
+                            int32_t inf) {
+  const size_t n = (size_t)nV * kWave;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    dist[i] = inf;
+    if ((i & 63) == 0) chg[i >> 6] = 0;
+  }
+}
+
+__global__ void k_mssp_sources(int ns, const int32_t* __restrict__ sources,
+                               int32_t* __restrict__ dist, uint64_t* __restrict__ chg) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int s = 0; s < ns; ++s) {
+    const int v = sources[s];
+    dist[(size_t)v * kWave + s] = 0;
+    chg[v] |= 1ull << s;
+  }
+}
+
+// dist (vertex-major, 64 lanes) -> out[s * nV + v]
+__global__ void k_mssp_transpose(int nV, int ns, const int32_t* __restrict__ dist,
+                                 int32_t* __restrict__ out) {
+  const size_t n = (size_t)nV * ns;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const size_t s = i / nV, v = i % nV;
+    out[i] = dist[v * kWave + s];
+  }
+}
+
+__global__ void k_permute_i32(const int32_t* __restrict__ in, const int32_t* __restrict__ eid,
+                              int64_t n, int32_t* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[eid ? eid[i] : i];
+}
+
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  ~DBuf() {
+    if (p) (void)hipFree(p);
+  }
+  int alloc(size_t n) {
+    if (n < 1) n = 1;
+    hipError_t e = hipMalloc((void**)&p, sizeof(T) * n);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return set_errorf(GR_ERR_NOMEM, "hipMalloc(%zu bytes): %s", sizeof(T) * n,
+                        hipGetErrorString(e));
+    }
+    return GR_OK;
+  }
+};
+
+}  // namespace gr
+
+using namespace gr;
+
+extern "C" int gr_msbfs(gr_graph* g, const int32_t* sources, int n_sources, int32_t* depth_device,
+                        int direction, double alpha, gr_batch_result* result) {
+  if (!g || !sources || !depth_device || n_sources < 1 || n_sources > 64 || direction < 0 ||
+      direction > 2)
+    return set_error(GR_ERR_INVALID, "gr_msbfs: bad arguments");
+  const int nV = g->nV;
+  for (int s = 0; s < n_sources; ++s)
+    if (sources[s] < 0 || sources[s] >= nV)
+      return set_errorf(GR_ERR_INVALID, "gr_msbfs: source %d out of range", sources[s]);
+  DBuf<uint64_t> visit, frontier, next;
+  DBuf<int32_t> d_src;
+  DBuf<MsCounters> ctr;
+  GR_TRY(visit.alloc(nV));
+  GR_TRY(frontier.alloc(nV));
+  GR_TRY(next.alloc(nV));
+  GR_TRY(d_src.alloc(n_sources));
+  GR_TRY(ctr.alloc(1));
+  GR_HIP(hipMemcpy(d_src.p, sources, sizeof(int32_t) * n_sources, hipMemcpyHostToDevice));
+  // depth = -1 everywhere: the counterpart of the host-side vertex init before setGraph
+  GR_HIP(hipMemset(depth_device, 0xFF, sizeof(int32_t) * (size_t)nV * n_sources));
+  int dev = 0, cus = 0, per_cu = 0;
+  GR_HIP(hipGetDevice(&dev));
+  GR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  GR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_msbfs, kBatchNT, 0));
+  if (per_cu < 1) return set_error(GR_ERR_HIP, "gr_msbfs: kernel cannot be resident");
+  const int grid = cus * per_cu;
+  hipEvent_t e0, e1;
+  GR_HIP(hipEventCreate(&e0));
+  GR_HIP(hipEventCreate(&e1));
+  GR_HIP(hipDeviceSynchronize());
+  GR_HIP(hipEventRecord(e0, 0));
+  k_ms_init<<<1024, 256>>>(nV, visit.p, frontier.p, next.p);
+  k_ms_sources<<<1, 64>>>(n_sources, d_src.p, nV, visit.p, frontier.p, depth_device, g->off_out,
+                          ctr.p);
+  GR_HIP(hipGetLastError());
+  int nVa = nV, ns = n_sources, mode = direction, max_levels = nV + 1;
+  long long nE = g->nE;
+  double al = alpha;
+  const int32_t *off_out = g->off_out, *dsts = g->dsts, *off_in = g->off_in, *srcs = g->srcs;
+  uint64_t *pv = visit.p, *pf = frontier.p, *pn = next.p;
+  int32_t* pd = depth_device;
+  MsCounters* pc = ctr.p;
+  void* args[] = {&nVa, &nE, &ns, &mode, &al, &off_out, &dsts, &off_in, &srcs,
+                  &pv,  &pf, &pn, &pd,   &pc, &max_levels};
+  GR_HIP(hipLaunchCooperativeKernel((const void*)k_msbfs, dim3(grid), dim3(kBatchNT), args, 0, 0));
+  GR_HIP(hipEventRecord(e1, 0));
+  GR_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  GR_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  MsCounters hc;
+  GR_HIP(hipMemcpy(&hc, ctr.p, sizeof(hc), hipMemcpyDeviceToHost));
+  if (result) {
+    result->levels = hc.levels;
+    result->ms = ms;
+    result->grid = grid;
+  }
+  return GR_OK;
+}
+
+extern "C" int gr_mssp(gr_graph* g, const int32_t* weights_device, const int32_t* sources,
+                       int n_sources, int32_t* dist_device, gr_batch_result* result) {
+  if (!g || !weights_device || !sources || !dist_device || n_sources < 1 || n_sources > 64)
+    return set_error(GR_ERR_INVALID, "gr_mssp: bad arguments");
+  const int nV = g->nV;
+  for (int s = 0; s < n_sources; ++s)
+    if (sources[s] < 0 || sources[s] >= nV)
+      return set_errorf(GR_ERR_INVALID, "gr_mssp: source %d out of range", sources[s]);
+  const int32_t inf = 2147483647 - 100000;  // SSSP::gatherZero, sssp.cu:32-33
+  DBuf<int32_t> dist, w_csc, d_src;
+  DBuf<uint64_t> chg_a, chg_b;
+  DBuf<unsigned long long> cnt;
+  GR_TRY(dist.alloc((size_t)nV * kWave));
+  GR_TRY(w_csc.alloc((size_t)g->nE));
+  GR_TRY(d_src.alloc(n_sources));
+  GR_TRY(chg_a.alloc(nV));
+  GR_TRY(chg_b.alloc(nV));
+  GR_TRY(cnt.alloc(64));
+  GR_HIP(hipMemcpy(d_src.p, sources, sizeof(int32_t) * n_sources, hipMemcpyHostToDevice));
+  if (g->nE)
+    k_permute_i32<<<grid_1d(g->nE, 256), 256>>>(weights_device, g->csc_eid, g->nE, w_csc.p);
+  hipStream_t st;
+  GR_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  GR_HIP(hipDeviceSynchronize());
+  hipEvent_t e0, e1;
+  GR_HIP(hipEventCreate(&e0));
+  GR_HIP(hipEventCreate(&e1));
+  GR_HIP(hipEventRecord(e0, st));
+  k_mssp_init<<<4096, 256, 0, st>>>(nV, n_sources, d_src.p, dist.p, chg_a.p, inf);
+  k_mssp_sources<<<1, 64, 0, st>>>(n_sources, d_src.p, dist.p, chg_a.p);
+  GR_HIP(hipGetLastError());
+  // One graph holds kRounds rounds (ping-pong of the changed masks); replays
+  // continue until a round changes nothing.
+  constexpr int kRounds = 8;
+  const unsigned rgrid = grid_1d((int64_t)nV, kBatchNT / kWave);
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+  GR_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  for (int r = 0; r < kRounds; ++r) {
+    uint64_t* cin = (r & 1) ? chg_b.p : chg_a.p;
+    uint64_t* cout = (r & 1) ? chg_a.p : chg_b.p;
+    k_mssp_round<<<rgrid < 65536 ? rgrid : 65536, kBatchNT, 0, st>>>(
+        nV, g->off_in, g->srcs, w_csc.p, dist.p, cin, cout, cnt.p + r);
+  }
+  GR_HIP(hipStreamEndCapture(st, &graph));
+  GR_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  int rounds = 0;
+  unsigned long long hc[kRounds];
+  for (;;) {
+    // counters are cleared by a kernel-free copy from a zeroed host array
+    unsigned long long zeros[kRounds] = {0};
+    GR_HIP(hipMemcpyAsync(cnt.p, zeros, sizeof(zeros), hipMemcpyHostToDevice, st));
+    GR_HIP(hipGraphLaunch(exec, st));
+    GR_HIP(hipMemcpyAsync(hc, cnt.p, sizeof(hc), hipMemcpyDeviceToHost, st));
+    GR_HIP(hipStreamSynchronize(st));
+    int last = -1;
+    for (int r = 0; r < kRounds; ++r)
+      if (hc[r]) last = r;
+    if (last < kRounds - 1) {
+      rounds += last + 1;
+      break;
+    }
+    rounds += kRounds;
+  }
+  if (true) {
+    k_mssp_transpose<<<4096, 256, 0, st>>>(nV, n_sources, dist.p, dist_device);
+    GR_HIP(hipGetLastError());
+  }
+  GR_HIP(hipEventRecord(e1, st));
+  GR_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  GR_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipGraphExecDestroy(exec);
+  (void)hipGraphDestroy(graph);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(st);
+  if (result) {
+    result->levels = rounds;
+    result->ms = ms;
+    result->grid = (int)rgrid;
+  }
+  return GR_OK;
+}

@@ -13,8 +13,8 @@
 //                    to the changed list together with their out-edge offsets.
 //   scatterActivate  push: k_push_partition + k_push_tiles set one bit per
 //                    out-neighbour in the next-frontier bitmap; pull:
-//                    k_pull_activate tests each vertex's in-edges against the
-//                    changed bitmap.
+//                    k_pull_stream / k_pull_heavy test each vertex's in-edges
+//                    against the changed bitmap.
 //   nextIter         k_compact_count/scan/write: bitmap -> ascending active
 //                    list plus the exclusive scan of its in-degrees (the
 //                    gather's row offsets); one 32-byte device->host read
@@ -25,330 +25,32 @@
 #include <vector>
 
 #include "gr_device.h"
+#include "gr_gather.h"
 #include "gr_programs.h"
 
 namespace gr {
 
-constexpr int kNT = 256;            // threads per tile workgroup (4 waves)
-constexpr int kVT = 8;              // merge-path items per thread
-constexpr int kTile = kNT * kVT;    // merge-path items per tile
-constexpr int kCompactNT = 256;     // compaction: one bitmap word per thread
-constexpr int kMaxGrid = 1 << 20;
-
-struct DevCounters {
-  int32_t n_active;               // size of the active list built by nextIter
-  int32_t w_edges;                // sum of in-degrees over that list
-  unsigned long long chg;         // (#changed << 32) | out-edges, appended by apply
-  unsigned long long chg_last;    // chg as seen by the last nextIter
-  int32_t pull;                   // last scatterActivate ran in pull mode
-  int32_t pad[3];
-};
-
-template <class T>
-struct Part {
-  int32_t row;
-  int32_t has;
-  T val;
-};
-
-static inline unsigned grid_1d(int64_t n, int block) {
-  int64_t g = (n + block - 1) / block;
-  if (g < 1) g = 1;
-  if (g > kMaxGrid) g = kMaxGrid;
-  return (unsigned)g;
-}
-
-// ------------------------------------------------------------ merge path
-
-// Number of row ends consumed on diagonal d of the merge of rowEnd[0..n)
-// (inclusive-scan row ends) with the edge items [0, w).  Edge j belongs to row
-// i while j < rowEnd[i].
-__device__ __forceinline__ int merge_path_search(int d, const int32_t* __restrict__ rowEnd,
-                                                 int n, int w) {
-  int lo = max(d - w, 0), hi = min(d, n);
-  while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    if (rowEnd[mid] <= d - mid - 1)
-      lo = mid + 1;
-    else
-      hi = mid;
-  }
-  return lo;
-}
-
-__global__ void k_merge_partition(const int32_t* __restrict__ escan, int n, int w, int n_tiles,
-                                  int32_t* __restrict__ part) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t > n_tiles) return;
-  int64_t d = min((int64_t)t * kTile, (int64_t)n + w);
-  part[t] = merge_path_search((int)d, escan + 1, n, w);
-}
-
-// ------------------------------------------------- segmented lane scan
-
-template <class P>
-struct Seg {
-  typedef typename P::GatherResult G;
-  int flag;  // this lane emitted a row (its tail starts a new row)
-  int has;
-  G val;
-
-  __device__ static Seg identity() {
-    Seg s;
-    s.flag = 0;
-    s.has = 0;
-    s.val = G();
-    return s;
-  }
-  // a precedes b on the merge path
-  __device__ static Seg combine(const Seg& a, const Seg& b) {
-    Seg r;
-    r.flag = a.flag | b.flag;
-    if (b.flag || !a.has) {
-      r.has = b.has;
-      r.val = b.val;
-    } else if (!b.has) {
-      r.has = a.has;
-      r.val = a.val;
-    } else {
-      r.has = 1;
-      r.val = P::gatherReduce(a.val, b.val);
-    }
-    return r;
-  }
-  __device__ static Seg shfl_up(const Seg& s, int off) {
-    Seg r;
-    r.flag = __shfl_up(s.flag, off, kWave);
-    r.has = __shfl_up(s.has, off, kWave);
-    r.val = __shfl_up(s.val, off, kWave);
-    return r;
-  }
-};
-
-template <class P>
-__device__ __forceinline__ typename P::GatherResult finalize(int has,
-                                                             typename P::GatherResult v) {
-  return has ? P::gatherReduce(P::gatherZero, v) : P::gatherZero;
-}
-
-// ------------------------------------------------------------------ gather
-
-template <class P>
-__global__ __launch_bounds__(kNT) void k_gather_tiles(
-    int n, int w, const int32_t* __restrict__ active, const int32_t* __restrict__ escan,
-    const int32_t* __restrict__ part, const int32_t* __restrict__ off_in,
-    const int32_t* __restrict__ srcs, const typename P::VertexData* __restrict__ vd,
-    const typename P::EdgeData* __restrict__ ed, typename P::GatherResult* __restrict__ gtmp,
-    Part<typename P::GatherResult>* __restrict__ carry,
-    Part<typename P::GatherResult>* __restrict__ head) {
-  typedef typename P::GatherResult G;
-  typedef Seg<P> S;
-  __shared__ int32_t s_scan[kTile + 2];
-  __shared__ int32_t s_base[kTile + 1];
-  __shared__ uint16_t s_row[kTile];
-  __shared__ G s_val[kTile];
-  __shared__ S s_wave[kNT / kWave];
-
-  const int t = blockIdx.x;
-  const int tid = threadIdx.x;
-  const int total = n + w;
-  const int d0 = t * kTile;
-  const int d1 = min(d0 + kTile, total);
-  const int i0 = part[t], i1 = part[t + 1];
-  const int j0 = d0 - i0, j1 = d1 - i1;
-  const int n_rows = i1 - i0;                    // row ends inside this tile
-  const int n_ref = n_rows + (i1 < n ? 1 : 0);   // plus the row still open at the end
-  const int n_edges = j1 - j0;
-
-  for (int k = tid; k <= n_rows; k += kNT) s_scan[k] = escan[i0 + k];
-  for (int k = tid; k < n_ref; k += kNT) s_base[k] = off_in[active[i0 + k]] - escan[i0 + k];
-  __syncthreads();
-
-  // Phase 1: each lane walks its kVT path items and labels edges with rows.
-  const int dl = tid * kVT;
-  const int steps = max(0, min(kVT, (d1 - d0) - dl));
-  int ri = 0, ej = 0;
-  if (steps > 0) {
-    int lo = max(dl - n_edges, 0), hi = min(dl, n_rows);
-    while (lo < hi) {
-      int mid = (lo + hi) >> 1;
-      if (s_scan[mid + 1] - j0 <= dl - mid - 1)
-        lo = mid + 1;
-      else
-        hi = mid;
-    }
-    ri = lo;
-    ej = dl - lo;
-  }
-  const int ri0 = ri, ej0 = ej;
-  for (int s = 0; s < steps; ++s) {
-    if (ri < n_rows && s_scan[ri + 1] - j0 <= ej) {
-      ++ri;
-    } else {
-      s_row[ej] = (uint16_t)ri;
-      ++ej;
-    }
-  }
-  __syncthreads();
-
-  // Phase 2: coalesced pass over the tile's edges: CSC source, gatherMap.
-  {
-    int src_v[kVT], csc_v[kVT];
-#pragma unroll
-    for (int m = 0; m < kVT; ++m) {
-      int e = tid + m * kNT;
-      if (e < n_edges) {
-        int k = s_row[e];
-        csc_v[m] = s_base[k] + j0 + e;
-        src_v[m] = srcs[csc_v[m]];
-      }
-    }
-#pragma unroll
-    for (int m = 0; m < kVT; ++m) {
-      int e = tid + m * kNT;
-      if (e < n_edges) {
-        const typename P::VertexData* dstp = vd;
-        if constexpr (P::kGatherUsesDst) dstp = vd + active[i0 + s_row[e]];
-        s_val[e] = P::gatherMap(dstp, vd + src_v[m], ed + csc_v[m]);
-      }
-    }
-  }
-  __syncthreads();
-
-  // Phase 3: sequential reduction along the lane's path items.
-  ri = ri0;
-  ej = ej0;
-  G acc = G();
-  int has = 0;
-  int emitted = 0, first_row = 0, first_has = 0;
-  G first_val = G();
-  for (int s = 0; s < steps; ++s) {
-    if (ri < n_rows && s_scan[ri + 1] - j0 <= ej) {
-      if (!emitted) {
-        emitted = 1;
-        first_row = ri;
-        first_val = acc;
-        first_has = has;
-      } else {
-        gtmp[i0 + ri] = finalize<P>(has, acc);
-      }
-      has = 0;
-      ++ri;
-    } else {
-      G v = s_val[ej];
-      acc = has ? P::gatherReduce(acc, v) : v;
-      has = 1;
-      ++ej;
-    }
-  }
-
-  // Segmented exclusive scan of the lanes' open tails.
-  S mine;
-  mine.flag = emitted;
-  mine.has = has;
-  mine.val = acc;
-  const int lane = lane_id(), wv = tid / kWave;
-  S incl = mine;
-#pragma unroll
-  for (int off = 1; off < kWave; off <<= 1) {
-    S o = S::shfl_up(incl, off);
-    if (lane >= off) incl = S::combine(o, incl);
-  }
-  if (lane == kWave - 1) s_wave[wv] = incl;
-  __syncthreads();
-  S wave_prefix = S::identity();
-  for (int i = 0; i < wv; ++i) wave_prefix = S::combine(wave_prefix, s_wave[i]);
-  S prev = S::shfl_up(incl, 1);
-  S excl = (lane == 0) ? wave_prefix : S::combine(wave_prefix, prev);
-
-  const bool head_open = n_rows > 0 && s_scan[0] < j0;  // row i0 began in an earlier tile
-  if (emitted) {
-    S fr;
-    fr.flag = 0;
-    fr.has = first_has;
-    fr.val = first_val;
-    S rv = S::combine(excl, fr);
-    if (first_row == 0 && head_open) {
-      Part<G> p;
-      p.row = i0;
-      p.has = rv.has;
-      p.val = rv.val;
-      head[t] = p;
-    } else {
-      gtmp[i0 + first_row] = finalize<P>(rv.has, rv.val);
-    }
-  }
-  if (tid == 0 && !head_open) head[t].row = -1;
-  if (tid == kNT - 1) {
-    S all = S::identity();
-    for (int i = 0; i < kNT / kWave; ++i) all = S::combine(all, s_wave[i]);
-    Part<G> p;
-    // The row open at the end carries over when some of its edges were here.
-    if (i1 < n && escan[i1] < j1) {
-      p.row = i1;
-      p.has = all.has;
-      p.val = all.val;
-    } else {
-      p.row = -1;
-      p.has = 0;
-      p.val = G();
-    }
-    carry[t] = p;
-  }
-}
-
-// Rows spanning several tiles: carries in tile order, then the closing
-// tile's head partial.
-template <class P>
-__global__ void k_gather_fixup(int n_tiles, const Part<typename P::GatherResult>* __restrict__ carry,
-                               const Part<typename P::GatherResult>* __restrict__ head,
-                               typename P::GatherResult* __restrict__ gtmp) {
-  typedef typename P::GatherResult G;
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n_tiles) return;
-  const int r = carry[t].row;
-  if (r < 0 || (t > 0 && carry[t - 1].row == r)) return;
-  int has = carry[t].has;
-  G acc = carry[t].val;
-  int u = t + 1;
-  while (u < n_tiles && carry[u].row == r) {
-    if (carry[u].has) {
-      acc = has ? P::gatherReduce(acc, carry[u].val) : carry[u].val;
-      has = 1;
-    }
-    ++u;
-  }
-  if (u < n_tiles && head[u].row == r && head[u].has) {
-    acc = has ? P::gatherReduce(acc, head[u].val) : head[u].val;
-    has = 1;
-  }
-  gtmp[r] = finalize<P>(has, acc);
-}
-
 // ------------------------------------------------------------------- apply
 
-// Appends the lanes with want != 0 to the changed list, keeping the
-// out-edge offsets increasing with the list index.
-__device__ __forceinline__ void append_changed(bool want, int v, int deg,
-                                               int32_t* __restrict__ chg_v,
-                                               int32_t* __restrict__ chg_e,
-                                               unsigned long long* __restrict__ counter) {
-  const uint64_t mask = __ballot(want);
-  if (mask == 0) return;
+// Sets bit v of `bits` for the lanes with set != 0.  The lanes hold
+// ascending vertex ids (lanes without a vertex pass valid = false and come
+// last), so lanes sharing a 32-bit word are contiguous: a segmented OR over
+// the wave leaves one atomic per word instead of one per vertex.
+__device__ __forceinline__ void wave_set_bits_sorted(bool valid, bool set, int v,
+                                                     uint32_t* __restrict__ bits) {
+  if (!__ballot(set)) return;
   const int lane = lane_id();
-  const int e = want ? deg : 0;
-  const int incl = wave_incl_scan_add(e);
-  const int total = __shfl(incl, kWave - 1, kWave);
-  const int cnt = __popcll(mask);
-  unsigned long long base = 0;
-  if (lane == 0)
-    base = atomicAdd(counter, ((unsigned long long)cnt << 32) | (unsigned long long)(uint32_t)total);
-  base = __shfl(base, 0, kWave);
-  if (want) {
-    const int idx = (int)(base >> 32) + lanes_below(mask);
-    chg_v[idx] = v;
-    chg_e[idx] = (int)(uint32_t)base + incl - e;
+  const uint32_t w = valid ? ((uint32_t)v >> 5) : 0xFFFFFFFFu;
+  uint32_t b = set ? (1u << (v & 31)) : 0u;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint32_t ow = __shfl_down(w, off, kWave);
+    const uint32_t ob = __shfl_down(b, off, kWave);
+    if (lane + off < kWave && ow == w) b |= ob;
   }
+  const uint32_t pw = __shfl_up(w, 1, kWave);
+  const bool leader = (lane == 0) || pw != w;
+  if (leader && b && w != 0xFFFFFFFFu) atomicOr(&bits[w], b);
 }
 
 template <class T>
@@ -362,33 +64,75 @@ __device__ __forceinline__ bool pod_equal(const T& a, const T& b) {
   return eq;
 }
 
+constexpr int kApplyNT = 256;
+constexpr int kApplyVT = 8;
+constexpr int kApplyChunk = kApplyNT * kApplyVT;
+
+// Program::apply over the active list.  Each workgroup takes a chunk of
+// kApplyChunk entries; the vertices whose apply returned true and that have
+// out-edges are appended to the changed list with ONE atomic per chunk (a
+// block scan assigns positions), keeping out-edge offsets increasing with the
+// list index; their bits are set in the changed bitmap for pull mode.
 template <class P>
-__global__ __launch_bounds__(256) void k_apply(
+__global__ __launch_bounds__(kApplyNT) void k_apply(
     int n, const int32_t* __restrict__ active, const typename P::GatherResult* __restrict__ gtmp,
     int use_gather, typename P::VertexData* __restrict__ vd, const int32_t* __restrict__ off_out,
     int iteration, int32_t* __restrict__ chg_v, int32_t* __restrict__ chg_e,
     uint32_t* __restrict__ chg_bits, DevCounters* __restrict__ ctr) {
-  const int lane = lane_id();
-  const int wpb = blockDim.x / kWave;
-  const int nw = gridDim.x * wpb;
-  for (int base = (blockIdx.x * wpb + threadIdx.x / kWave) * kWave; base < n; base += nw * kWave) {
-    const int i = base + lane;
-    bool want = false;
-    int dv = 0, deg = 0;
-    if (i < n) {
-      dv = active[i];
-      typename P::GatherResult g = use_gather ? gtmp[i] : P::gatherZero;
-      typename P::VertexData v = vd[dv];
-      const typename P::VertexData old = v;
-      bool r = P::apply(&v, g, iteration);
-      if (!pod_equal(v, old)) vd[dv] = v;
-      if (r) {
-        deg = off_out[dv + 1] - off_out[dv];
-        want = deg > 0;
-        if (want && chg_bits) atomicOr(&chg_bits[dv >> 5], 1u << (dv & 31));
+  __shared__ int s_a[kApplyNT / kWave], s_b[kApplyNT / kWave];
+  __shared__ unsigned long long s_base;
+  const int tid = threadIdx.x;
+  for (int chunk = blockIdx.x * kApplyChunk; chunk < n; chunk += gridDim.x * kApplyChunk) {
+    int dvs[kApplyVT], degs[kApplyVT];
+    unsigned want_mask = 0;
+    int my_cnt = 0, my_edges = 0;
+#pragma unroll
+    for (int m = 0; m < kApplyVT; ++m) {
+      const int i = chunk + m * kApplyNT + tid;
+      const bool valid = i < n;
+      bool r = false;
+      int dv = 0, deg = 0;
+      if (valid) {
+        dv = active[i];
+        typename P::GatherResult g = use_gather ? gtmp[i] : P::gatherZero;
+        typename P::VertexData v = vd[dv];
+        const typename P::VertexData old = v;
+        r = P::apply(&v, g, iteration);
+        if (!pod_equal(v, old)) vd[dv] = v;
+        if (r) deg = off_out[dv + 1] - off_out[dv];
+      }
+      const bool want = r && deg > 0;
+      if (chg_bits) wave_set_bits_sorted(valid, want, dv, chg_bits);
+      dvs[m] = dv;
+      degs[m] = deg;
+      if (want) {
+        want_mask |= 1u << m;
+        my_cnt += 1;
+        my_edges += deg;
       }
     }
-    append_changed(want, dv, deg, chg_v, chg_e, &ctr->chg);
+    int tc, te;
+    const int ec = block_excl_scan_add<kApplyNT>(my_cnt, s_a, &tc);
+    const int ee = block_excl_scan_add<kApplyNT>(my_edges, s_b, &te);
+    if (tc > 0) {
+      if (tid == 0)
+        s_base = atomicAdd(&ctr->chg, ((unsigned long long)tc << 32) |
+                                          (unsigned long long)(uint32_t)te);
+      __syncthreads();
+      const unsigned long long base = s_base;
+      int idx = (int)(base >> 32) + ec;
+      int eo = (int)(uint32_t)base + ee;
+#pragma unroll
+      for (int m = 0; m < kApplyVT; ++m) {
+        if (want_mask & (1u << m)) {
+          chg_v[idx] = dvs[m];
+          chg_e[idx] = eo;
+          ++idx;
+          eo += degs[m];
+        }
+      }
+      __syncthreads();
+    }
   }
 }
 
@@ -477,12 +221,26 @@ __global__ __launch_bounds__(kNT) void k_push_tiles(const DevCounters* __restric
 
 // ------------------------------------------------------- scatter: pull
 
-// next[v] = some in-neighbour of v is in the changed bitmap.  One wave owns
-// 64 consecutive vertices: light vertices are scanned by their own lane,
-// heavy ones (in-degree > kPullLight) by the whole wave with early exit.
-constexpr int kPullLight = 32;
+// next[v] = some in-neighbour of v is in the changed bitmap.
+//
+// k_pull_stream: one wave owns 64 consecutive vertices (lane l <-> vertex
+// base + l), whose in-edges form one contiguous CSC segment.  The wave
+// streams that segment 64 edges at a time (coalesced srcs reads); a ballot
+// gives the hit lanes, and each lane tests whether its own vertex's slice of
+// the chunk contains a hit (a second ballot), so no cross-lane shuffles are
+// needed.  The row holding the chunk start is found from a ballot over the
+// lanes' row offsets; rows already hit, and heavy rows (in-degree above
+// kPullHeavy, done by k_pull_heavy with a whole workgroup and early exit),
+// are jumped over.
+constexpr int kPullHeavy = 1024;
 
-__global__ __launch_bounds__(256) void k_pull_activate(
+__device__ __forceinline__ uint64_t lane_range_mask(int lo, int hi) {  // bits [lo, hi)
+  const uint64_t upto = hi >= 64 ? ~0ull : ((1ull << hi) - 1);
+  const uint64_t below = lo <= 0 ? 0ull : ((1ull << lo) - 1);
+  return upto & ~below;
+}
+
+__global__ __launch_bounds__(256) void k_pull_stream(
     const DevCounters* __restrict__ ctr, int mode, long long threshold, int nV,
     const int32_t* __restrict__ off_in, const int32_t* __restrict__ srcs,
     const uint32_t* __restrict__ chg_bits, uint32_t* __restrict__ bitmap) {
@@ -491,49 +249,68 @@ __global__ __launch_bounds__(256) void k_pull_activate(
   const int wpb = blockDim.x / kWave;
   const int nw = gridDim.x * wpb;
   for (int base = (blockIdx.x * wpb + threadIdx.x / kWave) * kWave; base < nV; base += nw * kWave) {
-    const int v = base + lane;
-    int b = 0, e = 0;
-    if (v < nV) {
-      b = off_in[v];
-      e = off_in[v + 1];
-    }
-    const bool light = (e - b) <= kPullLight;
-    bool hit = false;
-    if (v < nV && light) {
-      for (int p = b; p < e; ++p) {
+    const int cnt = min(kWave, nV - base);
+    const bool mine = lane < cnt;
+    const int my_off = mine ? off_in[base + lane] : INT_MAX;
+    const int my_end = mine ? off_in[base + lane + 1] : INT_MAX;
+    const bool heavy = mine && (my_end - my_off) > kPullHeavy;
+    const uint64_t need = __ballot(mine && my_end > my_off && !heavy);
+    const uint64_t heavy_mask = __ballot(heavy);
+    const int seg0 = __shfl(my_off, 0, kWave);
+    const int seg1 = __shfl(my_end, cnt - 1, kWave);
+    uint64_t hits = 0;
+    int pb = seg0;
+    while (pb < seg1 && (hits & need) != need) {
+      // row holding edge pb: the last lane whose row starts at or before pb
+      const uint64_t started = __ballot(mine && my_off <= pb);
+      const int r0 = 63 - __builtin_clzll(started);
+      if (((heavy_mask | hits) >> r0) & 1ull) {
+        pb = __shfl(my_end, r0, kWave);
+        continue;
+      }
+      const int p = pb + lane;
+      bool h = false;
+      if (p < seg1) {
         const int u = srcs[p];
-        if ((chg_bits[u >> 5] >> (u & 31)) & 1u) {
-          hit = true;
-          break;
-        }
+        h = (chg_bits[u >> 5] >> (u & 31)) & 1u;
       }
-    }
-    uint64_t heavy = __ballot(v < nV && !light);
-    while (heavy) {
-      const int hl = __builtin_ctzll(heavy);
-      heavy &= heavy - 1;
-      const int hb = __shfl(b, hl, kWave), he = __shfl(e, hl, kWave);
-      // wave-uniform trip count, so every lane sees the same `found`
-      bool found = false;
-      for (int pb = hb; pb < he; pb += kWave) {
-        const int p = pb + lane;
-        bool h = false;
-        if (p < he) {
-          const int u = srcs[p];
-          h = (chg_bits[u >> 5] >> (u & 31)) & 1u;
-        }
-        if (__ballot(h)) {
-          found = true;
-          break;
-        }
+      const uint64_t hit_lanes = __ballot(h);
+      bool row_hit = false;
+      if (mine && !heavy && hit_lanes) {
+        const int lo = my_off - pb, hi = my_end - pb;
+        if (hi > 0 && lo < kWave) row_hit = (hit_lanes & lane_range_mask(lo, hi)) != 0;
       }
-      if (lane == hl) hit = found;
+      hits |= __ballot(row_hit);
+      pb += kWave;
     }
-    const uint64_t hits = __ballot(hit);
-    // lanes 0 and 32 write the two 32-bit words of this wave's 64 vertices
-    if (v < nV && (lane & 31) == 0) {
+    // lanes 0 and 32 write the two 32-bit words of this wave's vertices
+    if ((lane & 31) == 0 && base + lane < nV) {
       const uint32_t word = (uint32_t)(hits >> lane);
-      if (word) bitmap[v >> 5] = word;
+      if (word) bitmap[(base + lane) >> 5] = word;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_pull_heavy(
+    const DevCounters* __restrict__ ctr, int mode, long long threshold,
+    const int32_t* __restrict__ heavy, int n_heavy, const int32_t* __restrict__ off_in,
+    const int32_t* __restrict__ srcs, const uint32_t* __restrict__ chg_bits,
+    uint32_t* __restrict__ bitmap) {
+  if (!use_pull(ctr->chg, mode, threshold)) return;
+  for (int h = blockIdx.x; h < n_heavy; h += gridDim.x) {
+    const int v = heavy[h];
+    const int b = off_in[v], e = off_in[v + 1];
+    for (int pb = b; pb < e; pb += blockDim.x) {
+      const int p = pb + threadIdx.x;
+      bool hit = false;
+      if (p < e) {
+        const int u = srcs[p];
+        hit = (chg_bits[u >> 5] >> (u & 31)) & 1u;
+      }
+      if (__syncthreads_or(hit)) {
+        if (threadIdx.x == 0) atomicOr(&bitmap[v >> 5], 1u << (v & 31));
+        break;
+      }
     }
   }
 }
@@ -609,6 +386,7 @@ __global__ __launch_bounds__(1024) void k_compact_scan(int2* __restrict__ blk, i
 __global__ __launch_bounds__(kCompactNT) void k_compact_write(
     uint32_t* __restrict__ bitmap, int n_words, const int32_t* __restrict__ off_in, int need_deg,
     const int2* __restrict__ blk, int32_t* __restrict__ active, int32_t* __restrict__ escan,
+    int32_t* __restrict__ cbase,
     uint32_t* __restrict__ chg_bits) {
   __shared__ int s_a[kCompactNT / kWave], s_b[kCompactNT / kWave];
   const int w = blockIdx.x * kCompactNT + threadIdx.x;
@@ -631,6 +409,7 @@ __global__ __launch_bounds__(kCompactNT) void k_compact_write(
     active[pos] = v;
     if (need_deg) {
       escan[pos] = dpos;
+      cbase[pos] = off_in[v] - dpos;
       dpos += off_in[v + 1] - off_in[v];
     }
     ++pos;
@@ -639,11 +418,15 @@ __global__ __launch_bounds__(kCompactNT) void k_compact_write(
 
 __global__ void k_set_active_range(int start, int n, const int32_t* __restrict__ off_in,
                                    int32_t* __restrict__ active, int32_t* __restrict__ escan,
+                                   int32_t* __restrict__ cbase,
                                    DevCounters* __restrict__ ctr) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i > n) return;
   const int base = off_in[start];
-  if (i < n) active[i] = start + i;
+  if (i < n) {
+    active[i] = start + i;
+    cbase[i] = base;
+  }
   escan[i] = off_in[start + i] - base;
   if (i == n) {
     ctr->n_active = n;
@@ -685,6 +468,7 @@ struct EngineBase {
   bool bound = false;
   int dir_mode = 0;
   double dir_alpha = 15.0;
+  int gather_vt = 8;  // merge-path items per thread in the gather tiles (4, 8 or 16)
   bool timing = false;
   gr_engine_stats stats{};
 };
@@ -705,6 +489,7 @@ struct Engine final : EngineBase {
   ED* d_ed = nullptr;  // CSC order
   int32_t* d_active = nullptr;
   int32_t* d_escan = nullptr;
+  int32_t* d_cbase = nullptr;  // per active row: CSC offset minus its escan value
   G* d_gtmp = nullptr;
   int32_t* d_chg_v = nullptr;
   int32_t* d_chg_e = nullptr;
@@ -718,6 +503,8 @@ struct Engine final : EngineBase {
   DevCounters* d_ctr = nullptr;
   DevCounters* h_ctr = nullptr;  // pinned
   int n_words = 0, n_cblocks = 0, max_tiles = 0, max_push_tiles = 0;
+  int32_t* d_heavy = nullptr;  // vertices with in-degree > kPullHeavy (pull mode)
+  int n_heavy = 0;
   // hipEvent start/stop pairs per timed phase
   enum { kPhGather = 0, kPhScatter, kPhApply, kPhNext, kPhases };
   std::vector<hipEvent_t> ev[kPhases];
@@ -729,7 +516,7 @@ struct Engine final : EngineBase {
   size_t edge_size() const override { return kEdgeEmpty ? 0 : sizeof(ED); }
 
   void release() {
-    void* ptrs[] = {d_vd,    d_vd_init, d_ed,      d_active,    d_escan, d_gtmp,  d_chg_v,
+    void* ptrs[] = {d_cbase, d_heavy, d_vd,    d_vd_init, d_ed,      d_active,    d_escan, d_gtmp,  d_chg_v,
                     d_chg_e, d_bitmap,  d_chg_bits,  d_blk,   d_part,  d_push_part,
                     d_carry, d_head,    d_ctr};
     for (void* p : ptrs)
@@ -743,7 +530,10 @@ struct Engine final : EngineBase {
     if (owns_graph && g) graph_free(g);
     d_vd = nullptr;
     d_vd_init = nullptr;
+    d_heavy = nullptr;
+    n_heavy = 0;
     d_ed = nullptr;
+    d_cbase = nullptr;
     d_active = d_escan = d_chg_v = d_chg_e = d_part = d_push_part = nullptr;
     d_gtmp = nullptr;
     d_bitmap = d_chg_bits = nullptr;
@@ -783,17 +573,19 @@ struct Engine final : EngineBase {
     owns_graph = owns;
     host_vdata = vdata;
     const int64_t V = g->nV, E = g->nE;
-    if (V + E >= ((int64_t)1 << 31) - kTile)
+    if (V + E >= ((int64_t)1 << 31) - kNT * 16)
       return set_error(GR_ERR_INVALID, "graph too large for int32 merge-path coordinates");
     GR_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     n_words = (int)((V + 31) / 32);
     n_cblocks = (n_words + kCompactNT - 1) / kCompactNT;
-    max_tiles = (int)((V + E + kTile - 1) / kTile) + 1;
+    // sized for the smallest tile (gather_vt = 4)
+    max_tiles = (int)((V + E + kNT * 4 - 1) / (kNT * 4)) + 1;
     max_push_tiles = (int)((E + kTile - 1) / kTile) + 1;
     GR_TRY(dalloc(&d_vd, V));
     GR_TRY(dalloc(&d_vd_init, V));
     GR_TRY(dalloc(&d_active, V));
     GR_TRY(dalloc(&d_escan, V + 1));
+    GR_TRY(dalloc(&d_cbase, V + 1));
     GR_TRY(dalloc(&d_gtmp, V));
     GR_TRY(dalloc(&d_chg_v, V));
     GR_TRY(dalloc(&d_chg_e, V));
@@ -809,6 +601,15 @@ struct Engine final : EngineBase {
     GR_HIP(hipMemsetAsync(d_bitmap, 0, sizeof(uint32_t) * (n_words + 1), stream));
     GR_HIP(hipMemsetAsync(d_chg_bits, 0, sizeof(uint32_t) * (n_words + 1), stream));
     GR_HIP(hipMemsetAsync(d_ctr, 0, sizeof(DevCounters), stream));
+    {
+      std::vector<int32_t> hv;
+      for (int32_t v = 0; v < g->nV; ++v)
+        if (g->h_off_in[v + 1] - g->h_off_in[v] > kPullHeavy) hv.push_back(v);
+      n_heavy = (int)hv.size();
+      GR_TRY(dalloc(&d_heavy, n_heavy));
+      if (n_heavy)
+        GR_HIP(hipMemcpy(d_heavy, hv.data(), sizeof(int32_t) * n_heavy, hipMemcpyHostToDevice));
+    }
     GR_HIP(hipMemcpyAsync(d_vd, vdata, sizeof(VD) * V, hipMemcpyHostToDevice, stream));
     GR_HIP(hipMemcpyAsync(d_vd_init, d_vd, sizeof(VD) * V, hipMemcpyDeviceToDevice, stream));
     if constexpr (!kEdgeEmpty) {
@@ -852,7 +653,8 @@ struct Engine final : EngineBase {
     GR_HIP(hipMemsetAsync(d_bitmap, 0, sizeof(uint32_t) * (n_words + 1), stream));
     GR_HIP(hipMemsetAsync(d_chg_bits, 0, sizeof(uint32_t) * (n_words + 1), stream));
     k_set_active_range<<<grid_1d((int64_t)n + 1, 256), 256, 0, stream>>>(start, n, g->off_in,
-                                                                         d_active, d_escan, d_ctr);
+                                                                         d_active, d_escan, d_cbase,
+                                                                         d_ctr);
     GR_HIP(hipGetLastError());
     n_active = n;
     w_edges = g->h_off_in[end] - g->h_off_in[start];
@@ -880,13 +682,21 @@ struct Engine final : EngineBase {
   int gather() override {
     if (!P::kHasGather || n_active == 0) return GR_OK;
     const int n = n_active, w = w_edges;
-    const int n_tiles = (int)(((int64_t)n + w + kTile - 1) / kTile);
+    const int tile = kNT * gather_vt;
+    const int n_tiles = (int)(((int64_t)n + w + tile - 1) / tile);
     k_merge_partition<<<grid_1d((int64_t)n_tiles + 1, 256), 256, 0, stream>>>(d_escan, n, w,
-                                                                              n_tiles, d_part);
+                                                                              n_tiles, tile, d_part);
     tick();
-    k_gather_tiles<P><<<n_tiles, kNT, 0, stream>>>(n, w, d_active, d_escan, d_part, g->off_in,
-                                                    g->srcs, d_vd, edge_ptr(), d_gtmp, d_carry,
-                                                    d_head);
+#define GR_GATHER_LAUNCH(VT)                                                                \
+  k_gather_tiles<P, VT><<<n_tiles, kNT, 0, stream>>>(n, w, d_active, d_escan, d_part, d_cbase, \
+                                                     g->srcs, d_vd, edge_ptr(), d_gtmp,       \
+                                                     d_carry, d_head)
+    switch (gather_vt) {
+      case 4: GR_GATHER_LAUNCH(4); break;
+      case 16: GR_GATHER_LAUNCH(16); break;
+      default: GR_GATHER_LAUNCH(8); break;
+    }
+#undef GR_GATHER_LAUNCH
     tock(kPhGather);
     k_gather_fixup<P><<<grid_1d(n_tiles, 256), 256, 0, stream>>>(n_tiles, d_carry, d_head, d_gtmp);
     GR_HIP(hipGetLastError());
@@ -899,7 +709,7 @@ struct Engine final : EngineBase {
     if (n_active == 0) return GR_OK;
     const bool pull_possible = dir_mode != 1;
     tick();
-    k_apply<P><<<grid_1d(n_active, 256), 256, 0, stream>>>(
+    k_apply<P><<<grid_1d(n_active, kApplyChunk), kApplyNT, 0, stream>>>(
         n_active, d_active, d_gtmp, P::kHasGather ? 1 : 0, d_vd, g->off_out, iteration, d_chg_v,
         d_chg_e, pull_possible ? d_chg_bits : nullptr, d_ctr);
     tock(kPhApply);
@@ -917,9 +727,12 @@ struct Engine final : EngineBase {
     const long long thr = pull_threshold();
     tick();
     if (dir_mode != 1) {
-      k_pull_activate<<<grid_1d(g->nV, 256), 256, 0, stream>>>(d_ctr, dir_mode, thr, g->nV,
-                                                                g->off_in, g->srcs, d_chg_bits,
-                                                                d_bitmap);
+      k_pull_stream<<<grid_1d(g->nV, 256), 256, 0, stream>>>(d_ctr, dir_mode, thr, g->nV,
+                                                              g->off_in, g->srcs, d_chg_bits,
+                                                              d_bitmap);
+      if (n_heavy > 0)
+        k_pull_heavy<<<n_heavy < 4096 ? n_heavy : 4096, 256, 0, stream>>>(
+            d_ctr, dir_mode, thr, d_heavy, n_heavy, g->off_in, g->srcs, d_chg_bits, d_bitmap);
     }
     if (dir_mode != 2) {
       k_push_partition<<<grid_1d((int64_t)max_push_tiles + 1, 256), 256, 0, stream>>>(
@@ -942,7 +755,7 @@ struct Engine final : EngineBase {
     k_compact_scan<<<1, 1024, 0, stream>>>(d_blk, n_cblocks, d_ctr, need_deg ? d_escan : nullptr,
                                            dir_mode, pull_threshold());
     k_compact_write<<<n_cblocks, kCompactNT, 0, stream>>>(d_bitmap, n_words, g->off_in, need_deg,
-                                                          d_blk, d_active, d_escan,
+                                                          d_blk, d_active, d_escan, d_cbase,
                                                           dir_mode != 1 ? d_chg_bits : nullptr);
     tock(kPhNext);
     GR_HIP(hipGetLastError());
@@ -1154,6 +967,14 @@ extern "C" int gr_engine_set_direction(gr_engine* e, int mode, double alpha) {
   if (mode < 0 || mode > 2) return set_error(GR_ERR_INVALID, "direction mode must be 0, 1 or 2");
   e->impl->dir_mode = mode;
   if (alpha > 0) e->impl->dir_alpha = alpha;
+  return GR_OK;
+}
+
+extern "C" int gr_engine_set_gather_tile(gr_engine* e, int items_per_thread) {
+  GR_TRY(check(e, false));
+  if (items_per_thread != 4 && items_per_thread != 8 && items_per_thread != 16)
+    return set_error(GR_ERR_INVALID, "gather items per thread must be 4, 8 or 16");
+  e->impl->gather_vt = items_per_thread;
   return GR_OK;
 }
 

@@ -1,0 +1,334 @@
+// Merge-path gather: the load-balanced, segmented reduction of a vertex
+// program's gatherMap values over the CSC in-edges of an active list.  Shared
+// by the GAS engine (gr_engine.hip) and the PageRank precision form
+// (gr_pagerank.hip).
+//
+// Work is the merge of the active rows' end offsets (exclusive scan `escan`
+// of their in-degrees) with the edge items; fixed tiles of kNT * VT path items
+// go to one workgroup each (k_merge_partition finds the tile starts).  Inside
+// a tile: (1) each lane walks its VT items and labels edges with rows,
+// (2) a coalesced pass reads the CSC sources and edge data and applies
+// gatherMap into LDS, (3) each lane reduces its items sequentially and a
+// wave64 segmented scan joins the lanes' partial rows.  Rows that span tiles
+// leave a carry and a head partial, joined in tile order by k_gather_fixup,
+// so results are deterministic.
+#ifndef GRAPHREDUCE_AMD_GR_GATHER_H
+#define GRAPHREDUCE_AMD_GR_GATHER_H
+
+#include "gr_device.h"
+
+namespace gr {
+
+constexpr int kNT = 256;            // threads per tile workgroup (4 waves)
+constexpr int kVT = 8;              // merge-path items per thread (push tiles)
+constexpr int kTile = kNT * kVT;    // path items per push tile
+constexpr int kCompactNT = 256;     // compaction: one bitmap word per thread
+constexpr int kMaxGrid = 1 << 20;
+
+struct DevCounters {
+  int32_t n_active;               // size of the active list built by nextIter
+  int32_t w_edges;                // sum of in-degrees over that list
+  unsigned long long chg;         // (#changed << 32) | out-edges, appended by apply
+  unsigned long long chg_last;    // chg as seen by the last nextIter
+  int32_t pull;                   // last scatterActivate ran in pull mode
+  int32_t pad[3];
+};
+
+template <class T>
+struct Part {
+  int32_t row;
+  int32_t has;
+  T val;
+};
+
+static inline unsigned grid_1d(int64_t n, int block) {
+  int64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > kMaxGrid) g = kMaxGrid;
+  return (unsigned)g;
+}
+
+
+// Shuffle of a plain-old-data value, 32 bits at a time.
+template <class T>
+__device__ __forceinline__ T shfl_up_pod(const T& v, int off) {
+  static_assert(sizeof(T) % 4 == 0, "shuffled values must be a multiple of 4 bytes");
+  T r;
+  const int* a = reinterpret_cast<const int*>(&v);
+  int* b = reinterpret_cast<int*>(&r);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) b[i] = __shfl_up(a[i], off, kWave);
+  return r;
+}
+
+// ------------------------------------------------------------ merge path
+
+// Number of row ends consumed on diagonal d of the merge of rowEnd[0..n)
+// (inclusive-scan row ends) with the edge items [0, w).  Edge j belongs to row
+// i while j < rowEnd[i].
+__device__ __forceinline__ int merge_path_search(int d, const int32_t* __restrict__ rowEnd,
+                                                 int n, int w) {
+  int lo = max(d - w, 0), hi = min(d, n);
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (rowEnd[mid] <= d - mid - 1)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+static __global__ void k_merge_partition(const int32_t* __restrict__ escan, int n, int w, int n_tiles,
+                                  int tile, int32_t* __restrict__ part) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > n_tiles) return;
+  int64_t d = min((int64_t)t * tile, (int64_t)n + w);
+  part[t] = merge_path_search((int)d, escan + 1, n, w);
+}
+
+// ------------------------------------------------- segmented lane scan
+
+template <class P>
+struct Seg {
+  typedef typename P::GatherResult G;
+  int flag;  // this lane emitted a row (its tail starts a new row)
+  int has;
+  G val;
+
+  __device__ static Seg identity() {
+    Seg s;
+    s.flag = 0;
+    s.has = 0;
+    s.val = G();
+    return s;
+  }
+  // a precedes b on the merge path
+  __device__ static Seg combine(const Seg& a, const Seg& b) {
+    Seg r;
+    r.flag = a.flag | b.flag;
+    if (b.flag || !a.has) {
+      r.has = b.has;
+      r.val = b.val;
+    } else if (!b.has) {
+      r.has = a.has;
+      r.val = a.val;
+    } else {
+      r.has = 1;
+      r.val = P::gatherReduce(a.val, b.val);
+    }
+    return r;
+  }
+  __device__ static Seg shfl_up(const Seg& s, int off) {
+    Seg r;
+    r.flag = __shfl_up(s.flag, off, kWave);
+    r.has = __shfl_up(s.has, off, kWave);
+    r.val = shfl_up_pod(s.val, off);
+    return r;
+  }
+};
+
+template <class P>
+__device__ __forceinline__ typename P::GatherResult finalize(int has,
+                                                             typename P::GatherResult v) {
+  return has ? P::gatherReduce(P::gatherZero, v) : P::gatherZero;
+}
+
+// ------------------------------------------------------------------ gather
+
+template <class P, int VT>
+__global__ __launch_bounds__(kNT) void k_gather_tiles(
+    int n, int w, const int32_t* __restrict__ active, const int32_t* __restrict__ escan,
+    const int32_t* __restrict__ part, const int32_t* __restrict__ cbase,
+    const int32_t* __restrict__ srcs, const typename P::VertexData* __restrict__ vd,
+    const typename P::EdgeData* __restrict__ ed, typename P::GatherResult* __restrict__ gtmp,
+    Part<typename P::GatherResult>* __restrict__ carry,
+    Part<typename P::GatherResult>* __restrict__ head) {
+  typedef typename P::GatherResult G;
+  typedef Seg<P> S;
+  constexpr int TILE = kNT * VT;
+  __shared__ int32_t s_scan[TILE + 2];
+  __shared__ int32_t s_base[TILE + 1];
+  __shared__ uint16_t s_row[TILE];
+  __shared__ G s_val[TILE];
+  __shared__ S s_wave[kNT / kWave];
+
+  const int t = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int total = n + w;
+  const int d0 = t * TILE;
+  const int d1 = min(d0 + TILE, total);
+  const int i0 = part[t], i1 = part[t + 1];
+  const int j0 = d0 - i0, j1 = d1 - i1;
+  const int n_rows = i1 - i0;                    // row ends inside this tile
+  const int n_ref = n_rows + (i1 < n ? 1 : 0);   // plus the row still open at the end
+  const int n_edges = j1 - j0;
+
+  for (int k = tid; k <= n_rows; k += kNT) s_scan[k] = escan[i0 + k];
+  for (int k = tid; k < n_ref; k += kNT) s_base[k] = cbase[i0 + k];
+  __syncthreads();
+
+  // Phase 1: each lane walks its VT path items and labels edges with rows.
+  const int dl = tid * VT;
+  const int steps = max(0, min(VT, (d1 - d0) - dl));
+  int ri = 0, ej = 0;
+  if (steps > 0) {
+    int lo = max(dl - n_edges, 0), hi = min(dl, n_rows);
+    while (lo < hi) {
+      int mid = (lo + hi) >> 1;
+      if (s_scan[mid + 1] - j0 <= dl - mid - 1)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    ri = lo;
+    ej = dl - lo;
+  }
+  const int ri0 = ri, ej0 = ej;
+  for (int s = 0; s < steps; ++s) {
+    if (ri < n_rows && s_scan[ri + 1] - j0 <= ej) {
+      ++ri;
+    } else {
+      s_row[ej] = (uint16_t)ri;
+      ++ej;
+    }
+  }
+  __syncthreads();
+
+  // Phase 2: coalesced pass over the tile's edges: CSC source, gatherMap.
+  {
+    int src_v[VT], csc_v[VT];
+#pragma unroll
+    for (int m = 0; m < VT; ++m) {
+      int e = tid + m * kNT;
+      if (e < n_edges) {
+        int k = s_row[e];
+        csc_v[m] = s_base[k] + j0 + e;
+        src_v[m] = srcs[csc_v[m]];
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < VT; ++m) {
+      int e = tid + m * kNT;
+      if (e < n_edges) {
+        const typename P::VertexData* dstp = vd;
+        if constexpr (P::kGatherUsesDst) dstp = vd + active[i0 + s_row[e]];
+        s_val[e] = P::gatherMap(dstp, vd + src_v[m], ed + csc_v[m]);
+      }
+    }
+  }
+  __syncthreads();
+
+  // Phase 3: sequential reduction along the lane's path items.
+  ri = ri0;
+  ej = ej0;
+  G acc = G();
+  int has = 0;
+  int emitted = 0, first_row = 0, first_has = 0;
+  G first_val = G();
+  for (int s = 0; s < steps; ++s) {
+    if (ri < n_rows && s_scan[ri + 1] - j0 <= ej) {
+      if (!emitted) {
+        emitted = 1;
+        first_row = ri;
+        first_val = acc;
+        first_has = has;
+      } else {
+        gtmp[i0 + ri] = finalize<P>(has, acc);
+      }
+      has = 0;
+      ++ri;
+    } else {
+      G v = s_val[ej];
+      acc = has ? P::gatherReduce(acc, v) : v;
+      has = 1;
+      ++ej;
+    }
+  }
+
+  // Segmented exclusive scan of the lanes' open tails.
+  S mine;
+  mine.flag = emitted;
+  mine.has = has;
+  mine.val = acc;
+  const int lane = lane_id(), wv = tid / kWave;
+  S incl = mine;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    S o = S::shfl_up(incl, off);
+    if (lane >= off) incl = S::combine(o, incl);
+  }
+  if (lane == kWave - 1) s_wave[wv] = incl;
+  __syncthreads();
+  S wave_prefix = S::identity();
+  for (int i = 0; i < wv; ++i) wave_prefix = S::combine(wave_prefix, s_wave[i]);
+  S prev = S::shfl_up(incl, 1);
+  S excl = (lane == 0) ? wave_prefix : S::combine(wave_prefix, prev);
+
+  const bool head_open = n_rows > 0 && s_scan[0] < j0;  // row i0 began in an earlier tile
+  if (emitted) {
+    S fr;
+    fr.flag = 0;
+    fr.has = first_has;
+    fr.val = first_val;
+    S rv = S::combine(excl, fr);
+    if (first_row == 0 && head_open) {
+      Part<G> p;
+      p.row = i0;
+      p.has = rv.has;
+      p.val = rv.val;
+      head[t] = p;
+    } else {
+      gtmp[i0 + first_row] = finalize<P>(rv.has, rv.val);
+    }
+  }
+  if (tid == 0 && !head_open) head[t].row = -1;
+  if (tid == kNT - 1) {
+    S all = S::identity();
+    for (int i = 0; i < kNT / kWave; ++i) all = S::combine(all, s_wave[i]);
+    Part<G> p;
+    // The row open at the end carries over when some of its edges were here.
+    if (i1 < n && escan[i1] < j1) {
+      p.row = i1;
+      p.has = all.has;
+      p.val = all.val;
+    } else {
+      p.row = -1;
+      p.has = 0;
+      p.val = G();
+    }
+    carry[t] = p;
+  }
+}
+
+// Rows spanning several tiles: carries in tile order, then the closing
+// tile's head partial.
+template <class P>
+__global__ void k_gather_fixup(int n_tiles, const Part<typename P::GatherResult>* __restrict__ carry,
+                               const Part<typename P::GatherResult>* __restrict__ head,
+                               typename P::GatherResult* __restrict__ gtmp) {
+  typedef typename P::GatherResult G;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_tiles) return;
+  const int r = carry[t].row;
+  if (r < 0 || (t > 0 && carry[t - 1].row == r)) return;
+  int has = carry[t].has;
+  G acc = carry[t].val;
+  int u = t + 1;
+  while (u < n_tiles && carry[u].row == r) {
+    if (carry[u].has) {
+      acc = has ? P::gatherReduce(acc, carry[u].val) : carry[u].val;
+      has = 1;
+    }
+    ++u;
+  }
+  if (u < n_tiles && head[u].row == r && head[u].has) {
+    acc = has ? P::gatherReduce(acc, head[u].val) : head[u].val;
+    has = 1;
+  }
+  gtmp[r] = finalize<P>(has, acc);
+}
+
+}  // namespace gr
+
+#endif

@@ -297,6 +297,34 @@ extern "C" int gr_device_free(void* device_ptr) {
   return GR_OK;
 }
 
+namespace {
+__global__ void k_stream_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+}  // namespace
+
+extern "C" int gr_device_copy(void* dst, const void* src, size_t bytes, float* ms) {
+  if ((bytes & 15) || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15))
+    return set_error(GR_ERR_INVALID, "gr_device_copy: 16-byte aligned sizes and pointers only");
+  const int64_t n = (int64_t)(bytes / 16);
+  hipEvent_t e0, e1;
+  GR_HIP(hipEventCreate(&e0));
+  GR_HIP(hipEventCreate(&e1));
+  GR_HIP(hipEventRecord(e0, 0));
+  k_stream_copy<<<4096, 256>>>(static_cast<const uint4*>(src), static_cast<uint4*>(dst), n);
+  GR_HIP(hipGetLastError());
+  GR_HIP(hipEventRecord(e1, 0));
+  GR_HIP(hipEventSynchronize(e1));
+  float t = 0;
+  GR_HIP(hipEventElapsedTime(&t, e0, e1));
+  if (ms) *ms = t;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return GR_OK;
+}
+
 extern "C" int gr_memcpy_h2d(void* dst, const void* src, size_t bytes) {
   if (bytes) GR_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
   return GR_OK;

@@ -1,0 +1,133 @@
+"""Autotuner of the engine's launch parameters.
+
+Chooses, per vertex program and graph, the gather tile shape (merge-path
+items per thread: 4, 8 or 16 in a 256-thread tile) and the push/pull switch
+point alpha of scatterActivate (pull when the out-edges of the changed
+vertices exceed E / alpha), by timing complete runs of the reference loop on
+the device graph.  Results are written as JSON and applied with
+:func:`apply_tuning`.
+
+Usage:  python -m graphreduce_amd.tune --scale 24 --seed 6 --out graphreduce_amd/tuned.json
+"""
+from __future__ import annotations
+
+import argparse
+import itertools
+import json
+import time
+from pathlib import Path
+
+import numpy as np
+
+import graphreduce_amd as gr
+
+DEFAULT_FILE = Path(__file__).resolve().parent / "tuned.json"
+TILES = (4, 8, 16)
+ALPHAS = (4.0, 8.0, 15.0, 30.0, 60.0)
+
+
+def _loop(eng, n, source=None):
+    eng.resetVertexData()
+    if source is None:
+        eng.setActive(0, n)
+    else:
+        eng.setActive(source, source + 1)
+    it = 0
+    eng.setIterationCount(0)
+    while eng.countActive():
+        eng.gather()
+        eng.apply()
+        eng.scatterActivate()
+        eng.nextIter()
+        it += 1
+        eng.setIterationCount(it)
+    return it
+
+
+def _engine(program, g, weights=None):
+    n = g.n_vertices
+    if program == gr.BFS:
+        vd = np.zeros(n, gr.PROGRAMS[gr.BFS]["vertex"])
+        vd["depth"] = -1
+    elif program == gr.SSSP:
+        vd = np.full(n, gr.SSSP_INF, np.int32)
+    elif program == gr.CC:
+        vd = np.arange(n, dtype=np.int32)
+    else:
+        off, _ = g.csr()
+        vd = np.zeros(n, gr.PROGRAMS[gr.PAGERANK]["vertex"])
+        vd["rank"] = np.float32(0.15)
+        vd["numOutEdges"] = np.diff(off)
+    eng = gr.GASEngine(program)
+    return eng, vd
+
+
+def tune_program(program, g, source, weights=None, repeats=2, log=print):
+    """Returns {"gather_tile": vt, "alpha": a, "ms": best} for one program."""
+    eng, vd = _engine(program, g)
+    if program == gr.SSSP:
+        vd[source] = 0
+    eng.bindGraph(g, vd, weights)
+    src = source if program == gr.BFS else None
+    _loop(eng, g.n_vertices, src)  # warm
+    best = None
+    tiles = TILES if program != gr.BFS else (8,)
+    for vt, alpha in itertools.product(tiles, ALPHAS):
+        eng.setGatherTile(vt)
+        eng.setDirection(0, alpha)
+        ts = []
+        for _ in range(repeats):
+            gr.synchronize()
+            t0 = time.perf_counter()
+            _loop(eng, g.n_vertices, src)
+            gr.synchronize()
+            ts.append(time.perf_counter() - t0)
+        ms = min(ts) * 1e3
+        log(f"  {gr.PROGRAMS[program]['name']} tile={vt} alpha={alpha}: {ms:.3f} ms")
+        if best is None or ms < best["ms"]:
+            best = {"gather_tile": vt, "alpha": alpha, "ms": round(ms, 4)}
+    eng.close()
+    return best
+
+
+def apply_tuning(eng: "gr.GASEngine", program: int, path: Path | str = DEFAULT_FILE) -> bool:
+    """Applies the tuned parameters of `program` from `path` (if present)."""
+    p = Path(path)
+    if not p.exists():
+        return False
+    data = json.loads(p.read_text())
+    entry = data.get("programs", {}).get(gr.PROGRAMS[program]["name"])
+    if not entry:
+        return False
+    eng.setGatherTile(int(entry["gather_tile"]))
+    eng.setDirection(0, float(entry["alpha"]))
+    return True
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--seed", type=int, default=6)
+    ap.add_argument("--out", default=str(DEFAULT_FILE))
+    ap.add_argument("--programs", default="BFS,SSSP,CC,PageRank")
+    args = ap.parse_args()
+    g = gr.Graph.rmat(args.scale, 16, seed=args.seed)
+    src, _ = g.max_out_degree_vertex()
+    w = g.hash_weights_device(seed=args.seed)
+    out = {"graph": f"rmat{args.scale} seed {args.seed}", "device": gr.device_info()["name"],
+           "programs": {}}
+    names = {v["name"]: k for k, v in gr.PROGRAMS.items()}
+    for name in args.programs.split(","):
+        prog = names[name]
+        gg = g
+        if prog == gr.CC:
+            gg = gr.Graph.rmat(args.scale, 16, seed=args.seed, symmetrize=True)
+        print(f"tuning {name}", flush=True)
+        out["programs"][name] = tune_program(prog, gg, src, w if prog == gr.SSSP else None)
+        print(f"  -> {out['programs'][name]}", flush=True)
+    Path(args.out).write_text(json.dumps(out, indent=1))
+    print(f"wrote {args.out}")
+
+
+if __name__ == "__main__":
+    main()

@@ -71,6 +71,10 @@ int gr_device_free(void* device_ptr);
 int gr_memcpy_h2d(void* device_dst, const void* host_src, size_t bytes);
 int gr_memcpy_d2h(void* host_dst, const void* device_src, size_t bytes);
 
+/* Streaming device-to-device copy with a plain kernel (16-byte aligned), timed
+ * with hipEvents: the reference point for bandwidth and HBM-counter scaling. */
+int gr_device_copy(void* device_dst, const void* device_src, size_t bytes, float* ms);
+
 /* ------------------------------------------------------- host generators */
 /* Deterministic synthetic inputs.  All draws are counter-based, so host and
  * device generators give identical edge lists. */
@@ -200,6 +204,10 @@ int gr_engine_reset_vertex_data(gr_engine* e);
  * vertices exceed n_edges / alpha. */
 int gr_engine_set_direction(gr_engine* e, int mode, double alpha);
 
+/* Gather tile shape: merge-path items per thread (4, 8 or 16; a tile is 256
+ * threads).  Chosen by the autotuner; default 8. */
+int gr_engine_set_gather_tile(gr_engine* e, int items_per_thread);
+
 /* Work and timing counters, summed since the last reset. */
 typedef struct gr_engine_stats {
   int64_t iterations;        /* nextIter calls with a non-empty active set */
@@ -222,6 +230,50 @@ typedef struct gr_engine_stats {
 int gr_engine_enable_timing(gr_engine* e, int on);
 int gr_engine_get_stats(gr_engine* e, gr_engine_stats* out);
 int gr_engine_reset_stats(gr_engine* e);
+
+/* ------------------------------------------------ PageRank precision form */
+/* The recurrence of pagerank.cu:46-66 with every vertex active:
+ * r_0 = 0.15, r_{t+1}[v] = 0.15 + 0.85 * sum_{u->v} r_t[u] / outdeg[u], in FP32;
+ * the in-edge sums are compensated (TwoSum) when compensated != 0, plain FP32
+ * otherwise.  Stops when the FP64 L1 residual sum_v |r_{t+1} - r_t| < tol or
+ * after max_iters steps.  The reference's activation-driven variant
+ * (tol 0.01 per vertex) is the engine with GR_PROGRAM_PAGERANK. */
+typedef struct gr_pagerank_result {
+  int32_t iterations;     /* steps run */
+  double residual;        /* FP64 L1 residual of the last step */
+  double ms;              /* device time of the iteration loop */
+  double gather_ms;       /* of which merge-path gather kernels */
+  int64_t gathered_edges; /* iterations * E */
+} gr_pagerank_result;
+
+/* ranks (host, V floats) and residuals (host, max_iters doubles) may be NULL. */
+int gr_pagerank_precise(gr_graph* g, int max_iters, double tol, int compensated, float* ranks,
+                        double* residuals, gr_pagerank_result* result);
+
+/* ----------------------------------------------- batched multi-source runs */
+/* Up to 64 sources share one traversal (no reference counterpart: the
+ * reference runs one source per engine, bfs.cu:93-134, sssp.cu:69-111).
+ * Results equal independent plain runs of each source. */
+typedef struct gr_batch_result {
+  int32_t levels; /* BFS levels / SSSP relaxation rounds run */
+  double ms;      /* device time from state init to the last kernel */
+  int32_t grid;   /* workgroups of the persistent (BFS) or round (SSSP) kernel */
+} gr_batch_result;
+
+/* BFS from sources[0..n) in ONE persistent cooperative launch with 64-bit
+ * visit/frontier masks per vertex.  depth_device: device array n*V int32,
+ * row s = depths from sources[s] (-1 = unreached), filled by this call.
+ * direction: 0 automatic (pull when the frontier's out-edges exceed E/alpha),
+ * 1 push, 2 pull. */
+int gr_msbfs(gr_graph* g, const int32_t* sources, int n_sources, int32_t* depth_device,
+             int direction, double alpha, gr_batch_result* result);
+
+/* SSSP (sssp.cu semantics: distances, unreached = INT_MAX - 100000) from
+ * sources[0..n): lane s of a wave relaxes source s; rounds are replayed from a
+ * hipGraph until nothing changes.  weights_device: int32 per edge in the
+ * graph's edge-list order; dist_device: n*V int32, row s = source s. */
+int gr_mssp(gr_graph* g, const int32_t* weights_device, const int32_t* sources, int n_sources,
+            int32_t* dist_device, gr_batch_result* result);
 
 #ifdef __cplusplus
 }

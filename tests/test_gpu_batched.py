@@ -1,0 +1,84 @@
+"""Batched multi-source BFS / SSSP: every row must equal the plain engine's
+result for that source (and the CPU oracle's), bit for bit."""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _sources(n, k, seed, extra=()):
+    rng = np.random.default_rng(seed)
+    return list(extra) + rng.integers(0, n, k - len(extra)).tolist()
+
+
+@pytest.mark.parametrize("direction", [0, 1, 2])
+def test_msbfs_equals_plain_runs(gpu, direction):
+    gr = gpu
+    g = gr.Graph.rmat(14, 16, seed=6)
+    n = g.n_vertices
+    s, d = g.edges()
+    hub, _ = g.max_out_degree_vertex()
+    srcs = _sources(n, 64, 11, extra=(hub, hub))  # a hub and a duplicated source
+    depth, info = gr.msbfs(g, srcs, direction=direction)
+    assert info["levels"] >= 1
+    for i, src in enumerate(srcs):
+        ref, _, _ = oracle.bfs(n, s, d, src)
+        assert np.array_equal(depth[i], ref), f"source {src} (row {i})"
+    g.close()
+
+
+def test_msbfs_partial_batch_and_plain_engine(gpu):
+    gr = gpu
+    g = gr.Graph.rmat(12, 16, seed=1)
+    n = g.n_vertices
+    srcs = _sources(n, 5, 3)
+    depth, _ = gr.msbfs(g, srcs)
+    for i, src in enumerate(srcs):
+        plain, _ = gr.run_bfs(g, src)
+        assert np.array_equal(depth[i], plain)
+    g.close()
+
+
+def test_msbfs_grid_deep_levels(gpu):
+    gr = gpu
+    g = gr.Graph.grid(96, 0.01, seed=5)
+    s, d = g.edges()
+    srcs = [0, 95, 96 * 96 - 1, 4000]
+    depth, info = gr.msbfs(g, srcs)
+    assert info["levels"] > 150
+    for i, src in enumerate(srcs):
+        ref, _, _ = oracle.bfs(g.n_vertices, s, d, src)
+        assert np.array_equal(depth[i], ref)
+    g.close()
+
+
+def test_mssp_equals_plain_runs(gpu):
+    gr = gpu
+    g = gr.Graph.rmat(13, 16, seed=2)
+    n = g.n_vertices
+    s, d = g.edges()
+    w = gr.hash_weights(len(s), seed=2)
+    wd = gr.upload_i32(w)
+    hub, _ = g.max_out_degree_vertex()
+    srcs = _sources(n, 64, 5, extra=(hub,))
+    dist, info = gr.mssp(g, wd, srcs)
+    assert info["levels"] >= 1
+    for i, src in enumerate(srcs[:16]):
+        ref, _, _ = oracle.sssp(n, s, d, w, src)
+        assert np.array_equal(dist[i], ref), f"source {src}"
+    for i in (20, 40, 63):
+        plain, _ = gr.run_sssp(g, wd, srcs[i])
+        assert np.array_equal(dist[i], plain)
+    g.close()
+
+
+def test_batched_rejects_bad_sources(gpu):
+    gr = gpu
+    g = gr.Graph.rmat(8, 16, seed=1)
+    with pytest.raises(gr.GRError):
+        gr.msbfs(g, [0] * 65)
+    with pytest.raises(gr.GRError):
+        gr.msbfs(g, [g.n_vertices])
+    g.close()
