@@ -123,6 +123,9 @@ def main() -> int:
     vd[src] = 0
     eng = gr.GASEngine(gr.SSSP)
     eng.bindGraph(g, vd, w_dev)
+    from graphreduce_amd import tune
+
+    tuned = tune.apply_tuning(eng, gr.SSSP)
     step = lambda: run_loop(eng, all_active_n=n)  # noqa: E731
     trips = 0
     for _ in range(args.warmup):
@@ -150,13 +153,15 @@ def main() -> int:
     step_alg_gbs = gather_bytes / sec / 1e9
     log(f"  phases ms {phases} launches {launches}; gather {achieved:.1f} GB/s algorithmic")
 
+    # HBM traffic per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    # summary (calibrated against a plain copy kernel)
     prof_traffic = None
-    tfile = ROOT / "profiles" / "gather_tiles_sssp_rmat24_traffic.json"
+    tfile = ROOT / "profiles" / "gather_tiles_sssp_rmat24_traffic.md"
     if tfile.exists():
-        try:
-            prof_traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
-        except Exception:
-            prof_traffic = None
+        import re
+
+        m = re.search(r"hbm_bytes_per_launch: (\d+)", tfile.read_text())
+        prof_traffic = int(m.group(1)) if m else None
 
     result = {
         "metric": METRIC,
@@ -197,6 +202,7 @@ def main() -> int:
                                  "distance) + 16 B per active vertex",
         },
         "phase_ms_per_step": phases,
+        "tuning": "graphreduce_amd/tuned.json (autotuner)" if tuned else "defaults",
         "step_algorithmic_gbs": round(step_alg_gbs, 1),
     }
 
@@ -288,6 +294,162 @@ def run_extras(gr, g, args, src, outdeg):
                      gather_alg_gbs=round((8 * ein + 16 * st["active_vertices"])
                                           / (st["gather_ms"] / 1e3) / 1e9, 1))
     log(f"CC: {out['cc']}")
+    out["batched"] = run_batched(gr, g, args, outdeg)
+    out["cc_grid_4096"] = run_cc_grid(gr)
+    if not args.no_cpu_baseline:
+        out["pagerank_precision_rmat22"] = run_pr_precision(gr)
+    return out
+
+
+def _plain_loop_engine(gr, g, program, weights=None):
+    n = g.n_vertices
+    if program == gr.BFS:
+        vd = np.zeros(n, gr.PROGRAMS[gr.BFS]["vertex"])
+        vd["depth"] = -1
+    else:
+        vd = np.full(n, gr.SSSP_INF, np.int32)
+    eng = gr.GASEngine(program)
+    return eng, vd
+
+
+def run_batched(gr, g, args, outdeg):
+    """64-source BFS / SSSP in one batch vs 64 sequential plain runs (same
+    sources: seeded random vertices with out-degree > 0)."""
+    n = g.n_vertices
+    rng = np.random.default_rng(args.seed)
+    cand = np.flatnonzero(outdeg > 0)
+    srcs = rng.choice(cand, 64, replace=False).astype(np.int32).tolist()
+    res = {"sources": "64 seeded random vertices with out-degree > 0"}
+    # --- BFS
+    plain_rows, t_plain, er_tot = [], 0.0, 0
+    for s in srcs:
+        vd = np.zeros(n, gr.PROGRAMS[gr.BFS]["vertex"])
+        vd["depth"] = -1
+        eng = gr.GASEngine(gr.BFS)
+        eng.bindGraph(g, vd)
+        run_loop(eng, source=s)  # warm
+        gr.synchronize()
+        t0 = time.perf_counter()
+        run_loop(eng, source=s)
+        gr.synchronize()
+        t_plain += time.perf_counter() - t0
+        row = eng.readVertexData()["depth"]
+        er_tot += int(outdeg[row >= 0].sum())
+        plain_rows.append(row)
+        eng.close()
+    depth, info = gr.msbfs(g, srcs)  # warm
+    depth, info = gr.msbfs(g, srcs)
+    same = all(np.array_equal(depth[i], plain_rows[i]) for i in range(64))
+    t_b = info["ms"] / 1e3
+    res["bfs"] = dict(plain_sequential_ms=round(t_plain * 1e3, 3), batched_ms=round(t_b * 1e3, 3),
+                      plain_aggregate_gteps=round(er_tot / t_plain / 1e9, 3),
+                      batched_aggregate_gteps=round(er_tot / t_b / 1e9, 3),
+                      speedup=round(t_plain / t_b, 2), levels=info["levels"],
+                      identical_to_plain=bool(same))
+    log(f"batched BFS: {res['bfs']}")
+    del plain_rows, depth
+    # --- SSSP (16 sequential plain runs timed, scaled to 64; all 64 rows checked
+    # against the batch for the timed ones)
+    w = g.hash_weights_device(seed=args.seed)
+    dist, info = gr.mssp(g, w, srcs)  # warm
+    dist, info = gr.mssp(g, w, srcs)
+    t_b = info["ms"] / 1e3
+    k = 16
+    t_plain, er_tot_k, same = 0.0, 0, True
+    for i in range(k):
+        s = srcs[i]
+        vd = np.full(n, gr.SSSP_INF, np.int32)
+        vd[s] = 0
+        eng = gr.GASEngine(gr.SSSP)
+        eng.bindGraph(g, vd, w)
+        gr.synchronize()
+        t0 = time.perf_counter()
+        run_loop(eng, all_active_n=n)
+        gr.synchronize()
+        t_plain += time.perf_counter() - t0
+        row = eng.readVertexData()
+        er_tot_k += int(outdeg[row < gr.SSSP_INF].sum())
+        same = same and bool(np.array_equal(row, dist[i]))
+        eng.close()
+    er_all = int(sum(int(outdeg[dist[i] < gr.SSSP_INF].sum()) for i in range(64)))
+    t_plain64 = t_plain / k * 64
+    res["sssp"] = dict(plain_sequential_ms_64=round(t_plain64 * 1e3, 3),
+                       plain_runs_timed=k, batched_ms=round(t_b * 1e3, 3),
+                       plain_aggregate_gteps=round(er_tot_k / t_plain / 1e9, 3),
+                       batched_aggregate_gteps=round(er_all / t_b / 1e9, 3),
+                       speedup=round(t_plain64 / t_b, 2), rounds=info["levels"],
+                       identical_to_plain_for_timed_rows=same)
+    log(f"batched SSSP: {res['sssp']}")
+    return res
+
+
+def run_cc_grid(gr):
+    """CC on the 4096x4096 grid with 1 % of its edges deleted (seed 5)."""
+    gg = gr.Graph.grid(4096, 0.01, seed=5)
+    n = gg.n_vertices
+    vd = np.arange(n, dtype=np.int32)
+    eng = gr.GASEngine(gr.CC)
+    eng.bindGraph(gg, vd)
+    gr.synchronize()
+    t0 = time.perf_counter()
+    trips = run_loop(eng, all_active_n=n)
+    gr.synchronize()
+    sec = time.perf_counter() - t0
+    lab = eng.readVertexData()
+    st = eng.stats()
+    eng.close()
+    import scipy.sparse as sp
+    import scipy.sparse.csgraph as cg
+
+    s, d = gg.edges()
+    nc, lbl = cg.connected_components(sp.csr_matrix((np.ones(len(s)), (s, d)), shape=(n, n)),
+                                      directed=False)
+    mins = np.full(nc, n)
+    np.minimum.at(mins, lbl, np.arange(n))
+    ok = bool(np.array_equal(lab, mins[lbl]))
+    out = dict(ms=round(sec * 1e3, 2), trips=trips, n_edges=int(gg.n_edges),
+               components=int(nc), gathered_edges=st["gathered_edges"],
+               gteps=round(st["gathered_edges"] / sec / 1e9, 3),
+               us_per_trip=round(sec / max(1, trips) * 1e6, 1),
+               labels_equal_scipy_min_id=ok)
+    gg.close()
+    log(f"CC grid: {out}")
+    return out
+
+
+def run_pr_precision(gr):
+    """PageRank precision form on RMAT-22 seed 3 vs the FP64 CPU oracle."""
+    import oracle
+
+    g = gr.Graph.rmat(22, 16, seed=3)
+    out = {}
+    ranks = {}
+    for comp in (True, False):
+        gr.pagerank_precise(g, max_iters=2, tol=0.0, compensated=comp)  # warm
+        rk, info = gr.pagerank_precise(g, max_iters=50, tol=1e-7, compensated=comp)
+        ranks[comp] = (rk, info)
+    iters = ranks[True][1]["iterations"]
+    off_out, _ = g.csr()
+    off_in, srcs = g.csc()
+    t0 = time.perf_counter()
+    ref, l1 = oracle.pagerank_fp64_csr(g.n_vertices, off_out, off_in, srcs, iters)
+    cpu_s = time.perf_counter() - t0
+    for comp, name in ((True, "compensated"), (False, "plain_fp32")):
+        rk, info = ranks[comp]
+        rel = np.abs(rk.astype(np.float64) - ref) / ref
+        out[name] = dict(iterations=info["iterations"], residual_fp64=info["residual"],
+                         ms=round(info["ms"], 3),
+                         gteps=round(info["gathered_edges"] / (info["ms"] / 1e3) / 1e9, 2),
+                         gather_alg_gbs=round((8 * info["gathered_edges"] + 16 * g.n_vertices *
+                                               info["iterations"]) / (info["gather_ms"] / 1e3)
+                                              / 1e9, 1),
+                         max_rel_err_vs_fp64=float(rel.max()),
+                         p99_rel_err_vs_fp64=float(np.percentile(rel, 99)))
+    out["stop_rule"] = "absolute FP64 L1 residual < 1e-7 or 50 iterations"
+    out["fp64_oracle_cpu_s"] = round(cpu_s, 2)
+    out["n_edges"] = int(g.n_edges)
+    g.close()
+    log(f"PR precision: {out}")
     return out
 
 

@@ -22,7 +22,7 @@ __all__ = [
     "GRError", "lib", "Graph", "GASEngine", "PROGRAMS", "BFS", "SSSP", "PAGERANK", "CC",
     "rmat_edges", "grid_edges", "hash_weights", "device_count", "device_info",
     "run_bfs", "run_sssp", "run_cc", "run_pagerank", "SSSP_INF", "pagerank_precise",
-    "msbfs", "mssp", "upload_i32",
+    "msbfs", "mssp", "upload_i32", "Server", "QUERY_BFS", "QUERY_SSSP", "QUERY_PPR",
 ]
 
 _PKG = Path(__file__).resolve().parent
@@ -88,6 +88,18 @@ class _BatchResult(C.Structure):
     _fields_ = [("levels", C.c_int32), ("ms", C.c_double), ("grid", C.c_int32)]
 
 
+class _ServerConfig(C.Structure):
+    _fields_ = [("memory_budget_bytes", C.c_size_t), ("max_lanes_per_kind", C.c_int),
+                ("ppr_iterations", C.c_int)]
+
+
+class _ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("steps", C.c_int64),
+                ("max_lanes_in_use", C.c_int32), ("peak_bytes", C.c_int64),
+                ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("max_ms", C.c_double),
+                ("throughput_rps", C.c_double)]
+
+
 _P = C.c_void_p
 _I32P = C.POINTER(C.c_int32)
 _I64P = C.POINTER(C.c_int64)
@@ -147,6 +159,13 @@ _SIGS = {
     "gr_msbfs": (C.c_int, [_P, _I32P, C.c_int, _P, C.c_int, C.c_double,
                             C.POINTER(_BatchResult)]),
     "gr_mssp": (C.c_int, [_P, _P, _I32P, C.c_int, _P, C.POINTER(_BatchResult)]),
+    "gr_server_create": (C.c_int, [_P, _P, C.POINTER(_ServerConfig), C.POINTER(_P)]),
+    "gr_server_destroy": (C.c_int, [_P]),
+    "gr_server_submit": (C.c_int, [_P, C.c_int, C.c_int32, C.c_int, _I64P]),
+    "gr_server_wait": (C.c_int, [_P, C.c_int64, _P, C.POINTER(C.c_uint64),
+                                 C.POINTER(C.c_double), _I32P]),
+    "gr_server_get_stats": (C.c_int, [_P, C.POINTER(_ServerStats)]),
+    "gr_server_load_test": (C.c_int, [_P, C.c_double, C.c_int, _I32P, _I32P, C.c_int, _I64P]),
     "gr_pagerank_precise": (C.c_int, [_P, C.c_int, C.c_double, C.c_int,
                                        C.POINTER(C.c_float), C.POINTER(C.c_double),
                                        C.POINTER(_PRResult)]),
@@ -663,3 +682,63 @@ def upload_i32(a) -> DeviceArray:
     if h.nbytes:
         _check(lib().gr_memcpy_h2d(d, h.ctypes.data, h.nbytes))
     return DeviceArray(d, h.nbytes)
+
+
+QUERY_BFS, QUERY_SSSP, QUERY_PPR = 1, 2, 3
+
+
+class Server:
+    """Continuous-batching query server over a device graph (see gr_abi.h)."""
+
+    def __init__(self, graph: Graph, weights: "DeviceArray | None" = None,
+                 memory_budget_bytes: int = 0, max_lanes_per_kind: int = 0,
+                 ppr_iterations: int = 0):
+        cfg = _ServerConfig(memory_budget_bytes, max_lanes_per_kind, ppr_iterations)
+        h = _P()
+        _check(lib().gr_server_create(graph.handle, weights.ptr if weights else None,
+                                      C.byref(cfg), C.byref(h)))
+        self._h = h
+        self.graph = graph
+        self.n = graph.n_vertices
+
+    def close(self):
+        if self._h:
+            lib().gr_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, kind: int, vertex: int, keep: bool = True) -> int:
+        rid = C.c_int64(0)
+        _check(lib().gr_server_submit(self._h, kind, vertex, int(keep), C.byref(rid)))
+        return rid.value
+
+    def wait(self, rid: int, kind: int | None = None, want_result: bool = True):
+        """Returns (result or None, checksum, latency_ms, steps)."""
+        out = None
+        ptr = None
+        if want_result:
+            out = np.empty(self.n, np.float32 if kind == QUERY_PPR else np.int32)
+            ptr = out.ctypes.data
+        cs = C.c_uint64(0)
+        lat = C.c_double(0)
+        steps = C.c_int32(0)
+        _check(lib().gr_server_wait(self._h, rid, ptr, C.byref(cs), C.byref(lat), C.byref(steps)))
+        return out, cs.value, lat.value, steps.value
+
+    def stats(self) -> dict:
+        st = _ServerStats()
+        _check(lib().gr_server_get_stats(self._h, C.byref(st)))
+        return {name: getattr(st, name) for name, _ in _ServerStats._fields_}
+
+    def load_test(self, rate_per_s: float, vertices, kinds, keep: bool = False):
+        v = _i32(vertices)
+        k = _i32(kinds)
+        ids = np.empty(len(v), np.int64)
+        _check(lib().gr_server_load_test(self._h, rate_per_s, len(v), _p32(v), _p32(k), int(keep),
+                                         ids.ctypes.data_as(_I64P)))
+        return ids

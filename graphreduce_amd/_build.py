@@ -47,6 +47,7 @@ SOURCES = [
     "gr_engine.hip",
     "gr_pagerank.hip",
     "gr_batched.hip",
+    "gr_serve.hip",
 ]
 
 APP_SOURCES = ["bfs.cpp", "sssp.cpp", "pagerank.cpp", "connected_component.cpp"]

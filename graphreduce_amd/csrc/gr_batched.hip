@@ -18,18 +18,19 @@
 // Depths equal those of independent plain runs (the BFS level of a vertex
 // does not depend on the schedule), which the tests check.
 //
-// Multi-source SSSP (k_mssp_round): Bellman-Ford rounds over 64 lanes, one
-// wave per destination vertex, lane s relaxing source s; only in-edges whose
-// source changed in the previous round are read (a source that did not
-// change cannot lower a distance it already lowered), so distances equal the
-// plain engine's fixed point.  Rounds are recorded once into a hipGraph and
-// replayed.
+// Multi-source SSSP (k_lane_sssp_light/heavy in gr_lanes.h): Bellman-Ford
+// rounds over 64 lanes, one wave per destination vertex, lane s relaxing
+// source s; only in-edges whose source changed in the previous round are read
+// (a source that did not change cannot lower a distance it already lowered),
+// so distances equal the plain engine's fixed point.  Rounds are recorded
+// once into a hipGraph and replayed.
 #include <hip/hip_cooperative_groups.h>
 
 #include <vector>
 
 #include "gr_device.h"
 #include "gr_gather.h"
+#include "gr_lanes.h"
 
 namespace cg = cooperative_groups;
 
@@ -103,12 +104,20 @@ __global__ __launch_bounds__(kBatchNT) void k_msbfs(
   const long long threshold = (long long)((double)nE / (alpha > 0 ? alpha : 1.0));
   for (int level = 0; level < max_levels; ++level) {
     const int slot = level % 3;
-    if (ctr->new_vertices[slot] == 0) break;  // uniform: read after a grid barrier
-    const bool pull = mode == 2 || (mode == 0 && (long long)ctr->frontier_edges[slot] > threshold);
+    // Coherent (atomic) loads after the grid barrier: every workgroup must see
+    // the same counts so that all of them leave the loop at the same level
+    // and pick the same direction.
+    const unsigned long long nv_count =
+        __hip_atomic_load(&ctr->new_vertices[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long fe_count =
+        __hip_atomic_load(&ctr->frontier_edges[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (nv_count == 0) break;
+    const bool pull = mode == 2 || (mode == 0 && (long long)fe_count > threshold);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       const int nx = (level + 1) % 3;
-      ctr->new_vertices[nx] = 0;
-      ctr->frontier_edges[nx] = 0;
+      __hip_atomic_store(&ctr->new_vertices[nx], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ctr->frontier_edges[nx], 0ull, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
     }
     // ---------------------------------------------------------- expand
     for (int base = wave_id * kWave; base < nV; base += n_waves * kWave) {
@@ -216,58 +225,6 @@ __global__ __launch_bounds__(kBatchNT) void k_msbfs(
 }
 
 // ------------------------------------------------------------ batched SSSP
-
-// dist is vertex-major: dist[v * 64 + s].  One wave per destination vertex.
-__global__ __launch_bounds__(kBatchNT) void k_mssp_round(
-    int nV, const int32_t* __restrict__ off_in, const int32_t* __restrict__ srcs,
-    const int32_t* __restrict__ w_csc, int32_t* __restrict__ dist,
-    const uint64_t* __restrict__ chg_in, uint64_t* __restrict__ chg_out,
-    unsigned long long* __restrict__ changed_count) {
-  const int lane = lane_id();
-  const int wpb = blockDim.x / kWave;
-  const int n_waves = gridDim.x * wpb;
-  const uint64_t my_bit = 1ull << lane;
-  unsigned long long cnt = 0;
-  for (int v = blockIdx.x * wpb + threadIdx.x / kWave; v < nV; v += n_waves) {
-    const int b = off_in[v], e = off_in[v + 1];
-    int32_t best = dist[(size_t)v * kWave + lane];
-    const int32_t before = best;
-    for (int pb = b; pb < e; pb += kWave) {
-      // each lane fetches one in-edge; the wave then walks the changed ones
-      const int p = pb + lane;
-      int u = 0, wt = 0;
-      uint64_t cu = 0;
-      if (p < e) {
-        u = srcs[p];
-        cu = chg_in[u];
-        if (cu) wt = w_csc[p];
-      }
-      uint64_t todo = __ballot(cu != 0);
-      while (todo) {
-        const int k = __builtin_ctzll(todo);
-        todo &= todo - 1;
-        const int uu = __shfl(u, k, kWave);
-        const int ww = __shfl(wt, k, kWave);
-        const uint32_t clo = __shfl((uint32_t)cu, k, kWave);
-        const uint32_t chi = __shfl((uint32_t)(cu >> 32), k, kWave);
-        const uint64_t cm = ((uint64_t)chi << 32) | clo;
-        if (cm & my_bit) {
-          const int32_t du = dist[(size_t)uu * kWave + lane];
-          const int32_t cand = du + ww;
-          if (cand < best) best = cand;
-        }
-      }
-    }
-    const bool improved = best < before;
-    if (improved) dist[(size_t)v * kWave + lane] = best;
-    const uint64_t m = __ballot(improved);
-    if (lane == 0) {
-      chg_out[v] = m;
-      if (m) cnt += 1;
-    }
-  }
-  if (lane == 0 && cnt) atomicAdd(changed_count, cnt);
-}
 
 __global__ void k_mssp_init(int nV, int ns, const int32_t* __restrict__ sources,
                             int32_t* __restrict__ dist, uint64_t* __restrict__ chg,
@@ -401,16 +358,19 @@ extern "C" int gr_mssp(gr_graph* g, const int32_t* weights_device, const int32_t
   const int32_t inf = 2147483647 - 100000;  // SSSP::gatherZero, sssp.cu:32-33
   DBuf<int32_t> dist, w_csc, d_src;
   DBuf<uint64_t> chg_a, chg_b;
-  DBuf<unsigned long long> cnt;
+  DBuf<unsigned long long> cnt, any;
   GR_TRY(dist.alloc((size_t)nV * kWave));
   GR_TRY(w_csc.alloc((size_t)g->nE));
   GR_TRY(d_src.alloc(n_sources));
   GR_TRY(chg_a.alloc(nV));
   GR_TRY(chg_b.alloc(nV));
   GR_TRY(cnt.alloc(64));
+  GR_TRY(any.alloc(1));
   GR_HIP(hipMemcpy(d_src.p, sources, sizeof(int32_t) * n_sources, hipMemcpyHostToDevice));
   if (g->nE)
     k_permute_i32<<<grid_1d(g->nE, 256), 256>>>(weights_device, g->csc_eid, g->nE, w_csc.p);
+  LaneTables lt;
+  GR_TRY(lt.build(g->h_off_in));
   hipStream_t st;
   GR_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   GR_HIP(hipDeviceSynchronize());
@@ -424,15 +384,19 @@ extern "C" int gr_mssp(gr_graph* g, const int32_t* weights_device, const int32_t
   // One graph holds kRounds rounds (ping-pong of the changed masks); replays
   // continue until a round changes nothing.
   constexpr int kRounds = 8;
-  const unsigned rgrid = grid_1d((int64_t)nV, kBatchNT / kWave);
+  const unsigned rgrid = grid_1d((int64_t)nV, kLaneNT / kWave);
+  const unsigned hgrid = grid_1d((int64_t)lt.n_chunks, kLaneNT / kWave);
   hipGraph_t graph;
   hipGraphExec_t exec;
   GR_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
   for (int r = 0; r < kRounds; ++r) {
     uint64_t* cin = (r & 1) ? chg_b.p : chg_a.p;
     uint64_t* cout = (r & 1) ? chg_a.p : chg_b.p;
-    k_mssp_round<<<rgrid < 65536 ? rgrid : 65536, kBatchNT, 0, st>>>(
-        nV, g->off_in, g->srcs, w_csc.p, dist.p, cin, cout, cnt.p + r);
+    k_lane_sssp_light<<<rgrid < 65536 ? rgrid : 65536, kLaneNT, 0, st>>>(
+        nV, g->off_in, g->srcs, w_csc.p, dist.p, cin, cout, any.p, cnt.p + r);
+    if (lt.n_chunks)
+      k_lane_sssp_heavy<<<hgrid, kLaneNT, 0, st>>>(lt.chunks, lt.n_chunks, g->srcs, w_csc.p,
+                                                   dist.p, cin, cout, any.p, cnt.p + r);
   }
   GR_HIP(hipStreamEndCapture(st, &graph));
   GR_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));

@@ -275,6 +275,54 @@ int gr_msbfs(gr_graph* g, const int32_t* sources, int n_sources, int32_t* depth_
 int gr_mssp(gr_graph* g, const int32_t* weights_device, const int32_t* sources, int n_sources,
             int32_t* dist_device, gr_batch_result* result);
 
+/* ------------------------------------------------------------------ serving */
+/* Continuous-batching query server (no reference counterpart).  Each query
+ * kind has a pool of up to 64 resident lanes; a host thread admits queued
+ * requests into free lanes as lanes and the memory budget allow, advances
+ * every pool by one step per loop (BFS level, SSSP round, PPR iteration; the
+ * pools run on separate streams and overlap), and retires finished lanes.
+ * Each result equals the query's solo result. */
+#define GR_QUERY_BFS 1  /* int32 depth per vertex, -1 unreached */
+#define GR_QUERY_SSSP 2 /* int32 distance per vertex, INT_MAX - 100000 unreached */
+#define GR_QUERY_PPR 3  /* float rank: r = 0.15 e_seed + 0.85 A^T (r / outdeg), r_0 = e_seed */
+
+typedef struct gr_server gr_server;
+
+typedef struct gr_server_config {
+  size_t memory_budget_bytes; /* 0 = unlimited; a lane is admitted only if it fits */
+  int max_lanes_per_kind;     /* 0 = 64 */
+  int ppr_iterations;         /* 0 = 10 */
+} gr_server_config;
+
+typedef struct gr_server_stats {
+  int64_t submitted;
+  int64_t completed;
+  int64_t steps;            /* server loop iterations */
+  int32_t max_lanes_in_use; /* over all kinds */
+  int64_t peak_bytes;       /* lane bytes admitted at once */
+  double p50_ms, p99_ms, max_ms; /* submit -> completion latency */
+  double throughput_rps;    /* completed / (last completion - first submit) */
+} gr_server_stats;
+
+/* weights_device (int32 per edge, edge-list order) may be NULL when no SSSP
+ * query will be submitted.  The graph must outlive the server. */
+int gr_server_create(gr_graph* g, const int32_t* weights_device, const gr_server_config* config,
+                     gr_server** out);
+int gr_server_destroy(gr_server* s);
+/* Thread-safe.  keep_result != 0 keeps the V-element result for gr_server_wait;
+ * otherwise only its checksum is kept.  GR_ERR_BUDGET if one lane of this kind
+ * can never fit the memory budget. */
+int gr_server_submit(gr_server* s, int kind, int32_t vertex, int keep_result, int64_t* request_id);
+/* Blocks until the request completed.  result (V elements, may be NULL),
+ * checksum (order-independent hash of the result row), latency, steps. */
+int gr_server_wait(gr_server* s, int64_t request_id, void* result, uint64_t* checksum,
+                   double* latency_ms, int32_t* steps);
+int gr_server_get_stats(gr_server* s, gr_server_stats* out);
+/* Submits n requests (kinds[i], vertices[i]) with exponential inter-arrival
+ * gaps of mean 1/rate (a Poisson stream) from the calling thread. */
+int gr_server_load_test(gr_server* s, double rate_per_s, int n_requests, const int32_t* vertices,
+                        const int32_t* kinds, int keep_results, int64_t* request_ids);
+
 #ifdef __cplusplus
 }
 #endif
