@@ -635,10 +635,17 @@ def _batch_sources(sources):
     return src
 
 
-def msbfs(graph: Graph, sources, direction: int = 0, alpha: float = 20.0):
+MSBFS_STREAM_PULL = 4
+
+
+def msbfs(graph: Graph, sources, direction: int = 0, alpha: float = 20.0,
+          stream_pull: bool = False):
     """BFS from up to 64 sources in one persistent launch.
 
-    Returns (depth int32[len(sources), V], info)."""
+    stream_pull asks for the streaming pull kernel, which the library
+    currently refuses (GRError).  Returns (depth int32[len(sources), V], info)."""
+    if stream_pull:
+        direction |= MSBFS_STREAM_PULL
     src = _batch_sources(sources)
     n = graph.n_vertices
     nbytes = 4 * n * len(src)

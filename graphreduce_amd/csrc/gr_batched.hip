@@ -11,7 +11,8 @@
 //                 sources already visited);
 //   expand, pull  every vertex with unvisited sources ORs the frontier masks of
 //                 its CSC in-neighbours, stopping once all of its unvisited
-//                 sources are covered;
+//                 sources are covered (thread per vertex; a wave per vertex
+//                 above kMsHeavy in-edges);
 //   update        new = next & ~visit; visit |= new; frontier = new; the
 //                 level is written for every new (source, vertex) pair.
 // The direction is chosen per level from the frontier's out-edge count.
@@ -46,6 +47,18 @@ struct MsCounters {
   int32_t error;
 };
 
+// Grid barrier that first completes this wave's own memory operations.
+// The workgroup barrier inside grid.sync() does not wait for the
+// fire-and-forget atomics (counter adds, atomicOr into next) of the waves
+// that are not the workgroup's barrier leader, so such an atomic could land
+// after another workgroup has already read the location.  Every workgroup
+// must read the same level counts (they decide when all leave the level
+// loop), hence the agent-scope fence in every wave before the barrier.
+__device__ __forceinline__ void grid_barrier(cg::grid_group& grid) {
+  __threadfence();
+  grid.sync();
+}
+
 // OR-reduction of a 64-bit value over the wave.
 __device__ __forceinline__ uint64_t wave_or64(uint64_t v) {
   uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
@@ -58,18 +71,19 @@ __device__ __forceinline__ uint64_t wave_or64(uint64_t v) {
 }
 
 __global__ void k_ms_init(int nV, uint64_t* __restrict__ visit, uint64_t* __restrict__ frontier,
-                          uint64_t* __restrict__ next) {
+                          uint64_t* __restrict__ next, uint32_t* __restrict__ fbits) {
   for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nV; v += gridDim.x * blockDim.x) {
     visit[v] = 0;
     frontier[v] = 0;
     next[v] = 0;
+    if (fbits && (v & 31) == 0) fbits[v >> 5] = 0;
   }
 }
 
 __global__ void k_ms_sources(int ns, const int32_t* __restrict__ sources, int nV,
                              uint64_t* __restrict__ visit, uint64_t* __restrict__ frontier,
-                             int32_t* __restrict__ depth, const int32_t* __restrict__ off_out,
-                             MsCounters* __restrict__ ctr) {
+                             uint32_t* __restrict__ fbits, int32_t* __restrict__ depth,
+                             const int32_t* __restrict__ off_out, MsCounters* __restrict__ ctr) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   unsigned long long fe = 0;
   for (int s = 0; s < ns; ++s) {
@@ -77,6 +91,7 @@ __global__ void k_ms_sources(int ns, const int32_t* __restrict__ sources, int nV
     if (frontier[v] == 0) fe += (unsigned long long)(off_out[v + 1] - off_out[v]);
     visit[v] |= 1ull << s;
     frontier[v] |= 1ull << s;
+    if (fbits) fbits[v >> 5] |= 1u << (v & 31);
     depth[(size_t)s * nV + v] = 0;
   }
   for (int i = 0; i < 3; ++i) {
@@ -185,7 +200,7 @@ __global__ __launch_bounds__(kBatchNT) void k_msbfs(
         }
       }
     }
-    grid.sync();
+    grid_barrier(grid);
     // ---------------------------------------------------------- update
     const int slot_next = (level + 1) % 3;
     for (int base = wave_id * kWave; base < nV; base += n_waves * kWave) {
@@ -219,7 +234,219 @@ __global__ __launch_bounds__(kBatchNT) void k_msbfs(
         atomicAdd(&ctr->frontier_edges[slot_next], fe);
       }
     }
-    grid.sync();
+    grid_barrier(grid);
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctr->levels = level + 1;
+  }
+}
+
+// Pull for the 64 vertices base .. base+63 (lane l <-> vertex base + l): the
+// wave streams their contiguous CSC segment; an edge contributes
+// frontier[u] when u's bit is set in the frontier bitmap; per-row ORs are
+// formed with a segmented shuffle reduction and land in the row's lane.
+// Rows whose unvisited sources are all covered, and heavy rows (done as
+// chunks), are jumped over.  Returns next for the lane's own vertex.
+__device__ __forceinline__ uint64_t ms_pull_rows(int base, int nV, uint64_t all, int heavy_deg,
+                                                 const int32_t* __restrict__ off_in,
+                                                 const int32_t* __restrict__ srcs,
+                                                 const uint64_t* __restrict__ visit,
+                                                 const uint64_t* __restrict__ frontier,
+                                                 const uint32_t* __restrict__ fbits) {
+  const int lane = lane_id();
+  const int cnt = min(kWave, nV - base);
+  const bool mine = lane < cnt;
+  const uint64_t need = mine ? (all & ~visit[base + lane]) : 0ull;
+  int my_off = INT_MAX, my_end = INT_MAX;
+  if (mine) {
+    my_off = off_in[base + lane];
+    my_end = off_in[base + lane + 1];
+  }
+  const bool heavy = mine && (my_end - my_off) > heavy_deg;
+  const uint64_t todo_rows = __ballot(need != 0 && my_end > my_off && !heavy);
+  uint64_t acc = 0;
+  if (!todo_rows) return 0;
+  uint64_t open = todo_rows;  // rows still to be completed
+  const int first = __builtin_ctzll(todo_rows);
+  const int last = 63 - __builtin_clzll(todo_rows);
+  int pb = __shfl(my_off, first, kWave);
+  const int seg1 = __shfl(my_end, last, kWave);
+  while (pb < seg1 && open) {
+    const uint64_t started = __ballot(mine && my_off <= pb);
+    const int r0 = 63 - __builtin_clzll(started);
+    if (!((open >> r0) & 1ull)) {  // row done, heavy, empty or needless: jump
+      pb = __shfl(my_end, r0, kWave);
+      continue;
+    }
+    const int p = pb + lane;
+    uint64_t val = 0;
+    if (p < seg1) {
+      const int u = srcs[p];
+      if ((fbits[u >> 5] >> (u & 31)) & 1u) val = frontier[u];
+    }
+    // row of each lane's edge: last row starting at or before p
+    int r = 0;
+#pragma unroll
+    for (int st = 32; st >= 1; st >>= 1) {
+      const int c = r + st;
+      const int oc = __shfl(my_off, c & 63, kWave);
+      if (c < cnt && oc <= p) r = c;
+    }
+    // segmented OR over runs of equal rows (suffix OR within each run)
+    uint32_t lo = (uint32_t)val, hi = (uint32_t)(val >> 32);
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const int orow = __shfl_down(r, off, kWave);
+      const uint32_t olo = __shfl_down(lo, off, kWave);
+      const uint32_t ohi = __shfl_down(hi, off, kWave);
+      if (lane + off < kWave && orow == r) {
+        lo |= olo;
+        hi |= ohi;
+      }
+    }
+    // the head lane of row l's run is max(off_l - pb, 0)
+    const int head = min(max(my_off - pb, 0), 63);
+    const uint32_t hlo = __shfl(lo, head, kWave);
+    const uint32_t hhi = __shfl(hi, head, kWave);
+    const uint32_t hrow = __shfl(r, head, kWave);
+    if (mine && my_off < pb + kWave && my_end > pb && (int)hrow == lane)
+      acc |= ((uint64_t)hhi << 32) | hlo;
+    const bool done_row = (acc & need) == need;
+    open &= ~__ballot(done_row);
+    pb += kWave;
+  }
+  // a heavy row may have picked up part of its edges here; its result comes
+  // only from the chunk pass (atomicOr), never from a plain store
+  return heavy ? 0ull : (acc & need);
+}
+
+// Streaming-pull variant (frontier bitmap, wave per 64 rows, heavy-row
+// chunks).  Disabled: see kStreamPullEnabled.
+__global__ __launch_bounds__(kBatchNT) void k_msbfs_stream(
+    int nV, long long nE, int ns, int mode, double alpha, const int32_t* __restrict__ off_out,
+    const int32_t* __restrict__ dsts, const int32_t* __restrict__ off_in,
+    const int32_t* __restrict__ srcs, uint64_t* __restrict__ visit,
+    uint64_t* __restrict__ frontier, uint64_t* __restrict__ next, uint32_t* __restrict__ fbits,
+    int32_t* __restrict__ depth, const LaneChunk* __restrict__ chunks, int n_chunks,
+    MsCounters* __restrict__ ctr, int max_levels) {
+  cg::grid_group grid = cg::this_grid();
+  const uint64_t all = ns >= 64 ? ~0ull : ((1ull << ns) - 1);
+  const int lane = lane_id();
+  const int wpb = blockDim.x / kWave;
+  const int wave_id = blockIdx.x * wpb + threadIdx.x / kWave;
+  const int n_waves = gridDim.x * wpb;
+  const long long threshold = (long long)((double)nE / (alpha > 0 ? alpha : 1.0));
+  for (int level = 0; level < max_levels; ++level) {
+    const int slot = level % 3;
+    // Coherent (atomic) loads after the grid barrier: every workgroup must see
+    // the same counts so that all of them leave the loop at the same level
+    // and pick the same direction.
+    const unsigned long long nv_count =
+        __hip_atomic_load(&ctr->new_vertices[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long fe_count =
+        __hip_atomic_load(&ctr->frontier_edges[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (nv_count == 0) break;
+    const bool pull = mode == 2 || (mode == 0 && (long long)fe_count > threshold);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      const int nx = (level + 1) % 3;
+      __hip_atomic_store(&ctr->new_vertices[nx], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ctr->frontier_edges[nx], 0ull, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // ---------------------------------------------------------- expand
+    if (!pull) {
+      for (int base = wave_id * kWave; base < nV; base += n_waves * kWave) {
+        const int v = base + lane;
+        const uint32_t word = (v < nV) ? fbits[v >> 5] : 0u;
+        const bool inf = (word >> (v & 31)) & 1u;
+        if (!__ballot(inf)) continue;
+        const uint64_t f = inf ? frontier[v] : 0ull;
+        int b = 0, e = 0;
+        if (f) {
+          b = off_out[v];
+          e = off_out[v + 1];
+        }
+        const bool light = (e - b) <= kMsHeavy;
+        if (f && light) {
+          for (int p = b; p < e; ++p) {
+            const int u = dsts[p];
+            if (f & ~visit[u]) atomicOr((unsigned long long*)&next[u], (unsigned long long)f);
+          }
+        }
+        uint64_t heavy = __ballot(f && !light);
+        while (heavy) {
+          const int hl = __builtin_ctzll(heavy);
+          heavy &= heavy - 1;
+          const int hb = __shfl(b, hl, kWave), he = __shfl(e, hl, kWave);
+          const uint32_t flo = __shfl((uint32_t)f, hl, kWave);
+          const uint32_t fhi = __shfl((uint32_t)(f >> 32), hl, kWave);
+          const uint64_t hf = ((uint64_t)fhi << 32) | flo;
+          for (int p = hb + lane; p < he; p += kWave) {
+            const int u = dsts[p];
+            if (hf & ~visit[u]) atomicOr((unsigned long long*)&next[u], (unsigned long long)hf);
+          }
+        }
+      }
+    } else {
+      for (int base = wave_id * kWave; base < nV; base += n_waves * kWave) {
+        const uint64_t nx = ms_pull_rows(base, nV, all, kLaneHeavy, off_in, srcs, visit,
+                                         frontier, fbits);
+        if (nx) next[base + lane] = nx;  // heavy rows are left to the chunk pass
+      }
+      // heavy rows: chunks of in-edges spread over all waves, ORed in atomically
+      for (int c = wave_id; c < n_chunks; c += n_waves) {
+        const LaneChunk ch = chunks[c];
+        const uint64_t need = all & ~visit[ch.v];
+        if (!need) continue;
+        uint64_t acc = 0;
+        for (int pb = ch.b; pb < ch.e; pb += kWave) {  // wave-uniform trip count
+          const int p = pb + lane;
+          uint64_t m = 0;
+          if (p < ch.e) {
+            const int u = srcs[p];
+            if ((fbits[u >> 5] >> (u & 31)) & 1u) m = frontier[u];
+          }
+          acc |= wave_or64(m);
+          if ((acc & need) == need) break;
+        }
+        if (lane == 0 && (acc & need))
+          atomicOr((unsigned long long*)&next[ch.v], (unsigned long long)(acc & need));
+      }
+    }
+    grid_barrier(grid);
+    // ---------------------------------------------------------- update
+    const int slot_next = (level + 1) % 3;
+    for (int base = wave_id * kWave; base < nV; base += n_waves * kWave) {
+      const int v = base + lane;
+      uint64_t nw = 0;
+      int deg = 0;
+      if (v < nV) {
+        const uint64_t vis = visit[v];
+        const uint64_t nxt = next[v];
+        nw = nxt & ~vis & all;
+        if (nxt) next[v] = 0;
+        if (nw || frontier[v]) frontier[v] = nw;
+        if (nw) {
+          visit[v] = vis | nw;
+          deg = off_out[v + 1] - off_out[v];
+          uint64_t bits = nw;
+          while (bits) {
+            const int s = __builtin_ctzll(bits);
+            bits &= bits - 1;
+            depth[(size_t)s * nV + v] = level + 1;
+          }
+        }
+      }
+      const uint64_t nb = __ballot(nw != 0);
+      // frontier bitmap: lanes 0 and 32 write the two words of these vertices
+      if ((lane & 31) == 0 && v < nV) fbits[v >> 5] = (uint32_t)(nb >> lane);
+      unsigned long long fe = (unsigned long long)deg;
+#pragma unroll
+      for (int off = kWave / 2; off > 0; off >>= 1) fe += __shfl_xor(fe, off, kWave);
+      if (lane == 0 && nb) {
+        atomicAdd(&ctr->new_vertices[slot_next], (unsigned long long)__popcll(nb));
+        atomicAdd(&ctr->frontier_edges[slot_next], fe);
+      }
+    }
+    grid_barrier(grid);
     if (blockIdx.x == 0 && threadIdx.x == 0) ctr->levels = level + 1;
   }
 }
@@ -286,30 +513,51 @@ struct DBuf {
 
 using namespace gr;
 
+// The streaming-pull kernel (k_msbfs_stream) stays off: its cooperative
+// launch did not finish on an MI355X (RMAT-14, 64 sources, automatic
+// direction) although a lane-level CPU emulation of the same code terminates
+// with exact depths on that graph.  Unlike k_msbfs it needs 86 SGPRs, so one
+// open question is whether all of the 8 workgroups per CU that the occupancy
+// query reports are resident at once (grid.sync waits for every workgroup).
+constexpr bool kStreamPullEnabled = false;
+
 extern "C" int gr_msbfs(gr_graph* g, const int32_t* sources, int n_sources, int32_t* depth_device,
                         int direction, double alpha, gr_batch_result* result) {
+  const bool stream = (direction & GR_MSBFS_STREAM_PULL) != 0;
+  direction &= ~GR_MSBFS_STREAM_PULL;
   if (!g || !sources || !depth_device || n_sources < 1 || n_sources > 64 || direction < 0 ||
       direction > 2)
     return set_error(GR_ERR_INVALID, "gr_msbfs: bad arguments");
+  if (stream && !kStreamPullEnabled)
+    return set_error(GR_ERR_INVALID,
+                     "gr_msbfs: the streaming pull kernel is disabled (its launch did not finish "
+                     "on the GPU; cause not established)");
   const int nV = g->nV;
   for (int s = 0; s < n_sources; ++s)
     if (sources[s] < 0 || sources[s] >= nV)
       return set_errorf(GR_ERR_INVALID, "gr_msbfs: source %d out of range", sources[s]);
   DBuf<uint64_t> visit, frontier, next;
+  DBuf<uint32_t> fbits;
   DBuf<int32_t> d_src;
   DBuf<MsCounters> ctr;
+  LaneTables lt;
   GR_TRY(visit.alloc(nV));
   GR_TRY(frontier.alloc(nV));
   GR_TRY(next.alloc(nV));
+  if (stream) {
+    GR_TRY(fbits.alloc((size_t)nV / 32 + 2));
+    GR_TRY(lt.build(g->h_off_in));
+  }
   GR_TRY(d_src.alloc(n_sources));
   GR_TRY(ctr.alloc(1));
   GR_HIP(hipMemcpy(d_src.p, sources, sizeof(int32_t) * n_sources, hipMemcpyHostToDevice));
   // depth = -1 everywhere: the counterpart of the host-side vertex init before setGraph
   GR_HIP(hipMemset(depth_device, 0xFF, sizeof(int32_t) * (size_t)nV * n_sources));
+  const void* kernel = stream ? (const void*)k_msbfs_stream : (const void*)k_msbfs;
   int dev = 0, cus = 0, per_cu = 0;
   GR_HIP(hipGetDevice(&dev));
   GR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  GR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_msbfs, kBatchNT, 0));
+  GR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBatchNT, 0));
   if (per_cu < 1) return set_error(GR_ERR_HIP, "gr_msbfs: kernel cannot be resident");
   const int grid = cus * per_cu;
   hipEvent_t e0, e1;
@@ -317,20 +565,29 @@ extern "C" int gr_msbfs(gr_graph* g, const int32_t* sources, int n_sources, int3
   GR_HIP(hipEventCreate(&e1));
   GR_HIP(hipDeviceSynchronize());
   GR_HIP(hipEventRecord(e0, 0));
-  k_ms_init<<<1024, 256>>>(nV, visit.p, frontier.p, next.p);
-  k_ms_sources<<<1, 64>>>(n_sources, d_src.p, nV, visit.p, frontier.p, depth_device, g->off_out,
-                          ctr.p);
+  k_ms_init<<<1024, 256>>>(nV, visit.p, frontier.p, next.p, fbits.p);
+  k_ms_sources<<<1, 64>>>(n_sources, d_src.p, nV, visit.p, frontier.p, fbits.p, depth_device,
+                          g->off_out, ctr.p);
   GR_HIP(hipGetLastError());
   int nVa = nV, ns = n_sources, mode = direction, max_levels = nV + 1;
   long long nE = g->nE;
   double al = alpha;
   const int32_t *off_out = g->off_out, *dsts = g->dsts, *off_in = g->off_in, *srcs = g->srcs;
   uint64_t *pv = visit.p, *pf = frontier.p, *pn = next.p;
+  uint32_t* pfb = fbits.p;
   int32_t* pd = depth_device;
+  const LaneChunk* pch = lt.chunks;
+  int nch = lt.n_chunks;
   MsCounters* pc = ctr.p;
-  void* args[] = {&nVa, &nE, &ns, &mode, &al, &off_out, &dsts, &off_in, &srcs,
-                  &pv,  &pf, &pn, &pd,   &pc, &max_levels};
-  GR_HIP(hipLaunchCooperativeKernel((const void*)k_msbfs, dim3(grid), dim3(kBatchNT), args, 0, 0));
+  if (stream) {
+    void* args[] = {&nVa, &nE, &ns, &mode, &al, &off_out, &dsts, &off_in, &srcs, &pv,
+                    &pf,  &pn, &pfb, &pd,  &pch, &nch,   &pc,   &max_levels};
+    GR_HIP(hipLaunchCooperativeKernel(kernel, dim3(grid), dim3(kBatchNT), args, 0, 0));
+  } else {
+    void* args[] = {&nVa, &nE, &ns, &mode, &al, &off_out, &dsts, &off_in, &srcs,
+                    &pv,  &pf, &pn, &pd,   &pc, &max_levels};
+    GR_HIP(hipLaunchCooperativeKernel(kernel, dim3(grid), dim3(kBatchNT), args, 0, 0));
+  }
   GR_HIP(hipEventRecord(e1, 0));
   GR_HIP(hipEventSynchronize(e1));
   float ms = 0;

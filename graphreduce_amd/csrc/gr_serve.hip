@@ -3,12 +3,13 @@
 //
 // Each query kind has a pool of up to 64 lanes whose state stays resident in
 // HBM between steps (one bit per lane for BFS masks, one lane of a wave64 for
-// SSSP distances and PPR ranks).  A host thread runs the loop:
+// SSSP distances and PPR ranks).  Per pool, a host thread runs the loop:
 //   1. admit queued requests into free lanes, as long as the pool has a free
 //      lane and the device-memory budget covers the lane's bytes;
 //   2. run ONE step (a BFS level, an SSSP relaxation round, a PPR power
-//      iteration) of every pool with active lanes, each pool on its own HIP
-//      stream so the three kinds overlap on the device;
+//      iteration) of the pool's active lanes; every pool has its own host
+//      thread and HIP stream, so the three kinds overlap on the device and
+//      each advances at its own rate;
 //   3. retire the lanes that finished: the lane's result row is reduced to a
 //      64-bit checksum on the device and, when the request asked for it,
 //      copied to the host.
@@ -413,13 +414,14 @@ struct gr_server {
   std::deque<int64_t> queue[3];
   std::unordered_map<int64_t, gr::Request> reqs;
   int64_t next_id = 1;
-  std::thread worker;
+  std::thread workers[3];
   std::atomic<bool> stop{false};
   std::atomic<int> error{GR_OK};
   std::string error_msg;
   size_t bytes_in_use = 0, peak_bytes = 0;
   int max_lanes_in_use = 0;
-  int64_t steps_run = 0, completed = 0;
+  std::atomic<int64_t> steps_run{0};
+  int64_t completed = 0;
   std::vector<double> latencies_ms;
   gr::Clock::time_point first_submit, last_finish;
   bool any_submit = false;
@@ -660,44 +662,37 @@ int retire(gr_server* sv, Pool& p, uint64_t done) {
       sv->completed += 1;
       sv->last_finish = now;
       sv->bytes_in_use -= p.lane_bytes;
+      p.active &= ~(1ull << s);  // read by the other pools' threads under the lock
+      p.lane_req[s] = -1;
     }
-    p.active &= ~(1ull << s);
-    p.lane_req[s] = -1;
   }
   sv->cv.notify_all();
   return GR_OK;
 }
 
-void worker_loop(gr_server* sv) {
+// One host thread per pool: each kind advances at its own step rate, on its
+// own stream, so a slow PPR iteration does not hold BFS levels back.
+void worker_loop(gr_server* sv, int k) {
   (void)hipSetDevice(0);
+  Pool& p = sv->pools[k];
   while (!sv->stop.load()) {
-    uint64_t adm[3] = {0, 0, 0};
-    bool any_active = false;
+    uint64_t adm = 0;
     {
       std::unique_lock<std::mutex> lk(sv->mu);
-      for (int k = 0; k < 3; ++k) adm[k] = admit(sv, sv->pools[k]);
-      for (int k = 0; k < 3; ++k) any_active = any_active || sv->pools[k].active;
-      if (!any_active) {
+      adm = admit(sv, p);
+      if (!p.active) {
         sv->cv.wait_for(lk, std::chrono::milliseconds(2));
         continue;
       }
       int used = 0;
-      for (int k = 0; k < 3; ++k) used += __builtin_popcountll(sv->pools[k].active);
+      for (int j = 0; j < 3; ++j) used += __builtin_popcountll(sv->pools[j].active);
       sv->max_lanes_in_use = std::max(sv->max_lanes_in_use, used);
     }
-    int rc = GR_OK;
-    for (int k = 0; k < 3 && rc == GR_OK; ++k) rc = launch_admit(sv, sv->pools[k], adm[k]);
-    for (int k = 0; k < 3 && rc == GR_OK; ++k)
-      if (sv->pools[k].active) rc = launch_step(sv, sv->pools[k]);
-    for (int k = 0; k < 3 && rc == GR_OK; ++k) {
-      Pool& p = sv->pools[k];
-      if (!p.active) continue;
-      if (hipStreamSynchronize(p.stream) != hipSuccess) {
-        rc = set_error(GR_ERR_HIP, "server: step failed");
-        break;
-      }
-      rc = retire(sv, p, finished_lanes(sv, p));
-    }
+    int rc = launch_admit(sv, p, adm);
+    if (rc == GR_OK) rc = launch_step(sv, p);
+    if (rc == GR_OK && hipStreamSynchronize(p.stream) != hipSuccess)
+      rc = set_error(GR_ERR_HIP, "server: step failed");
+    if (rc == GR_OK) rc = retire(sv, p, finished_lanes(sv, p));
     sv->steps_run += 1;
     if (rc != GR_OK) {
       std::lock_guard<std::mutex> lk(sv->mu);
@@ -740,7 +735,7 @@ extern "C" int gr_server_create(gr_graph* g, const int32_t* weights_device,
     delete sv;
     return rc;
   }
-  sv->worker = std::thread(worker_loop, sv);
+  for (int k = 0; k < 3; ++k) sv->workers[k] = std::thread(worker_loop, sv, k);
   *out = sv;
   return GR_OK;
 }
@@ -749,7 +744,8 @@ extern "C" int gr_server_destroy(gr_server* sv) {
   if (!sv) return GR_OK;
   sv->stop = true;
   sv->cv.notify_all();
-  if (sv->worker.joinable()) sv->worker.join();
+  for (auto& w : sv->workers)
+    if (w.joinable()) w.join();
   for (auto& p : sv->pools) pool_free(p);
   if (sv->w_csc) (void)hipFree(sv->w_csc);
   if (sv->ppr_partial) (void)hipFree(sv->ppr_partial);
@@ -815,7 +811,7 @@ extern "C" int gr_server_get_stats(gr_server* sv, gr_server_stats* out) {
   gr_server_stats st{};
   st.completed = sv->completed;
   st.submitted = sv->next_id - 1;
-  st.steps = sv->steps_run;
+  st.steps = sv->steps_run.load();
   st.max_lanes_in_use = sv->max_lanes_in_use;
   st.peak_bytes = (int64_t)sv->peak_bytes;
   if (!sv->latencies_ms.empty()) {

@@ -264,7 +264,9 @@ typedef struct gr_batch_result {
  * visit/frontier masks per vertex.  depth_device: device array n*V int32,
  * row s = depths from sources[s] (-1 = unreached), filled by this call.
  * direction: 0 automatic (pull when the frontier's out-edges exceed E/alpha),
- * 1 push, 2 pull. */
+ * 1 push, 2 pull; OR GR_MSBFS_STREAM_PULL to ask for the streaming pull
+ * kernel, which is currently disabled (the call returns GR_ERR_INVALID). */
+#define GR_MSBFS_STREAM_PULL 4
 int gr_msbfs(gr_graph* g, const int32_t* sources, int n_sources, int32_t* depth_device,
              int direction, double alpha, gr_batch_result* result);
 
