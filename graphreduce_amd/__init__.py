@@ -71,6 +71,7 @@ class _Stats(C.Structure):
         ("apply_ms", C.c_double),
         ("next_launches", C.c_int64),
         ("next_ms", C.c_double),
+        ("delta_gathers", C.c_int64),
     ]
 
 
@@ -153,6 +154,7 @@ _SIGS = {
     "gr_engine_reset_vertex_data": (C.c_int, [_P]),
     "gr_engine_set_direction": (C.c_int, [_P, C.c_int, C.c_double]),
     "gr_engine_set_gather_tile": (C.c_int, [_P, C.c_int]),
+    "gr_engine_set_delta_gather": (C.c_int, [_P, C.c_int]),
     "gr_engine_enable_timing": (C.c_int, [_P, C.c_int]),
     "gr_engine_get_stats": (C.c_int, [_P, C.POINTER(_Stats)]),
     "gr_engine_reset_stats": (C.c_int, [_P]),
@@ -499,6 +501,11 @@ class GASEngine:
         """0 = automatic push/pull, 1 = push only, 2 = pull only."""
         _check(lib().gr_engine_set_direction(self._h, mode, alpha))
 
+    def setDeltaGather(self, on: bool = True) -> None:
+        """SSSP/CC: after an all-active iteration, gather only from sources
+        changed in the previous iteration (same results; default on)."""
+        _check(lib().gr_engine_set_delta_gather(self._h, 1 if on else 0))
+
     def setGatherTile(self, items_per_thread: int) -> None:
         """Merge-path items per thread of the gather tiles (4, 8 or 16)."""
         _check(lib().gr_engine_set_gather_tile(self._h, items_per_thread))
@@ -557,13 +564,14 @@ def run_bfs(graph_or_edges, source: int, direction: int = 0):
     return out, iteration
 
 
-def run_sssp(graph_or_edges, weights, source: int, direction: int = 0):
+def run_sssp(graph_or_edges, weights, source: int, direction: int = 0, delta: bool = True):
     """sssp.cu:69-111 loop (all vertices active first); returns (dist, trips)."""
     n = graph_or_edges.n_vertices if isinstance(graph_or_edges, Graph) else graph_or_edges[0]
     vd = np.full(n, SSSP_INF, np.int32)
     vd[source] = 0
     eng = GASEngine(SSSP)
     eng.setDirection(direction)
+    eng.setDeltaGather(delta)
     _bind(eng, graph_or_edges, vd, weights)
     eng.setActive(0, n)
     it = 0
@@ -579,12 +587,13 @@ def run_sssp(graph_or_edges, weights, source: int, direction: int = 0):
     return out, it
 
 
-def run_cc(graph_or_edges, direction: int = 0):
+def run_cc(graph_or_edges, direction: int = 0, delta: bool = True):
     """connected_component.cu:66-81 (label = own id, all active, run())."""
     n = graph_or_edges.n_vertices if isinstance(graph_or_edges, Graph) else graph_or_edges[0]
     vd = np.arange(n, dtype=np.int32)
     eng = GASEngine(CC)
     eng.setDirection(direction)
+    eng.setDeltaGather(delta)
     _bind(eng, graph_or_edges, vd)
     eng.setActive(0, n)
     it = eng.run()

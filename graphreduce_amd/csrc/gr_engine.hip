@@ -390,7 +390,8 @@ __global__ __launch_bounds__(kCompactNT) void k_compact_write(
     uint32_t* __restrict__ chg_bits) {
   __shared__ int s_a[kCompactNT / kWave], s_b[kCompactNT / kWave];
   const int w = blockIdx.x * kCompactNT + threadIdx.x;
-  // the changed bitmap (pull-mode input) starts empty for the next apply
+  // clears the changed bitmap of the previous iteration: the host swaps it in
+  // as the buffer that the next apply fills
   if (chg_bits && w < n_words) chg_bits[w] = 0u;
   uint32_t word = (w < n_words) ? bitmap[w] : 0u;
   const int cnt = __popc(word);
@@ -458,6 +459,7 @@ struct EngineBase {
   virtual int get_results() = 0;
   virtual int read_vertex_data(void* out, size_t velem) = 0;
   virtual int reset_vertex_data() = 0;
+  virtual int set_delta_gather(bool on) = 0;
   virtual int sync_stats(gr_engine_stats* out) = 0;
   virtual size_t vertex_size() const = 0;
   virtual size_t edge_size() const = 0;  // 0 for an empty EdgeData
@@ -494,7 +496,8 @@ struct Engine final : EngineBase {
   int32_t* d_chg_v = nullptr;
   int32_t* d_chg_e = nullptr;
   uint32_t* d_bitmap = nullptr;
-  uint32_t* d_chg_bits = nullptr;
+  uint32_t* d_chg_bits = nullptr;  // vertices changed by this iteration's apply
+  uint32_t* d_chg_prev = nullptr;  // ... by the previous iteration's apply
   int2* d_blk = nullptr;
   int32_t* d_part = nullptr;
   int32_t* d_push_part = nullptr;
@@ -505,6 +508,13 @@ struct Engine final : EngineBase {
   int n_words = 0, n_cblocks = 0, max_tiles = 0, max_push_tiles = 0;
   int32_t* d_heavy = nullptr;  // vertices with in-degree > kPullHeavy (pull mode)
   int n_heavy = 0;
+  // Delta gather (programs with kDeltaGather): valid once an all-active
+  // iteration has run and every later active set came from scatterActivate.
+  static constexpr bool kDelta = delta_gather_v<P>;
+  bool delta_enabled = true;
+  bool delta_chain = false;  // the invariant holds for the current state
+  bool delta_ready = false;  // d_chg_prev holds the previous iteration's changes
+  bool it_gather = false, it_apply = false, it_scatter = false;
   // hipEvent start/stop pairs per timed phase
   enum { kPhGather = 0, kPhScatter, kPhApply, kPhNext, kPhases };
   std::vector<hipEvent_t> ev[kPhases];
@@ -516,9 +526,10 @@ struct Engine final : EngineBase {
   size_t edge_size() const override { return kEdgeEmpty ? 0 : sizeof(ED); }
 
   void release() {
-    void* ptrs[] = {d_cbase, d_heavy, d_vd,    d_vd_init, d_ed,      d_active,    d_escan, d_gtmp,  d_chg_v,
-                    d_chg_e, d_bitmap,  d_chg_bits,  d_blk,   d_part,  d_push_part,
-                    d_carry, d_head,    d_ctr};
+    void* ptrs[] = {d_cbase, d_heavy,   d_vd,       d_vd_init, d_ed,   d_active,
+                    d_escan, d_gtmp,    d_chg_v,    d_chg_e,   d_bitmap, d_chg_bits,
+                    d_chg_prev, d_blk,  d_part,     d_push_part, d_carry, d_head,
+                    d_ctr};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     if (h_ctr) (void)hipHostFree(h_ctr);
@@ -536,7 +547,7 @@ struct Engine final : EngineBase {
     d_cbase = nullptr;
     d_active = d_escan = d_chg_v = d_chg_e = d_part = d_push_part = nullptr;
     d_gtmp = nullptr;
-    d_bitmap = d_chg_bits = nullptr;
+    d_bitmap = d_chg_bits = d_chg_prev = nullptr;
     d_blk = nullptr;
     d_carry = d_head = nullptr;
     d_ctr = h_ctr = nullptr;
@@ -591,6 +602,7 @@ struct Engine final : EngineBase {
     GR_TRY(dalloc(&d_chg_e, V));
     GR_TRY(dalloc(&d_bitmap, n_words + 1));
     GR_TRY(dalloc(&d_chg_bits, n_words + 1));
+    GR_TRY(dalloc(&d_chg_prev, n_words + 1));
     GR_TRY(dalloc(&d_blk, n_cblocks));
     GR_TRY(dalloc(&d_part, max_tiles + 2));
     GR_TRY(dalloc(&d_push_part, max_push_tiles + 2));
@@ -600,6 +612,7 @@ struct Engine final : EngineBase {
     GR_HIP(hipHostMalloc((void**)&h_ctr, sizeof(DevCounters), hipHostMallocDefault));
     GR_HIP(hipMemsetAsync(d_bitmap, 0, sizeof(uint32_t) * (n_words + 1), stream));
     GR_HIP(hipMemsetAsync(d_chg_bits, 0, sizeof(uint32_t) * (n_words + 1), stream));
+    GR_HIP(hipMemsetAsync(d_chg_prev, 0, sizeof(uint32_t) * (n_words + 1), stream));
     GR_HIP(hipMemsetAsync(d_ctr, 0, sizeof(DevCounters), stream));
     {
       std::vector<int32_t> hv;
@@ -634,6 +647,8 @@ struct Engine final : EngineBase {
     n_active = 0;
     w_edges = 0;
     iteration = 0;
+    delta_chain = delta_ready = false;
+    it_gather = it_apply = it_scatter = false;
     bound = true;
     return GR_OK;
   }
@@ -652,6 +667,12 @@ struct Engine final : EngineBase {
     const int n = end - start;
     GR_HIP(hipMemsetAsync(d_bitmap, 0, sizeof(uint32_t) * (n_words + 1), stream));
     GR_HIP(hipMemsetAsync(d_chg_bits, 0, sizeof(uint32_t) * (n_words + 1), stream));
+    GR_HIP(hipMemsetAsync(d_chg_prev, 0, sizeof(uint32_t) * (n_words + 1), stream));
+    // An all-active iteration gathers every in-edge of every vertex, which
+    // establishes the delta-gather invariant; a partial range does not.
+    delta_chain = (start == 0 && end == g->nV);
+    delta_ready = false;
+    it_gather = it_apply = it_scatter = false;
     k_set_active_range<<<grid_1d((int64_t)n + 1, 256), 256, 0, stream>>>(start, n, g->off_in,
                                                                          d_active, d_escan, d_cbase,
                                                                          d_ctr);
@@ -680,8 +701,10 @@ struct Engine final : EngineBase {
   }
 
   int gather() override {
+    it_gather = true;
     if (!P::kHasGather || n_active == 0) return GR_OK;
     const int n = n_active, w = w_edges;
+    const uint32_t* dbits = (kDelta && delta_enabled && delta_ready) ? d_chg_prev : nullptr;
     const int tile = kNT * gather_vt;
     const int n_tiles = (int)(((int64_t)n + w + tile - 1) / tile);
     k_merge_partition<<<grid_1d((int64_t)n_tiles + 1, 256), 256, 0, stream>>>(d_escan, n, w,
@@ -690,7 +713,7 @@ struct Engine final : EngineBase {
 #define GR_GATHER_LAUNCH(VT)                                                                \
   k_gather_tiles<P, VT><<<n_tiles, kNT, 0, stream>>>(n, w, d_active, d_escan, d_part, d_cbase, \
                                                      g->srcs, d_vd, edge_ptr(), d_gtmp,       \
-                                                     d_carry, d_head)
+                                                     d_carry, d_head, dbits)
     switch (gather_vt) {
       case 4: GR_GATHER_LAUNCH(4); break;
       case 16: GR_GATHER_LAUNCH(16); break;
@@ -702,16 +725,20 @@ struct Engine final : EngineBase {
     GR_HIP(hipGetLastError());
     stats.active_vertices += n;
     stats.gathered_edges += w;
+    if (dbits) stats.delta_gathers += 1;
     return GR_OK;
   }
 
+  // The changed bitmap feeds pull activation and the next delta gather.
+  bool changed_bits_on() const { return dir_mode != 1 || kDelta; }
+
   int apply() override {
+    it_apply = true;
     if (n_active == 0) return GR_OK;
-    const bool pull_possible = dir_mode != 1;
     tick();
     k_apply<P><<<grid_1d(n_active, kApplyChunk), kApplyNT, 0, stream>>>(
         n_active, d_active, d_gtmp, P::kHasGather ? 1 : 0, d_vd, g->off_out, iteration, d_chg_v,
-        d_chg_e, pull_possible ? d_chg_bits : nullptr, d_ctr);
+        d_chg_e, changed_bits_on() ? d_chg_bits : nullptr, d_ctr);
     tock(kPhApply);
     GR_HIP(hipGetLastError());
     return GR_OK;
@@ -724,6 +751,7 @@ struct Engine final : EngineBase {
   int scatter_activate(bool) override {
     // Program::scatter is a no-op for every compiled program (kHasScatter).
     static_assert(!P::kHasScatter, "non-trivial scatter is not compiled in");
+    it_scatter = true;
     const long long thr = pull_threshold();
     tick();
     if (dir_mode != 1) {
@@ -756,9 +784,15 @@ struct Engine final : EngineBase {
                                            dir_mode, pull_threshold());
     k_compact_write<<<n_cblocks, kCompactNT, 0, stream>>>(d_bitmap, n_words, g->off_in, need_deg,
                                                           d_blk, d_active, d_escan, d_cbase,
-                                                          dir_mode != 1 ? d_chg_bits : nullptr);
+                                                          changed_bits_on() ? d_chg_prev : nullptr);
     tock(kPhNext);
     GR_HIP(hipGetLastError());
+    // this iteration's changes become the previous ones; the cleared buffer
+    // receives the next apply's
+    std::swap(d_chg_bits, d_chg_prev);
+    delta_chain = delta_chain && it_gather && it_apply && it_scatter;
+    delta_ready = kDelta && delta_chain && changed_bits_on();
+    it_gather = it_apply = it_scatter = false;
     GR_HIP(hipMemcpyAsync(h_ctr, d_ctr, sizeof(DevCounters), hipMemcpyDeviceToHost, stream));
     GR_HIP(hipStreamSynchronize(stream));
     const unsigned long long c = h_ctr->chg_last;
@@ -780,6 +814,12 @@ struct Engine final : EngineBase {
 
   int reset_vertex_data() override {
     GR_HIP(hipMemcpyAsync(d_vd, d_vd_init, sizeof(VD) * g->nV, hipMemcpyDeviceToDevice, stream));
+    delta_chain = delta_ready = false;
+    return GR_OK;
+  }
+
+  int set_delta_gather(bool on) override {
+    delta_enabled = on;
     return GR_OK;
   }
 
@@ -968,6 +1008,11 @@ extern "C" int gr_engine_set_direction(gr_engine* e, int mode, double alpha) {
   e->impl->dir_mode = mode;
   if (alpha > 0) e->impl->dir_alpha = alpha;
   return GR_OK;
+}
+
+extern "C" int gr_engine_set_delta_gather(gr_engine* e, int enable) {
+  GR_TRY(check(e, false));
+  return e->impl->set_delta_gather(enable != 0);
 }
 
 extern "C" int gr_engine_set_gather_tile(gr_engine* e, int items_per_thread) {

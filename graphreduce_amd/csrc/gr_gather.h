@@ -15,6 +15,8 @@
 #ifndef GRAPHREDUCE_AMD_GR_GATHER_H
 #define GRAPHREDUCE_AMD_GR_GATHER_H
 
+#include <type_traits>
+
 #include "gr_device.h"
 
 namespace gr {
@@ -128,6 +130,18 @@ struct Seg {
   }
 };
 
+// Delta gather: a program opts in with kDeltaGather = true when gatherReduce
+// is min, gatherMap depends only on the source value and the edge, and apply
+// keeps min(current, g) and reports exactly a decrease.  Programs without the
+// member are treated as false.
+template <class P, class = void>
+struct delta_gather : std::false_type {};
+template <class P>
+struct delta_gather<P, std::void_t<decltype(P::kDeltaGather)>>
+    : std::integral_constant<bool, P::kDeltaGather> {};
+template <class P>
+constexpr bool delta_gather_v = delta_gather<P>::value;
+
 template <class P>
 __device__ __forceinline__ typename P::GatherResult finalize(int has,
                                                              typename P::GatherResult v) {
@@ -143,7 +157,8 @@ __global__ __launch_bounds__(kNT) void k_gather_tiles(
     const int32_t* __restrict__ srcs, const typename P::VertexData* __restrict__ vd,
     const typename P::EdgeData* __restrict__ ed, typename P::GatherResult* __restrict__ gtmp,
     Part<typename P::GatherResult>* __restrict__ carry,
-    Part<typename P::GatherResult>* __restrict__ head) {
+    Part<typename P::GatherResult>* __restrict__ head,
+    const uint32_t* __restrict__ delta_bits) {
   typedef typename P::GatherResult G;
   typedef Seg<P> S;
   constexpr int TILE = kNT * VT;
@@ -196,6 +211,11 @@ __global__ __launch_bounds__(kNT) void k_gather_tiles(
   __syncthreads();
 
   // Phase 2: coalesced pass over the tile's edges: CSC source, gatherMap.
+  // With delta_bits (delta-gather programs only), an edge whose source is not
+  // in the bitmap contributes gatherZero instead of gatherMap: the row value
+  // is already folded with gatherZero, and such a source cannot lower the
+  // destination (it last gathered from the source after its last change), so
+  // the result of apply is unchanged while the random source read is saved.
   {
     int src_v[VT], csc_v[VT];
 #pragma unroll
@@ -207,13 +227,36 @@ __global__ __launch_bounds__(kNT) void k_gather_tiles(
         src_v[m] = srcs[csc_v[m]];
       }
     }
+    bool mapped = false;
+    if constexpr (delta_gather_v<P>) {
+      if (delta_bits) {
+        mapped = true;
+        uint32_t word[VT];
 #pragma unroll
-    for (int m = 0; m < VT; ++m) {
-      int e = tid + m * kNT;
-      if (e < n_edges) {
-        const typename P::VertexData* dstp = vd;
-        if constexpr (P::kGatherUsesDst) dstp = vd + active[i0 + s_row[e]];
-        s_val[e] = P::gatherMap(dstp, vd + src_v[m], ed + csc_v[m]);
+        for (int m = 0; m < VT; ++m) {
+          int e = tid + m * kNT;
+          word[m] = e < n_edges ? delta_bits[src_v[m] >> 5] : 0u;
+        }
+#pragma unroll
+        for (int m = 0; m < VT; ++m) {
+          int e = tid + m * kNT;
+          if (e < n_edges) {
+            s_val[e] = ((word[m] >> (src_v[m] & 31)) & 1u)
+                           ? P::gatherMap(vd, vd + src_v[m], ed + csc_v[m])
+                           : P::gatherZero;
+          }
+        }
+      }
+    }
+    if (!mapped) {
+#pragma unroll
+      for (int m = 0; m < VT; ++m) {
+        int e = tid + m * kNT;
+        if (e < n_edges) {
+          const typename P::VertexData* dstp = vd;
+          if constexpr (P::kGatherUsesDst) dstp = vd + active[i0 + s_row[e]];
+          s_val[e] = P::gatherMap(dstp, vd + src_v[m], ed + csc_v[m]);
+        }
       }
     }
   }

@@ -8,6 +8,14 @@
 //                     apply ignores the value).
 //   kHasScatter       false when scatter is a no-op.
 //   kGatherUsesDst    false when gatherMap ignores its dst argument.
+//   kDeltaGather      true when gatherReduce is min, gatherMap reads only the
+//                     source value and the edge, and apply keeps
+//                     min(current, g) and reports exactly a decrease.  After an
+//                     all-active iteration the engine may then gather only
+//                     from sources that changed in the previous iteration:
+//                     every other source was already gathered by the
+//                     destination after its last change and cannot lower it,
+//                     so apply sees the same outcome and value.
 // and apply() receives the engine's iteration number as a third argument in
 // place of the reference's __constant__ g_iterationCountGPU (bfs.cu:27-28).
 #ifndef GRAPHREDUCE_AMD_GR_PROGRAMS_H
@@ -29,6 +37,7 @@ struct BFSProgram {
   static constexpr bool kHasGather = false;
   static constexpr bool kHasScatter = false;
   static constexpr bool kGatherUsesDst = false;
+  static constexpr bool kDeltaGather = false;
 
   __device__ static int gatherReduce(const int&, const int&) { return 0; }
   __device__ static int gatherMap(const VertexData*, const VertexData*, const EdgeData*) {
@@ -55,6 +64,7 @@ struct SSSPProgram {
   static constexpr bool kHasGather = true;
   static constexpr bool kHasScatter = false;
   static constexpr bool kGatherUsesDst = false;
+  static constexpr bool kDeltaGather = true;
 
   __device__ static int gatherReduce(const int& left, const int& right) {
     return min(left, right);
@@ -87,6 +97,7 @@ struct PageRankProgram {
   static constexpr bool kHasGather = true;
   static constexpr bool kHasScatter = false;
   static constexpr bool kGatherUsesDst = false;
+  static constexpr bool kDeltaGather = false;
 
   __device__ static float gatherMap(const VertexData*, const VertexData* src, const EdgeData*) {
     return __fdiv_rn(src->rank, (float)src->numOutEdges);
@@ -113,6 +124,7 @@ struct CCProgram {
   static constexpr bool kHasGather = true;
   static constexpr bool kHasScatter = false;
   static constexpr bool kGatherUsesDst = false;
+  static constexpr bool kDeltaGather = true;
 
   __device__ static int gatherReduce(const int& left, const int& right) {
     return min(left, right);

@@ -208,6 +208,17 @@ int gr_engine_set_direction(gr_engine* e, int mode, double alpha);
  * threads).  Chosen by the autotuner; default 8. */
 int gr_engine_set_gather_tile(gr_engine* e, int items_per_thread);
 
+/* Delta gather (default on; no reference counterpart).  For SSSP and CC,
+ * once an iteration with every vertex active has run (setActive(0, V)) and
+ * every later active set came from scatterActivate, gather reads only the
+ * in-edges whose source changed in the previous iteration and treats the
+ * others as gatherZero.  Such a source was gathered by the destination after
+ * its last change and cannot lower it, so every apply sees the same outcome
+ * and value as with the full gather of refgas.h:141-160: results, iteration
+ * counts and the gathered_edges counter are unchanged.  enable = 0 gathers
+ * every in-edge always. */
+int gr_engine_set_delta_gather(gr_engine* e, int enable);
+
 /* Work and timing counters, summed since the last reset. */
 typedef struct gr_engine_stats {
   int64_t iterations;        /* nextIter calls with a non-empty active set */
@@ -224,6 +235,8 @@ typedef struct gr_engine_stats {
   double apply_ms;
   int64_t next_launches;     /* nextIter compaction kernels */
   double next_ms;
+  int64_t delta_gathers;     /* gathers that read only sources changed in the
+                                previous iteration (see gr_engine_set_delta_gather) */
 } gr_engine_stats;
 
 /* Enables hipEvent timing of the engine phases on the engine's stream. */

@@ -270,3 +270,72 @@ def test_bound_graph_shared_by_engines(gpu):
     ref, rt, _ = oracle.bfs(n, s, d, src)
     assert np.array_equal(dep, ref) and t == rt
     g.close()
+
+
+# ------------------------------------------------------------ delta gather
+
+def _engine_run(gr, prog, n, vd, s, d, w=None, start=None, delta=True):
+    eng = gr.GASEngine(prog)
+    eng.setDeltaGather(delta)
+    # getResults writes back into the array given to setGraph (the engine owns
+    # it until then, refgas.h:69-72), so each run gets its own copy
+    eng.setGraph(n, vd.copy(), len(s), w, s, d)
+    if start is None:
+        eng.setActive(0, n)
+    else:
+        eng.setActive(start, start + 1)
+    trips = eng.run()
+    eng.getResults()
+    out = eng.vertexData.copy()
+    st = eng.stats()
+    eng.close()
+    return out, trips, st
+
+
+@pytest.mark.parametrize("case", ["sssp_rmat14", "cc_sym_rmat14", "cc_grid96", "sssp_grid96"])
+def test_delta_gather_equals_full_gather(gpu, case):
+    """Delta gather reads fewer source values but gives the same values, trips
+    and work counters as the full gather of refgas.h:141-160."""
+    gr = gpu
+    if case.endswith("rmat14"):
+        n, s, d = _rmat(gr, 14, seed=3, sym=case.startswith("cc"))
+    else:
+        n, (s, d) = 96 * 96, gr.grid_edges(96, 0.02, seed=5)
+    if case.startswith("sssp"):
+        w = gr.hash_weights(len(s), seed=3)
+        vd = np.full(n, gr.SSSP_INF, np.int32)
+        vd[_max_out_src(n, s)] = 0
+        prog = gr.SSSP
+        ref, ref_trips, _ = oracle.sssp(n, s, d, w, _max_out_src(n, s))
+    else:
+        w, vd, prog = None, np.arange(n, dtype=np.int32), gr.CC
+        ref, ref_trips, _ = oracle.cc(n, s, d)
+    on, t_on, st_on = _engine_run(gr, prog, n, vd, s, d, w, delta=True)
+    off, t_off, st_off = _engine_run(gr, prog, n, vd, s, d, w, delta=False)
+    assert np.array_equal(on, ref) and np.array_equal(off, ref)
+    assert t_on == t_off == ref_trips
+    for k in ("iterations", "active_vertices", "gathered_edges", "changed_vertices",
+              "pushed_edges"):
+        assert st_on[k] == st_off[k], k
+    assert st_off["delta_gathers"] == 0
+    assert st_on["delta_gathers"] == t_on - 1  # every gather after the all-active one
+
+
+def test_delta_gather_not_used_after_partial_start(gpu):
+    """With setActive on a sub-range, a vertex may be activated whose other
+    in-neighbours it never gathered from: vertex 1 must see source 2 (value 0,
+    never changed), so its distance is 1, not 6 as a delta gather would give."""
+    gr = gpu
+    s = np.array([3, 0, 2], np.int32)
+    d = np.array([0, 1, 1], np.int32)
+    w = np.array([1, 5, 1], np.int32)
+    vd = np.array([gr.SSSP_INF, gr.SSSP_INF, 0, 0], np.int32)
+    for delta in (True, False):
+        out, trips, st = _engine_run(gr, gr.SSSP, 4, vd, s, d, w, start=0, delta=delta)
+        assert out.tolist() == [1, 1, 0, 0]
+        assert trips == 2
+        assert st["delta_gathers"] == 0
+    # the same graph from an all-active start: delta is valid and used
+    out, trips, st = _engine_run(gr, gr.SSSP, 4, vd, s, d, w, delta=True)
+    assert out.tolist() == [1, 1, 0, 0]
+    assert st["delta_gathers"] == trips - 1
