@@ -229,7 +229,7 @@ def main() -> int:
     extra = {}
     if not args.no_extras:
         try:
-            extra.update(run_extras(gr, g, args, src, outdeg))
+            extra.update(run_extras(gr, g, args, src, outdeg, w_dev))
         except Exception as exc:  # extras never hide the headline
             extra["error"] = repr(exc)
     result["extra"] = extra
@@ -245,7 +245,7 @@ def main() -> int:
     return 0
 
 
-def run_extras(gr, g, args, src, outdeg):
+def run_extras(gr, g, args, src, outdeg, w_dev):
     out = {}
     n = g.n_vertices
     # plain BFS (bfs.cu loop)
@@ -315,6 +315,7 @@ def run_extras(gr, g, args, src, outdeg):
                                           / (st["gather_ms"] / 1e3) / 1e9, 1))
     log(f"CC: {out['cc']}")
     out["batched"] = run_batched(gr, g, args, outdeg)
+    out["serving"] = run_serving(gr, g, args, outdeg, w_dev)
     out["cc_grid_4096"] = run_cc_grid(gr)
     if not args.no_cpu_baseline:
         out["pagerank_precision_rmat22"] = run_pr_precision(gr)
@@ -400,6 +401,74 @@ def run_batched(gr, g, args, outdeg):
                        speedup=round(t_plain64 / t_b, 2), rounds=info["levels"],
                        identical_to_plain_for_timed_rows=same)
     log(f"batched SSSP: {res['sssp']}")
+    return res
+
+
+def run_serving(gr, g, args, outdeg, w_dev, n_req=384, rate=2000.0, per_kind_sample=2):
+    """Poisson stream of mixed BFS / SSSP / PPR queries (uniform kinds, seeded
+    random sources with out-degree > 0) at `rate` requests/s mean into the
+    continuous-batching server; latency is submit -> result ready.  Then a
+    sample of the same queries is resubmitted one at a time on the idle server
+    (alone) and their checksums must equal the ones from the mix; one BFS and
+    one SSSP alone result are also compared with plain engine runs."""
+    n = g.n_vertices
+    rng = np.random.default_rng(args.seed + 101)
+    cand = np.flatnonzero(outdeg > 0)
+    verts = rng.choice(cand, n_req).astype(np.int32)
+    kinds = rng.integers(1, 4, n_req).astype(np.int32)
+    names = {gr.QUERY_BFS: "bfs", gr.QUERY_SSSP: "sssp", gr.QUERY_PPR: "ppr"}
+    srv = gr.Server(g, w_dev)
+    t0 = time.perf_counter()
+    ids = srv.load_test(rate, verts, kinds, keep=False)
+    lat = {k: [] for k in names}
+    cs_mix = []
+    for i, rid in enumerate(ids):
+        _, cs, ms, _ = srv.wait(int(rid), want_result=False)
+        lat[int(kinds[i])].append(ms)
+        cs_mix.append(cs)
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    res = {
+        "graph": f"RMAT-{args.scale} seed {args.seed}",
+        "offered_rate_rps": rate,
+        "requests": n_req,
+        "kinds": "uniform over BFS, SSSP, PPR (10 power iterations)",
+        "completed": st["completed"],
+        "throughput_rps": round(st["throughput_rps"], 1),
+        "wall_s": round(wall, 3),
+        "p50_ms": round(st["p50_ms"], 2),
+        "p99_ms": round(st["p99_ms"], 2),
+        "max_ms": round(st["max_ms"], 2),
+        "max_lanes_in_use": st["max_lanes_in_use"],
+        "peak_lane_bytes": st["peak_bytes"],
+        "steps": st["steps"],
+    }
+    for k, name in names.items():
+        if lat[k]:
+            a = np.sort(np.array(lat[k]))
+            res[f"{name}_p50_ms"] = round(float(np.percentile(a, 50)), 2)
+            res[f"{name}_p99_ms"] = round(float(np.percentile(a, 99)), 2)
+    # alone vs mix
+    sample = []
+    for k in names:
+        sample += [i for i in range(n_req) if kinds[i] == k][:per_kind_sample]
+    equal = True
+    alone_rows = {}
+    for i in sample:
+        rid = srv.submit(int(kinds[i]), int(verts[i]), keep=True)
+        row, cs, _, _ = srv.wait(rid, int(kinds[i]))
+        equal &= cs == cs_mix[i]
+        alone_rows.setdefault(int(kinds[i]), (int(verts[i]), row))
+    srv.close()
+    res["alone_equals_mix"] = bool(equal)
+    res["alone_vs_mix_sample"] = len(sample)
+    v, row = alone_rows[gr.QUERY_BFS]
+    plain, _ = gr.run_bfs(g, v)
+    res["bfs_equals_plain_engine"] = bool(np.array_equal(row, plain))
+    v, row = alone_rows[gr.QUERY_SSSP]
+    plain, _ = gr.run_sssp(g, w_dev, v)
+    res["sssp_equals_plain_engine"] = bool(np.array_equal(row, plain))
+    log(f"serving: {res}")
     return res
 
 

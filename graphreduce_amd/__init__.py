@@ -155,6 +155,7 @@ _SIGS = {
     "gr_engine_set_direction": (C.c_int, [_P, C.c_int, C.c_double]),
     "gr_engine_set_gather_tile": (C.c_int, [_P, C.c_int]),
     "gr_engine_set_delta_gather": (C.c_int, [_P, C.c_int]),
+    "gr_engine_set_vertex_order": (C.c_int, [_P, C.c_int]),
     "gr_engine_enable_timing": (C.c_int, [_P, C.c_int]),
     "gr_engine_get_stats": (C.c_int, [_P, C.POINTER(_Stats)]),
     "gr_engine_reset_stats": (C.c_int, [_P]),
@@ -500,6 +501,12 @@ class GASEngine:
     def setDirection(self, mode: int = 0, alpha: float = 15.0) -> None:
         """0 = automatic push/pull, 1 = push only, 2 = pull only."""
         _check(lib().gr_engine_set_direction(self._h, mode, alpha))
+
+    def setVertexOrder(self, order: int = 1) -> None:
+        """1 (default): run on the degree-ordered copy of the graph; 0: as
+        given.  Results are in the caller's ids either way; takes effect at
+        the next setGraph/bindGraph."""
+        _check(lib().gr_engine_set_vertex_order(self._h, order))
 
     def setDeltaGather(self, on: bool = True) -> None:
         """SSSP/CC: after an all-active iteration, gather only from sources

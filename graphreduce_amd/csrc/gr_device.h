@@ -92,10 +92,19 @@ struct gr_graph {
   int32_t* csr_eid = nullptr;  // CSR position -> edge-list index (nullptr: identity)
   int32_t* csc_eid = nullptr;  // CSC position -> edge-list index
   std::vector<int32_t> h_off_out, h_off_in;  // host copies of the offsets
+  // Degree-ordered copy for the engines, built on first use: vertex v of
+  // `ordered` is vertex perm[v] of this graph (descending in+out degree,
+  // ties by id); its rows are sorted by the new neighbour ids and its eid
+  // maps index this graph's edge list.
+  gr_graph* ordered = nullptr;
+  int32_t* perm = nullptr;  // ordered id -> id   [V], device
+  int32_t* inv = nullptr;   // id -> ordered id   [V], device
 };
 
 namespace gr {
 int graph_free(gr_graph* g);
+// The degree-ordered copy of g (cached in g; freed with g).
+int graph_ordered(gr_graph* g, gr_graph** out);
 }
 
 #endif

@@ -21,6 +21,7 @@
 //                    returns the new counts.
 #include <string.h>
 
+#include <algorithm>
 #include <type_traits>
 #include <vector>
 
@@ -436,7 +437,15 @@ __global__ void k_set_active_range(int start, int n, const int32_t* __restrict__
   }
 }
 
-// Edge data from edge-list order into CSC order, in 4-byte words.
+__global__ void k_set_counters(DevCounters* __restrict__ ctr, int n, int w) {
+  ctr->n_active = n;
+  ctr->w_edges = w;
+  ctr->chg = 0;
+}
+
+// Gathers n elements of `words` 4-byte words: out[i] = in[eid[i]] (eid may be
+// nullptr: identity).  Moves edge data from edge-list order into CSC order,
+// and vertex data between user order and the degree-ordered layout.
 __global__ void k_permute_words(const uint32_t* __restrict__ in, const int32_t* __restrict__ eid,
                                 int64_t n, int words, uint32_t* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -471,6 +480,7 @@ struct EngineBase {
   int dir_mode = 0;
   double dir_alpha = 15.0;
   int gather_vt = 8;  // merge-path items per thread in the gather tiles (4, 8 or 16)
+  int vertex_order = 1;  // 1: engine works on the degree-ordered copy of the graph
   bool timing = false;
   gr_engine_stats stats{};
 };
@@ -483,11 +493,15 @@ struct Engine final : EngineBase {
   static constexpr bool kEdgeEmpty = std::is_empty<ED>::value;
 
   hipStream_t stream = nullptr;
-  gr_graph* g = nullptr;
+  gr_graph* g = nullptr;       // the layout the kernels run on (user_g or its ordered copy)
+  gr_graph* user_g = nullptr;  // the graph as given
+  const int32_t* d_perm = nullptr;  // layout id -> user id (nullptr: same ids)
+  const int32_t* d_inv = nullptr;   // user id -> layout id
   bool owns_graph = false;
   void* host_vdata = nullptr;
   VD* d_vd = nullptr;
   VD* d_vd_init = nullptr;  // vertex data as given at bind time
+  VD* d_vd_user = nullptr;  // staging in user order (ordered layout only)
   ED* d_ed = nullptr;  // CSC order
   int32_t* d_active = nullptr;
   int32_t* d_escan = nullptr;
@@ -526,7 +540,7 @@ struct Engine final : EngineBase {
   size_t edge_size() const override { return kEdgeEmpty ? 0 : sizeof(ED); }
 
   void release() {
-    void* ptrs[] = {d_cbase, d_heavy,   d_vd,       d_vd_init, d_ed,   d_active,
+    void* ptrs[] = {d_cbase, d_heavy,   d_vd,       d_vd_init, d_vd_user, d_ed,   d_active,
                     d_escan, d_gtmp,    d_chg_v,    d_chg_e,   d_bitmap, d_chg_bits,
                     d_chg_prev, d_blk,  d_part,     d_push_part, d_carry, d_head,
                     d_ctr};
@@ -538,9 +552,12 @@ struct Engine final : EngineBase {
       v.clear();
     }
     if (stream) (void)hipStreamDestroy(stream);
-    if (owns_graph && g) graph_free(g);
+    if (owns_graph && user_g) graph_free(user_g);
+    user_g = nullptr;
+    d_perm = d_inv = nullptr;
     d_vd = nullptr;
     d_vd_init = nullptr;
+    d_vd_user = nullptr;
     d_heavy = nullptr;
     n_heavy = 0;
     d_ed = nullptr;
@@ -580,9 +597,15 @@ struct Engine final : EngineBase {
       return set_error(GR_ERR_INVALID, "this program needs edge data");
     if (!vdata) return set_error(GR_ERR_INVALID, "vertex data is NULL");
     release();
+    user_g = graph;
     g = graph;
     owns_graph = owns;
     host_vdata = vdata;
+    if (vertex_order == 1 && graph->nV > 0) {
+      GR_TRY(graph_ordered(graph, &g));
+      d_perm = graph->perm;
+      d_inv = graph->inv;
+    }
     const int64_t V = g->nV, E = g->nE;
     if (V + E >= ((int64_t)1 << 31) - kNT * 16)
       return set_error(GR_ERR_INVALID, "graph too large for int32 merge-path coordinates");
@@ -623,7 +646,13 @@ struct Engine final : EngineBase {
       if (n_heavy)
         GR_HIP(hipMemcpy(d_heavy, hv.data(), sizeof(int32_t) * n_heavy, hipMemcpyHostToDevice));
     }
-    GR_HIP(hipMemcpyAsync(d_vd, vdata, sizeof(VD) * V, hipMemcpyHostToDevice, stream));
+    if (d_perm) {
+      GR_TRY(dalloc(&d_vd_user, V));
+      GR_HIP(hipMemcpyAsync(d_vd_user, vdata, sizeof(VD) * V, hipMemcpyHostToDevice, stream));
+      to_layout(d_vd_user, d_vd);
+    } else {
+      GR_HIP(hipMemcpyAsync(d_vd, vdata, sizeof(VD) * V, hipMemcpyHostToDevice, stream));
+    }
     GR_HIP(hipMemcpyAsync(d_vd_init, d_vd, sizeof(VD) * V, hipMemcpyDeviceToDevice, stream));
     if constexpr (!kEdgeEmpty) {
       GR_TRY(dalloc(&d_ed, E));
@@ -653,6 +682,23 @@ struct Engine final : EngineBase {
     return GR_OK;
   }
 
+  static_assert(sizeof(VD) % 4 == 0, "vertex data is moved in 4-byte words");
+  // user order -> layout order: out[i] = in[perm[i]]
+  void to_layout(const VD* in, VD* out) {
+    k_permute_words<<<grid_1d(g->nV, 256), 256, 0, stream>>>(
+        reinterpret_cast<const uint32_t*>(in), d_perm, g->nV, (int)(sizeof(VD) / 4),
+        reinterpret_cast<uint32_t*>(out));
+  }
+  // Device pointer to the vertex data in user order (d_vd itself without a
+  // reordered layout).
+  const VD* user_order_vd() {
+    if (!d_inv) return d_vd;
+    k_permute_words<<<grid_1d(g->nV, 256), 256, 0, stream>>>(
+        reinterpret_cast<const uint32_t*>(d_vd), d_inv, g->nV, (int)(sizeof(VD) / 4),
+        reinterpret_cast<uint32_t*>(d_vd_user));
+    return d_vd_user;
+  }
+
   const ED* edge_ptr() const {
     if constexpr (kEdgeEmpty)
       return reinterpret_cast<const ED*>(d_gtmp);  // never dereferenced
@@ -665,6 +711,7 @@ struct Engine final : EngineBase {
       return set_errorf(GR_ERR_INVALID, "setActive range [%d, %d) outside [0, %d)", start, end,
                         g->nV);
     const int n = end - start;
+    if (d_inv && n < g->nV) return set_active_list(start, end);
     GR_HIP(hipMemsetAsync(d_bitmap, 0, sizeof(uint32_t) * (n_words + 1), stream));
     GR_HIP(hipMemsetAsync(d_chg_bits, 0, sizeof(uint32_t) * (n_words + 1), stream));
     GR_HIP(hipMemsetAsync(d_chg_prev, 0, sizeof(uint32_t) * (n_words + 1), stream));
@@ -680,6 +727,45 @@ struct Engine final : EngineBase {
     n_active = n;
     w_edges = g->h_off_in[end] - g->h_off_in[start];
     iteration = 0;
+    return GR_OK;
+  }
+
+  // setActive on a sub-range of user ids with the ordered layout: the layout
+  // ids of the range, ascending, with their in-edge scan built on the host.
+  int set_active_list(int32_t start, int32_t end) {
+    const int n = end - start;
+    std::vector<int32_t> ids((size_t)n), escan((size_t)n + 1), cbase((size_t)n + 1);
+    if (n > 0)
+      GR_HIP(hipMemcpy(ids.data(), d_inv + start, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    std::sort(ids.begin(), ids.end());
+    int64_t acc = 0;
+    for (int i = 0; i < n; ++i) {
+      escan[i] = (int32_t)acc;
+      cbase[i] = g->h_off_in[ids[i]] - (int32_t)acc;
+      acc += g->h_off_in[ids[i] + 1] - g->h_off_in[ids[i]];
+    }
+    escan[n] = (int32_t)acc;
+    cbase[n] = 0;
+    GR_HIP(hipMemsetAsync(d_bitmap, 0, sizeof(uint32_t) * (n_words + 1), stream));
+    GR_HIP(hipMemsetAsync(d_chg_bits, 0, sizeof(uint32_t) * (n_words + 1), stream));
+    GR_HIP(hipMemsetAsync(d_chg_prev, 0, sizeof(uint32_t) * (n_words + 1), stream));
+    if (n > 0) {
+      GR_HIP(hipMemcpyAsync(d_active, ids.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice,
+                            stream));
+      GR_HIP(hipMemcpyAsync(d_cbase, cbase.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice,
+                            stream));
+    }
+    GR_HIP(hipMemcpyAsync(d_escan, escan.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice,
+                          stream));
+    k_set_counters<<<1, 1, 0, stream>>>(d_ctr, n, (int32_t)acc);
+    GR_HIP(hipGetLastError());
+    GR_HIP(hipStreamSynchronize(stream));  // the host vectors go out of scope
+    n_active = n;
+    w_edges = (int32_t)acc;
+    iteration = 0;
+    delta_chain = false;
+    delta_ready = false;
+    it_gather = it_apply = it_scatter = false;
     return GR_OK;
   }
 
@@ -807,7 +893,8 @@ struct Engine final : EngineBase {
   }
 
   int get_results() override {
-    GR_HIP(hipMemcpyAsync(host_vdata, d_vd, sizeof(VD) * g->nV, hipMemcpyDeviceToHost, stream));
+    const VD* src = user_order_vd();
+    GR_HIP(hipMemcpyAsync(host_vdata, src, sizeof(VD) * g->nV, hipMemcpyDeviceToHost, stream));
     GR_HIP(hipStreamSynchronize(stream));
     return GR_OK;
   }
@@ -825,7 +912,8 @@ struct Engine final : EngineBase {
 
   int read_vertex_data(void* out, size_t velem) override {
     if (velem != sizeof(VD)) return set_error(GR_ERR_INVALID, "vertex element size mismatch");
-    GR_HIP(hipMemcpyAsync(out, d_vd, sizeof(VD) * g->nV, hipMemcpyDeviceToHost, stream));
+    const VD* src = user_order_vd();
+    GR_HIP(hipMemcpyAsync(out, src, sizeof(VD) * g->nV, hipMemcpyDeviceToHost, stream));
     GR_HIP(hipStreamSynchronize(stream));
     return GR_OK;
   }
@@ -1007,6 +1095,14 @@ extern "C" int gr_engine_set_direction(gr_engine* e, int mode, double alpha) {
   if (mode < 0 || mode > 2) return set_error(GR_ERR_INVALID, "direction mode must be 0, 1 or 2");
   e->impl->dir_mode = mode;
   if (alpha > 0) e->impl->dir_alpha = alpha;
+  return GR_OK;
+}
+
+extern "C" int gr_engine_set_vertex_order(gr_engine* e, int order) {
+  GR_TRY(check(e, false));
+  if (order != 0 && order != 1)
+    return set_error(GR_ERR_INVALID, "vertex order must be 0 (as given) or 1 (by degree)");
+  e->impl->vertex_order = order;
   return GR_OK;
 }
 

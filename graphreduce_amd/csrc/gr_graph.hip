@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 #include "gr_device.h"
 #include "gr_hash.h"
@@ -180,9 +181,121 @@ int finish_graph(gr_graph* g) {
 
 int graph_free(gr_graph* g) {
   if (!g) return GR_OK;
-  for (int32_t* p : {g->off_out, g->dsts, g->off_in, g->srcs, g->csr_eid, g->csc_eid})
+  if (g->ordered) graph_free(g->ordered);
+  for (int32_t* p : {g->off_out, g->dsts, g->off_in, g->srcs, g->csr_eid, g->csc_eid, g->perm,
+                     g->inv})
     if (p) (void)hipFree(p);
   delete g;
+  return GR_OK;
+}
+
+namespace {
+
+__global__ void k_degree_keys(int32_t nV, const int32_t* __restrict__ off_out,
+                              const int32_t* __restrict__ off_in, uint64_t* __restrict__ keys,
+                              int32_t* __restrict__ ids) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nV) return;
+  const uint32_t deg = (uint32_t)(off_out[v + 1] - off_out[v]) + (uint32_t)(off_in[v + 1] - off_in[v]);
+  keys[v] = (uint64_t)(0xFFFFFFFFu - deg);  // ascending key = descending degree
+  ids[v] = (int32_t)v;
+}
+
+__global__ void k_inverse(int32_t nV, const int32_t* __restrict__ perm, int32_t* __restrict__ inv) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nV) inv[perm[i]] = (int32_t)i;
+}
+
+// rowid[off[v]] = v for every non-empty row (rowid zeroed before)
+__global__ void k_row_marks(int32_t nV, const int32_t* __restrict__ off, int32_t* __restrict__ rowid) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v < nV && off[v + 1] > off[v]) rowid[off[v]] = (int32_t)v;
+}
+
+// The edge list in the new ids, by edge-list position: CSR position p is
+// edge eid[p] (identity when eid is nullptr) from row rowid[p] to adj[p].
+__global__ void k_relabel_edges(int64_t nE, const int32_t* __restrict__ rowid,
+                                const int32_t* __restrict__ adj, const int32_t* __restrict__ eid,
+                                const int32_t* __restrict__ inv, int32_t* __restrict__ s_new,
+                                int32_t* __restrict__ d_new) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nE) return;
+  const int64_t i = eid ? (int64_t)eid[p] : p;
+  s_new[i] = inv[rowid[p]];
+  d_new[i] = inv[adj[p]];
+}
+
+// Row of every CSR position (max-scan over the row starts).
+int row_ids(int32_t nV, int64_t nE, const int32_t* off, int32_t* rowid) {
+  if (nE == 0) return GR_OK;
+  DevBuf<int32_t> marks;
+  GR_TRY(marks.alloc(nE));
+  GR_HIP(hipMemset(marks.p, 0, sizeof(int32_t) * (size_t)nE));
+  k_row_marks<<<grid_for(nV), kBlock>>>(nV, off, marks.p);
+  GR_HIP(hipGetLastError());
+  size_t temp = 0;
+  GR_HIP(hipcub::DeviceScan::InclusiveScan(nullptr, temp, marks.p, rowid, hipcub::Max(), (int)nE));
+  DevBuf<uint8_t> t;
+  GR_TRY(t.alloc((int64_t)temp));
+  GR_HIP(hipcub::DeviceScan::InclusiveScan(t.p, temp, marks.p, rowid, hipcub::Max(), (int)nE));
+  return GR_OK;
+}
+
+std::mutex g_order_mu;
+
+}  // namespace
+
+int graph_from_device_edges(int32_t nV, int64_t nE, const int32_t* d_src,
+                            const int32_t* d_dst, gr_graph** out);
+
+int graph_ordered(gr_graph* g, gr_graph** out) {
+  std::lock_guard<std::mutex> lk(g_order_mu);
+  if (g->ordered) {
+    *out = g->ordered;
+    return GR_OK;
+  }
+  const int32_t nV = g->nV;
+  const int64_t nE = g->nE;
+  DevBuf<int32_t> perm, inv;
+  GR_TRY(perm.alloc(nV));
+  GR_TRY(inv.alloc(nV));
+  if (nV > 0) {
+    DevBuf<uint64_t> keys, ksorted;
+    DevBuf<int32_t> ids;
+    GR_TRY(keys.alloc(nV));
+    GR_TRY(ksorted.alloc(nV));
+    GR_TRY(ids.alloc(nV));
+    k_degree_keys<<<grid_for(nV), kBlock>>>(nV, g->off_out, g->off_in, keys.p, ids.p);
+    GR_HIP(hipGetLastError());
+    GR_TRY(radix_sort_pairs(keys.p, ksorted.p, ids.p, perm.p, nV, 32));  // stable
+    k_inverse<<<grid_for(nV), kBlock>>>(nV, perm.p, inv.p);
+    GR_HIP(hipGetLastError());
+  }
+  // Rebuilt from the renumbered edge list (same edge-list positions, so the
+  // eid maps still index the caller's edge data); rows come out sorted by the
+  // new neighbour ids, which puts the hubs of every row next to each other.
+  gr_graph* o = nullptr;
+  {
+    DevBuf<int32_t> rowid, s_new, d_new;
+    GR_TRY(rowid.alloc(nE));
+    GR_TRY(s_new.alloc(nE));
+    GR_TRY(d_new.alloc(nE));
+    GR_TRY(row_ids(nV, nE, g->off_out, rowid.p));
+    if (nE > 0) {
+      k_relabel_edges<<<grid_for(nE), kBlock>>>(nE, rowid.p, g->dsts, g->csr_eid, inv.p, s_new.p,
+                                                d_new.p);
+      GR_HIP(hipGetLastError());
+    }
+    GR_TRY(graph_from_device_edges(nV, nE, s_new.p, d_new.p, &o));
+  }
+  if (hipDeviceSynchronize() != hipSuccess) {
+    graph_free(o);
+    return set_error(GR_ERR_HIP, "ordered graph build failed");
+  }
+  g->ordered = o;
+  g->perm = perm.release();
+  g->inv = inv.release();
+  *out = o;
   return GR_OK;
 }
 

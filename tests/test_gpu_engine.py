@@ -339,3 +339,80 @@ def test_delta_gather_not_used_after_partial_start(gpu):
     out, trips, st = _engine_run(gr, gr.SSSP, 4, vd, s, d, w, delta=True)
     assert out.tolist() == [1, 1, 0, 0]
     assert st["delta_gathers"] == trips - 1
+
+
+# ------------------------------------------------------- vertex order
+
+def _ordered_run(gr, prog, n, vd, s, d, w=None, start=None, order=1):
+    eng = gr.GASEngine(prog)
+    eng.setVertexOrder(order)
+    eng.setGraph(n, vd.copy(), len(s), w, s, d)
+    if start is None:
+        eng.setActive(0, n)
+    else:
+        eng.setActive(*start)
+    trips = 0
+    while eng.countActive():
+        eng.setIterationCount(trips)
+        eng.gather()
+        eng.apply()
+        eng.scatterActivate()
+        eng.nextIter()
+        trips += 1
+    first = eng.readVertexData()
+    eng.getResults()
+    out = eng.vertexData.copy()
+    st = eng.stats()
+    eng.close()
+    assert np.array_equal(first.view(np.uint8), out.view(np.uint8))
+    return out, trips, st
+
+
+@pytest.mark.parametrize("case", ["bfs", "sssp", "sssp_range", "cc"])
+def test_vertex_order_gives_the_same_results(gpu, case):
+    """The degree-ordered layout is internal: values in the caller's ids,
+    trips and work counters equal those of the layout as given."""
+    gr = gpu
+    n, s, d = _rmat(gr, 14, seed=8, sym=(case == "cc"))
+    w = gr.hash_weights(len(s), seed=8)
+    src = _max_out_src(n, s)
+    start = None
+    if case == "bfs":
+        prog, vd = gr.BFS, np.zeros(n, gr.PROGRAMS[gr.BFS]["vertex"])
+        vd["depth"] = -1
+        start, w = (src, src + 1), None
+        ref, ref_trips, _ = oracle.bfs(n, s, d, src)
+    elif case.startswith("sssp"):
+        prog, vd = gr.SSSP, np.full(n, gr.SSSP_INF, np.int32)
+        vd[src] = 0
+        ref, ref_trips, _ = oracle.sssp(n, s, d, w, src)
+        if case == "sssp_range":  # sources 100..163 at distance 0, start on them only
+            vd[100:164] = 0
+            start, ref, ref_trips = (100, 164), None, None
+    else:
+        prog, vd, w = gr.CC, np.arange(n, dtype=np.int32), None
+        ref, ref_trips, _ = oracle.cc(n, s, d)
+    a, ta, sa = _ordered_run(gr, prog, n, vd, s, d, w, start, order=0)
+    b, tb, sb = _ordered_run(gr, prog, n, vd, s, d, w, start, order=1)
+    assert np.array_equal(a.view(np.int32), b.view(np.int32))
+    assert ta == tb
+    for k in ("iterations", "active_vertices", "gathered_edges", "changed_vertices",
+              "pushed_edges"):
+        assert sa[k] == sb[k], k
+    if ref is not None:
+        got = b["depth"] if case == "bfs" else b
+        assert np.array_equal(got, ref)
+        assert tb == ref_trips
+
+
+def test_vertex_order_pagerank_within_tolerance(gpu):
+    gr = gpu
+    n, s, d = _rmat(gr, 12, seed=3)
+    outdeg = np.bincount(s, minlength=n).astype(np.int32)
+    vd = np.zeros(n, gr.PROGRAMS[gr.PAGERANK]["vertex"])
+    vd["rank"] = 1.0
+    vd["numOutEdges"] = outdeg
+    a, ta, _ = _ordered_run(gr, gr.PAGERANK, n, vd, s, d, order=0)
+    b, tb, _ = _ordered_run(gr, gr.PAGERANK, n, vd, s, d, order=1)
+    assert np.array_equal(a["numOutEdges"], b["numOutEdges"])
+    assert np.allclose(a["rank"], b["rank"], rtol=1e-5, atol=1e-6)
