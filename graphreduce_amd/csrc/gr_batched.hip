@@ -474,15 +474,35 @@ __global__ void k_mssp_sources(int ns, const int32_t* __restrict__ sources,
   }
 }
 
-// dist (vertex-major, 64 lanes) -> out[s * nV + v]
+// dist (vertex-major, 64 lanes, layout ids) -> out[s * nV + u] in the
+// caller's ids: u's layout id is inv[u] (inv nullptr: the same)
 __global__ void k_mssp_transpose(int nV, int ns, const int32_t* __restrict__ dist,
-                                 int32_t* __restrict__ out) {
+                                 const int32_t* __restrict__ inv, int32_t* __restrict__ out) {
   const size_t n = (size_t)nV * ns;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
-    const size_t s = i / nV, v = i % nV;
+    const size_t s = i / nV, u = i % nV;
+    const size_t v = inv ? (size_t)inv[u] : u;
     out[i] = dist[v * kWave + s];
   }
+}
+
+// rows[s * nV + v] in layout ids -> out[s * nV + u] in the caller's ids
+__global__ void k_rows_to_user(int nV, int ns, const int32_t* __restrict__ rows,
+                               const int32_t* __restrict__ inv, int32_t* __restrict__ out) {
+  const size_t n = (size_t)nV * ns;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const size_t s = i / nV, u = i % nV;
+    out[i] = rows[s * nV + inv[u]];
+  }
+}
+
+// sources in the caller's ids -> layout ids
+__global__ void k_map_ids(int n, const int32_t* __restrict__ in, const int32_t* __restrict__ inv,
+                          int32_t* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = inv[in[i]];
 }
 
 __global__ void k_permute_i32(const int32_t* __restrict__ in, const int32_t* __restrict__ eid,
@@ -536,9 +556,13 @@ extern "C" int gr_msbfs(gr_graph* g, const int32_t* sources, int n_sources, int3
   for (int s = 0; s < n_sources; ++s)
     if (sources[s] < 0 || sources[s] >= nV)
       return set_errorf(GR_ERR_INVALID, "gr_msbfs: source %d out of range", sources[s]);
+  // runs on the degree-ordered copy of the graph; depth rows are produced in
+  // its ids and moved to the caller's ids at the end
+  gr_graph* user_g = g;
+  GR_TRY(graph_ordered(user_g, &g));
   DBuf<uint64_t> visit, frontier, next;
   DBuf<uint32_t> fbits;
-  DBuf<int32_t> d_src;
+  DBuf<int32_t> d_src, d_src_user, depth_layout;
   DBuf<MsCounters> ctr;
   LaneTables lt;
   GR_TRY(visit.alloc(nV));
@@ -549,10 +573,14 @@ extern "C" int gr_msbfs(gr_graph* g, const int32_t* sources, int n_sources, int3
     GR_TRY(lt.build(g->h_off_in));
   }
   GR_TRY(d_src.alloc(n_sources));
+  GR_TRY(d_src_user.alloc(n_sources));
+  GR_TRY(depth_layout.alloc((size_t)nV * n_sources));
   GR_TRY(ctr.alloc(1));
-  GR_HIP(hipMemcpy(d_src.p, sources, sizeof(int32_t) * n_sources, hipMemcpyHostToDevice));
+  GR_HIP(hipMemcpy(d_src_user.p, sources, sizeof(int32_t) * n_sources, hipMemcpyHostToDevice));
+  k_map_ids<<<1, 64>>>(n_sources, d_src_user.p, user_g->inv, d_src.p);
+  GR_HIP(hipGetLastError());
   // depth = -1 everywhere: the counterpart of the host-side vertex init before setGraph
-  GR_HIP(hipMemset(depth_device, 0xFF, sizeof(int32_t) * (size_t)nV * n_sources));
+  GR_HIP(hipMemset(depth_layout.p, 0xFF, sizeof(int32_t) * (size_t)nV * n_sources));
   const void* kernel = stream ? (const void*)k_msbfs_stream : (const void*)k_msbfs;
   int dev = 0, cus = 0, per_cu = 0;
   GR_HIP(hipGetDevice(&dev));
@@ -566,8 +594,8 @@ extern "C" int gr_msbfs(gr_graph* g, const int32_t* sources, int n_sources, int3
   GR_HIP(hipDeviceSynchronize());
   GR_HIP(hipEventRecord(e0, 0));
   k_ms_init<<<1024, 256>>>(nV, visit.p, frontier.p, next.p, fbits.p);
-  k_ms_sources<<<1, 64>>>(n_sources, d_src.p, nV, visit.p, frontier.p, fbits.p, depth_device,
-                          g->off_out, ctr.p);
+  k_ms_sources<<<1, 64>>>(n_sources, d_src.p, nV, visit.p, frontier.p, fbits.p,
+                          depth_layout.p, g->off_out, ctr.p);
   GR_HIP(hipGetLastError());
   int nVa = nV, ns = n_sources, mode = direction, max_levels = nV + 1;
   long long nE = g->nE;
@@ -575,7 +603,7 @@ extern "C" int gr_msbfs(gr_graph* g, const int32_t* sources, int n_sources, int3
   const int32_t *off_out = g->off_out, *dsts = g->dsts, *off_in = g->off_in, *srcs = g->srcs;
   uint64_t *pv = visit.p, *pf = frontier.p, *pn = next.p;
   uint32_t* pfb = fbits.p;
-  int32_t* pd = depth_device;
+  int32_t* pd = depth_layout.p;
   const LaneChunk* pch = lt.chunks;
   int nch = lt.n_chunks;
   MsCounters* pc = ctr.p;
@@ -588,6 +616,8 @@ extern "C" int gr_msbfs(gr_graph* g, const int32_t* sources, int n_sources, int3
                     &pv,  &pf, &pn, &pd,   &pc, &max_levels};
     GR_HIP(hipLaunchCooperativeKernel(kernel, dim3(grid), dim3(kBatchNT), args, 0, 0));
   }
+  k_rows_to_user<<<4096, 256>>>(nV, n_sources, depth_layout.p, user_g->inv, depth_device);
+  GR_HIP(hipGetLastError());
   GR_HIP(hipEventRecord(e1, 0));
   GR_HIP(hipEventSynchronize(e1));
   float ms = 0;
@@ -613,17 +643,22 @@ extern "C" int gr_mssp(gr_graph* g, const int32_t* weights_device, const int32_t
     if (sources[s] < 0 || sources[s] >= nV)
       return set_errorf(GR_ERR_INVALID, "gr_mssp: source %d out of range", sources[s]);
   const int32_t inf = 2147483647 - 100000;  // SSSP::gatherZero, sssp.cu:32-33
-  DBuf<int32_t> dist, w_csc, d_src;
+  // runs on the degree-ordered copy of the graph (hot vertices share lines)
+  gr_graph* user_g = g;
+  GR_TRY(graph_ordered(user_g, &g));
+  DBuf<int32_t> dist, w_csc, d_src, d_src_user;
   DBuf<uint64_t> chg_a, chg_b;
   DBuf<unsigned long long> cnt, any;
   GR_TRY(dist.alloc((size_t)nV * kWave));
   GR_TRY(w_csc.alloc((size_t)g->nE));
   GR_TRY(d_src.alloc(n_sources));
+  GR_TRY(d_src_user.alloc(n_sources));
   GR_TRY(chg_a.alloc(nV));
   GR_TRY(chg_b.alloc(nV));
   GR_TRY(cnt.alloc(64));
   GR_TRY(any.alloc(1));
-  GR_HIP(hipMemcpy(d_src.p, sources, sizeof(int32_t) * n_sources, hipMemcpyHostToDevice));
+  GR_HIP(hipMemcpy(d_src_user.p, sources, sizeof(int32_t) * n_sources, hipMemcpyHostToDevice));
+  k_map_ids<<<1, 64>>>(n_sources, d_src_user.p, user_g->inv, d_src.p);
   if (g->nE)
     k_permute_i32<<<grid_1d(g->nE, 256), 256>>>(weights_device, g->csc_eid, g->nE, w_csc.p);
   LaneTables lt;
@@ -676,7 +711,7 @@ extern "C" int gr_mssp(gr_graph* g, const int32_t* weights_device, const int32_t
     rounds += kRounds;
   }
   if (true) {
-    k_mssp_transpose<<<4096, 256, 0, st>>>(nV, n_sources, dist.p, dist_device);
+    k_mssp_transpose<<<4096, 256, 0, st>>>(nV, n_sources, dist.p, user_g->inv, dist_device);
     GR_HIP(hipGetLastError());
   }
   GR_HIP(hipEventRecord(e1, st));

@@ -314,6 +314,7 @@ __global__ __launch_bounds__(kSrvNT) void k_srv_ppr_heavy_combine(
 template <class T>
 __global__ __launch_bounds__(kSrvNT) void k_srv_checksum_lanes(int nV, uint64_t mask,
                                                                const T* __restrict__ data,
+                                                               const int32_t* __restrict__ perm,
                                                                SrvCounters* __restrict__ ctr) {
   const int lane = lane_id();
   const int wpb = blockDim.x / kWave;
@@ -325,15 +326,18 @@ __global__ __launch_bounds__(kSrvNT) void k_srv_checksum_lanes(int nV, uint64_t 
     const T x = data[(size_t)v * kLanes + lane];
     uint32_t bits;
     memcpy(&bits, &x, 4);
-    h += mix64(((uint64_t)(uint32_t)v << 32) | bits);
+    const uint32_t id = perm ? (uint32_t)perm[v] : (uint32_t)v;  // the caller's vertex id
+    h += mix64(((uint64_t)id << 32) | bits);
   }
   if (mine && h) atomicAdd(&ctr->checksum[lane], h);
 }
 
-// Order-independent checksum of one lane's result row: sum of mix64(v, value).
+// Order-independent checksum of one lane's result row: sum of mix64(v, value)
+// over the caller's vertex ids v (perm maps layout ids to them).
 template <class T>
 __global__ void k_srv_checksum(int nV, int lane_count, uint64_t mask, const T* __restrict__ data,
-                               int lane_major, SrvCounters* __restrict__ ctr) {
+                               int lane_major, const int32_t* __restrict__ perm,
+                               SrvCounters* __restrict__ ctr) {
   // lane_major: data[s * nV + v]; otherwise data[v * 64 + s]
   for (int s = 0; s < lane_count; ++s) {
     if (!((mask >> s) & 1ull)) continue;
@@ -342,7 +346,8 @@ __global__ void k_srv_checksum(int nV, int lane_count, uint64_t mask, const T* _
       const T x = lane_major ? data[(size_t)s * nV + v] : data[(size_t)v * kLanes + s];
       uint32_t bits;
       memcpy(&bits, &x, 4);
-      h += mix64(((uint64_t)(uint32_t)v << 32) | bits);
+      const uint32_t id = perm ? (uint32_t)perm[v] : (uint32_t)v;
+      h += mix64(((uint64_t)id << 32) | bits);
     }
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) h += __shfl_xor(h, off, kWave);
@@ -350,11 +355,14 @@ __global__ void k_srv_checksum(int nV, int lane_count, uint64_t mask, const T* _
   }
 }
 
+// Result row of lane s in the caller's ids: out[u] = value at layout id inv[u].
 template <class T>
 __global__ void k_srv_extract(int nV, int s, const T* __restrict__ data, int lane_major,
-                              T* __restrict__ out) {
-  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nV; v += gridDim.x * blockDim.x)
-    out[v] = lane_major ? data[(size_t)s * nV + v] : data[(size_t)v * kLanes + s];
+                              const int32_t* __restrict__ inv, T* __restrict__ out) {
+  for (int u = blockIdx.x * blockDim.x + threadIdx.x; u < nV; u += gridDim.x * blockDim.x) {
+    const size_t v = inv ? (size_t)inv[u] : (size_t)u;
+    out[u] = lane_major ? data[(size_t)s * nV + v] : data[v * kLanes + s];
+  }
 }
 
 __global__ void k_permute_w(const int32_t* __restrict__ in, const int32_t* __restrict__ eid,
@@ -403,7 +411,10 @@ struct Pool {
 }  // namespace gr
 
 struct gr_server {
-  gr_graph* g = nullptr;
+  gr_graph* g = nullptr;              // degree-ordered copy the kernels run on
+  const int32_t* perm = nullptr;      // its ids -> the caller's ids (device)
+  const int32_t* inv = nullptr;       // the caller's ids -> its ids (device)
+  std::vector<int32_t> h_inv;         // host copy of inv (query seeds)
   gr_server_config cfg{};
   int32_t* w_csc = nullptr;
   gr::Pool pools[3];
@@ -519,7 +530,7 @@ uint64_t admit(gr_server* sv, Pool& p) {
     p.active |= 1ull << s;
     p.lane_req[s] = id;
     p.lane_steps[s] = 0;
-    p.seeds.vertex[s] = r.vertex;
+    p.seeds.vertex[s] = sv->h_inv[r.vertex];
     p.seeds.start[s] = p.step;
     sv->bytes_in_use += p.lane_bytes;
     sv->peak_bytes = std::max(sv->peak_bytes, sv->bytes_in_use);
@@ -615,11 +626,14 @@ int retire(gr_server* sv, Pool& p, uint64_t done) {
   const int nV = sv->g->nV;
   GR_HIP(hipMemsetAsync(p.ctr, 0, sizeof(SrvCounters), p.stream));
   if (p.kind == GR_QUERY_BFS)
-    k_srv_checksum<int32_t><<<1024, 256, 0, p.stream>>>(nV, kLanes, done, p.i32, 1, p.ctr);
+    k_srv_checksum<int32_t><<<1024, 256, 0, p.stream>>>(nV, kLanes, done, p.i32, 1, sv->perm,
+                                                         p.ctr);
   else if (p.kind == GR_QUERY_SSSP)
-    k_srv_checksum_lanes<int32_t><<<grid_v(nV), kSrvNT, 0, p.stream>>>(nV, done, p.i32, p.ctr);
+    k_srv_checksum_lanes<int32_t><<<grid_v(nV), kSrvNT, 0, p.stream>>>(nV, done, p.i32, sv->perm,
+                                                                        p.ctr);
   else
-    k_srv_checksum_lanes<float><<<grid_v(nV), kSrvNT, 0, p.stream>>>(nV, done, p.f0, p.ctr);
+    k_srv_checksum_lanes<float><<<grid_v(nV), kSrvNT, 0, p.stream>>>(nV, done, p.f0, sv->perm,
+                                                                      p.ctr);
   GR_HIP(hipGetLastError());
   GR_HIP(hipMemcpyAsync(p.h_ctr, p.ctr, sizeof(SrvCounters), hipMemcpyDeviceToHost, p.stream));
   GR_HIP(hipStreamSynchronize(p.stream));
@@ -638,11 +652,12 @@ int retire(gr_server* sv, Pool& p, uint64_t done) {
       int32_t* scratch = nullptr;
       GR_TRY(alloc_dev((void**)&scratch, 4 * (size_t)nV));
       if (p.kind == GR_QUERY_BFS)
-        k_srv_extract<int32_t><<<1024, 256, 0, p.stream>>>(nV, s, p.i32, 1, scratch);
+        k_srv_extract<int32_t><<<1024, 256, 0, p.stream>>>(nV, s, p.i32, 1, sv->inv, scratch);
       else if (p.kind == GR_QUERY_SSSP)
-        k_srv_extract<int32_t><<<1024, 256, 0, p.stream>>>(nV, s, p.i32, 0, scratch);
+        k_srv_extract<int32_t><<<1024, 256, 0, p.stream>>>(nV, s, p.i32, 0, sv->inv, scratch);
       else
-        k_srv_extract<float><<<1024, 256, 0, p.stream>>>(nV, s, p.f0, 0, (float*)scratch);
+        k_srv_extract<float><<<1024, 256, 0, p.stream>>>(nV, s, p.f0, 0, sv->inv,
+                                                         (float*)scratch);
       GR_HIP(hipMemcpyAsync(p.h_row, scratch, 4 * (size_t)nV, hipMemcpyDeviceToHost, p.stream));
       GR_HIP(hipStreamSynchronize(p.stream));
       (void)hipFree(scratch);
@@ -714,10 +729,22 @@ extern "C" int gr_server_create(gr_graph* g, const int32_t* weights_device,
                                 const gr_server_config* config, gr_server** out) {
   if (!g || !out) return set_error(GR_ERR_INVALID, "gr_server_create: bad arguments");
   gr_server* sv = new gr_server();
-  sv->g = g;
   if (config) sv->cfg = *config;
-  int rc = GR_OK;
-  if (weights_device && g->nE) {
+  // queries run on the degree-ordered copy of the graph (shared with engines)
+  gr_graph* user_g = g;
+  int rc = graph_ordered(user_g, &g);
+  if (rc != GR_OK) {
+    delete sv;
+    return rc;
+  }
+  sv->g = g;
+  sv->perm = user_g->perm;
+  sv->inv = user_g->inv;
+  sv->h_inv.resize((size_t)g->nV);
+  if (g->nV && hipMemcpy(sv->h_inv.data(), sv->inv, sizeof(int32_t) * (size_t)g->nV,
+                         hipMemcpyDeviceToHost) != hipSuccess)
+    rc = set_error(GR_ERR_HIP, "copy of the vertex order");
+  if (rc == GR_OK && weights_device && g->nE) {
     rc = alloc_dev((void**)&sv->w_csc, 4 * (size_t)g->nE);
     if (rc == GR_OK) {
       k_permute_w<<<grid_1d(g->nE, 256), 256>>>(weights_device, g->csc_eid, g->nE, sv->w_csc);
