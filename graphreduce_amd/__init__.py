@@ -508,10 +508,12 @@ class GASEngine:
         the next setGraph/bindGraph."""
         _check(lib().gr_engine_set_vertex_order(self._h, order))
 
-    def setDeltaGather(self, on: bool = True) -> None:
+    def setDeltaGather(self, mode=1) -> None:
         """SSSP/CC: after an all-active iteration, gather only from sources
-        changed in the previous iteration (same results; default on)."""
-        _check(lib().gr_engine_set_delta_gather(self._h, 1 if on else 0))
+        changed in the previous iteration (same results).  0/False: off;
+        1/True (default): when under 60 % of the in-edges come from changed
+        sources; 2: whenever valid."""
+        _check(lib().gr_engine_set_delta_gather(self._h, int(mode)))
 
     def setGatherTile(self, items_per_thread: int) -> None:
         """Merge-path items per thread of the gather tiles (4, 8 or 16)."""
@@ -571,7 +573,7 @@ def run_bfs(graph_or_edges, source: int, direction: int = 0):
     return out, iteration
 
 
-def run_sssp(graph_or_edges, weights, source: int, direction: int = 0, delta: bool = True):
+def run_sssp(graph_or_edges, weights, source: int, direction: int = 0, delta: int = 1):
     """sssp.cu:69-111 loop (all vertices active first); returns (dist, trips)."""
     n = graph_or_edges.n_vertices if isinstance(graph_or_edges, Graph) else graph_or_edges[0]
     vd = np.full(n, SSSP_INF, np.int32)
@@ -594,7 +596,7 @@ def run_sssp(graph_or_edges, weights, source: int, direction: int = 0, delta: bo
     return out, it
 
 
-def run_cc(graph_or_edges, direction: int = 0, delta: bool = True):
+def run_cc(graph_or_edges, direction: int = 0, delta: int = 1):
     """connected_component.cu:66-81 (label = own id, all active, run())."""
     n = graph_or_edges.n_vertices if isinstance(graph_or_edges, Graph) else graph_or_edges[0]
     vd = np.arange(n, dtype=np.int32)

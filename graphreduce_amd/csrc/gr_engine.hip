@@ -468,7 +468,7 @@ struct EngineBase {
   virtual int get_results() = 0;
   virtual int read_vertex_data(void* out, size_t velem) = 0;
   virtual int reset_vertex_data() = 0;
-  virtual int set_delta_gather(bool on) = 0;
+  virtual int set_delta_gather(int mode) = 0;
   virtual int sync_stats(gr_engine_stats* out) = 0;
   virtual size_t vertex_size() const = 0;
   virtual size_t edge_size() const = 0;  // 0 for an empty EdgeData
@@ -525,9 +525,14 @@ struct Engine final : EngineBase {
   // Delta gather (programs with kDeltaGather): valid once an all-active
   // iteration has run and every later active set came from scatterActivate.
   static constexpr bool kDelta = delta_gather_v<P>;
-  bool delta_enabled = true;
+  int delta_mode = 1;  // 0 off, 1 when the changed share is small, 2 always
   bool delta_chain = false;  // the invariant holds for the current state
   bool delta_ready = false;  // d_chg_prev holds the previous iteration's changes
+  // Share of the next gather's in-edges whose source changed (E_out(C) / E_in(A)).
+  // Delta gather adds a bitmap lookup per in-edge and pays off only when most
+  // sources are unchanged: at RMAT-24 it is slower from a share of about 0.6 up.
+  double changed_share = 1.0;
+  static constexpr double kDeltaMaxChangedShare = 0.6;
   bool it_gather = false, it_apply = false, it_scatter = false;
   // hipEvent start/stop pairs per timed phase
   enum { kPhGather = 0, kPhScatter, kPhApply, kPhNext, kPhases };
@@ -790,7 +795,11 @@ struct Engine final : EngineBase {
     it_gather = true;
     if (!P::kHasGather || n_active == 0) return GR_OK;
     const int n = n_active, w = w_edges;
-    const uint32_t* dbits = (kDelta && delta_enabled && delta_ready) ? d_chg_prev : nullptr;
+    const uint32_t* dbits =
+        (kDelta && delta_ready &&
+         (delta_mode == 2 || (delta_mode == 1 && changed_share < kDeltaMaxChangedShare)))
+            ? d_chg_prev
+            : nullptr;
     const int tile = kNT * gather_vt;
     const int n_tiles = (int)(((int64_t)n + w + tile - 1) / tile);
     k_merge_partition<<<grid_1d((int64_t)n_tiles + 1, 256), 256, 0, stream>>>(d_escan, n, w,
@@ -888,6 +897,7 @@ struct Engine final : EngineBase {
     if (n_active > 0) stats.iterations += 1;
     n_active = h_ctr->n_active;
     w_edges = h_ctr->w_edges;
+    changed_share = w_edges > 0 ? (double)(uint32_t)c / (double)w_edges : 1.0;
     ++iteration;
     return GR_OK;
   }
@@ -905,8 +915,8 @@ struct Engine final : EngineBase {
     return GR_OK;
   }
 
-  int set_delta_gather(bool on) override {
-    delta_enabled = on;
+  int set_delta_gather(int mode) override {
+    delta_mode = mode;
     return GR_OK;
   }
 
@@ -1106,9 +1116,11 @@ extern "C" int gr_engine_set_vertex_order(gr_engine* e, int order) {
   return GR_OK;
 }
 
-extern "C" int gr_engine_set_delta_gather(gr_engine* e, int enable) {
+extern "C" int gr_engine_set_delta_gather(gr_engine* e, int mode) {
   GR_TRY(check(e, false));
-  return e->impl->set_delta_gather(enable != 0);
+  if (mode < 0 || mode > 2)
+    return set_error(GR_ERR_INVALID, "delta gather mode must be 0 (off), 1 (adaptive) or 2 (always)");
+  return e->impl->set_delta_gather(mode);
 }
 
 extern "C" int gr_engine_set_gather_tile(gr_engine* e, int items_per_thread) {

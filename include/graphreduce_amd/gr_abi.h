@@ -219,16 +219,18 @@ int gr_engine_set_gather_tile(gr_engine* e, int items_per_thread);
  * the next setGraph / bindGraph. */
 int gr_engine_set_vertex_order(gr_engine* e, int order);
 
-/* Delta gather (default on; no reference counterpart).  For SSSP and CC,
- * once an iteration with every vertex active has run (setActive(0, V)) and
- * every later active set came from scatterActivate, gather reads only the
- * in-edges whose source changed in the previous iteration and treats the
+/* Delta gather (no reference counterpart).  For SSSP and CC, once an
+ * iteration with every vertex active has run (setActive(0, V)) and every
+ * later active set came from scatterActivate, gather may read only the
+ * in-edges whose source changed in the previous iteration and treat the
  * others as gatherZero.  Such a source was gathered by the destination after
  * its last change and cannot lower it, so every apply sees the same outcome
  * and value as with the full gather of refgas.h:141-160: results, iteration
- * counts and the gathered_edges counter are unchanged.  enable = 0 gathers
- * every in-edge always. */
-int gr_engine_set_delta_gather(gr_engine* e, int enable);
+ * counts and the gathered_edges counter are unchanged.
+ * mode 0: always gather every in-edge; 1 (default): delta gather when fewer
+ * than 60 % of the in-edges come from changed sources; 2: delta gather
+ * whenever it is valid. */
+int gr_engine_set_delta_gather(gr_engine* e, int mode);
 
 /* Work and timing counters, summed since the last reset. */
 typedef struct gr_engine_stats {

@@ -274,7 +274,7 @@ def test_bound_graph_shared_by_engines(gpu):
 
 # ------------------------------------------------------------ delta gather
 
-def _engine_run(gr, prog, n, vd, s, d, w=None, start=None, delta=True):
+def _engine_run(gr, prog, n, vd, s, d, w=None, start=None, delta=1):
     eng = gr.GASEngine(prog)
     eng.setDeltaGather(delta)
     # getResults writes back into the array given to setGraph (the engine owns
@@ -310,15 +310,18 @@ def test_delta_gather_equals_full_gather(gpu, case):
     else:
         w, vd, prog = None, np.arange(n, dtype=np.int32), gr.CC
         ref, ref_trips, _ = oracle.cc(n, s, d)
-    on, t_on, st_on = _engine_run(gr, prog, n, vd, s, d, w, delta=True)
-    off, t_off, st_off = _engine_run(gr, prog, n, vd, s, d, w, delta=False)
-    assert np.array_equal(on, ref) and np.array_equal(off, ref)
-    assert t_on == t_off == ref_trips
+    on, t_on, st_on = _engine_run(gr, prog, n, vd, s, d, w, delta=2)  # whenever valid
+    ad, t_ad, st_ad = _engine_run(gr, prog, n, vd, s, d, w, delta=1)  # adaptive
+    off, t_off, st_off = _engine_run(gr, prog, n, vd, s, d, w, delta=0)
+    for got in (on, ad, off):
+        assert np.array_equal(got, ref)
+    assert t_on == t_ad == t_off == ref_trips
     for k in ("iterations", "active_vertices", "gathered_edges", "changed_vertices",
               "pushed_edges"):
-        assert st_on[k] == st_off[k], k
+        assert st_on[k] == st_ad[k] == st_off[k], k
     assert st_off["delta_gathers"] == 0
     assert st_on["delta_gathers"] == t_on - 1  # every gather after the all-active one
+    assert st_ad["delta_gathers"] <= t_on - 1
 
 
 def test_delta_gather_not_used_after_partial_start(gpu):
@@ -330,13 +333,13 @@ def test_delta_gather_not_used_after_partial_start(gpu):
     d = np.array([0, 1, 1], np.int32)
     w = np.array([1, 5, 1], np.int32)
     vd = np.array([gr.SSSP_INF, gr.SSSP_INF, 0, 0], np.int32)
-    for delta in (True, False):
+    for delta in (2, 1, 0):
         out, trips, st = _engine_run(gr, gr.SSSP, 4, vd, s, d, w, start=0, delta=delta)
         assert out.tolist() == [1, 1, 0, 0]
         assert trips == 2
         assert st["delta_gathers"] == 0
     # the same graph from an all-active start: delta is valid and used
-    out, trips, st = _engine_run(gr, gr.SSSP, 4, vd, s, d, w, delta=True)
+    out, trips, st = _engine_run(gr, gr.SSSP, 4, vd, s, d, w, delta=2)
     assert out.tolist() == [1, 1, 0, 0]
     assert st["delta_gathers"] == trips - 1
 
