@@ -318,27 +318,51 @@ __global__ __launch_bounds__(256) void k_pull_heavy(
 
 // -------------------------------------------------------------- compaction
 
-__device__ __forceinline__ int degree_of_bits(uint32_t word, int base_v,
-                                              const int32_t* __restrict__ off) {
-  int deg = 0;
-  while (word) {
-    const int b = __builtin_ctz(word);
-    word &= word - 1;
-    const int v = base_v + b;
-    deg += off[v + 1] - off[v];
+// Compaction works on 8 consecutive vertices (one byte of the bitmap) per
+// thread: a workgroup covers kCompactNT words = 32 * kCompactNT vertices in
+// four rounds of 8 * kCompactNT.  The in-degrees come from one read of the 9
+// CSC offsets around the byte (coalesced across the wave).
+constexpr int kCompactRounds = 4;
+
+struct ByteWork {
+  int v0;         // first vertex of the byte
+  uint32_t bits;  // its 8 bits
+  int cnt, deg;   // set bits and the sum of their in-degrees
+  int o[9];       // off_in[v0 .. v0 + 8] (clamped to nV), valid when bits != 0
+};
+
+__device__ __forceinline__ ByteWork byte_work(const uint32_t* __restrict__ bitmap, int n_words,
+                                              int nV, const int32_t* __restrict__ off_in,
+                                              int need_deg, int byte_index) {
+  ByteWork b;
+  b.v0 = byte_index * 8;
+  const int wi = byte_index >> 2;
+  const uint32_t word = (wi < n_words) ? bitmap[wi] : 0u;
+  b.bits = (word >> ((byte_index & 3) * 8)) & 0xFFu;
+  b.cnt = __popc(b.bits);
+  b.deg = 0;
+  if (need_deg && b.bits) {
+#pragma unroll
+    for (int j = 0; j < 9; ++j) b.o[j] = off_in[min(b.v0 + j, nV)];
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if ((b.bits >> j) & 1u) b.deg += b.o[j + 1] - b.o[j];  // bit set => v0 + j < nV
   }
-  return deg;
+  return b;
 }
 
 __global__ __launch_bounds__(kCompactNT) void k_compact_count(const uint32_t* __restrict__ bitmap,
-                                                              int n_words,
+                                                              int n_words, int nV,
                                                               const int32_t* __restrict__ off_in,
                                                               int need_deg, int2* __restrict__ blk) {
   __shared__ int s_a[kCompactNT / kWave], s_b[kCompactNT / kWave];
-  const int w = blockIdx.x * kCompactNT + threadIdx.x;
-  const uint32_t word = (w < n_words) ? bitmap[w] : 0u;
-  int cnt = __popc(word);
-  int deg = need_deg ? degree_of_bits(word, w * 32, off_in) : 0;
+  int cnt = 0, deg = 0;
+  for (int r = 0; r < kCompactRounds; ++r) {
+    const int bi = (blockIdx.x * kCompactRounds + r) * kCompactNT + threadIdx.x;
+    const ByteWork b = byte_work(bitmap, n_words, nV, off_in, need_deg, bi);
+    cnt += b.cnt;
+    deg += b.deg;
+  }
   cnt = wave_reduce_add(cnt);
   deg = wave_reduce_add(deg);
   if (lane_id() == 0) {
@@ -384,37 +408,45 @@ __global__ __launch_bounds__(1024) void k_compact_scan(int2* __restrict__ blk, i
   }
 }
 
+// Writes the ascending active list, its exclusive in-degree scan and the CSC
+// base per row; clears the frontier bitmap and the previous iteration's
+// changed bitmap (the host swaps it in as the buffer the next apply fills).
 __global__ __launch_bounds__(kCompactNT) void k_compact_write(
-    uint32_t* __restrict__ bitmap, int n_words, const int32_t* __restrict__ off_in, int need_deg,
-    const int2* __restrict__ blk, int32_t* __restrict__ active, int32_t* __restrict__ escan,
-    int32_t* __restrict__ cbase,
-    uint32_t* __restrict__ chg_bits) {
-  __shared__ int s_a[kCompactNT / kWave], s_b[kCompactNT / kWave];
-  const int w = blockIdx.x * kCompactNT + threadIdx.x;
-  // clears the changed bitmap of the previous iteration: the host swaps it in
-  // as the buffer that the next apply fills
-  if (chg_bits && w < n_words) chg_bits[w] = 0u;
-  uint32_t word = (w < n_words) ? bitmap[w] : 0u;
-  const int cnt = __popc(word);
-  const int deg = need_deg ? degree_of_bits(word, w * 32, off_in) : 0;
-  int ta, tb;
-  const int ea = block_excl_scan_add<kCompactNT>(cnt, s_a, &ta);
-  const int eb = block_excl_scan_add<kCompactNT>(deg, s_b, &tb);
-  if (!word) return;
-  const int2 b = blk[blockIdx.x];
-  int pos = b.x + ea, dpos = b.y + eb;
-  bitmap[w] = 0u;
-  while (word) {
-    const int bit = __builtin_ctz(word);
-    word &= word - 1;
-    const int v = w * 32 + bit;
-    active[pos] = v;
-    if (need_deg) {
-      escan[pos] = dpos;
-      cbase[pos] = off_in[v] - dpos;
-      dpos += off_in[v + 1] - off_in[v];
+    uint32_t* __restrict__ bitmap, int n_words, int nV, const int32_t* __restrict__ off_in,
+    int need_deg, const int2* __restrict__ blk, int32_t* __restrict__ active,
+    int32_t* __restrict__ escan, int32_t* __restrict__ cbase, uint32_t* __restrict__ chg_bits) {
+  __shared__ unsigned long long s_scan[kCompactNT / kWave];
+  const int2 base = blk[blockIdx.x];
+  int pos_run = base.x, deg_run = base.y;
+  for (int r = 0; r < kCompactRounds; ++r) {
+    const int bi = (blockIdx.x * kCompactRounds + r) * kCompactNT + threadIdx.x;
+    const ByteWork b = byte_work(bitmap, n_words, nV, off_in, need_deg, bi);
+    // (deg << 16 | cnt): cnt <= 8 per thread and <= 2048 per round
+    unsigned long long tot;
+    const unsigned long long ex = block_excl_scan_add<kCompactNT>(
+        ((unsigned long long)(uint32_t)b.deg << 16) | (unsigned long long)b.cnt, s_scan, &tot);
+    int pos = pos_run + (int)(ex & 0xFFFFu);
+    int dpos = deg_run + (int)(ex >> 16);
+    pos_run += (int)(tot & 0xFFFFu);
+    deg_run += (int)(tot >> 16);
+    // every thread has read its byte before the scan's barriers
+    const int wi = bi >> 2;
+    if ((bi & 3) == 0 && wi < n_words) {
+      if (chg_bits) chg_bits[wi] = 0u;
+      bitmap[wi] = 0u;
     }
-    ++pos;
+    uint32_t bits = b.bits;
+    while (bits) {
+      const int j = __builtin_ctz(bits);
+      bits &= bits - 1;
+      active[pos] = b.v0 + j;
+      if (need_deg) {
+        escan[pos] = dpos;
+        cbase[pos] = b.o[j] - dpos;
+        dpos += b.o[j + 1] - b.o[j];
+      }
+      ++pos;
+    }
   }
 }
 
@@ -873,12 +905,13 @@ struct Engine final : EngineBase {
   int next_iter() override {
     const int need_deg = P::kHasGather ? 1 : 0;
     tick();
-    k_compact_count<<<n_cblocks, kCompactNT, 0, stream>>>(d_bitmap, n_words, g->off_in, need_deg,
-                                                          d_blk);
+    k_compact_count<<<n_cblocks, kCompactNT, 0, stream>>>(d_bitmap, n_words, g->nV, g->off_in,
+                                                          need_deg, d_blk);
     k_compact_scan<<<1, 1024, 0, stream>>>(d_blk, n_cblocks, d_ctr, need_deg ? d_escan : nullptr,
                                            dir_mode, pull_threshold());
-    k_compact_write<<<n_cblocks, kCompactNT, 0, stream>>>(d_bitmap, n_words, g->off_in, need_deg,
-                                                          d_blk, d_active, d_escan, d_cbase,
+    k_compact_write<<<n_cblocks, kCompactNT, 0, stream>>>(d_bitmap, n_words, g->nV, g->off_in,
+                                                          need_deg, d_blk, d_active, d_escan,
+                                                          d_cbase,
                                                           changed_bits_on() ? d_chg_prev : nullptr);
     tock(kPhNext);
     GR_HIP(hipGetLastError());
