@@ -145,25 +145,21 @@ def main() -> int:
     st = eng.stats()
     eng.enableTiming(False)
     eng.resetStats()
-    gather_bytes = 12 * st["gathered_edges"] + 16 * st["active_vertices"]
+    # Dominant kernel: the merge-path gather tile kernel, which runs the full
+    # gathers (the all-active first one and the dense iterations); the sparse
+    # iterations are gathered by pushing the changed out-edges (push_*).
+    tile_bytes = 12 * st["tile_gather_edges"] + 16 * st["tile_gather_rows"]
     gather_s = st["gather_ms"] / 1e3
-    achieved = gather_bytes / gather_s / 1e9 if gather_s > 0 else 0.0
-    # Bytes the delta-gather kernel itself must move: the all-active first
-    # gather reads 12 B per in-edge; every later gather reads the 4-B source id
-    # of each in-edge and 8 B (weight + source distance) only for in-edges
-    # whose source changed in the previous iteration -- together exactly the
-    # out-edges of the changed sets, sum E_out(C) = pushed_edges.
-    if st["delta_gathers"] > 0:
-        e_all = int(g.n_edges)
-        own_bytes = (12 * e_all + 4 * (st["gathered_edges"] - e_all) + 8 * st["pushed_edges"]
-                     + 16 * st["active_vertices"])
-    else:
-        own_bytes = gather_bytes
-    achieved_own = own_bytes / gather_s / 1e9 if gather_s > 0 else 0.0
+    achieved = tile_bytes / gather_s / 1e9 if gather_s > 0 else 0.0
+    ref_bytes = 12 * st["gathered_edges"] + 16 * st["active_vertices"]
+    phase_s = (st["gather_ms"] + st["push_ms"]) / 1e3
     phases = {k: round(st[k + "_ms"], 4) for k in ("gather", "apply", "scatter", "next")}
+    phases["gather"] = round(st["gather_ms"] + st["push_ms"], 4)
+    phases["gather_tiles"] = round(st["gather_ms"], 4)
+    phases["gather_push"] = round(st["push_ms"], 4)
     launches = {k: st[k + "_launches"] for k in ("gather", "apply", "scatter", "next")}
-    step_alg_gbs = gather_bytes / sec / 1e9
-    log(f"  phases ms {phases} launches {launches}; gather {achieved:.1f} GB/s algorithmic")
+    step_alg_gbs = ref_bytes / sec / 1e9
+    log(f"  phases ms {phases} launches {launches}; tile kernel {achieved:.1f} GB/s algorithmic")
 
     # HBM traffic per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     # summary (calibrated against a plain copy kernel)
@@ -200,26 +196,32 @@ def main() -> int:
             "edges_reached": e_reached,
         },
         "roofline": {
-            "kernel": "k_gather_tiles<SSSPProgram>",
+            "kernel": "k_gather_tiles<SSSPProgram> (the full gathers of the run)",
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": prof_traffic,
-            "bytes_per_launch": round(gather_bytes / max(1, launches["gather"])),
+            "bytes_per_launch": round(tile_bytes / max(1, launches["gather"])),
             "avg_launch_ms": round(st["gather_ms"] / max(1, launches["gather"]), 5),
             "launches": launches["gather"],
             "algorithmic_bytes": "12 B per gathered in-edge (source id, weight, source "
-                                 "distance) + 16 B per active vertex, over the reference's "
-                                 "iteration sets (full gather)",
-            "delta_gathers": st["delta_gathers"],
-            "kernel_bytes_per_launch": round(own_bytes / max(1, launches["gather"])),
-            "achieved_kernel_bytes": round(achieved_own, 1),
-            "frac_kernel_bytes": round(achieved_own / HBM_PEAK_GBS, 4),
-            "kernel_bytes": "delta gather: 12 B per in-edge in the all-active first gather; "
-                            "then 4 B per in-edge + 8 B per in-edge from a source changed in "
-                            "the previous iteration + 16 B per active vertex",
+                                 "distance) + 16 B per active vertex",
+            "push_gathers": {
+                "launches": st["push_gathers"],
+                "edges": st["push_edges"],
+                "ms": round(st["push_ms"], 4),
+                "alg_gbs": round(12 * st["push_edges"] / (st["push_ms"] / 1e3) / 1e9, 1)
+                if st["push_ms"] > 0 else None,
+                "algorithmic_bytes": "12 B per pushed out-edge (target id, weight, candidate)",
+            },
+            "gather_phase_reference_gbs": round(ref_bytes / phase_s / 1e9, 1) if phase_s else None,
+            "gather_phase_reference_frac": round(ref_bytes / phase_s / 1e9 / HBM_PEAK_GBS, 4)
+            if phase_s else None,
+            "reference_bytes": "12 B per in-edge of every active vertex + 16 B per active "
+                               "vertex over all 9 gathers, as the reference's full gathers "
+                               "would move them",
         },
         "phase_ms_per_step": phases,
         "tuning": "graphreduce_amd/tuned.json (autotuner)" if tuned else "defaults",
@@ -289,7 +291,7 @@ def run_extras(gr, g, args, src, outdeg, w_dev):
     out["pagerank"] = dict(gteps=round(ein / sec / 1e9, 3), ms=round(sec * 1e3, 3), trips=trips,
                            gathered_edges=ein,
                            gather_alg_gbs=round((8 * ein + 16 * st["active_vertices"])
-                                                / (st["gather_ms"] / 1e3) / 1e9, 1))
+                                                / ((st["gather_ms"] + st["push_ms"]) / 1e3) / 1e9, 1))
     log(f"PR: {out['pagerank']}")
     # plain CC on the symmetrised graph
     gs = gr.Graph.rmat(args.scale, 16, seed=args.seed, symmetrize=True)
@@ -312,7 +314,7 @@ def run_extras(gr, g, args, src, outdeg, w_dev):
     out["cc"] = dict(gteps=round(ein / sec / 1e9, 3), ms=round(sec * 1e3, 3), trips=trips,
                      n_edges=sym_edges, gathered_edges=ein,
                      gather_alg_gbs=round((8 * ein + 16 * st["active_vertices"])
-                                          / (st["gather_ms"] / 1e3) / 1e9, 1))
+                                          / ((st["gather_ms"] + st["push_ms"]) / 1e3) / 1e9, 1))
     log(f"CC: {out['cc']}")
     out["batched"] = run_batched(gr, g, args, outdeg)
     out["serving"] = run_serving(gr, g, args, outdeg, w_dev)

@@ -72,6 +72,11 @@ class _Stats(C.Structure):
         ("next_launches", C.c_int64),
         ("next_ms", C.c_double),
         ("delta_gathers", C.c_int64),
+        ("push_gathers", C.c_int64),
+        ("push_edges", C.c_int64),
+        ("push_ms", C.c_double),
+        ("tile_gather_rows", C.c_int64),
+        ("tile_gather_edges", C.c_int64),
     ]
 
 
@@ -511,8 +516,9 @@ class GASEngine:
     def setDeltaGather(self, mode=1) -> None:
         """SSSP/CC: after an all-active iteration, gather only from sources
         changed in the previous iteration (same results).  0/False: off;
-        1/True (default): when under 60 % of the in-edges come from changed
-        sources; 2: whenever valid."""
+        1/True (default): push under 60 % of the in-edges from changed
+        sources, else the full gather; 2: pull-delta whenever valid;
+        3: push whenever valid."""
         _check(lib().gr_engine_set_delta_gather(self._h, int(mode)))
 
     def setGatherTile(self, items_per_thread: int) -> None:

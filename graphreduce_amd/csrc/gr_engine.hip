@@ -220,6 +220,88 @@ __global__ __launch_bounds__(kNT) void k_push_tiles(const DevCounters* __restric
   }
 }
 
+// ---------------------------------------------- delta gather by push
+//
+// For delta-gather programs the gather of the next iteration only needs the
+// in-edges whose source changed, i.e. the out-edges of the changed list.  When
+// those are few, they are pushed: every out-edge (u -> v) of a changed u
+// lowers cand[v] to gatherMap(u, edge) with atomicMin, and the gather result of
+// an active row is then cand of its vertex.  cand holds gatherZero everywhere
+// between gathers: only out-neighbours of changed vertices receive values and
+// they are exactly the next active list, whose entries are reset when read.
+
+// part[t] = changed-list entry holding out-edge t * kTile (n_c entries, w_c edges)
+__global__ void k_relax_partition(int n_c, int w_c, const int32_t* __restrict__ chg_e,
+                                  int32_t* __restrict__ part) {
+  const int n_tiles = (w_c + kTile - 1) / kTile;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t <= n_tiles; t += gridDim.x * blockDim.x)
+    part[t] = max(0, upper_bound_dev(chg_e, n_c, min(t * kTile, w_c)) - 1);
+}
+
+template <class P>
+__global__ __launch_bounds__(kNT) void k_relax_tiles(
+    int w_c, const int32_t* __restrict__ part, const int32_t* __restrict__ chg_v,
+    const int32_t* __restrict__ chg_e, const int32_t* __restrict__ off_out,
+    const int32_t* __restrict__ dsts, const typename P::VertexData* __restrict__ vd,
+    const typename P::EdgeData* __restrict__ ed_csr, typename P::GatherResult* __restrict__ cand) {
+  static_assert(!P::kGatherUsesDst, "push relax needs a gatherMap without the destination");
+  __shared__ int32_t s_start[kTile + 2];
+  __shared__ int32_t s_csr[kTile + 2];
+  __shared__ typename P::VertexData s_vd[kTile + 2];
+  const int n_tiles = (w_c + kTile - 1) / kTile;
+  const int tid = threadIdx.x;
+  for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    const int k0 = part[t], k1 = part[t + 1];
+    const int nr = k1 - k0 + 1;
+    const int e0 = t * kTile, e1 = min(e0 + kTile, w_c);
+    for (int k = tid; k < nr; k += kNT) {
+      const int u = chg_v[k0 + k];
+      s_start[k] = chg_e[k0 + k];
+      s_csr[k] = off_out[u];
+      s_vd[k] = vd[u];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < kVT; ++m) {
+      const int e = e0 + tid + m * kNT;
+      if (e < e1) {
+        int lo = 0, hi = nr - 1;
+        while (lo < hi) {
+          int mid = (lo + hi + 1) >> 1;
+          if (s_start[mid] <= e)
+            lo = mid;
+          else
+            hi = mid - 1;
+        }
+        const int pos = s_csr[lo] + (e - s_start[lo]);
+        const int v = dsts[pos];
+        const typename P::GatherResult val = P::gatherMap(nullptr, &s_vd[lo], ed_csr + pos);
+        if (val < cand[v]) atomicMin(&cand[v], val);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// gtmp[i] = cand[active[i]] (already folded with gatherZero); resets cand.
+template <class P>
+__global__ void k_relax_collect(int n, const int32_t* __restrict__ active,
+                                typename P::GatherResult* __restrict__ cand,
+                                typename P::GatherResult* __restrict__ gtmp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int v = active[i];
+  gtmp[i] = cand[v];
+  cand[v] = P::gatherZero;
+}
+
+template <class T>
+__global__ void k_fill(int64_t n, T value, T* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = value;
+}
+
 // ------------------------------------------------------- scatter: pull
 
 // next[v] = some in-neighbour of v is in the changed bitmap.
@@ -535,6 +617,8 @@ struct Engine final : EngineBase {
   VD* d_vd_init = nullptr;  // vertex data as given at bind time
   VD* d_vd_user = nullptr;  // staging in user order (ordered layout only)
   ED* d_ed = nullptr;  // CSC order
+  ED* d_ed_csr = nullptr;  // CSR order (delta-gather programs: push relax)
+  G* d_cand = nullptr;     // push-relax candidates per vertex (gatherZero between gathers)
   int32_t* d_active = nullptr;
   int32_t* d_escan = nullptr;
   int32_t* d_cbase = nullptr;  // per active row: CSC offset minus its escan value
@@ -557,17 +641,23 @@ struct Engine final : EngineBase {
   // Delta gather (programs with kDeltaGather): valid once an all-active
   // iteration has run and every later active set came from scatterActivate.
   static constexpr bool kDelta = delta_gather_v<P>;
-  int delta_mode = 1;  // 0 off, 1 when the changed share is small, 2 always
+  // 0 off; 1 adaptive (push below kPushMaxChangedShare, else full); 2 delta
+  // pull whenever valid; 3 push whenever valid
+  int delta_mode = 1;
+  int64_t chg_count = 0, chg_edges = 0;  // changed list of the previous iteration
   bool delta_chain = false;  // the invariant holds for the current state
   bool delta_ready = false;  // d_chg_prev holds the previous iteration's changes
   // Share of the next gather's in-edges whose source changed (E_out(C) / E_in(A)).
-  // Delta gather adds a bitmap lookup per in-edge and pays off only when most
-  // sources are unchanged: at RMAT-24 it is slower from a share of about 0.6 up.
+  // Measured per iteration on RMAT-24 SSSP: pushing the changed out-edges is
+  // the fastest gather below a share of about 0.6 (1.6 ms vs 1.7 ms pull-delta
+  // and 1.8 ms full at 0.56; 0.7 ms vs 1.3 and 1.7 at 0.16), the full gather
+  // above it (1.8 ms vs 1.85 push and 2.1 pull-delta at 0.65; 1.8 vs 2.4 and
+  // 2.5 at 0.94); the pull-delta gather is never the fastest there.
   double changed_share = 1.0;
-  static constexpr double kDeltaMaxChangedShare = 0.6;
+  static constexpr double kPushMaxChangedShare = 0.6;
   bool it_gather = false, it_apply = false, it_scatter = false;
   // hipEvent start/stop pairs per timed phase
-  enum { kPhGather = 0, kPhScatter, kPhApply, kPhNext, kPhases };
+  enum { kPhGather = 0, kPhScatter, kPhApply, kPhNext, kPhPush, kPhases };
   std::vector<hipEvent_t> ev[kPhases];
   hipEvent_t ev_open = nullptr;
 
@@ -577,7 +667,7 @@ struct Engine final : EngineBase {
   size_t edge_size() const override { return kEdgeEmpty ? 0 : sizeof(ED); }
 
   void release() {
-    void* ptrs[] = {d_cbase, d_heavy,   d_vd,       d_vd_init, d_vd_user, d_ed,   d_active,
+    void* ptrs[] = {d_cbase, d_heavy,   d_vd,       d_vd_init, d_vd_user, d_ed,   d_ed_csr, d_cand, d_active,
                     d_escan, d_gtmp,    d_chg_v,    d_chg_e,   d_bitmap, d_chg_bits,
                     d_chg_prev, d_blk,  d_part,     d_push_part, d_carry, d_head,
                     d_ctr};
@@ -598,6 +688,8 @@ struct Engine final : EngineBase {
     d_heavy = nullptr;
     n_heavy = 0;
     d_ed = nullptr;
+    d_ed_csr = nullptr;
+    d_cand = nullptr;
     d_cbase = nullptr;
     d_active = d_escan = d_chg_v = d_chg_e = d_part = d_push_part = nullptr;
     d_gtmp = nullptr;
@@ -705,9 +797,21 @@ struct Engine final : EngineBase {
             reinterpret_cast<const uint32_t*>(src), g->csc_eid, E, (int)(sizeof(ED) / 4),
             reinterpret_cast<uint32_t*>(d_ed));
         GR_HIP(hipGetLastError());
+        if constexpr (kDelta) {
+          GR_TRY(dalloc(&d_ed_csr, E));
+          k_permute_words<<<grid_1d(E, 256), 256, 0, stream>>>(
+              reinterpret_cast<const uint32_t*>(src), g->csr_eid, E, (int)(sizeof(ED) / 4),
+              reinterpret_cast<uint32_t*>(d_ed_csr));
+          GR_HIP(hipGetLastError());
+        }
         GR_HIP(hipStreamSynchronize(stream));
         if (tmp) (void)hipFree(tmp);
       }
+    }
+    if constexpr (kDelta) {
+      GR_TRY(dalloc(&d_cand, V));
+      k_fill<G><<<1024, 256, 0, stream>>>(V, P::gatherZero, d_cand);
+      GR_HIP(hipGetLastError());
     }
     GR_HIP(hipStreamSynchronize(stream));
     n_active = 0;
@@ -734,6 +838,13 @@ struct Engine final : EngineBase {
         reinterpret_cast<const uint32_t*>(d_vd), d_inv, g->nV, (int)(sizeof(VD) / 4),
         reinterpret_cast<uint32_t*>(d_vd_user));
     return d_vd_user;
+  }
+
+  const ED* edge_csr_ptr() const {
+    if constexpr (kEdgeEmpty)
+      return reinterpret_cast<const ED*>(d_gtmp);  // never dereferenced
+    else
+      return d_ed_csr;
   }
 
   const ED* edge_ptr() const {
@@ -827,11 +938,12 @@ struct Engine final : EngineBase {
     it_gather = true;
     if (!P::kHasGather || n_active == 0) return GR_OK;
     const int n = n_active, w = w_edges;
-    const uint32_t* dbits =
-        (kDelta && delta_ready &&
-         (delta_mode == 2 || (delta_mode == 1 && changed_share < kDeltaMaxChangedShare)))
-            ? d_chg_prev
-            : nullptr;
+    if constexpr (kDelta) {
+      if (delta_ready &&
+          (delta_mode == 3 || (delta_mode == 1 && changed_share < kPushMaxChangedShare)))
+        return gather_push();
+    }
+    const uint32_t* dbits = (kDelta && delta_ready && delta_mode == 2) ? d_chg_prev : nullptr;
     const int tile = kNT * gather_vt;
     const int n_tiles = (int)(((int64_t)n + w + tile - 1) / tile);
     k_merge_partition<<<grid_1d((int64_t)n_tiles + 1, 256), 256, 0, stream>>>(d_escan, n, w,
@@ -852,7 +964,34 @@ struct Engine final : EngineBase {
     GR_HIP(hipGetLastError());
     stats.active_vertices += n;
     stats.gathered_edges += w;
+    stats.tile_gather_rows += n;
+    stats.tile_gather_edges += w;
     if (dbits) stats.delta_gathers += 1;
+    return GR_OK;
+  }
+
+  // Delta gather by push: relax the out-edges of the previous iteration's
+  // changed list into d_cand, then read the candidates of the active rows.
+  int gather_push() {
+    const int n = n_active;
+    const int n_c = (int)chg_count, w_c = (int)chg_edges;
+    tick();
+    if (w_c > 0) {
+      const int n_tiles = (w_c + kTile - 1) / kTile;
+      k_relax_partition<<<grid_1d((int64_t)n_tiles + 1, 256), 256, 0, stream>>>(n_c, w_c, d_chg_e,
+                                                                                d_push_part);
+      const int grid = n_tiles < 8192 ? n_tiles : 8192;
+      k_relax_tiles<P><<<grid, kNT, 0, stream>>>(w_c, d_push_part, d_chg_v, d_chg_e, g->off_out,
+                                                 g->dsts, d_vd, edge_csr_ptr(), d_cand);
+    }
+    k_relax_collect<P><<<grid_1d(n, 256), 256, 0, stream>>>(n, d_active, d_cand, d_gtmp);
+    tock(kPhPush);
+    GR_HIP(hipGetLastError());
+    stats.active_vertices += n;
+    stats.gathered_edges += w_edges;
+    stats.delta_gathers += 1;
+    stats.push_gathers += 1;
+    stats.push_edges += w_c;
     return GR_OK;
   }
 
@@ -931,6 +1070,8 @@ struct Engine final : EngineBase {
     n_active = h_ctr->n_active;
     w_edges = h_ctr->w_edges;
     changed_share = w_edges > 0 ? (double)(uint32_t)c / (double)w_edges : 1.0;
+    chg_count = (int64_t)(c >> 32);
+    chg_edges = (int64_t)(uint32_t)c;
     ++iteration;
     return GR_OK;
   }
@@ -977,6 +1118,10 @@ struct Engine final : EngineBase {
     sum_pairs(ev[kPhScatter], &stats.scatter_launches, &stats.scatter_ms);
     sum_pairs(ev[kPhApply], &stats.apply_launches, &stats.apply_ms);
     sum_pairs(ev[kPhNext], &stats.next_launches, &stats.next_ms);
+    {
+      int64_t pushes = 0;
+      sum_pairs(ev[kPhPush], &pushes, &stats.push_ms);
+    }
     if (out) *out = stats;
     return GR_OK;
   }
@@ -1151,8 +1296,9 @@ extern "C" int gr_engine_set_vertex_order(gr_engine* e, int order) {
 
 extern "C" int gr_engine_set_delta_gather(gr_engine* e, int mode) {
   GR_TRY(check(e, false));
-  if (mode < 0 || mode > 2)
-    return set_error(GR_ERR_INVALID, "delta gather mode must be 0 (off), 1 (adaptive) or 2 (always)");
+  if (mode < 0 || mode > 3)
+    return set_error(GR_ERR_INVALID,
+                     "delta gather mode must be 0 (off), 1 (adaptive), 2 (pull) or 3 (push)");
   return e->impl->set_delta_gather(mode);
 }
 

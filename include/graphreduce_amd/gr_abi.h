@@ -227,9 +227,13 @@ int gr_engine_set_vertex_order(gr_engine* e, int order);
  * its last change and cannot lower it, so every apply sees the same outcome
  * and value as with the full gather of refgas.h:141-160: results, iteration
  * counts and the gathered_edges counter are unchanged.
- * mode 0: always gather every in-edge; 1 (default): delta gather when fewer
- * than 60 % of the in-edges come from changed sources; 2: delta gather
- * whenever it is valid. */
+ * The delta gather is done either as a pull over the active rows' in-edges
+ * that skips unchanged sources, or as a push: the out-edges of the changed
+ * vertices lower a per-vertex candidate with atomicMin, read back per active
+ * row (the same minimum over the same edges).
+ * mode 0: always gather every in-edge; 1 (default): push when under 60 % of
+ * the in-edges come from changed sources, else the full gather; 2: pull-delta
+ * whenever valid; 3: push whenever valid. */
 int gr_engine_set_delta_gather(gr_engine* e, int mode);
 
 /* Work and timing counters, summed since the last reset. */
@@ -241,7 +245,7 @@ typedef struct gr_engine_stats {
   int64_t pushed_edges;      /* sum of E_out(C) */
   int64_t pull_iterations;   /* scatterActivate calls that ran pull mode */
   int64_t gather_launches;   /* gather tile-kernel launches timed */
-  double gather_ms;          /* summed duration of the gather tile kernel */
+  double gather_ms;          /* summed duration of the gather tile kernel (pushes: push_ms) */
   int64_t scatter_launches;  /* scatterActivate calls timed (push + pull kernels) */
   double scatter_ms;
   int64_t apply_launches;    /* apply kernel */
@@ -250,6 +254,11 @@ typedef struct gr_engine_stats {
   double next_ms;
   int64_t delta_gathers;     /* gathers that read only sources changed in the
                                 previous iteration (see gr_engine_set_delta_gather) */
+  int64_t push_gathers;      /* ... of which done by pushing the changed out-edges */
+  int64_t push_edges;        /* out-edges relaxed by those pushes */
+  double push_ms;            /* their summed duration (not in gather_ms) */
+  int64_t tile_gather_rows;  /* sum of #A over gathers by the merge-path tile kernel */
+  int64_t tile_gather_edges; /* sum of E_in(A) over those gathers */
 } gr_engine_stats;
 
 /* Enables hipEvent timing of the engine phases on the engine's stream. */

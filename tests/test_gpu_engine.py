@@ -310,17 +310,19 @@ def test_delta_gather_equals_full_gather(gpu, case):
     else:
         w, vd, prog = None, np.arange(n, dtype=np.int32), gr.CC
         ref, ref_trips, _ = oracle.cc(n, s, d)
-    on, t_on, st_on = _engine_run(gr, prog, n, vd, s, d, w, delta=2)  # whenever valid
+    on, t_on, st_on = _engine_run(gr, prog, n, vd, s, d, w, delta=2)  # pull-delta if valid
+    pu, t_pu, st_pu = _engine_run(gr, prog, n, vd, s, d, w, delta=3)  # push if valid
     ad, t_ad, st_ad = _engine_run(gr, prog, n, vd, s, d, w, delta=1)  # adaptive
     off, t_off, st_off = _engine_run(gr, prog, n, vd, s, d, w, delta=0)
-    for got in (on, ad, off):
+    for got in (on, pu, ad, off):
         assert np.array_equal(got, ref)
-    assert t_on == t_ad == t_off == ref_trips
+    assert t_on == t_pu == t_ad == t_off == ref_trips
     for k in ("iterations", "active_vertices", "gathered_edges", "changed_vertices",
               "pushed_edges"):
-        assert st_on[k] == st_ad[k] == st_off[k], k
+        assert st_on[k] == st_pu[k] == st_ad[k] == st_off[k], k
     assert st_off["delta_gathers"] == 0
-    assert st_on["delta_gathers"] == t_on - 1  # every gather after the all-active one
+    assert st_on["delta_gathers"] == t_on - 1 and st_on["push_gathers"] == 0
+    assert st_pu["delta_gathers"] == st_pu["push_gathers"] == t_on - 1
     assert st_ad["delta_gathers"] <= t_on - 1
 
 
@@ -333,15 +335,16 @@ def test_delta_gather_not_used_after_partial_start(gpu):
     d = np.array([0, 1, 1], np.int32)
     w = np.array([1, 5, 1], np.int32)
     vd = np.array([gr.SSSP_INF, gr.SSSP_INF, 0, 0], np.int32)
-    for delta in (2, 1, 0):
+    for delta in (3, 2, 1, 0):
         out, trips, st = _engine_run(gr, gr.SSSP, 4, vd, s, d, w, start=0, delta=delta)
         assert out.tolist() == [1, 1, 0, 0]
         assert trips == 2
         assert st["delta_gathers"] == 0
     # the same graph from an all-active start: delta is valid and used
-    out, trips, st = _engine_run(gr, gr.SSSP, 4, vd, s, d, w, delta=2)
-    assert out.tolist() == [1, 1, 0, 0]
-    assert st["delta_gathers"] == trips - 1
+    for delta in (2, 3):
+        out, trips, st = _engine_run(gr, gr.SSSP, 4, vd, s, d, w, delta=delta)
+        assert out.tolist() == [1, 1, 0, 0]
+        assert st["delta_gathers"] == trips - 1
 
 
 # ------------------------------------------------------- vertex order
