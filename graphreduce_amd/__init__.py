@@ -659,17 +659,22 @@ def _batch_sources(sources):
     return src
 
 
-MSBFS_STREAM_PULL = 4
+MSBFS_STREAM_PULL, MSBFS_PERSISTENT, MSBFS_FILTER = 4, 8, 16
 
 
 def msbfs(graph: Graph, sources, direction: int = 0, alpha: float = 20.0,
-          stream_pull: bool = False):
-    """BFS from up to 64 sources in one persistent launch.
+          stream_pull: bool = False, persistent: bool = False, pull_filter: bool = False):
+    """BFS from up to 64 sources in one batch (one expand + one update launch
+    per level; persistent=True: every level in one cooperative launch).
 
     stream_pull asks for the streaming pull kernel, which the library
     currently refuses (GRError).  Returns (depth int32[len(sources), V], info)."""
     if stream_pull:
         direction |= MSBFS_STREAM_PULL
+    if persistent:
+        direction |= MSBFS_PERSISTENT
+    if pull_filter:
+        direction |= MSBFS_FILTER
     src = _batch_sources(sources)
     n = graph.n_vertices
     nbytes = 4 * n * len(src)

@@ -27,6 +27,7 @@
 // once into a hipGraph and replayed.
 #include <hip/hip_cooperative_groups.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "gr_device.h"
@@ -80,10 +81,12 @@ __global__ void k_ms_init(int nV, uint64_t* __restrict__ visit, uint64_t* __rest
   }
 }
 
+// depth_vs != 0: depth is vertex-major [V][64]; otherwise source-major [ns][V]
 __global__ void k_ms_sources(int ns, const int32_t* __restrict__ sources, int nV,
                              uint64_t* __restrict__ visit, uint64_t* __restrict__ frontier,
                              uint32_t* __restrict__ fbits, int32_t* __restrict__ depth,
-                             const int32_t* __restrict__ off_out, MsCounters* __restrict__ ctr) {
+                             int depth_vs, const int32_t* __restrict__ off_out,
+                             MsCounters* __restrict__ ctr) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   unsigned long long fe = 0;
   for (int s = 0; s < ns; ++s) {
@@ -92,7 +95,7 @@ __global__ void k_ms_sources(int ns, const int32_t* __restrict__ sources, int nV
     visit[v] |= 1ull << s;
     frontier[v] |= 1ull << s;
     if (fbits) fbits[v >> 5] |= 1u << (v & 31);
-    depth[(size_t)s * nV + v] = 0;
+    depth[depth_vs ? (size_t)v * kWave + s : (size_t)s * nV + v] = 0;
   }
   for (int i = 0; i < 3; ++i) {
     ctr->new_vertices[i] = 0;
@@ -236,6 +239,277 @@ __global__ __launch_bounds__(kBatchNT) void k_msbfs(
     }
     grid_barrier(grid);
     if (blockIdx.x == 0 && threadIdx.x == 0) ctr->levels = level + 1;
+  }
+}
+
+// ---------------------------------------------- multi-kernel MS-BFS
+//
+// The same level structure as k_msbfs as ordinary launches (no grid
+// barrier): one expand kernel (push or pull) and one update kernel per level;
+// the host reads the level's two counters after the update.  fbits is the
+// frontier bitmap (bit v = frontier[v] != 0), written by the update: push reads
+// the 8-byte frontier mask only of frontier vertices, and pull may skip the
+// mask of in-neighbours outside the frontier.
+
+struct MsLevelCounters {
+  unsigned long long new_vertices;
+  unsigned long long frontier_edges;
+};
+
+__global__ __launch_bounds__(kBatchNT) void k_ms_push(int nV, const int32_t* __restrict__ off_out,
+                                                      const int32_t* __restrict__ dsts,
+                                                      const uint64_t* __restrict__ visit,
+                                                      const uint64_t* __restrict__ frontier,
+                                                      const uint32_t* __restrict__ fbits,
+                                                      uint64_t* __restrict__ next) {
+  const int lane = lane_id();
+  const int wpb = blockDim.x / kWave;
+  const int n_waves = gridDim.x * wpb;
+  for (int base = (blockIdx.x * wpb + threadIdx.x / kWave) * kWave; base < nV;
+       base += n_waves * kWave) {
+    const int v = base + lane;
+    const uint32_t word = (v < nV) ? fbits[v >> 5] : 0u;
+    const bool inf = (word >> (v & 31)) & 1u;
+    if (!__ballot(inf)) continue;
+    uint64_t f = inf ? frontier[v] : 0ull;
+    int b = 0, e = 0;
+    if (f) {
+      b = off_out[v];
+      e = off_out[v + 1];
+      if (e - b > kLaneHeavy) f = 0;  // chunked: k_ms_push_chunks
+    }
+    const bool light = (e - b) <= kMsHeavy;
+    if (f && light) {
+      // four out-edges per step (independent loads in flight); an atomic only
+      // when next[u] does not already hold the bits (hub targets get many)
+      for (int p0 = b; p0 < e; p0 += 4) {
+        int u[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) u[j] = p0 + j < e ? dsts[p0 + j] : -1;
+        uint64_t want[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) want[j] = u[j] >= 0 ? (f & ~visit[u[j]] & ~next[u[j]]) : 0ull;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (want[j]) atomicOr((unsigned long long*)&next[u[j]], (unsigned long long)want[j]);
+      }
+    }
+    uint64_t heavy = __ballot(f && !light);
+    while (heavy) {
+      const int hl = __builtin_ctzll(heavy);
+      heavy &= heavy - 1;
+      const int hb = __shfl(b, hl, kWave), he = __shfl(e, hl, kWave);
+      const uint32_t flo = __shfl((uint32_t)f, hl, kWave);
+      const uint32_t fhi = __shfl((uint32_t)(f >> 32), hl, kWave);
+      const uint64_t hf = ((uint64_t)fhi << 32) | flo;
+      for (int p = hb + lane; p < he; p += kWave) {
+        const int u = dsts[p];
+        const uint64_t want = hf & ~visit[u] & ~next[u];
+        if (want) atomicOr((unsigned long long*)&next[u], (unsigned long long)want);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ uint64_t ms_mask_of(int u, const uint64_t* __restrict__ frontier,
+                                               const uint32_t* __restrict__ filter) {
+  if (filter && !((filter[u >> 5] >> (u & 31)) & 1u)) return 0ull;
+  return frontier[u];
+}
+
+// filter: the frontier bitmap, or nullptr to read every in-neighbour's mask
+__global__ __launch_bounds__(kBatchNT) void k_ms_pull(int nV, uint64_t all,
+                                                      const int32_t* __restrict__ off_in,
+                                                      const int32_t* __restrict__ srcs,
+                                                      const uint64_t* __restrict__ visit,
+                                                      const uint64_t* __restrict__ frontier,
+                                                      const uint32_t* __restrict__ filter,
+                                                      uint64_t* __restrict__ next) {
+  const int lane = lane_id();
+  const int wpb = blockDim.x / kWave;
+  const int n_waves = gridDim.x * wpb;
+  for (int base = (blockIdx.x * wpb + threadIdx.x / kWave) * kWave; base < nV;
+       base += n_waves * kWave) {
+    const int v = base + lane;
+    uint64_t need = v < nV ? (all & ~visit[v]) : 0ull;
+    int b = 0, e = 0;
+    if (need) {
+      b = off_in[v];
+      e = off_in[v + 1];
+      if (e - b > kLaneHeavy) need = 0;  // chunked: k_ms_pull_chunks (atomicOr)
+    }
+    const bool light = (e - b) <= kMsHeavy;
+    if (need && light) {
+      uint64_t acc = 0;
+      for (int p0 = b; p0 < e; p0 += 4) {  // four in-edges per step: loads in flight
+        int u[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) u[j] = p0 + j < e ? srcs[p0 + j] : -1;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (u[j] >= 0) acc |= ms_mask_of(u[j], frontier, filter);
+        if ((acc & need) == need) break;
+      }
+      if (acc & need) next[v] = acc & need;
+    }
+    uint64_t heavy = __ballot(need && !light);
+    while (heavy) {
+      const int hl = __builtin_ctzll(heavy);
+      heavy &= heavy - 1;
+      const int hb = __shfl(b, hl, kWave), he = __shfl(e, hl, kWave);
+      const uint32_t nlo = __shfl((uint32_t)need, hl, kWave);
+      const uint32_t nhi = __shfl((uint32_t)(need >> 32), hl, kWave);
+      const uint64_t hn = ((uint64_t)nhi << 32) | nlo;
+      uint64_t hacc = 0;
+      for (int pb = hb; pb < he; pb += kWave) {  // wave-uniform trip count
+        const int p = pb + lane;
+        const uint64_t m = p < he ? ms_mask_of(srcs[p], frontier, filter) : 0ull;
+        hacc |= wave_or64(m);
+        if ((hacc & hn) == hn) break;
+      }
+      if (lane == hl && (hacc & hn)) next[v] = hacc & hn;
+    }
+  }
+}
+
+// Vertices with more than kLaneHeavy edges, in chunks of kLaneChunk edges
+// spread over all waves (a single wave per hub would be the critical path).
+__global__ __launch_bounds__(kBatchNT) void k_ms_push_chunks(
+    const LaneChunk* __restrict__ chunks, int n_chunks, const int32_t* __restrict__ dsts,
+    const uint64_t* __restrict__ visit, const uint64_t* __restrict__ frontier,
+    const uint32_t* __restrict__ fbits, uint64_t* __restrict__ next) {
+  const int lane = lane_id();
+  const int wpb = blockDim.x / kWave;
+  const int n_waves = gridDim.x * wpb;
+  for (int c = blockIdx.x * wpb + threadIdx.x / kWave; c < n_chunks; c += n_waves) {
+    const LaneChunk ch = chunks[c];
+    if (!((fbits[ch.v >> 5] >> (ch.v & 31)) & 1u)) continue;
+    const uint64_t f = frontier[ch.v];
+    for (int p = ch.b + lane; p < ch.e; p += kWave) {
+      const int u = dsts[p];
+      const uint64_t want = f & ~visit[u] & ~next[u];
+      if (want) atomicOr((unsigned long long*)&next[u], (unsigned long long)want);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBatchNT) void k_ms_pull_chunks(
+    const LaneChunk* __restrict__ chunks, int n_chunks, uint64_t all,
+    const int32_t* __restrict__ srcs, const uint64_t* __restrict__ visit,
+    const uint64_t* __restrict__ frontier, uint64_t* __restrict__ next) {
+  const int lane = lane_id();
+  const int wpb = blockDim.x / kWave;
+  const int n_waves = gridDim.x * wpb;
+  for (int c = blockIdx.x * wpb + threadIdx.x / kWave; c < n_chunks; c += n_waves) {
+    const LaneChunk ch = chunks[c];
+    const uint64_t need = all & ~visit[ch.v];
+    if (!need) continue;
+    uint64_t acc = 0;
+    for (int pb = ch.b; pb < ch.e; pb += kWave) {  // wave-uniform trip count
+      const int p = pb + lane;
+      acc |= wave_or64(p < ch.e ? frontier[srcs[p]] : 0ull);
+      if ((acc & need) == need) break;
+    }
+    if (lane == 0 && (acc & need))
+      atomicOr((unsigned long long*)&next[ch.v], (unsigned long long)(acc & need));
+  }
+}
+
+__global__ __launch_bounds__(kBatchNT) void k_ms_update(
+    int nV, uint64_t all, int level, const int32_t* __restrict__ off_out,
+    uint64_t* __restrict__ visit, uint64_t* __restrict__ frontier, uint64_t* __restrict__ next,
+    uint32_t* __restrict__ fbits, int32_t* __restrict__ depth_vs,
+    MsLevelCounters* __restrict__ ctr) {
+  __shared__ unsigned long long s_nv[kBatchNT / kWave], s_fe[kBatchNT / kWave];
+  const int lane = lane_id();
+  const int wpb = blockDim.x / kWave;
+  const int n_waves = gridDim.x * wpb;
+  // the level counts are summed per wave over its vertices, then per
+  // workgroup: one atomic pair per workgroup (not per wave) on the counters
+  unsigned long long my_nv = 0, my_fe = 0;
+  for (int base = (blockIdx.x * wpb + threadIdx.x / kWave) * kWave; base < nV;
+       base += n_waves * kWave) {
+    const int v = base + lane;
+    uint64_t nw = 0;
+    int deg = 0;
+    if (v < nV) {
+      const uint64_t vis = visit[v];
+      const uint64_t nxt = next[v];
+      nw = nxt & ~vis & all;
+      if (nxt) next[v] = 0;
+      if (nw || frontier[v]) frontier[v] = nw;
+      if (nw) {
+        visit[v] = vis | nw;
+        deg = off_out[v + 1] - off_out[v];
+      }
+    }
+    const uint64_t nb = __ballot(nw != 0);
+    // depths of the new (source, vertex) pairs: for each vertex of the wave
+    // with new bits, lane s writes the vertex's row entry s (one masked,
+    // coalesced 256-B store per vertex)
+    uint64_t rows = nb;
+    while (rows) {
+      const int r = __builtin_ctzll(rows);
+      rows &= rows - 1;
+      const uint32_t lo = __shfl((uint32_t)nw, r, kWave);
+      const uint32_t hi = __shfl((uint32_t)(nw >> 32), r, kWave);
+      const uint64_t m = ((uint64_t)hi << 32) | lo;
+      if ((m >> lane) & 1ull) depth_vs[(size_t)(base + r) * kWave + lane] = level + 1;
+    }
+    if ((lane & 31) == 0 && v < nV) fbits[v >> 5] = (uint32_t)(nb >> lane);
+    unsigned long long fe = (unsigned long long)deg;
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) fe += __shfl_xor(fe, off, kWave);
+    my_nv += (unsigned long long)__popcll(nb);
+    my_fe += fe;
+  }
+  if (lane == 0) {
+    s_nv[threadIdx.x / kWave] = my_nv;
+    s_fe[threadIdx.x / kWave] = my_fe;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long nv = 0, fe = 0;
+    for (int i = 0; i < wpb; ++i) {
+      nv += s_nv[i];
+      fe += s_fe[i];
+    }
+    if (nv) {
+      atomicAdd(&ctr->new_vertices, nv);
+      atomicAdd(&ctr->frontier_edges, fe);
+    }
+  }
+}
+
+// depth_vs ([V][64], layout ids; valid where the visit bit is set) ->
+// out[s * nV + u] in the caller's ids, -1 where unreached.  A workgroup moves
+// 64 consecutive caller ids through an LDS tile: four waves read 16 vertex
+// rows of 256 B each, then write 64 consecutive depths per source.
+__global__ __launch_bounds__(256) void k_ms_depth_out(int nV, int ns,
+                                                      const int32_t* __restrict__ depth_vs,
+                                                      const uint64_t* __restrict__ visit,
+                                                      const int32_t* __restrict__ inv,
+                                                      int32_t* __restrict__ out) {
+  __shared__ int32_t tile[kWave][kWave + 1];
+  const int lane = lane_id();
+  const int w = threadIdx.x / kWave;
+  for (int u0 = blockIdx.x * kWave; u0 < nV; u0 += gridDim.x * kWave) {
+    for (int r = w; r < kWave; r += 4) {
+      const int u = u0 + r;
+      int32_t d = -1;
+      if (u < nV) {
+        const int v = inv[u];
+        const uint64_t vis = visit[v];
+        if ((vis >> lane) & 1ull) d = depth_vs[(size_t)v * kWave + lane];
+      }
+      tile[r][lane] = d;
+    }
+    __syncthreads();
+    for (int s = w; s < ns; s += 4) {
+      const int u = u0 + lane;
+      if (u < nV) out[(size_t)s * nV + u] = tile[lane][s];
+    }
+    __syncthreads();
   }
 }
 
@@ -544,7 +818,10 @@ constexpr bool kStreamPullEnabled = false;
 extern "C" int gr_msbfs(gr_graph* g, const int32_t* sources, int n_sources, int32_t* depth_device,
                         int direction, double alpha, gr_batch_result* result) {
   const bool stream = (direction & GR_MSBFS_STREAM_PULL) != 0;
-  direction &= ~GR_MSBFS_STREAM_PULL;
+  const bool persistent = (direction & GR_MSBFS_PERSISTENT) != 0;
+  const bool filter = (direction & GR_MSBFS_FILTER) != 0;
+  direction &= ~(GR_MSBFS_STREAM_PULL | GR_MSBFS_PERSISTENT | GR_MSBFS_FILTER);
+  const bool multi = !stream && !persistent;
   if (!g || !sources || !depth_device || n_sources < 1 || n_sources > 64 || direction < 0 ||
       direction > 2)
     return set_error(GR_ERR_INVALID, "gr_msbfs: bad arguments");
@@ -568,19 +845,84 @@ extern "C" int gr_msbfs(gr_graph* g, const int32_t* sources, int n_sources, int3
   GR_TRY(visit.alloc(nV));
   GR_TRY(frontier.alloc(nV));
   GR_TRY(next.alloc(nV));
-  if (stream) {
-    GR_TRY(fbits.alloc((size_t)nV / 32 + 2));
-    GR_TRY(lt.build(g->h_off_in));
-  }
+  if (stream || multi) GR_TRY(fbits.alloc((size_t)nV / 32 + 2));
+  if (stream) GR_TRY(lt.build(g->h_off_in));
   GR_TRY(d_src.alloc(n_sources));
   GR_TRY(d_src_user.alloc(n_sources));
-  GR_TRY(depth_layout.alloc((size_t)nV * n_sources));
+  // multi-kernel: [V][64] depths (no initialisation: the visit bits say which
+  // entries are valid); the single-launch kernels: [ns][V] set to -1 first
+  GR_TRY(depth_layout.alloc(multi ? (size_t)nV * kWave : (size_t)nV * n_sources));
   GR_TRY(ctr.alloc(1));
   GR_HIP(hipMemcpy(d_src_user.p, sources, sizeof(int32_t) * n_sources, hipMemcpyHostToDevice));
   k_map_ids<<<1, 64>>>(n_sources, d_src_user.p, user_g->inv, d_src.p);
   GR_HIP(hipGetLastError());
   // depth = -1 everywhere: the counterpart of the host-side vertex init before setGraph
-  GR_HIP(hipMemset(depth_layout.p, 0xFF, sizeof(int32_t) * (size_t)nV * n_sources));
+  if (!multi)
+    GR_HIP(hipMemset(depth_layout.p, 0xFF, sizeof(int32_t) * (size_t)nV * n_sources));
+  if (multi) {
+    DBuf<MsLevelCounters> lc;
+    GR_TRY(lc.alloc(1));
+    LaneTables in_chunks, out_chunks;  // heavy CSC rows / CSR rows
+    GR_TRY(in_chunks.build(g->h_off_in));
+    GR_TRY(out_chunks.build(g->h_off_out));
+    hipEvent_t e0, e1;
+    GR_HIP(hipEventCreate(&e0));
+    GR_HIP(hipEventCreate(&e1));
+    GR_HIP(hipDeviceSynchronize());
+    GR_HIP(hipEventRecord(e0, 0));
+    k_ms_init<<<1024, 256>>>(nV, visit.p, frontier.p, next.p, fbits.p);
+    k_ms_sources<<<1, 64>>>(n_sources, d_src.p, nV, visit.p, frontier.p, fbits.p,
+                            depth_layout.p, 1, g->off_out, ctr.p);
+    GR_HIP(hipGetLastError());
+    MsCounters hc0;
+    GR_HIP(hipMemcpy(&hc0, ctr.p, sizeof(hc0), hipMemcpyDeviceToHost));
+    unsigned long long nv_count = hc0.new_vertices[0], fe_count = hc0.frontier_edges[0];
+    const uint64_t all = n_sources >= 64 ? ~0ull : ((1ull << n_sources) - 1);
+    const long long threshold = (long long)((double)g->nE / (alpha > 0 ? alpha : 1.0));
+    const int grid = (int)std::min<int64_t>(((int64_t)nV + kBatchNT - 1) / kBatchNT, 32768);
+    int levels = 0;
+    for (int level = 0; level < nV + 1 && nv_count > 0; ++level) {
+      const bool pull = direction == 2 || (direction == 0 && (long long)fe_count > threshold);
+      if (pull) {
+        k_ms_pull<<<grid, kBatchNT>>>(nV, all, g->off_in, g->srcs, visit.p, frontier.p,
+                                      filter ? fbits.p : nullptr, next.p);
+        if (in_chunks.n_chunks)
+          k_ms_pull_chunks<<<grid_1d(in_chunks.n_chunks, kBatchNT / kWave), kBatchNT>>>(
+              in_chunks.chunks, in_chunks.n_chunks, all, g->srcs, visit.p, frontier.p, next.p);
+      } else {
+        k_ms_push<<<grid, kBatchNT>>>(nV, g->off_out, g->dsts, visit.p, frontier.p, fbits.p,
+                                      next.p);
+        if (out_chunks.n_chunks)
+          k_ms_push_chunks<<<grid_1d(out_chunks.n_chunks, kBatchNT / kWave), kBatchNT>>>(
+              out_chunks.chunks, out_chunks.n_chunks, g->dsts, visit.p, frontier.p, fbits.p,
+              next.p);
+      }
+      GR_HIP(hipMemsetAsync(lc.p, 0, sizeof(MsLevelCounters), 0));
+      k_ms_update<<<grid, kBatchNT>>>(nV, all, level, g->off_out, visit.p, frontier.p, next.p,
+                                      fbits.p, depth_layout.p, lc.p);
+      GR_HIP(hipGetLastError());
+      MsLevelCounters h;
+      GR_HIP(hipMemcpy(&h, lc.p, sizeof(h), hipMemcpyDeviceToHost));
+      nv_count = h.new_vertices;
+      fe_count = h.frontier_edges;
+      levels = level + 1;
+    }
+    k_ms_depth_out<<<(unsigned)std::min<int64_t>(((int64_t)nV + kWave - 1) / kWave, 65536), 256>>>(
+        nV, n_sources, depth_layout.p, visit.p, user_g->inv, depth_device);
+    GR_HIP(hipGetLastError());
+    GR_HIP(hipEventRecord(e1, 0));
+    GR_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    GR_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (result) {
+      result->levels = levels;
+      result->ms = ms;
+      result->grid = grid;
+    }
+    return GR_OK;
+  }
   const void* kernel = stream ? (const void*)k_msbfs_stream : (const void*)k_msbfs;
   int dev = 0, cus = 0, per_cu = 0;
   GR_HIP(hipGetDevice(&dev));
@@ -595,7 +937,7 @@ extern "C" int gr_msbfs(gr_graph* g, const int32_t* sources, int n_sources, int3
   GR_HIP(hipEventRecord(e0, 0));
   k_ms_init<<<1024, 256>>>(nV, visit.p, frontier.p, next.p, fbits.p);
   k_ms_sources<<<1, 64>>>(n_sources, d_src.p, nV, visit.p, frontier.p, fbits.p,
-                          depth_layout.p, g->off_out, ctr.p);
+                          depth_layout.p, 0, g->off_out, ctr.p);
   GR_HIP(hipGetLastError());
   int nVa = nV, ns = n_sources, mode = direction, max_levels = nV + 1;
   long long nE = g->nE;

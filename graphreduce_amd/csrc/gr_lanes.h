@@ -139,6 +139,7 @@ static __global__ __launch_bounds__(kLaneNT) void k_lane_sssp_light(
     const int32_t* __restrict__ w_csc, int32_t* __restrict__ dist,
     const uint64_t* __restrict__ chg_in, uint64_t* __restrict__ chg_out,
     unsigned long long* __restrict__ any, unsigned long long* __restrict__ count) {
+  __shared__ unsigned long long s_acc[kLaneNT / kWave], s_cnt[kLaneNT / kWave];
   const int lane = lane_id();
   const int wpb = blockDim.x / kWave;
   const int n_waves = gridDim.x * wpb;
@@ -159,8 +160,22 @@ static __global__ __launch_bounds__(kLaneNT) void k_lane_sssp_light(
     acc |= m;
     cnt += m ? 1 : 0;
   }
-  if (lane == 0 && acc) atomicOr(any, (unsigned long long)acc);
-  if (lane == 0 && cnt && count) atomicAdd(count, cnt);
+  // one atomic pair per workgroup: per-wave atomics on these two words
+  // serialise over hundreds of thousands of waves
+  if (lane == 0) {
+    s_acc[threadIdx.x / kWave] = acc;
+    s_cnt[threadIdx.x / kWave] = cnt;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long a = 0, c = 0;
+    for (int i = 0; i < wpb; ++i) {
+      a |= s_acc[i];
+      c += s_cnt[i];
+    }
+    if (a) atomicOr(any, a);
+    if (c && count) atomicAdd(count, c);
+  }
 }
 
 // Heavy vertices: one wave per chunk, merged with atomicMin.

@@ -295,13 +295,21 @@ typedef struct gr_batch_result {
   int32_t grid;   /* workgroups of the persistent (BFS) or round (SSSP) kernel */
 } gr_batch_result;
 
-/* BFS from sources[0..n) in ONE persistent cooperative launch with 64-bit
- * visit/frontier masks per vertex.  depth_device: device array n*V int32,
- * row s = depths from sources[s] (-1 = unreached), filled by this call.
- * direction: 0 automatic (pull when the frontier's out-edges exceed E/alpha),
- * 1 push, 2 pull; OR GR_MSBFS_STREAM_PULL to ask for the streaming pull
- * kernel, which is currently disabled (the call returns GR_ERR_INVALID). */
+/* BFS from sources[0..n) with 64-bit visit/frontier masks per vertex: one
+ * expand (push or pull) and one update launch per level.  depth_device:
+ * device array n*V int32, row s = depths from sources[s] (-1 = unreached),
+ * filled by this call.  direction: 0 automatic (pull when the frontier's
+ * out-edges exceed E/alpha), 1 push, 2 pull; OR-able options:
+ *   GR_MSBFS_PERSISTENT   every level in ONE cooperative launch (grid barriers)
+ *   GR_MSBFS_FILTER       pull first checks an in-neighbour's bit in the 1-bit
+ *                         frontier map before reading its 8-byte mask (at
+ *                         RMAT-24 slower: the hubs first in every row are
+ *                         mostly on the frontier)
+ *   GR_MSBFS_STREAM_PULL  the streaming pull kernel, currently disabled (the
+ *                         call returns GR_ERR_INVALID). */
 #define GR_MSBFS_STREAM_PULL 4
+#define GR_MSBFS_PERSISTENT 8
+#define GR_MSBFS_FILTER 16
 int gr_msbfs(gr_graph* g, const int32_t* sources, int n_sources, int32_t* depth_device,
              int direction, double alpha, gr_batch_result* result);
 

@@ -108,3 +108,23 @@ def test_batched_rejects_bad_sources(gpu):
     with pytest.raises(gr.GRError):
         gr.msbfs(g, [g.n_vertices])
     g.close()
+
+
+@pytest.mark.parametrize("direction", [0, 1, 2])
+def test_msbfs_variants_agree(gpu, direction):
+    """Multi-kernel (with and without the frontier-map filter) and the single
+    persistent launch give the same depths as the oracle."""
+    gr = gpu
+    g = gr.Graph.rmat(13, 16, seed=4)
+    n = g.n_vertices
+    s, d = g.edges()
+    hub, _ = g.max_out_degree_vertex()
+    srcs = _sources(n, 40, 7, extra=(hub,))
+    outs = [gr.msbfs(g, srcs, direction=direction)[0],
+            gr.msbfs(g, srcs, direction=direction, pull_filter=True)[0],
+            gr.msbfs(g, srcs, direction=direction, persistent=True)[0]]
+    for i, src in enumerate(srcs[:8]):
+        ref, _, _ = oracle.bfs(n, s, d, src)
+        assert np.array_equal(outs[0][i], ref)
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+    g.close()
