@@ -68,7 +68,16 @@ __global__ void k_srv_bfs_seed(uint64_t mask, LaneSeeds seeds, int nV, uint64_t*
   const int v = seeds.vertex[s];
   atomicOr((unsigned long long*)&visit[v], 1ull << s);
   atomicOr((unsigned long long*)&frontier[v], 1ull << s);
-  depth[(size_t)s * nV + v] = 0;
+  depth[(size_t)v * kLanes + s] = 0;
+}
+
+// BFS: depth column of the admitted lanes = -1 (vertex-major [V][64]).
+__global__ void k_srv_bfs_reset_depth(int nV, uint64_t mask, int32_t* __restrict__ depth) {
+  const int lane = lane_id();
+  const int wpb = blockDim.x / kWave;
+  const bool mine = (mask >> lane) & 1ull;
+  for (int v = blockIdx.x * wpb + threadIdx.x / kWave; v < nV; v += gridDim.x * wpb)
+    if (mine) depth[(size_t)v * kLanes + lane] = -1;
 }
 
 // Expand one level for every active lane (push when the frontier's out-edges
@@ -86,17 +95,19 @@ __global__ __launch_bounds__(kSrvNT) void k_srv_bfs_expand(
     const int v = base + lane;
     const bool valid = v < nV;
     if (!pull) {
-      const uint64_t f = valid ? frontier[v] : 0ull;
+      uint64_t f = valid ? frontier[v] : 0ull;
       int b = 0, e = 0;
       if (f) {
         b = off_out[v];
         e = off_out[v + 1];
+        if (e - b > kLaneHeavy) f = 0;  // chunked: k_srv_bfs_push_chunks
       }
       const bool light = (e - b) <= kSrvHeavy;
       if (f && light)
         for (int p = b; p < e; ++p) {
           const int u = dsts[p];
-          if (f & ~visit[u]) atomicOr((unsigned long long*)&next[u], (unsigned long long)f);
+          const uint64_t want = f & ~visit[u] & ~next[u];  // an atomic only for new bits
+          if (want) atomicOr((unsigned long long*)&next[u], (unsigned long long)want);
         }
       uint64_t heavy = __ballot(f && !light);
       while (heavy) {
@@ -107,15 +118,17 @@ __global__ __launch_bounds__(kSrvNT) void k_srv_bfs_expand(
                             __shfl((uint32_t)f, hl, kWave);
         for (int p = hb + lane; p < he; p += kWave) {
           const int u = dsts[p];
-          if (hf & ~visit[u]) atomicOr((unsigned long long*)&next[u], (unsigned long long)hf);
+          const uint64_t want = hf & ~visit[u] & ~next[u];
+          if (want) atomicOr((unsigned long long*)&next[u], (unsigned long long)want);
         }
       }
     } else {
-      const uint64_t need = valid ? (active & ~visit[v]) : 0ull;
+      uint64_t need = valid ? (active & ~visit[v]) : 0ull;
       int b = 0, e = 0;
       if (need) {
         b = off_in[v];
         e = off_in[v + 1];
+        if (e - b > kLaneHeavy) need = 0;  // chunked: k_srv_bfs_pull_chunks (atomicOr)
       }
       const bool light = (e - b) <= kSrvHeavy;
       if (need && light) {
@@ -158,34 +171,93 @@ struct SrvCounters {
   unsigned long long checksum[kLanes];
 };
 
+// Vertices with more than kLaneHeavy edges: chunks of kLaneChunk edges spread
+// over all waves (one wave per hub would be the critical path of a step).
+__global__ __launch_bounds__(kSrvNT) void k_srv_bfs_push_chunks(
+    const LaneChunk* __restrict__ chunks, int n_chunks, const int32_t* __restrict__ dsts,
+    const uint64_t* __restrict__ visit, const uint64_t* __restrict__ frontier,
+    uint64_t* __restrict__ next) {
+  const int lane = lane_id();
+  const int wpb = blockDim.x / kWave;
+  for (int c = blockIdx.x * wpb + threadIdx.x / kWave; c < n_chunks; c += gridDim.x * wpb) {
+    const LaneChunk ch = chunks[c];
+    const uint64_t f = frontier[ch.v];
+    if (!f) continue;
+    for (int p = ch.b + lane; p < ch.e; p += kWave) {
+      const int u = dsts[p];
+      const uint64_t want = f & ~visit[u] & ~next[u];
+      if (want) atomicOr((unsigned long long*)&next[u], (unsigned long long)want);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kSrvNT) void k_srv_bfs_pull_chunks(
+    const LaneChunk* __restrict__ chunks, int n_chunks, uint64_t active,
+    const int32_t* __restrict__ srcs, const uint64_t* __restrict__ visit,
+    const uint64_t* __restrict__ frontier, uint64_t* __restrict__ next) {
+  const int lane = lane_id();
+  const int wpb = blockDim.x / kWave;
+  for (int c = blockIdx.x * wpb + threadIdx.x / kWave; c < n_chunks; c += gridDim.x * wpb) {
+    const LaneChunk ch = chunks[c];
+    const uint64_t need = active & ~visit[ch.v];
+    if (!need) continue;
+    uint64_t acc = 0;
+    for (int pb = ch.b; pb < ch.e; pb += kWave) {  // wave-uniform trip count
+      const int p = pb + lane;
+      uint64_t m = p < ch.e ? frontier[srcs[p]] : 0ull;
+      uint32_t lo = (uint32_t)m, hi = (uint32_t)(m >> 32);
+#pragma unroll
+      for (int off = kWave / 2; off > 0; off >>= 1) {
+        lo |= __shfl_xor(lo, off, kWave);
+        hi |= __shfl_xor(hi, off, kWave);
+      }
+      acc |= ((uint64_t)hi << 32) | lo;
+      if ((acc & need) == need) break;
+    }
+    if (lane == 0 && (acc & need))
+      atomicOr((unsigned long long*)&next[ch.v], (unsigned long long)(acc & need));
+  }
+}
+
+// New (lane, vertex) pairs of the level: visit/frontier, depth (vertex-major
+// [V][64], one masked row store per vertex with new lanes), and the step's
+// counters summed per workgroup (one atomic pair per workgroup).
 __global__ __launch_bounds__(kSrvNT) void k_srv_bfs_update(
     int nV, int step, uint64_t active, LaneSeeds seeds, const int32_t* __restrict__ off_out,
     uint64_t* __restrict__ visit, uint64_t* __restrict__ frontier, uint64_t* __restrict__ next,
     int32_t* __restrict__ depth, SrvCounters* __restrict__ ctr) {
+  __shared__ unsigned long long s_or[kSrvNT / kWave], s_fe[kSrvNT / kWave];
   const int lane = lane_id();
   const int wpb = blockDim.x / kWave;
   const int n_waves = gridDim.x * wpb;
+  const int my_depth_base = step + 1 - seeds.start[lane];  // depth of this lane's new vertices
   uint64_t f_or = 0;
   unsigned long long fe = 0;
   for (int base = (blockIdx.x * wpb + threadIdx.x / kWave) * kWave; base < nV;
        base += n_waves * kWave) {
     const int v = base + lane;
-    if (v >= nV) continue;
-    const uint64_t vis = visit[v];
-    const uint64_t nw = next[v] & ~vis & active;
-    next[v] = 0;
-    // lanes that are not active keep no frontier
-    frontier[v] = nw;
-    if (nw) {
-      visit[v] = vis | nw;
-      f_or |= nw;
-      fe += (unsigned long long)(off_out[v + 1] - off_out[v]);
-      uint64_t bits = nw;
-      while (bits) {
-        const int s = __builtin_ctzll(bits);
-        bits &= bits - 1;
-        depth[(size_t)s * nV + v] = step + 1 - seeds.start[s];
+    uint64_t nw = 0;
+    if (v < nV) {
+      const uint64_t vis = visit[v];
+      const uint64_t nxt = next[v];
+      nw = nxt & ~vis & active;
+      if (nxt) next[v] = 0;
+      // lanes that are not active keep no frontier
+      if (nw || frontier[v]) frontier[v] = nw;
+      if (nw) {
+        visit[v] = vis | nw;
+        f_or |= nw;
+        fe += (unsigned long long)(off_out[v + 1] - off_out[v]);
       }
+    }
+    uint64_t rows = __ballot(nw != 0);
+    while (rows) {
+      const int r = __builtin_ctzll(rows);
+      rows &= rows - 1;
+      const uint32_t lo = __shfl((uint32_t)nw, r, kWave);
+      const uint32_t hi = __shfl((uint32_t)(nw >> 32), r, kWave);
+      const uint64_t m = ((uint64_t)hi << 32) | lo;
+      if ((m >> lane) & 1ull) depth[(size_t)(base + r) * kLanes + lane] = my_depth_base;
     }
   }
   uint32_t lo = (uint32_t)f_or, hi = (uint32_t)(f_or >> 32);
@@ -196,9 +268,18 @@ __global__ __launch_bounds__(kSrvNT) void k_srv_bfs_update(
     fe += __shfl_xor(fe, off, kWave);
   }
   if (lane == 0) {
-    const uint64_t o = ((uint64_t)hi << 32) | lo;
-    if (o) atomicOr(&ctr->frontier_or, (unsigned long long)o);
-    if (fe) atomicAdd(&ctr->frontier_edges, fe);
+    s_or[threadIdx.x / kWave] = ((uint64_t)hi << 32) | lo;
+    s_fe[threadIdx.x / kWave] = fe;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long o = 0, e = 0;
+    for (int i = 0; i < wpb; ++i) {
+      o |= s_or[i];
+      e += s_fe[i];
+    }
+    if (o) atomicOr(&ctr->frontier_or, o);
+    if (e) atomicAdd(&ctr->frontier_edges, e);
   }
 }
 
@@ -419,6 +500,7 @@ struct gr_server {
   int32_t* w_csc = nullptr;
   gr::Pool pools[3];
   gr::LaneTables lanes;       // heavy-vertex chunks of the CSC
+  gr::LaneTables out_lanes;   // ... of the CSR (BFS push)
   float* ppr_partial = nullptr;  // [chunks][64]
   std::mutex mu;
   std::condition_variable cv;
@@ -544,9 +626,7 @@ int launch_admit(gr_server* sv, Pool& p, uint64_t mask) {
   const int nV = sv->g->nV;
   if (p.kind == GR_QUERY_BFS) {
     k_srv_bfs_clear<<<grid_1d(nV, 256), 256, 0, p.stream>>>(nV, ~mask, p.m0, p.m1, p.m2);
-    for (int s = 0; s < kLanes; ++s)
-      if ((mask >> s) & 1ull)
-        GR_HIP(hipMemsetAsync(p.i32 + (size_t)s * nV, 0xFF, 4 * (size_t)nV, p.stream));
+    k_srv_bfs_reset_depth<<<grid_v(nV), kSrvNT, 0, p.stream>>>(nV, mask, p.i32);
     k_srv_bfs_seed<<<1, kLanes, 0, p.stream>>>(mask, p.seeds, nV, p.m0, p.m1, p.i32);
   } else if (p.kind == GR_QUERY_SSSP) {
     k_srv_sssp_reset<<<grid_v(nV), kSrvNT, 0, p.stream>>>(nV, mask, 2147483647 - 100000, p.i32,
@@ -570,6 +650,13 @@ int launch_step(gr_server* sv, Pool& p) {
     const int pull = (long long)p.last_fe > thr;
     k_srv_bfs_expand<<<grid_1d(nV, kSrvNT), kSrvNT, 0, p.stream>>>(
         nV, pull, p.active, g->off_out, g->dsts, g->off_in, g->srcs, p.m0, p.m1, p.m2);
+    if (pull && sv->lanes.n_chunks)
+      k_srv_bfs_pull_chunks<<<grid_1d(sv->lanes.n_chunks, kSrvNT / kWave), kSrvNT, 0, p.stream>>>(
+          sv->lanes.chunks, sv->lanes.n_chunks, p.active, g->srcs, p.m0, p.m1, p.m2);
+    if (!pull && sv->out_lanes.n_chunks)
+      k_srv_bfs_push_chunks<<<grid_1d(sv->out_lanes.n_chunks, kSrvNT / kWave), kSrvNT, 0,
+                              p.stream>>>(sv->out_lanes.chunks, sv->out_lanes.n_chunks, g->dsts,
+                                          p.m0, p.m1, p.m2);
     k_srv_bfs_update<<<grid_1d(nV, kSrvNT), kSrvNT, 0, p.stream>>>(
         nV, p.step, p.active, p.seeds, g->off_out, p.m0, p.m1, p.m2, p.i32, p.ctr);
   } else if (p.kind == GR_QUERY_SSSP) {
@@ -625,10 +712,7 @@ int retire(gr_server* sv, Pool& p, uint64_t done) {
   if (!done) return GR_OK;
   const int nV = sv->g->nV;
   GR_HIP(hipMemsetAsync(p.ctr, 0, sizeof(SrvCounters), p.stream));
-  if (p.kind == GR_QUERY_BFS)
-    k_srv_checksum<int32_t><<<1024, 256, 0, p.stream>>>(nV, kLanes, done, p.i32, 1, sv->perm,
-                                                         p.ctr);
-  else if (p.kind == GR_QUERY_SSSP)
+  if (p.kind == GR_QUERY_BFS || p.kind == GR_QUERY_SSSP)
     k_srv_checksum_lanes<int32_t><<<grid_v(nV), kSrvNT, 0, p.stream>>>(nV, done, p.i32, sv->perm,
                                                                         p.ctr);
   else
@@ -651,9 +735,7 @@ int retire(gr_server* sv, Pool& p, uint64_t done) {
     if (keep) {
       int32_t* scratch = nullptr;
       GR_TRY(alloc_dev((void**)&scratch, 4 * (size_t)nV));
-      if (p.kind == GR_QUERY_BFS)
-        k_srv_extract<int32_t><<<1024, 256, 0, p.stream>>>(nV, s, p.i32, 1, sv->inv, scratch);
-      else if (p.kind == GR_QUERY_SSSP)
+      if (p.kind == GR_QUERY_BFS || p.kind == GR_QUERY_SSSP)
         k_srv_extract<int32_t><<<1024, 256, 0, p.stream>>>(nV, s, p.i32, 0, sv->inv, scratch);
       else
         k_srv_extract<float><<<1024, 256, 0, p.stream>>>(nV, s, p.f0, 0, sv->inv,
@@ -752,6 +834,7 @@ extern "C" int gr_server_create(gr_graph* g, const int32_t* weights_device,
     }
   }
   if (rc == GR_OK) rc = sv->lanes.build(g->h_off_in);
+  if (rc == GR_OK) rc = sv->out_lanes.build(g->h_off_out);
   if (rc == GR_OK)
     rc = alloc_dev((void**)&sv->ppr_partial, sizeof(float) * kLanes * ((size_t)sv->lanes.n_chunks + 1));
   for (int k = 0; k < 3 && rc == GR_OK; ++k) rc = pool_init(sv, sv->pools[k], k + 1);
