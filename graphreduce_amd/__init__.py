@@ -77,6 +77,8 @@ class _Stats(C.Structure):
         ("push_ms", C.c_double),
         ("tile_gather_rows", C.c_int64),
         ("tile_gather_edges", C.c_int64),
+        ("inert_gathers", C.c_int64),
+        ("inert_ms", C.c_double),
     ]
 
 

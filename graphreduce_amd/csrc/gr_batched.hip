@@ -748,16 +748,28 @@ __global__ void k_mssp_sources(int ns, const int32_t* __restrict__ sources,
   }
 }
 
-// dist (vertex-major, 64 lanes, layout ids) -> out[s * nV + u] in the
-// caller's ids: u's layout id is inv[u] (inv nullptr: the same)
-__global__ void k_mssp_transpose(int nV, int ns, const int32_t* __restrict__ dist,
-                                 const int32_t* __restrict__ inv, int32_t* __restrict__ out) {
-  const size_t n = (size_t)nV * ns;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const size_t s = i / nV, u = i % nV;
-    const size_t v = inv ? (size_t)inv[u] : u;
-    out[i] = dist[v * kWave + s];
+// dist (vertex-major [V][64], layout ids) -> out[s * nV + u] in the caller's
+// ids (u's layout id is inv[u]): a workgroup moves 64 consecutive caller ids
+// through an LDS tile, reading 256-B vertex rows and writing 64 consecutive
+// values per source.
+__global__ __launch_bounds__(256) void k_mssp_transpose(int nV, int ns,
+                                                        const int32_t* __restrict__ dist,
+                                                        const int32_t* __restrict__ inv,
+                                                        int32_t* __restrict__ out) {
+  __shared__ int32_t tile[kWave][kWave + 1];
+  const int lane = lane_id();
+  const int w = threadIdx.x / kWave;
+  for (int u0 = blockIdx.x * kWave; u0 < nV; u0 += gridDim.x * kWave) {
+    for (int r = w; r < kWave; r += 4) {
+      const int u = u0 + r;
+      if (u < nV) tile[r][lane] = dist[(size_t)(inv ? inv[u] : u) * kWave + lane];
+    }
+    __syncthreads();
+    for (int s = w; s < ns; s += 4) {
+      const int u = u0 + lane;
+      if (u < nV) out[(size_t)s * nV + u] = tile[lane][s];
+    }
+    __syncthreads();
   }
 }
 
@@ -1026,11 +1038,14 @@ extern "C" int gr_mssp(gr_graph* g, const int32_t* weights_device, const int32_t
   for (int r = 0; r < kRounds; ++r) {
     uint64_t* cin = (r & 1) ? chg_b.p : chg_a.p;
     uint64_t* cout = (r & 1) ? chg_a.p : chg_b.p;
+    // a round after a round without change returns at once (exact: nothing
+    // changed feeds it), so the rest of a replay after convergence is free
+    const unsigned long long* prev = r ? cnt.p + (r - 1) : nullptr;
     k_lane_sssp_light<<<rgrid < 65536 ? rgrid : 65536, kLaneNT, 0, st>>>(
-        nV, g->off_in, g->srcs, w_csc.p, dist.p, cin, cout, any.p, cnt.p + r);
+        nV, g->off_in, g->srcs, w_csc.p, dist.p, cin, cout, any.p, cnt.p + r, prev);
     if (lt.n_chunks)
       k_lane_sssp_heavy<<<hgrid, kLaneNT, 0, st>>>(lt.chunks, lt.n_chunks, g->srcs, w_csc.p,
-                                                   dist.p, cin, cout, any.p, cnt.p + r);
+                                                   dist.p, cin, cout, any.p, cnt.p + r, prev);
   }
   GR_HIP(hipStreamEndCapture(st, &graph));
   GR_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
@@ -1052,10 +1067,9 @@ extern "C" int gr_mssp(gr_graph* g, const int32_t* weights_device, const int32_t
     }
     rounds += kRounds;
   }
-  if (true) {
-    k_mssp_transpose<<<4096, 256, 0, st>>>(nV, n_sources, dist.p, user_g->inv, dist_device);
-    GR_HIP(hipGetLastError());
-  }
+  k_mssp_transpose<<<(unsigned)std::min<int64_t>(((int64_t)nV + kWave - 1) / kWave, 65536), 256, 0,
+                     st>>>(nV, n_sources, dist.p, user_g->inv, dist_device);
+  GR_HIP(hipGetLastError());
   GR_HIP(hipEventRecord(e1, st));
   GR_HIP(hipEventSynchronize(e1));
   float ms = 0;

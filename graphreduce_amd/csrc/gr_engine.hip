@@ -137,6 +137,80 @@ __global__ __launch_bounds__(kApplyNT) void k_apply(
   }
 }
 
+// Vertex scan for a gather without a previous changed set, for programs with
+// inert sources: writes the bitmap of the non-inert vertices (every word < 
+// n_words, so nothing needs clearing first) and appends the non-inert ones
+// with out-edges to the changed list as k_apply does (one atomic per chunk,
+// out-edge offsets increasing with the list index), counted in ctr->inert.
+template <class P>
+__global__ __launch_bounds__(kApplyNT) void k_inert_scan(
+    int nV, int n_words, const typename P::VertexData* __restrict__ vd,
+    const int32_t* __restrict__ off_out, int32_t* __restrict__ chg_v, int32_t* __restrict__ chg_e,
+    uint32_t* __restrict__ bits, DevCounters* __restrict__ ctr) {
+  __shared__ int s_a[kApplyNT / kWave], s_b[kApplyNT / kWave];
+  __shared__ unsigned long long s_base;
+  const int tid = threadIdx.x, lane = lane_id();
+  for (int chunk = blockIdx.x * kApplyChunk; chunk < nV; chunk += gridDim.x * kApplyChunk) {
+    int degs[kApplyVT];
+    unsigned want_mask = 0;
+    int my_cnt = 0, my_edges = 0;
+#pragma unroll
+    for (int m = 0; m < kApplyVT; ++m) {
+      // a wave covers the 64 consecutive vertices from v - lane (a multiple of 64)
+      const int v = chunk + m * kApplyNT + tid;
+      bool live = false;
+      int deg = 0;
+      if (v < nV) {
+        live = !P::inertSource(vd[v]);
+        if (live) deg = off_out[v + 1] - off_out[v];
+      }
+      const uint64_t word2 = __ballot(live);
+      const int w0 = (v - lane) >> 5;
+      if (lane == 0 && w0 < n_words) bits[w0] = (uint32_t)word2;
+      if (lane == 32 && w0 + 1 < n_words) bits[w0 + 1] = (uint32_t)(word2 >> 32);
+      degs[m] = deg;
+      if (live && deg > 0) {
+        want_mask |= 1u << m;
+        my_cnt += 1;
+        my_edges += deg;
+      }
+    }
+    int tc, te;
+    const int ec = block_excl_scan_add<kApplyNT>(my_cnt, s_a, &tc);
+    const int ee = block_excl_scan_add<kApplyNT>(my_edges, s_b, &te);
+    if (tc > 0) {
+      if (tid == 0)
+        s_base = atomicAdd(&ctr->inert, ((unsigned long long)tc << 32) |
+                                            (unsigned long long)(uint32_t)te);
+      __syncthreads();
+      const unsigned long long base = s_base;
+      int idx = (int)(base >> 32) + ec;
+      int eo = (int)(uint32_t)base + ee;
+#pragma unroll
+      for (int m = 0; m < kApplyVT; ++m) {
+        if (want_mask & (1u << m)) {
+          chg_v[idx] = chunk + m * kApplyNT + tid;
+          chg_e[idx] = eo;
+          ++idx;
+          eo += degs[m];
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// *bad != 0 when some edge's data fails P::edgeKeepsInert.
+template <class P>
+__global__ void k_check_inert_edges(int64_t E, const typename P::EdgeData* __restrict__ ed,
+                                    unsigned int* __restrict__ bad) {
+  bool b = false;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < E;
+       i += (int64_t)gridDim.x * blockDim.x)
+    b = b || !P::edgeKeepsInert(ed[i]);
+  if (__ballot(b) && lane_id() == 0) atomicOr(bad, 1u);
+}
+
 // ------------------------------------------------------- scatter: push
 
 // Direction of scatterActivate, decided on the device from the changed
@@ -655,9 +729,14 @@ struct Engine final : EngineBase {
   // 2.5 at 0.94); the pull-delta gather is never the fastest there.
   double changed_share = 1.0;
   static constexpr double kPushMaxChangedShare = 0.6;
+  // Inert sources (programs with kInertSources): with every bound edge passing
+  // P::edgeKeepsInert, a gather without a previous changed set reads only the
+  // non-inert sources (SSSP's first all-active gather: the start vertex).
+  static constexpr bool kInert = kDelta && inert_sources_v<P> && !kEdgeEmpty;
+  bool inert_ok = false;
   bool it_gather = false, it_apply = false, it_scatter = false;
   // hipEvent start/stop pairs per timed phase
-  enum { kPhGather = 0, kPhScatter, kPhApply, kPhNext, kPhPush, kPhases };
+  enum { kPhGather = 0, kPhScatter, kPhApply, kPhNext, kPhPush, kPhInert, kPhases };
   std::vector<hipEvent_t> ev[kPhases];
   hipEvent_t ev_open = nullptr;
 
@@ -808,6 +887,21 @@ struct Engine final : EngineBase {
         if (tmp) (void)hipFree(tmp);
       }
     }
+    inert_ok = false;
+    if constexpr (kInert) {
+      unsigned int* d_bad = nullptr;
+      unsigned int h_bad = 1;
+      GR_TRY(dalloc(&d_bad, 1));
+      GR_HIP(hipMemsetAsync(d_bad, 0, sizeof(unsigned int), stream));
+      if (E > 0)
+        k_check_inert_edges<P><<<grid_1d(E, 256 * 16) < 8192 ? grid_1d(E, 256 * 16) : 8192, 256, 0,
+                                 stream>>>(E, d_ed, d_bad);
+      GR_HIP(hipGetLastError());
+      GR_HIP(hipMemcpyAsync(&h_bad, d_bad, sizeof(unsigned int), hipMemcpyDeviceToHost, stream));
+      GR_HIP(hipStreamSynchronize(stream));
+      (void)hipFree(d_bad);
+      inert_ok = h_bad == 0;
+    }
     if constexpr (kDelta) {
       GR_TRY(dalloc(&d_cand, V));
       k_fill<G><<<1024, 256, 0, stream>>>(V, P::gatherZero, d_cand);
@@ -938,12 +1032,25 @@ struct Engine final : EngineBase {
     it_gather = true;
     if (!P::kHasGather || n_active == 0) return GR_OK;
     const int n = n_active, w = w_edges;
+    const uint32_t* dbits = nullptr;
     if constexpr (kDelta) {
-      if (delta_ready &&
-          (delta_mode == 3 || (delta_mode == 1 && changed_share < kPushMaxChangedShare)))
-        return gather_push();
+      if (delta_ready) {
+        if (delta_mode == 3 || (delta_mode == 1 && changed_share < kPushMaxChangedShare))
+          return gather_push();
+        if (delta_mode == 2) dbits = d_chg_prev;
+      } else if (kInert && inert_ok && delta_mode != 0) {
+        // The non-inert vertices stand in for the changed set: the gather's
+        // values are those of the full gather, so an all-active start still
+        // establishes the delta invariant.  Pushing needs every vertex active
+        // (a pushed candidate is collected and reset by its active row only).
+        GR_TRY(inert_scan());
+        const double share = w > 0 ? (double)chg_edges / (double)w : 1.0;
+        if (n == g->nV &&
+            (delta_mode == 3 || (delta_mode == 1 && share < kPushMaxChangedShare)))
+          return gather_push();
+        if (delta_mode != 1 || share < kPushMaxChangedShare) dbits = d_chg_prev;
+      }
     }
-    const uint32_t* dbits = (kDelta && delta_ready && delta_mode == 2) ? d_chg_prev : nullptr;
     const int tile = kNT * gather_vt;
     const int n_tiles = (int)(((int64_t)n + w + tile - 1) / tile);
     k_merge_partition<<<grid_1d((int64_t)n_tiles + 1, 256), 256, 0, stream>>>(d_escan, n, w,
@@ -966,7 +1073,27 @@ struct Engine final : EngineBase {
     stats.gathered_edges += w;
     stats.tile_gather_rows += n;
     stats.tile_gather_edges += w;
-    if (dbits) stats.delta_gathers += 1;
+    if (dbits && delta_ready) stats.delta_gathers += 1;
+    return GR_OK;
+  }
+
+  // Bitmap (in d_chg_prev) and list (d_chg_v / d_chg_e, chg_count / chg_edges)
+  // of the non-inert vertices.
+  int inert_scan() {
+    if constexpr (kInert) {
+      tick();
+      GR_HIP(hipMemsetAsync(&d_ctr->inert, 0, sizeof(unsigned long long), stream));
+      k_inert_scan<P><<<grid_1d(g->nV, kApplyChunk), kApplyNT, 0, stream>>>(
+          g->nV, n_words, d_vd, g->off_out, d_chg_v, d_chg_e, d_chg_prev, d_ctr);
+      GR_HIP(hipGetLastError());
+      GR_HIP(hipMemcpyAsync(&h_ctr->inert, &d_ctr->inert, sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, stream));
+      tock(kPhInert);
+      GR_HIP(hipStreamSynchronize(stream));
+      chg_count = (int64_t)(h_ctr->inert >> 32);
+      chg_edges = (int64_t)(uint32_t)h_ctr->inert;
+      stats.inert_gathers += 1;
+    }
     return GR_OK;
   }
 
@@ -989,7 +1116,7 @@ struct Engine final : EngineBase {
     GR_HIP(hipGetLastError());
     stats.active_vertices += n;
     stats.gathered_edges += w_edges;
-    stats.delta_gathers += 1;
+    if (delta_ready) stats.delta_gathers += 1;  // else the list holds the non-inert vertices
     stats.push_gathers += 1;
     stats.push_edges += w_c;
     return GR_OK;
@@ -1119,8 +1246,9 @@ struct Engine final : EngineBase {
     sum_pairs(ev[kPhApply], &stats.apply_launches, &stats.apply_ms);
     sum_pairs(ev[kPhNext], &stats.next_launches, &stats.next_ms);
     {
-      int64_t pushes = 0;
+      int64_t pushes = 0, scans = 0;
       sum_pairs(ev[kPhPush], &pushes, &stats.push_ms);
+      sum_pairs(ev[kPhInert], &scans, &stats.inert_ms);
     }
     if (out) *out = stats;
     return GR_OK;

@@ -33,7 +33,8 @@ struct DevCounters {
   unsigned long long chg;         // (#changed << 32) | out-edges, appended by apply
   unsigned long long chg_last;    // chg as seen by the last nextIter
   int32_t pull;                   // last scatterActivate ran in pull mode
-  int32_t pad[3];
+  int32_t pad;
+  unsigned long long inert;       // (#non-inert << 32) | out-edges, from k_inert_scan
 };
 
 template <class T>
@@ -141,6 +142,16 @@ struct delta_gather<P, std::void_t<decltype(P::kDeltaGather)>>
     : std::integral_constant<bool, P::kDeltaGather> {};
 template <class P>
 constexpr bool delta_gather_v = delta_gather<P>::value;
+
+// Programs with kInertSources name vertex values whose gatherMap contribution
+// gatherZero absorbs on every edge whose data passes P::edgeKeepsInert.
+template <class P, class = void>
+struct inert_sources : std::false_type {};
+template <class P>
+struct inert_sources<P, std::void_t<decltype(P::kInertSources)>>
+    : std::integral_constant<bool, P::kInertSources> {};
+template <class P>
+constexpr bool inert_sources_v = inert_sources<P>::value;
 
 template <class P>
 __device__ __forceinline__ typename P::GatherResult finalize(int has,

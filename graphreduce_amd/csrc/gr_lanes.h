@@ -134,11 +134,16 @@ __device__ __forceinline__ int32_t lane_min_in_edges(int b, int e, const int32_t
 // One relaxation round over the light vertices (in-degree <= kLaneHeavy).
 // chg_out[v] = lanes whose distance at v dropped; heavy vertices get 0 here
 // and their bits from k_lane_sssp_heavy.  *any |= lanes changed anywhere.
+// prev_count (may be nullptr): the previous round's changed count; when it is
+// 0 nothing can change in this round and the kernel returns at once (chg_out
+// is then not rewritten: the caller stops after such a round).
 static __global__ __launch_bounds__(kLaneNT) void k_lane_sssp_light(
     int nV, const int32_t* __restrict__ off_in, const int32_t* __restrict__ srcs,
     const int32_t* __restrict__ w_csc, int32_t* __restrict__ dist,
     const uint64_t* __restrict__ chg_in, uint64_t* __restrict__ chg_out,
-    unsigned long long* __restrict__ any, unsigned long long* __restrict__ count) {
+    unsigned long long* __restrict__ any, unsigned long long* __restrict__ count,
+    const unsigned long long* __restrict__ prev_count) {
+  if (prev_count && *prev_count == 0) return;
   __shared__ unsigned long long s_acc[kLaneNT / kWave], s_cnt[kLaneNT / kWave];
   const int lane = lane_id();
   const int wpb = blockDim.x / kWave;
@@ -183,7 +188,9 @@ static __global__ __launch_bounds__(kLaneNT) void k_lane_sssp_heavy(
     const LaneChunk* __restrict__ chunks, int n_chunks, const int32_t* __restrict__ srcs,
     const int32_t* __restrict__ w_csc, int32_t* __restrict__ dist,
     const uint64_t* __restrict__ chg_in, uint64_t* __restrict__ chg_out,
-    unsigned long long* __restrict__ any, unsigned long long* __restrict__ count) {
+    unsigned long long* __restrict__ any, unsigned long long* __restrict__ count,
+    const unsigned long long* __restrict__ prev_count) {
+  if (prev_count && *prev_count == 0) return;
   const int lane = lane_id();
   const int wpb = blockDim.x / kWave;
   const int n_waves = gridDim.x * wpb;

@@ -65,6 +65,13 @@ struct SSSPProgram {
   static constexpr bool kHasScatter = false;
   static constexpr bool kGatherUsesDst = false;
   static constexpr bool kDeltaGather = true;
+  // An unreached source (distance gatherZero, the sssp.cu:45-52 initial
+  // value) maps to gatherZero + w on an edge of weight w in [0, maxLength]:
+  // at least gatherZero and without overflow, so min() with gatherZero
+  // absorbs it and it can lower no distance.
+  static constexpr bool kInertSources = true;
+  __device__ static bool inertSource(const VertexData& d) { return d == gatherZero; }
+  __device__ static bool edgeKeepsInert(const EdgeData& w) { return w >= 0 && w <= maxLength; }
 
   __device__ static int gatherReduce(const int& left, const int& right) {
     return min(left, right);

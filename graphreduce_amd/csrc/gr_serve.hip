@@ -664,10 +664,12 @@ int launch_step(gr_server* sv, Pool& p) {
     uint64_t* cout = p.parity ? p.m0 : p.m1;
     unsigned long long* any = &p.ctr->frontier_or;
     k_lane_sssp_light<<<grid_v(nV), kLaneNT, 0, p.stream>>>(nV, g->off_in, g->srcs, sv->w_csc,
-                                                            p.i32, cin, cout, any, nullptr);
+                                                            p.i32, cin, cout, any, nullptr,
+                                                            nullptr);
     if (sv->lanes.n_chunks)
       k_lane_sssp_heavy<<<grid_1d(sv->lanes.n_chunks, kLaneNT / kWave), kLaneNT, 0, p.stream>>>(
-          sv->lanes.chunks, sv->lanes.n_chunks, g->srcs, sv->w_csc, p.i32, cin, cout, any, nullptr);
+          sv->lanes.chunks, sv->lanes.n_chunks, g->srcs, sv->w_csc, p.i32, cin, cout, any, nullptr,
+          nullptr);
     p.parity ^= 1;
   } else {
     const uint64_t* nzi = p.parity ? p.m1 : p.m0;

@@ -254,11 +254,16 @@ typedef struct gr_engine_stats {
   double next_ms;
   int64_t delta_gathers;     /* gathers that read only sources changed in the
                                 previous iteration (see gr_engine_set_delta_gather) */
-  int64_t push_gathers;      /* ... of which done by pushing the changed out-edges */
+  int64_t push_gathers;      /* gathers done by pushing the out-edges of the changed
+                                (or non-inert) vertices */
   int64_t push_edges;        /* out-edges relaxed by those pushes */
   double push_ms;            /* their summed duration (not in gather_ms) */
   int64_t tile_gather_rows;  /* sum of #A over gathers by the merge-path tile kernel */
   int64_t tile_gather_edges; /* sum of E_in(A) over those gathers */
+  int64_t inert_gathers;     /* first gathers that skipped inert sources (SSSP:
+                                unreached vertices, weights in [0, 100000]) */
+  double inert_ms;           /* their vertex scans (the gather itself is in
+                                gather_ms or push_ms) */
 } gr_engine_stats;
 
 /* Enables hipEvent timing of the engine phases on the engine's stream. */

@@ -17,12 +17,13 @@ SSSP_ZERO = 2**31 - 1 - 100000  # sssp.cu gatherZero
 CC_ZERO = 2**31 - 1             # connected_component.cu gatherZero
 
 
-def _delta_loop(n, s, d, vals, mapped, zero):
+def _delta_loop(n, s, d, vals, mapped, zero, inert=None):
     """Returns (values, trips) of the loop with the delta rule; `mapped(vals)`
-    gives gatherMap per edge-list position."""
+    gives gatherMap per edge-list position.  With `inert`, the first gather
+    substitutes gatherZero for the sources whose value is `inert`."""
     vals = vals.astype(np.int64).copy()
     active = np.ones(n, bool)
-    changed_prev = None
+    changed_prev = None if inert is None else vals != inert
     trips = 0
     while active.any():
         trips += 1
@@ -68,6 +69,34 @@ def test_delta_rule_sssp_matches_oracle(name, n, s, d):
     got, trips = _delta_loop(n, s, d, init, lambda v: v[s] + w, SSSP_ZERO)
     assert np.array_equal(got, ref)
     assert trips == ref_trips
+
+
+@pytest.mark.parametrize("name,n,s,d", _graphs(), ids=lambda x: x if isinstance(x, str) else "")
+def test_inert_first_gather_sssp_matches_oracle(name, n, s, d):
+    """The first gather reads only sources off gatherZero (weights in
+    [0, 100000]: an unreached source maps to at least gatherZero)."""
+    w = gr.hash_weights(len(s), seed=12, lo=0, span_log2=4)
+    w[::7] = 100000  # the largest weight that keeps gatherZero + w in range
+    src = int(np.argmax(np.bincount(s, minlength=n)))
+    ref, ref_trips, _ = oracle.sssp(n, s, d, w, src)
+    init = np.full(n, SSSP_ZERO, np.int64)
+    init[src] = 0
+    got, trips = _delta_loop(n, s, d, init, lambda v: v[s] + w, SSSP_ZERO, inert=SSSP_ZERO)
+    assert np.array_equal(got, ref)
+    assert trips == ref_trips
+
+
+def test_inert_rule_needs_the_weight_bound():
+    """A negative weight lets an unreached source lower its target below
+    gatherZero in the full gather; skipping inert sources would miss that."""
+    s, d, w = np.array([0, 1]), np.array([1, 2]), np.array([5, -1])
+    init = np.array([0, SSSP_ZERO, SSSP_ZERO], np.int64)
+    full, _ = _delta_loop(3, s, d, init, lambda v: v[s] + w, SSSP_ZERO)
+    skip, _ = _delta_loop(3, s, d, init, lambda v: v[s] + w, SSSP_ZERO, inert=SSSP_ZERO)
+    assert full[2] == 4 and skip[2] == 4  # both reach the fixed point ...
+    g = np.full(3, SSSP_ZERO, np.int64)  # ... but the first full gather differs:
+    np.minimum.at(g, d, init[s] + w)
+    assert g[2] == SSSP_ZERO - 1  # vertex 2 changes in trip 1 of the full gather only
 
 
 @pytest.mark.parametrize("name,n,s,d", _graphs(), ids=lambda x: x if isinstance(x, str) else "")

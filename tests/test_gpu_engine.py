@@ -320,10 +320,15 @@ def test_delta_gather_equals_full_gather(gpu, case):
     for k in ("iterations", "active_vertices", "gathered_edges", "changed_vertices",
               "pushed_edges"):
         assert st_on[k] == st_pu[k] == st_ad[k] == st_off[k], k
-    assert st_off["delta_gathers"] == 0
+    assert st_off["delta_gathers"] == st_off["inert_gathers"] == 0
     assert st_on["delta_gathers"] == t_on - 1 and st_on["push_gathers"] == 0
-    assert st_pu["delta_gathers"] == st_pu["push_gathers"] == t_on - 1
+    assert st_pu["delta_gathers"] == t_on - 1
     assert st_ad["delta_gathers"] <= t_on - 1
+    # SSSP's first gather skips the unreached (inert) sources, by push when
+    # forced; CC has no inert values
+    inert = 1 if prog == gr.SSSP else 0
+    assert st_on["inert_gathers"] == st_pu["inert_gathers"] == st_ad["inert_gathers"] == inert
+    assert st_pu["push_gathers"] == t_on - 1 + inert
 
 
 def test_delta_gather_not_used_after_partial_start(gpu):
@@ -340,11 +345,38 @@ def test_delta_gather_not_used_after_partial_start(gpu):
         assert out.tolist() == [1, 1, 0, 0]
         assert trips == 2
         assert st["delta_gathers"] == 0
+        # every gather of the broken chain may still skip the inert sources
+        assert st["inert_gathers"] == (trips if delta else 0)
+        assert st["push_gathers"] == 0  # never pushes with a partial active set
     # the same graph from an all-active start: delta is valid and used
     for delta in (2, 3):
         out, trips, st = _engine_run(gr, gr.SSSP, 4, vd, s, d, w, delta=delta)
         assert out.tolist() == [1, 1, 0, 0]
         assert st["delta_gathers"] == trips - 1
+
+
+@pytest.mark.parametrize("bad", [None, 100000, 100001, -1])
+def test_inert_sources_need_bounded_weights(gpu, bad):
+    """SSSP's first gather skips sources at gatherZero only while every weight
+    is in [0, 100000] (gatherZero + w then neither overflows nor drops below
+    gatherZero); with another weight bound, every gather is the plain one and
+    the values are those of the full-gather run."""
+    gr = gpu
+    n, s, d = _rmat(gr, 12, seed=8)
+    w = gr.hash_weights(len(s), seed=8)
+    if bad is not None:
+        w[len(w) // 3] = bad
+    src = _max_out_src(n, s)
+    vd = np.full(n, gr.SSSP_INF, np.int32)
+    vd[src] = 0
+    full, t_full, _ = _engine_run(gr, gr.SSSP, n, vd, s, d, w, delta=0)
+    for delta in (1, 2, 3):
+        got, t, st = _engine_run(gr, gr.SSSP, n, vd, s, d, w, delta=delta)
+        assert np.array_equal(got, full) and t == t_full
+        assert st["inert_gathers"] == (1 if bad in (None, 100000) else 0)
+    if bad in (None, 100000):
+        ref, ref_trips, _ = oracle.sssp(n, s, d, w, src)
+        assert np.array_equal(full, ref) and t_full == ref_trips
 
 
 # ------------------------------------------------------- vertex order
