@@ -179,6 +179,17 @@ _SIGS = {
     "gr_pagerank_precise": (C.c_int, [_P, C.c_int, C.c_double, C.c_int,
                                        C.POINTER(C.c_float), C.POINTER(C.c_double),
                                        C.POINTER(_PRResult)]),
+    # gr_graphio.h
+    "gr_graphfile_load": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(_P)]),
+    "gr_graphfile_n_vertices": (C.c_int32, [_P]),
+    "gr_graphfile_n_edges": (C.c_int64, [_P]),
+    "gr_graphfile_srcs": (_I32P, [_P]),
+    "gr_graphfile_dsts": (_I32P, [_P]),
+    "gr_graphfile_values": (_I32P, [_P]),
+    "gr_graphfile_free": (None, [_P]),
+    "gr_write_graph_gr": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, _I32P, _I32P, _I32P]),
+    "gr_write_graph_mtx": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, _I32P, _I32P, _I32P]),
+    "gr_edge_list_to_csr": (C.c_int, [C.c_int32, C.c_int32, _I32P, _I32P, _I32P, _I32P, _I32P]),
 }
 
 _lib = None
@@ -283,6 +294,68 @@ def hash_weights(n: int, seed: int, lo: int = 1, span_log2: int = 6) -> np.ndarr
     return w
 
 
+# ------------------------------------------------------------- graph files
+
+def _path(p) -> bytes:
+    return os.fsencode(os.fspath(p))
+
+
+def read_graph(path, values: bool = False):
+    """loadGraph (graphio.cpp:441-473) for .edge, .mtx and .gr files (each
+    optionally gzip-compressed): returns (n_vertices, srcs, dsts, vals), vals
+    None unless `values` is set and the file has edge values."""
+    h = _P()
+    _check(lib().gr_graphfile_load(_path(path), int(values), C.byref(h)))
+    try:
+        L = lib()
+        n = int(L.gr_graphfile_n_vertices(h))
+        m = int(L.gr_graphfile_n_edges(h))
+
+        def take(ptr):
+            if not ptr or m == 0:
+                return None if not ptr else np.empty(0, np.int32)
+            return np.ctypeslib.as_array(ptr, shape=(m,)).copy()
+
+        s = take(L.gr_graphfile_srcs(h))
+        d = take(L.gr_graphfile_dsts(h))
+        v = take(L.gr_graphfile_values(h))
+        if s is None:
+            s = np.empty(0, np.int32)
+            d = np.empty(0, np.int32)
+        return n, s, d, v
+    finally:
+        lib().gr_graphfile_free(h)
+
+
+def edge_list_to_csr(n_vertices: int, srcs, dsts):
+    """edgeListToCSR (util.cuh:79-108), stable within a source: returns
+    (offsets, csr_dsts, sort_indices)."""
+    s, d = _i32(srcs), _i32(dsts)
+    off = np.empty(n_vertices + 1, np.int32)
+    out = np.empty(len(s), np.int32)
+    idx = np.empty(len(s), np.int32)
+    _check(lib().gr_edge_list_to_csr(n_vertices, len(s), _p32(s), _p32(d), _p32(off), _p32(out),
+                                     _p32(idx)))
+    return off, out, idx
+
+
+def write_gr(path, n_vertices: int, srcs, dsts, values=None) -> None:
+    """Edge list -> Lonestar binary CSR file, as the mtx2gr tool does
+    (mtx2gr.cpp:37-52, writeGraph_binaryCSR graphio.cpp:365-413)."""
+    off, out, idx = edge_list_to_csr(n_vertices, srcs, dsts)
+    vals = None if values is None else _i32(np.asarray(values)[idx])
+    _check(lib().gr_write_graph_gr(_path(path), n_vertices, len(out), _p32(off), _p32(out),
+                                   None if vals is None else _p32(vals)))
+
+
+def write_mtx(path, n_vertices: int, srcs, dsts, values=None) -> None:
+    """writeGraph_mtx (graphio.cpp:416-438): 1-based 'src dst [value]' lines."""
+    s, d = _i32(srcs), _i32(dsts)
+    vals = None if values is None else _i32(values)
+    _check(lib().gr_write_graph_mtx(_path(path), n_vertices, len(s), _p32(s), _p32(d),
+                                    None if vals is None else _p32(vals)))
+
+
 # ------------------------------------------------------------------- graph
 
 class Graph:
@@ -302,6 +375,12 @@ class Graph:
         h = _P()
         _check(lib().gr_graph_from_edges(n_vertices, len(s), _p32(s), _p32(d), C.byref(h)))
         return cls(h)
+
+    @classmethod
+    def from_file(cls, path):
+        """Graph of a .edge / .mtx / .gr file (read_graph)."""
+        n, s, d, _ = read_graph(path)
+        return cls.from_edges(n, s, d)
 
     @classmethod
     def rmat(cls, scale: int, edge_factor: int = 16, seed: int = 1, abc=RMAT_ABCD,

@@ -48,9 +48,11 @@ SOURCES = [
     "gr_pagerank.hip",
     "gr_batched.hip",
     "gr_serve.hip",
+    "gr_graphio.cpp",
 ]
 
-APP_SOURCES = ["bfs.cpp", "sssp.cpp", "pagerank.cpp", "connected_component.cpp"]
+APP_SOURCES = ["bfs.cpp", "sssp.cpp", "pagerank.cpp", "connected_component.cpp", "mtx2gr.cpp",
+               "gr2mtx.cpp"]
 
 
 def hipcc() -> str:
@@ -88,7 +90,7 @@ def build_library(verbose: bool = False) -> Path:
             pass
     lib = LIBDIR / LIBNAME
     tmp = LIBDIR / (LIBNAME + ".tmp")
-    _run([cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
+    _run([cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs), "-lz"])
     os.replace(tmp, lib)
     if verbose:
         print(f"built {lib}")

@@ -7,10 +7,9 @@
 //   rmat:SCALE:SEED[:sym]   RMAT, edge factor 16, a/b/c = .57/.19/.19,
 //                           self-loops and duplicates removed
 //   grid:SIDE:SEED          4-neighbour grid, 1 % of edges deleted, both arcs
-//   FILE.mtx                MatrixMarket coordinate file: 1-based ids,
-//                           self-loops dropped, "symmetric" expanded to both
-//                           arcs, a value column read with %f and truncated
-//                           to int (graphio.cpp:133-166)
+//   FILE.edge / FILE.mtx / FILE.gr [.gz]
+//                           graph files as graphio.cpp:441-473 reads them
+//                           (gr_graphio.h)
 // Synthetic inputs get hash weights uniform on [1, 64] (seed = SEED).
 #ifndef GRAPHREDUCE_AMD_APP_COMMON_H
 #define GRAPHREDUCE_AMD_APP_COMMON_H
@@ -25,6 +24,7 @@
 #include <vector>
 
 #include "graphreduce_amd/gr_abi.h"
+#include "graphreduce_amd/gr_graphio.h"
 
 namespace app {
 
@@ -116,56 +116,16 @@ inline void loadGraph(const std::string& name, int& nVertices, std::vector<int>&
     }
     return;
   }
-  // MatrixMarket coordinate file
-  FILE* fp = fopen(name.c_str(), "r");
-  if (!fp) {
-    fprintf(stderr, "cannot open %s\n", name.c_str());
-    exit(1);
-  }
-  char line[1024];
-  bool header = true, symmetric = false, first = true;
-  int maxV = 0;
-  while (fgets(line, sizeof(line), fp)) {
-    if (header) {
-      header = false;
-      if (strncmp(line, "%%MatrixMarket", 14) == 0) {
-        symmetric = strstr(line, "symmetric") != nullptr;
-        continue;
-      }
-    }
-    if (line[0] == '%' || line[0] == '\n') continue;
-    if (first) {  // "rows cols nnz"
-      first = false;
-      int r, c;
-      long long nz;
-      if (sscanf(line, "%d %d %lld", &r, &c, &nz) == 3) {
-        maxV = (r > c ? r : c);
-        continue;
-      }
-    }
-    int s, d, nb = 0;
-    if (sscanf(line, "%d %d%n", &s, &d, &nb) != 2) {
-      fprintf(stderr, "bad line in %s: %s", name.c_str(), line);
-      exit(1);
-    }
-    --s;
-    --d;
-    if (s == d) continue;  // self-loops dropped
-    float val = 1.0f;
-    if (edgeValues && sscanf(line + nb, "%f", &val) != 1) val = 1.0f;
-    srcs.push_back(s);
-    dsts.push_back(d);
-    if (edgeValues) edgeValues->push_back((int)val);
-    if (symmetric) {
-      srcs.push_back(d);
-      dsts.push_back(s);
-      if (edgeValues) edgeValues->push_back((int)val);
-    }
-    if (s + 1 > maxV) maxV = s + 1;
-    if (d + 1 > maxV) maxV = d + 1;
-  }
-  fclose(fp);
-  nVertices = maxV;
+  // graph file: .edge / .mtx / .gr, optionally .gz (gr_graphio.h)
+  gr_graphfile* gf = nullptr;
+  die(gr_graphfile_load(name.c_str(), edgeValues ? 1 : 0, &gf), "loadGraph");
+  const int64_t m = gr_graphfile_n_edges(gf);
+  nVertices = gr_graphfile_n_vertices(gf);
+  srcs.assign(gr_graphfile_srcs(gf), gr_graphfile_srcs(gf) + m);
+  dsts.assign(gr_graphfile_dsts(gf), gr_graphfile_dsts(gf) + m);
+  if (edgeValues && gr_graphfile_values(gf))
+    edgeValues->assign(gr_graphfile_values(gf), gr_graphfile_values(gf) + m);
+  gr_graphfile_free(gf);
 }
 
 // Vertex of maximum out-degree, first on ties (bfs.cu:175-195).

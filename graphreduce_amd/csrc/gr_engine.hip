@@ -286,8 +286,11 @@ __global__ __launch_bounds__(kNT) void k_push_tiles(const DevCounters* __restric
           else
             hi = mid - 1;
         }
-        const int dst = dsts[s_csr[lo] + (e - s_start[lo])];
-        atomicOr(&bitmap[dst >> 5], 1u << (dst & 31));
+        const int dst = __builtin_nontemporal_load(dsts + s_csr[lo] + (e - s_start[lo]));
+        // many changed vertices share a few hub targets: an atomic on a word
+        // whose bit is already visible set would only serialise behind others
+        const uint32_t bit = 1u << (dst & 31);
+        if (!(bitmap[dst >> 5] & bit)) atomicOr(&bitmap[dst >> 5], bit);
       }
     }
     __syncthreads();
@@ -348,8 +351,9 @@ __global__ __launch_bounds__(kNT) void k_relax_tiles(
             hi = mid - 1;
         }
         const int pos = s_csr[lo] + (e - s_start[lo]);
-        const int v = dsts[pos];
-        const typename P::GatherResult val = P::gatherMap(nullptr, &s_vd[lo], ed_csr + pos);
+        const int v = __builtin_nontemporal_load(dsts + pos);
+        const typename P::EdgeData ev = stream_load(ed_csr + pos);
+        const typename P::GatherResult val = P::gatherMap(nullptr, &s_vd[lo], &ev);
         if (val < cand[v]) atomicMin(&cand[v], val);
       }
     }

@@ -37,6 +37,25 @@ struct DevCounters {
   unsigned long long inert;       // (#non-inert << 32) | out-edges, from k_inert_scan
 };
 
+// Read-once operands (CSC sources, edge data) are loaded non-temporally so
+// that streaming them does not displace the randomly re-read vertex values
+// from L2.
+template <class T>
+__device__ __forceinline__ T stream_load(const T* p) {
+  if constexpr (std::is_empty<T>::value) {
+    return T();
+  } else if constexpr (sizeof(T) % 4 == 0) {
+    T out;
+    uint32_t* o = reinterpret_cast<uint32_t*>(&out);
+    const uint32_t* q = reinterpret_cast<const uint32_t*>(p);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) o[i] = __builtin_nontemporal_load(q + i);
+    return out;
+  } else {
+    return *p;
+  }
+}
+
 template <class T>
 struct Part {
   int32_t row;
@@ -235,7 +254,7 @@ __global__ __launch_bounds__(kNT) void k_gather_tiles(
       if (e < n_edges) {
         int k = s_row[e];
         csc_v[m] = s_base[k] + j0 + e;
-        src_v[m] = srcs[csc_v[m]];
+        src_v[m] = __builtin_nontemporal_load(srcs + csc_v[m]);
       }
     }
     bool mapped = false;
@@ -252,9 +271,12 @@ __global__ __launch_bounds__(kNT) void k_gather_tiles(
         for (int m = 0; m < VT; ++m) {
           int e = tid + m * kNT;
           if (e < n_edges) {
-            s_val[e] = ((word[m] >> (src_v[m] & 31)) & 1u)
-                           ? P::gatherMap(vd, vd + src_v[m], ed + csc_v[m])
-                           : P::gatherZero;
+            if ((word[m] >> (src_v[m] & 31)) & 1u) {
+              const typename P::EdgeData ev = stream_load(ed + csc_v[m]);
+              s_val[e] = P::gatherMap(vd, vd + src_v[m], &ev);
+            } else {
+              s_val[e] = P::gatherZero;
+            }
           }
         }
       }
@@ -266,7 +288,8 @@ __global__ __launch_bounds__(kNT) void k_gather_tiles(
         if (e < n_edges) {
           const typename P::VertexData* dstp = vd;
           if constexpr (P::kGatherUsesDst) dstp = vd + active[i0 + s_row[e]];
-          s_val[e] = P::gatherMap(dstp, vd + src_v[m], ed + csc_v[m]);
+          const typename P::EdgeData ev = stream_load(ed + csc_v[m]);
+          s_val[e] = P::gatherMap(dstp, vd + src_v[m], &ev);
         }
       }
     }

@@ -66,3 +66,27 @@ def test_mtx_input(gpu, tmp_path):
     stdout, dist = _run("sssp", [str(mtx), "0"], tmp_path)
     # symmetric expansion, self-loop (3,3) dropped, 1-based ids (graphio.cpp:133-166)
     assert dist.astype(np.int64).tolist() == [0, 7, 10, 2, 3]
+
+
+def test_gr_and_edge_file_inputs(gpu, tmp_path):
+    """SSSP over a .gr file written by mtx2gr's writer, BFS over a .edge.gz
+    file: the same results as the oracle on the file's edges."""
+    import gzip
+    s, d = gr.rmat_edges(11, 16, seed=7)
+    n = 1 << 11
+    w = gr.hash_weights(len(s), seed=7)
+    gr.write_gr(tmp_path / "g.gr", n, s, d, w)
+    n2, fs, fd, fw = gr.read_graph(tmp_path / "g.gr", True)
+    assert n2 == n
+    src = int(np.argmax(np.bincount(fs, minlength=n)))
+    stdout, dist = _run("sssp", ["-m", str(tmp_path / "g.gr"), "0"], tmp_path)
+    assert f"using vertex {src}" in stdout
+    ref, _, _ = oracle.sssp(n, fs, fd, fw, src)
+    assert np.array_equal(dist.astype(np.int64), ref.astype(np.int64))
+    body = "# rmat 11\n" + "".join(f"{a} {b}\n" for a, b in zip(s.tolist(), d.tolist()))
+    (tmp_path / "g.edge.gz").write_bytes(gzip.compress(body.encode()))
+    stdout, depth = _run("bfs", ["-m", str(tmp_path / "g.edge.gz"), "0"], tmp_path)
+    ne = int(max(s.max(), d.max())) + 1  # a text file has 1 + the largest id vertices
+    src = int(np.argmax(np.bincount(s, minlength=ne)))
+    ref, _, _ = oracle.bfs(ne, s, d, src)
+    assert np.array_equal(depth.astype(np.int32), ref)
