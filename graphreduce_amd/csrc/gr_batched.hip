@@ -1003,20 +1003,30 @@ extern "C" int gr_mssp(gr_graph* g, const int32_t* weights_device, const int32_t
   DBuf<int32_t> dist, w_csc, d_src, d_src_user;
   DBuf<uint64_t> chg_a, chg_b;
   DBuf<unsigned long long> cnt, any;
+  DBuf<uint8_t> dirty;
+  DBuf<int32_t> list;
   GR_TRY(dist.alloc((size_t)nV * kWave));
   GR_TRY(w_csc.alloc((size_t)g->nE));
   GR_TRY(d_src.alloc(n_sources));
   GR_TRY(d_src_user.alloc(n_sources));
   GR_TRY(chg_a.alloc(nV));
   GR_TRY(chg_b.alloc(nV));
-  GR_TRY(cnt.alloc(64));
+  // per round r of a replay: cnt[r] changed vertices, cnt[kCnt + r] their
+  // out-edges, cnt[2 kCnt + r] filtered-list length; cnt[3 kCnt + 0/1]: the
+  // previous replay's last round (changed vertices, out-edges)
+  constexpr int kCnt = 16;
+  GR_TRY(cnt.alloc(4 * kCnt));
   GR_TRY(any.alloc(1));
+  GR_TRY(dirty.alloc(nV));
+  GR_TRY(list.alloc(nV));
+  GR_HIP(hipMemset(dirty.p, 0, (size_t)nV));
   GR_HIP(hipMemcpy(d_src_user.p, sources, sizeof(int32_t) * n_sources, hipMemcpyHostToDevice));
   k_map_ids<<<1, 64>>>(n_sources, d_src_user.p, user_g->inv, d_src.p);
   if (g->nE)
     k_permute_i32<<<grid_1d(g->nE, 256), 256>>>(weights_device, g->csc_eid, g->nE, w_csc.p);
-  LaneTables lt;
+  LaneTables lt, lt_out;
   GR_TRY(lt.build(g->h_off_in));
+  GR_TRY(lt_out.build(g->h_off_out));
   hipStream_t st;
   GR_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   GR_HIP(hipDeviceSynchronize());
@@ -1030,8 +1040,13 @@ extern "C" int gr_mssp(gr_graph* g, const int32_t* weights_device, const int32_t
   // One graph holds kRounds rounds (ping-pong of the changed masks); replays
   // continue until a round changes nothing.
   constexpr int kRounds = 8;
+  static_assert(kRounds <= kCnt, "one counter slot per round");
   const unsigned rgrid = grid_1d((int64_t)nV, kLaneNT / kWave);
   const unsigned hgrid = grid_1d((int64_t)lt.n_chunks, kLaneNT / kWave);
+  const unsigned vgrid = std::min<unsigned>(grid_1d((int64_t)nV, kLaneNT), 16384);
+  // a filtered round pays one flag write per out-edge of the changed vertices;
+  // it replaces the scan of every in-edge when those are a small share
+  const long long filter_max = g->nE / 8;
   hipGraph_t graph;
   hipGraphExec_t exec;
   GR_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -1040,24 +1055,47 @@ extern "C" int gr_mssp(gr_graph* g, const int32_t* weights_device, const int32_t
     uint64_t* cout = (r & 1) ? chg_a.p : chg_b.p;
     // a round after a round without change returns at once (exact: nothing
     // changed feeds it), so the rest of a replay after convergence is free
-    const unsigned long long* prev = r ? cnt.p + (r - 1) : nullptr;
+    LaneRound rr;
+    rr.prev_count = r ? cnt.p + (r - 1) : cnt.p + 3 * kCnt;
+    rr.prev_edges = r ? cnt.p + kCnt + (r - 1) : cnt.p + 3 * kCnt + 1;
+    rr.count = cnt.p + r;
+    rr.edges = cnt.p + kCnt + r;
+    rr.dirty = dirty.p;
+    rr.list = list.p;
+    rr.list_len = reinterpret_cast<unsigned int*>(cnt.p + 2 * kCnt + r);
+    rr.filter_max = filter_max;
+    rr.host_filter = -1;
+    k_lane_sssp_mark<<<vgrid, kLaneNT, 0, st>>>(nV, g->off_out, g->dsts, lt_out.chunks,
+                                                lt_out.n_chunks, cin, rr);
+    k_lane_sssp_filter<<<vgrid, kLaneNT, 0, st>>>(nV, g->off_in, cout, rr);
     k_lane_sssp_light<<<rgrid < 65536 ? rgrid : 65536, kLaneNT, 0, st>>>(
-        nV, g->off_in, g->srcs, w_csc.p, dist.p, cin, cout, any.p, cnt.p + r, prev);
-    if (lt.n_chunks)
-      k_lane_sssp_heavy<<<hgrid, kLaneNT, 0, st>>>(lt.chunks, lt.n_chunks, g->srcs, w_csc.p,
-                                                   dist.p, cin, cout, any.p, cnt.p + r, prev);
+        nV, g->off_in, g->off_out, g->srcs, w_csc.p, dist.p, cin, cout, any.p, rr);
+    if (lt.n_chunks) {
+      k_lane_sssp_heavy<<<hgrid, kLaneNT, 0, st>>>(lt.chunks, lt.n_chunks, g->off_out, g->srcs,
+                                                   w_csc.p, dist.p, cin, cout, any.p, rr);
+      k_lane_sssp_clear_heavy<<<grid_1d(lt.n_heavy, 256), 256, 0, st>>>(lt.heavy, lt.n_heavy, rr);
+    }
   }
   GR_HIP(hipStreamEndCapture(st, &graph));
   GR_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   int rounds = 0;
   unsigned long long hc[kRounds];
+  // the first replay's first round has no previous round: not idle, unfiltered
+  unsigned long long carry[2] = {1ull, ~0ull};
   for (;;) {
-    // counters are cleared by a kernel-free copy from a zeroed host array
-    unsigned long long zeros[kRounds] = {0};
-    GR_HIP(hipMemcpyAsync(cnt.p, zeros, sizeof(zeros), hipMemcpyHostToDevice, st));
+    // counters are cleared by kernel-free copies (host arrays), and the last
+    // round's counts carried into the next replay's first round
+    unsigned long long init[4 * kCnt] = {0};
+    init[3 * kCnt] = carry[0];
+    init[3 * kCnt + 1] = carry[1];
+    GR_HIP(hipMemcpyAsync(cnt.p, init, sizeof(init), hipMemcpyHostToDevice, st));
     GR_HIP(hipGraphLaunch(exec, st));
-    GR_HIP(hipMemcpyAsync(hc, cnt.p, sizeof(hc), hipMemcpyDeviceToHost, st));
+    unsigned long long hall[4 * kCnt];
+    GR_HIP(hipMemcpyAsync(hall, cnt.p, sizeof(hall), hipMemcpyDeviceToHost, st));
     GR_HIP(hipStreamSynchronize(st));
+    for (int r = 0; r < kRounds; ++r) hc[r] = hall[r];
+    carry[0] = hall[kRounds - 1];
+    carry[1] = hall[kCnt + kRounds - 1];
     int last = -1;
     for (int r = 0; r < kRounds; ++r)
       if (hc[r]) last = r;

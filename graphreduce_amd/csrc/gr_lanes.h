@@ -131,26 +131,152 @@ __device__ __forceinline__ int32_t lane_min_in_edges(int b, int e, const int32_t
   return best;
 }
 
-// One relaxation round over the light vertices (in-degree <= kLaneHeavy).
-// chg_out[v] = lanes whose distance at v dropped; heavy vertices get 0 here
-// and their bits from k_lane_sssp_heavy.  *any |= lanes changed anywhere.
-// prev_count (may be nullptr): the previous round's changed count; when it is
-// 0 nothing can change in this round and the kernel returns at once (chg_out
-// is then not rewritten: the caller stops after such a round).
+// Round control of the lane SSSP.  A round only needs the vertices with an
+// in-neighbour whose mask changed in the previous round; when those sources
+// have few out-edges, k_lane_sssp_mark flags their out-neighbours in `dirty`,
+// k_lane_sssp_filter turns the flagged light vertices into a list (zeroing
+// chg_out of every other vertex and clearing the light flags), the relax
+// kernels visit only the listed vertices and the flagged heavy chunks, and
+// k_lane_sssp_clear_heavy clears the heavy flags, so `dirty` is all zero
+// between rounds.  A vertex outside the list has no changed in-neighbour and
+// would compute nothing, so the filter changes no value.
+struct LaneRound {
+  const unsigned long long* prev_count;  // previous round's changed vertices (nullptr: none)
+  const unsigned long long* prev_edges;  // ... and their out-edges
+  unsigned long long* count;             // this round's, counted when not nullptr
+  unsigned long long* edges;
+  uint8_t* dirty;                        // [V] flags; nullptr: never filter
+  int32_t* list;                         // [V] light vertices to visit (filtered rounds)
+  unsigned int* list_len;                // zeroed by the caller before the filter kernel
+  long long filter_max;                  // filter when prev_edges < filter_max ...
+  int host_filter;                       // ... (-1), or as the host says (0 / 1)
+};
+
+// A round after a round without change: nothing can change, every kernel of
+// the round returns at once (chg_out is not rewritten: the caller stops).
+__device__ __forceinline__ bool lane_round_idle(const LaneRound& r) {
+  return r.prev_count && *r.prev_count == 0;
+}
+__device__ __forceinline__ bool lane_round_filtered(const LaneRound& r) {
+  if (!r.dirty) return false;
+  if (r.host_filter >= 0) return r.host_filter != 0;
+  return r.prev_count && r.prev_edges && *r.prev_count != 0 &&
+         (long long)*r.prev_edges < r.filter_max;
+}
+
+// Flags the out-neighbours of every vertex whose mask is set in chg_in.  Light
+// out-degrees (<= kLaneHeavy) are walked by one wave per vertex; heavier
+// sources by one wave per out-edge chunk (out_chunks: LaneChunk over CSR).
+static __global__ __launch_bounds__(kLaneNT) void k_lane_sssp_mark(
+    int nV, const int32_t* __restrict__ off_out, const int32_t* __restrict__ dsts,
+    const LaneChunk* __restrict__ out_chunks, int n_out_chunks,
+    const uint64_t* __restrict__ chg_in, LaneRound rr) {
+  if (lane_round_idle(rr) || !lane_round_filtered(rr)) return;
+  const int lane = lane_id();
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  const int n_waves = gridDim.x * blockDim.x / kWave;
+  for (int base = wave * kWave; base < nV; base += n_waves * kWave) {
+    const int v = base + lane;
+    const uint64_t m = v < nV ? chg_in[v] : 0ull;
+    int b = 0, e = 0;
+    if (m) {
+      b = off_out[v];
+      e = off_out[v + 1];
+    }
+    uint64_t todo = __ballot(m != 0 && e - b <= kLaneHeavy);
+    while (todo) {
+      const int j = __builtin_ctzll(todo);
+      todo &= todo - 1;
+      const int bj = __shfl(b, j, kWave), ej = __shfl(e, j, kWave);
+      for (int p = bj + lane; p < ej; p += kWave) rr.dirty[dsts[p]] = 1;
+    }
+  }
+  for (int c = wave; c < n_out_chunks; c += n_waves) {
+    const LaneChunk ch = out_chunks[c];
+    if (!chg_in[ch.v]) continue;
+    for (int p = ch.b + lane; p < ch.e; p += kWave) rr.dirty[dsts[p]] = 1;
+  }
+}
+
+// Filtered rounds: flagged light vertices -> list; chg_out = 0 and flag
+// cleared elsewhere (heavy flags stay for k_lane_sssp_heavy).  A workgroup
+// owns a contiguous range of vertices: it counts its list entries, takes their
+// positions with ONE atomic (same-address atomics of every wave would
+// serialise), then writes them in ascending order.
+static __global__ __launch_bounds__(kLaneNT) void k_lane_sssp_filter(
+    int nV, const int32_t* __restrict__ off_in, uint64_t* __restrict__ chg_out, LaneRound rr) {
+  if (lane_round_idle(rr) || !lane_round_filtered(rr)) return;
+  __shared__ unsigned int s_w[kLaneNT / kWave];
+  __shared__ unsigned int s_base;
+  const int lane = lane_id(), w = threadIdx.x / kWave;
+  const int64_t per = ((int64_t)nV + gridDim.x - 1) / gridDim.x;
+  const int64_t span = (per + kLaneNT - 1) / kLaneNT * kLaneNT;
+  const int64_t lo64 = (int64_t)blockIdx.x * span;
+  const int lo = (int)(lo64 < nV ? lo64 : nV);
+  const int hi = (int)(lo64 + span < nV ? lo64 + span : nV);
+  auto wanted = [&](int v) {
+    return v < hi && rr.dirty[v] != 0 && off_in[v + 1] - off_in[v] <= kLaneHeavy;
+  };
+  unsigned int mine = 0;
+  for (int v = lo + (int)threadIdx.x; v < hi; v += kLaneNT) mine += wanted(v) ? 1u : 0u;
+  // block total
+  unsigned int tot = mine;
+  for (int o = kWave / 2; o > 0; o >>= 1) tot += __shfl_xor(tot, o, kWave);
+  if (lane == 0) s_w[w] = tot;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned int t = 0;
+    for (int i = 0; i < kLaneNT / kWave; ++i) t += s_w[i];
+    s_base = t ? atomicAdd(rr.list_len, t) : 0u;
+  }
+  __syncthreads();
+  unsigned int at = s_base;
+  for (int v0 = lo; v0 < hi; v0 += kLaneNT) {
+    const int v = v0 + (int)threadIdx.x;
+    bool take = false;
+    if (v < hi) {
+      const bool flagged = rr.dirty[v] != 0;
+      const bool heavy = off_in[v + 1] - off_in[v] > kLaneHeavy;
+      take = flagged && !heavy;
+      if (take) rr.dirty[v] = 0;
+      if (!take) chg_out[v] = 0;  // heavy: k_lane_sssp_heavy ORs its bits in
+    }
+    const uint64_t tk = __ballot(take);
+    __syncthreads();  // s_w of the previous step has been read
+    if (lane == 0) s_w[w] = (unsigned int)__popcll(tk);
+    __syncthreads();
+    unsigned int before = 0, step = 0;
+    for (int i = 0; i < kLaneNT / kWave; ++i) {
+      if (i < w) before += s_w[i];
+      step += s_w[i];
+    }
+    if (take) rr.list[at + before + __popcll(tk & ((1ull << lane) - 1))] = v;
+    at += step;
+  }
+}
+
+// One relaxation round over the light vertices (in-degree <= kLaneHeavy), all
+// of them or the filtered list.  chg_out[v] = lanes whose distance at v
+// dropped (heavy vertices get 0 here and their bits from k_lane_sssp_heavy);
+// *any |= lanes changed anywhere; rr.count / rr.edges += changed vertices and
+// their out-edges.
 static __global__ __launch_bounds__(kLaneNT) void k_lane_sssp_light(
-    int nV, const int32_t* __restrict__ off_in, const int32_t* __restrict__ srcs,
-    const int32_t* __restrict__ w_csc, int32_t* __restrict__ dist,
-    const uint64_t* __restrict__ chg_in, uint64_t* __restrict__ chg_out,
-    unsigned long long* __restrict__ any, unsigned long long* __restrict__ count,
-    const unsigned long long* __restrict__ prev_count) {
-  if (prev_count && *prev_count == 0) return;
-  __shared__ unsigned long long s_acc[kLaneNT / kWave], s_cnt[kLaneNT / kWave];
+    int nV, const int32_t* __restrict__ off_in, const int32_t* __restrict__ off_out,
+    const int32_t* __restrict__ srcs, const int32_t* __restrict__ w_csc,
+    int32_t* __restrict__ dist, const uint64_t* __restrict__ chg_in,
+    uint64_t* __restrict__ chg_out, unsigned long long* __restrict__ any, LaneRound rr) {
+  if (lane_round_idle(rr)) return;
+  const bool filtered = lane_round_filtered(rr);
+  const int n_items = filtered ? (int)*rr.list_len : nV;
+  __shared__ unsigned long long s_acc[kLaneNT / kWave], s_cnt[kLaneNT / kWave],
+      s_edg[kLaneNT / kWave];
   const int lane = lane_id();
   const int wpb = blockDim.x / kWave;
   const int n_waves = gridDim.x * wpb;
   uint64_t acc = 0;
-  unsigned long long cnt = 0;
-  for (int v = blockIdx.x * wpb + threadIdx.x / kWave; v < nV; v += n_waves) {
+  unsigned long long cnt = 0, edg = 0;
+  for (int i = blockIdx.x * wpb + threadIdx.x / kWave; i < n_items; i += n_waves) {
+    const int v = filtered ? rr.list[i] : i;
     const int b = off_in[v], e = off_in[v + 1];
     if (e - b > kLaneHeavy) {
       if (lane == 0) chg_out[v] = 0;
@@ -163,39 +289,47 @@ static __global__ __launch_bounds__(kLaneNT) void k_lane_sssp_light(
     const uint64_t m = __ballot(improved);
     if (lane == 0) chg_out[v] = m;
     acc |= m;
-    cnt += m ? 1 : 0;
+    if (m) {
+      cnt += 1;
+      if (lane == 0 && rr.edges) edg += (unsigned long long)(off_out[v + 1] - off_out[v]);
+    }
   }
-  // one atomic pair per workgroup: per-wave atomics on these two words
+  // one atomic per counter and workgroup: per-wave atomics on these words
   // serialise over hundreds of thousands of waves
   if (lane == 0) {
     s_acc[threadIdx.x / kWave] = acc;
     s_cnt[threadIdx.x / kWave] = cnt;
+    s_edg[threadIdx.x / kWave] = edg;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned long long a = 0, c = 0;
+    unsigned long long a = 0, c = 0, d = 0;
     for (int i = 0; i < wpb; ++i) {
       a |= s_acc[i];
       c += s_cnt[i];
+      d += s_edg[i];
     }
     if (a) atomicOr(any, a);
-    if (c && count) atomicAdd(count, c);
+    if (c && rr.count) atomicAdd(rr.count, c);
+    if (d && rr.edges) atomicAdd(rr.edges, d);
   }
 }
 
-// Heavy vertices: one wave per chunk, merged with atomicMin.
+// Heavy vertices: one wave per chunk, merged with atomicMin; in filtered
+// rounds only the chunks of flagged vertices.
 static __global__ __launch_bounds__(kLaneNT) void k_lane_sssp_heavy(
-    const LaneChunk* __restrict__ chunks, int n_chunks, const int32_t* __restrict__ srcs,
-    const int32_t* __restrict__ w_csc, int32_t* __restrict__ dist,
-    const uint64_t* __restrict__ chg_in, uint64_t* __restrict__ chg_out,
-    unsigned long long* __restrict__ any, unsigned long long* __restrict__ count,
-    const unsigned long long* __restrict__ prev_count) {
-  if (prev_count && *prev_count == 0) return;
+    const LaneChunk* __restrict__ chunks, int n_chunks, const int32_t* __restrict__ off_out,
+    const int32_t* __restrict__ srcs, const int32_t* __restrict__ w_csc,
+    int32_t* __restrict__ dist, const uint64_t* __restrict__ chg_in,
+    uint64_t* __restrict__ chg_out, unsigned long long* __restrict__ any, LaneRound rr) {
+  if (lane_round_idle(rr)) return;
+  const bool filtered = lane_round_filtered(rr);
   const int lane = lane_id();
   const int wpb = blockDim.x / kWave;
   const int n_waves = gridDim.x * wpb;
   for (int c = blockIdx.x * wpb + threadIdx.x / kWave; c < n_chunks; c += n_waves) {
     const LaneChunk ch = chunks[c];
+    if (filtered && !rr.dirty[ch.v]) continue;
     const int32_t inf = 2147483647;
     const int32_t best = lane_min_in_edges(ch.b, ch.e, srcs, w_csc, dist, chg_in, inf);
     bool improved = false;
@@ -205,11 +339,23 @@ static __global__ __launch_bounds__(kLaneNT) void k_lane_sssp_heavy(
     }
     const uint64_t m = __ballot(improved);
     if (lane == 0 && m) {
-      const unsigned long long prev = atomicOr((unsigned long long*)&chg_out[ch.v], (unsigned long long)m);
+      const unsigned long long prev =
+          atomicOr((unsigned long long*)&chg_out[ch.v], (unsigned long long)m);
       atomicOr(any, (unsigned long long)m);
-      if (count && prev == 0) atomicAdd(count, 1ull);
+      if (prev == 0) {
+        if (rr.count) atomicAdd(rr.count, 1ull);
+        if (rr.edges) atomicAdd(rr.edges, (unsigned long long)(off_out[ch.v + 1] - off_out[ch.v]));
+      }
     }
   }
+}
+
+// Filtered rounds: clears the flags of the heavy vertices after their chunks ran.
+static __global__ void k_lane_sssp_clear_heavy(const LaneHeavy* __restrict__ heavy, int n_heavy,
+                                               LaneRound rr) {
+  if (lane_round_idle(rr) || !lane_round_filtered(rr)) return;
+  for (int h = blockIdx.x * blockDim.x + threadIdx.x; h < n_heavy; h += gridDim.x * blockDim.x)
+    rr.dirty[heavy[h].v] = 0;
 }
 
 // PPR partial sum of contrib[u][lane] over in-edges [b, e) whose source has

@@ -168,6 +168,9 @@ __global__ __launch_bounds__(kSrvNT) void k_srv_bfs_expand(
 struct SrvCounters {
   unsigned long long frontier_or;     // lanes with a non-empty frontier after the step
   unsigned long long frontier_edges;  // out-edges of the new frontier
+  unsigned long long changed;         // SSSP: vertices changed by the step
+  unsigned int list_len;              // SSSP: filtered-list length of the step
+  unsigned int pad;
   unsigned long long checksum[kLanes];
 };
 
@@ -486,7 +489,7 @@ struct Pool {
   SrvCounters* h_ctr = nullptr;  // pinned
   int32_t* h_row = nullptr;      // pinned V ints for result copies
   int parity = 0;                // SSSP/PPR ping-pong
-  uint64_t last_fe = 0;
+  uint64_t last_fe = 0;          // BFS frontier / SSSP changed out-edges of the last step
 };
 
 }  // namespace gr
@@ -498,6 +501,8 @@ struct gr_server {
   std::vector<int32_t> h_inv;         // host copy of inv (query seeds)
   gr_server_config cfg{};
   int32_t* w_csc = nullptr;
+  uint8_t* sssp_dirty = nullptr;   // [V] flags of filtered SSSP steps (all zero between steps)
+  int32_t* sssp_list = nullptr;    // [V] their light vertices
   gr::Pool pools[3];
   gr::LaneTables lanes;       // heavy-vertex chunks of the CSC
   gr::LaneTables out_lanes;   // ... of the CSR (BFS push)
@@ -663,13 +668,32 @@ int launch_step(gr_server* sv, Pool& p) {
     const uint64_t* cin = p.parity ? p.m1 : p.m0;
     uint64_t* cout = p.parity ? p.m0 : p.m1;
     unsigned long long* any = &p.ctr->frontier_or;
-    k_lane_sssp_light<<<grid_v(nV), kLaneNT, 0, p.stream>>>(nV, g->off_in, g->srcs, sv->w_csc,
-                                                            p.i32, cin, cout, any, nullptr,
-                                                            nullptr);
-    if (sv->lanes.n_chunks)
+    // a step visits only the out-neighbours of the changed (and newly seeded)
+    // vertices when the last step's changes had few out-edges
+    LaneRound rr;
+    rr.prev_count = nullptr;
+    rr.prev_edges = nullptr;
+    rr.count = &p.ctr->changed;
+    rr.edges = &p.ctr->frontier_edges;
+    rr.dirty = sv->sssp_dirty;
+    rr.list = sv->sssp_list;
+    rr.list_len = &p.ctr->list_len;
+    rr.filter_max = g->nE / 8;
+    rr.host_filter = (long long)p.last_fe < rr.filter_max ? 1 : 0;
+    const unsigned vgrid = std::min<unsigned>(grid_1d((int64_t)nV, kLaneNT), 16384);
+    k_lane_sssp_mark<<<vgrid, kLaneNT, 0, p.stream>>>(nV, g->off_out, g->dsts,
+                                                      sv->out_lanes.chunks,
+                                                      sv->out_lanes.n_chunks, cin, rr);
+    k_lane_sssp_filter<<<vgrid, kLaneNT, 0, p.stream>>>(nV, g->off_in, cout, rr);
+    k_lane_sssp_light<<<grid_v(nV), kLaneNT, 0, p.stream>>>(nV, g->off_in, g->off_out, g->srcs,
+                                                            sv->w_csc, p.i32, cin, cout, any, rr);
+    if (sv->lanes.n_chunks) {
       k_lane_sssp_heavy<<<grid_1d(sv->lanes.n_chunks, kLaneNT / kWave), kLaneNT, 0, p.stream>>>(
-          sv->lanes.chunks, sv->lanes.n_chunks, g->srcs, sv->w_csc, p.i32, cin, cout, any, nullptr,
-          nullptr);
+          sv->lanes.chunks, sv->lanes.n_chunks, g->off_out, g->srcs, sv->w_csc, p.i32, cin, cout,
+          any, rr);
+      k_lane_sssp_clear_heavy<<<grid_1d(sv->lanes.n_heavy, 256), 256, 0, p.stream>>>(
+          sv->lanes.heavy, sv->lanes.n_heavy, rr);
+    }
     p.parity ^= 1;
   } else {
     const uint64_t* nzi = p.parity ? p.m1 : p.m0;
@@ -701,6 +725,7 @@ uint64_t finished_lanes(gr_server* sv, Pool& p) {
     p.last_fe = p.h_ctr->frontier_edges;
     done = p.active & ~p.h_ctr->frontier_or;
   } else if (p.kind == GR_QUERY_SSSP) {
+    p.last_fe = p.h_ctr->frontier_edges;
     done = p.active & ~p.h_ctr->frontier_or;
   } else {
     const int iters = sv->cfg.ppr_iterations > 0 ? sv->cfg.ppr_iterations : 10;
@@ -835,6 +860,12 @@ extern "C" int gr_server_create(gr_graph* g, const int32_t* weights_device,
       if (hipDeviceSynchronize() != hipSuccess) rc = set_error(GR_ERR_HIP, "weights permute");
     }
   }
+  if (rc == GR_OK && sv->w_csc) {
+    rc = alloc_dev((void**)&sv->sssp_dirty, (size_t)g->nV + 1);
+    if (rc == GR_OK) rc = alloc_dev((void**)&sv->sssp_list, 4 * ((size_t)g->nV + 1));
+    if (rc == GR_OK && hipMemset(sv->sssp_dirty, 0, (size_t)g->nV + 1) != hipSuccess)
+      rc = set_error(GR_ERR_HIP, "server: clearing the SSSP flags");
+  }
   if (rc == GR_OK) rc = sv->lanes.build(g->h_off_in);
   if (rc == GR_OK) rc = sv->out_lanes.build(g->h_off_out);
   if (rc == GR_OK)
@@ -843,6 +874,8 @@ extern "C" int gr_server_create(gr_graph* g, const int32_t* weights_device,
   if (rc != GR_OK) {
     for (auto& p : sv->pools) pool_free(p);
     if (sv->w_csc) (void)hipFree(sv->w_csc);
+    if (sv->sssp_dirty) (void)hipFree(sv->sssp_dirty);
+    if (sv->sssp_list) (void)hipFree(sv->sssp_list);
     if (sv->ppr_partial) (void)hipFree(sv->ppr_partial);
     delete sv;
     return rc;
@@ -860,6 +893,8 @@ extern "C" int gr_server_destroy(gr_server* sv) {
     if (w.joinable()) w.join();
   for (auto& p : sv->pools) pool_free(p);
   if (sv->w_csc) (void)hipFree(sv->w_csc);
+  if (sv->sssp_dirty) (void)hipFree(sv->sssp_dirty);
+  if (sv->sssp_list) (void)hipFree(sv->sssp_list);
   if (sv->ppr_partial) (void)hipFree(sv->ppr_partial);
   delete sv;
   return GR_OK;
