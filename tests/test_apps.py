@@ -90,3 +90,30 @@ def test_gr_and_edge_file_inputs(gpu, tmp_path):
     src = int(np.argmax(np.bincount(s, minlength=ne)))
     ref, _, _ = oracle.bfs(ne, s, d, src)
     assert np.array_equal(depth.astype(np.int32), ref)
+
+
+def test_regression_checkers_accept_app_output(gpu, tmp_path):
+    """The regression flow of regressions/Makefile:66-110: app output vs a gold
+    file (here the oracle's, with PowerGraph's 10000000 for unreachable) through
+    checkDist / checkPageRank."""
+    from graphreduce_amd import checkers
+    s, d = gr.rmat_edges(11, 16, seed=9)
+    n = 1 << 11
+    w = gr.hash_weights(len(s), seed=9)
+    gr.write_gr(tmp_path / "g.gr", n, s, d, w)
+    _, fs, fd, fw = gr.read_graph(tmp_path / "g.gr", True)
+    _run("sssp", [str(tmp_path / "g.gr"), "0"], tmp_path)
+    ref, _, _ = oracle.sssp(n, fs, fd, fw, 0)
+    gold = tmp_path / "sssp.gold"
+    gold.write_text("".join(f"{v} {10000000 if x == gr.SSSP_INF else x}\n"
+                            for v, x in enumerate(ref.tolist())))
+    assert checkers.check_dist(tmp_path / "sssp.txt", gold)[0]
+    bad = tmp_path / "sssp.bad"
+    bad.write_text("".join(f"{v} {x + (v == 7)}\n" for v, x in enumerate(ref.tolist())))
+    assert not checkers.check_dist(tmp_path / "sssp.txt", bad)[0]
+    _run("pagerank", [str(tmp_path / "g.gr")], tmp_path)
+    pr, _, _ = oracle.pagerank(n, fs, fd)
+    pgold = tmp_path / "pr.gold"
+    pgold.write_text("".join(f"{v} {x:.9f}\n" for v, x in enumerate(pr.tolist())))
+    ok, msg = checkers.check_pagerank(tmp_path / "pagerank.txt", pgold)
+    assert ok, msg
