@@ -79,7 +79,8 @@ __global__ __launch_bounds__(kApplyNT) void k_apply(
     int n, const int32_t* __restrict__ active, const typename P::GatherResult* __restrict__ gtmp,
     int use_gather, typename P::VertexData* __restrict__ vd, const int32_t* __restrict__ off_out,
     int iteration, int32_t* __restrict__ chg_v, int32_t* __restrict__ chg_e,
-    uint32_t* __restrict__ chg_bits, DevCounters* __restrict__ ctr) {
+    uint32_t* __restrict__ chg_bits, DevCounters* __restrict__ ctr,
+    source_value_t<P>* __restrict__ srcval) {
   __shared__ int s_a[kApplyNT / kWave], s_b[kApplyNT / kWave];
   __shared__ unsigned long long s_base;
   const int tid = threadIdx.x;
@@ -99,7 +100,10 @@ __global__ __launch_bounds__(kApplyNT) void k_apply(
         typename P::VertexData v = vd[dv];
         const typename P::VertexData old = v;
         r = P::apply(&v, g, iteration);
-        if (!pod_equal(v, old)) vd[dv] = v;
+        if (!pod_equal(v, old)) {
+          vd[dv] = v;
+          if constexpr (source_value_v<P>) srcval[dv] = P::sourceValue(v);
+        }
         if (r) deg = off_out[dv + 1] - off_out[dv];
       }
       const bool want = r && deg > 0;
@@ -134,6 +138,16 @@ __global__ __launch_bounds__(kApplyNT) void k_apply(
       }
       __syncthreads();
     }
+  }
+}
+
+// srcval[v] = P::sourceValue(vd[v]) for every vertex.
+template <class P>
+__global__ void k_source_values(int nV, const typename P::VertexData* __restrict__ vd,
+                                source_value_t<P>* __restrict__ srcval) {
+  if constexpr (source_value_v<P>) {
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < nV; v += gridDim.x * blockDim.x)
+      srcval[v] = P::sourceValue(vd[v]);
   }
 }
 
@@ -697,6 +711,7 @@ struct Engine final : EngineBase {
   ED* d_ed = nullptr;  // CSC order
   ED* d_ed_csr = nullptr;  // CSR order (delta-gather programs: push relax)
   G* d_cand = nullptr;     // push-relax candidates per vertex (gatherZero between gathers)
+  source_value_t<P>* d_srcval = nullptr;  // P::sourceValue of every vertex (programs with one)
   int32_t* d_active = nullptr;
   int32_t* d_escan = nullptr;
   int32_t* d_cbase = nullptr;  // per active row: CSC offset minus its escan value
@@ -750,7 +765,7 @@ struct Engine final : EngineBase {
   size_t edge_size() const override { return kEdgeEmpty ? 0 : sizeof(ED); }
 
   void release() {
-    void* ptrs[] = {d_cbase, d_heavy,   d_vd,       d_vd_init, d_vd_user, d_ed,   d_ed_csr, d_cand, d_active,
+    void* ptrs[] = {d_srcval, d_cbase, d_heavy,   d_vd,       d_vd_init, d_vd_user, d_ed,   d_ed_csr, d_cand, d_active,
                     d_escan, d_gtmp,    d_chg_v,    d_chg_e,   d_bitmap, d_chg_bits,
                     d_chg_prev, d_blk,  d_part,     d_push_part, d_carry, d_head,
                     d_ctr};
@@ -773,6 +788,7 @@ struct Engine final : EngineBase {
     d_ed = nullptr;
     d_ed_csr = nullptr;
     d_cand = nullptr;
+    d_srcval = nullptr;
     d_cbase = nullptr;
     d_active = d_escan = d_chg_v = d_chg_e = d_part = d_push_part = nullptr;
     d_gtmp = nullptr;
@@ -866,6 +882,10 @@ struct Engine final : EngineBase {
       GR_HIP(hipMemcpyAsync(d_vd, vdata, sizeof(VD) * V, hipMemcpyHostToDevice, stream));
     }
     GR_HIP(hipMemcpyAsync(d_vd_init, d_vd, sizeof(VD) * V, hipMemcpyDeviceToDevice, stream));
+    if constexpr (source_value_v<P>) {
+      GR_TRY(dalloc(&d_srcval, V));
+      GR_TRY(refresh_source_values());
+    }
     if constexpr (!kEdgeEmpty) {
       GR_TRY(dalloc(&d_ed, E));
       if (E > 0) {
@@ -1063,7 +1083,7 @@ struct Engine final : EngineBase {
 #define GR_GATHER_LAUNCH(VT)                                                                \
   k_gather_tiles<P, VT><<<n_tiles, kNT, 0, stream>>>(n, w, d_active, d_escan, d_part, d_cbase, \
                                                      g->srcs, d_vd, edge_ptr(), d_gtmp,       \
-                                                     d_carry, d_head, dbits)
+                                                     d_carry, d_head, dbits, d_srcval)
     switch (gather_vt) {
       case 4: GR_GATHER_LAUNCH(4); break;
       case 16: GR_GATHER_LAUNCH(16); break;
@@ -1135,7 +1155,7 @@ struct Engine final : EngineBase {
     tick();
     k_apply<P><<<grid_1d(n_active, kApplyChunk), kApplyNT, 0, stream>>>(
         n_active, d_active, d_gtmp, P::kHasGather ? 1 : 0, d_vd, g->off_out, iteration, d_chg_v,
-        d_chg_e, changed_bits_on() ? d_chg_bits : nullptr, d_ctr);
+        d_chg_e, changed_bits_on() ? d_chg_bits : nullptr, d_ctr, d_srcval);
     tock(kPhApply);
     GR_HIP(hipGetLastError());
     return GR_OK;
@@ -1214,8 +1234,19 @@ struct Engine final : EngineBase {
     return GR_OK;
   }
 
+  // d_srcval after the vertex data was written as a whole
+  int refresh_source_values() {
+    if constexpr (source_value_v<P>) {
+      k_source_values<P><<<grid_1d(g->nV, 256) < 8192 ? grid_1d(g->nV, 256) : 8192, 256, 0,
+                           stream>>>(g->nV, d_vd, d_srcval);
+      GR_HIP(hipGetLastError());
+    }
+    return GR_OK;
+  }
+
   int reset_vertex_data() override {
     GR_HIP(hipMemcpyAsync(d_vd, d_vd_init, sizeof(VD) * g->nV, hipMemcpyDeviceToDevice, stream));
+    GR_TRY(refresh_source_values());
     delta_chain = delta_ready = false;
     return GR_OK;
   }

@@ -109,6 +109,13 @@ struct PageRankProgram {
   __device__ static float gatherMap(const VertexData*, const VertexData* src, const EdgeData*) {
     return __fdiv_rn(src->rank, (float)src->numOutEdges);
   }
+  // gatherMap depends on the source through rank / numOutEdges only: the
+  // engine keeps that quotient per vertex (the same rounded division)
+  typedef float SourceValue;
+  __device__ static float sourceValue(const VertexData& v) {
+    return __fdiv_rn(v.rank, (float)v.numOutEdges);
+  }
+  __device__ static float gatherMapValue(const float& s, const EdgeData*) { return s; }
   __device__ static float gatherReduce(const float& left, const float& right) {
     return __fadd_rn(left, right);
   }
