@@ -23,6 +23,7 @@ __all__ = [
     "rmat_edges", "grid_edges", "hash_weights", "device_count", "device_info",
     "run_bfs", "run_sssp", "run_cc", "run_pagerank", "SSSP_INF", "pagerank_precise",
     "msbfs", "mssp", "upload_i32", "Server", "QUERY_BFS", "QUERY_SSSP", "QUERY_PPR",
+    "SSSP_CONGESTION", "read_graph", "write_gr", "write_mtx", "edge_list_to_csr",
 ]
 
 _PKG = Path(__file__).resolve().parent
@@ -30,6 +31,8 @@ LIB_PATH = _PKG / "lib" / "libgraphreduce_amd.so"
 
 # Program ids (include/graphreduce_amd/gr_abi.h)
 BFS, SSSP, PAGERANK, CC = 1, 2, 3, 4
+# SSSP on congesting links: a program whose scatter changes edge data
+SSSP_CONGESTION = 5
 
 # Vertex/edge data layouts of the reference programs (bfs.cu:33-38,
 # sssp.cu:28-29, pagerank.cu:32-39, connected_component.cu:28-29).
@@ -39,6 +42,7 @@ PROGRAMS = {
     PAGERANK: dict(name="PageRank",
                    vertex=np.dtype([("rank", "<f4"), ("numOutEdges", "<i4")]), edge=None),
     CC: dict(name="CC", vertex=np.dtype("<i4"), edge=None),
+    SSSP_CONGESTION: dict(name="SSSPCongestion", vertex=np.dtype("<i4"), edge=np.dtype("<i4")),
 }
 
 # SSSP::gatherZero = INT_MAX - maxLength (sssp.cu:32-33): the "unreached" distance.
@@ -158,6 +162,8 @@ _SIGS = {
     "gr_engine_set_iteration_count": (C.c_int, [_P, C.c_int32]),
     "gr_engine_get_iteration_count": (C.c_int, [_P, _I32P]),
     "gr_engine_read_vertex_data": (C.c_int, [_P, _P, C.c_size_t]),
+    "gr_engine_read_edge_data": (C.c_int, [_P, _P, C.c_size_t]),
+    "gr_program_has_scatter": (C.c_int, [C.c_int]),
     "gr_engine_reset_vertex_data": (C.c_int, [_P]),
     "gr_engine_set_direction": (C.c_int, [_P, C.c_int, C.c_double]),
     "gr_engine_set_gather_tile": (C.c_int, [_P, C.c_int]),
@@ -482,7 +488,9 @@ class GASEngine:
         _check(lib().gr_engine_create(program, C.byref(h)))
         self._h = h
         self._vdata = None
+        self._edata = None
         self._graph = None
+        self._scatter = lib().gr_program_has_scatter(program) == 1
 
     def close(self) -> None:
         if self._h:
@@ -519,6 +527,9 @@ class GASEngine:
         _check(lib().gr_engine_set_graph(self._h, nVertices, vd.ctypes.data_as(_P),
                                          vd.dtype.itemsize, nEdges, ep, esz, _p32(s), _p32(d)))
         self._vdata = vd
+        # a program whose scatter changes edge data gets it written back here
+        # by getResults (the reference engine works on the caller's array)
+        self._edata = keep if self._scatter else None
         self._graph = None
 
     def bindGraph(self, graph: Graph, vertexData, edgeData=None) -> None:
@@ -526,6 +537,7 @@ class GASEngine:
         vd = self._vertex_array(vertexData)
         if len(vd) != graph.n_vertices:
             raise GRError(1, "bindGraph: vertex data length != graph vertices")
+        _keep = None
         if isinstance(edgeData, DeviceArray):
             ep, esz, on_dev = edgeData.ptr, 4, 1
         else:
@@ -534,6 +546,7 @@ class GASEngine:
         _check(lib().gr_engine_bind_graph(self._h, graph.handle, vd.ctypes.data_as(_P),
                                           vd.dtype.itemsize, ep, esz, on_dev))
         self._vdata = vd
+        self._edata = _keep if (self._scatter and not on_dev) else None
         self._graph = graph
 
     @property
@@ -563,8 +576,33 @@ class GASEngine:
         _check(lib().gr_engine_next_iter(self._h, C.byref(n)))
         return n.value
 
+    @property
+    def edgeData(self) -> np.ndarray:
+        """The host edge array given to setGraph (programs with a scatter:
+        updated by getResults); None otherwise."""
+        return self._edata
+
     def getResults(self) -> None:
         _check(lib().gr_engine_get_results(self._h))
+        if self._edata is not None and len(self._edata):
+            _check(lib().gr_engine_read_edge_data(self._h, self._edata.ctypes.data_as(_P),
+                                                  self._edata.dtype.itemsize))
+
+    def readEdgeData(self) -> np.ndarray:
+        """The engine's edge data in edge-list order (a copy)."""
+        if self.edge_dtype is None:
+            raise GRError(1, f"{PROGRAMS[self.program]['name']} has no edge data")
+        if self._graph is not None:
+            m = self._graph.n_edges
+        elif self._edata is not None:
+            m = len(self._edata)
+        else:
+            raise GRError(4, "readEdgeData: edge count unknown for this binding")
+        out = np.empty(m, self.edge_dtype)
+        if m:
+            _check(lib().gr_engine_read_edge_data(self._h, out.ctypes.data_as(_P),
+                                                  out.dtype.itemsize))
+        return out
 
     def run(self) -> int:
         it = C.c_int32(0)

@@ -311,6 +311,55 @@ __global__ __launch_bounds__(kNT) void k_push_tiles(const DevCounters* __restric
   }
 }
 
+// Program::scatter over the out-edges of the changed list (refgas.h:172-196):
+// scatter(&vd[u], &vd[v], &edge) for every out-edge u -> v of a vertex whose
+// apply returned true.  Vertex data is read only and every edge is written by
+// its own source alone, so the calls are independent.  Edge data is in
+// edge-list order, reached through csr_eid.
+template <class P>
+__global__ __launch_bounds__(kNT) void k_scatter_tiles(
+    const DevCounters* __restrict__ ctr, const int32_t* __restrict__ part,
+    const int32_t* __restrict__ chg_v, const int32_t* __restrict__ chg_e,
+    const int32_t* __restrict__ off_out, const int32_t* __restrict__ dsts,
+    const int32_t* __restrict__ csr_eid, const typename P::VertexData* __restrict__ vd,
+    typename P::EdgeData* __restrict__ ed) {
+  __shared__ int32_t s_start[kTile + 2];
+  __shared__ int32_t s_csr[kTile + 2];
+  __shared__ int32_t s_u[kTile + 2];
+  const int w_c = (int)(uint32_t)ctr->chg;
+  const int n_tiles = (w_c + kTile - 1) / kTile;
+  const int tid = threadIdx.x;
+  for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    const int k0 = part[t], k1 = part[t + 1];
+    const int nr = k1 - k0 + 1;
+    const int e0 = t * kTile, e1 = min(e0 + kTile, w_c);
+    for (int k = tid; k < nr; k += kNT) {
+      const int u = chg_v[k0 + k];
+      s_u[k] = u;
+      s_start[k] = chg_e[k0 + k];
+      s_csr[k] = off_out[u];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < kVT; ++m) {
+      const int e = e0 + tid + m * kNT;
+      if (e < e1) {
+        int lo = 0, hi = nr - 1;
+        while (lo < hi) {
+          int mid = (lo + hi + 1) >> 1;
+          if (s_start[mid] <= e)
+            lo = mid;
+          else
+            hi = mid - 1;
+        }
+        const int pos = s_csr[lo] + (e - s_start[lo]);
+        P::scatter(vd + s_u[lo], vd + dsts[pos], ed + csr_eid[pos]);
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------- delta gather by push
 //
 // For delta-gather programs the gather of the next iteration only needs the
@@ -660,6 +709,15 @@ __global__ void k_permute_words(const uint32_t* __restrict__ in, const int32_t* 
   for (int k = 0; k < words; ++k) out[i * words + k] = in[s * words + k];
 }
 
+// out[eid[i]] = in[i] (the inverse of k_permute_words), n elements of `words` words.
+__global__ void k_unpermute_words(const uint32_t* __restrict__ in, const int32_t* __restrict__ eid,
+                                  int64_t n, int words, uint32_t* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t d = eid[i];
+  for (int k = 0; k < words; ++k) out[d * words + k] = in[i * words + k];
+}
+
 // ================================================================ host side
 
 struct EngineBase {
@@ -673,6 +731,7 @@ struct EngineBase {
   virtual int next_iter() = 0;
   virtual int get_results() = 0;
   virtual int read_vertex_data(void* out, size_t velem) = 0;
+  virtual int read_edge_data(void* out, size_t eelem) = 0;
   virtual int reset_vertex_data() = 0;
   virtual int set_delta_gather(int mode) = 0;
   virtual int sync_stats(gr_engine_stats* out) = 0;
@@ -697,6 +756,9 @@ struct Engine final : EngineBase {
   typedef typename P::EdgeData ED;
   typedef typename P::GatherResult G;
   static constexpr bool kEdgeEmpty = std::is_empty<ED>::value;
+  // Programs with a scatter keep ONE copy of their edge data, in edge-list
+  // order (the scatter changes it; the gather reads it through csc_eid).
+  static constexpr bool kScatter = P::kHasScatter && !std::is_empty<typename P::EdgeData>::value;
 
   hipStream_t stream = nullptr;
   gr_graph* g = nullptr;       // the layout the kernels run on (user_g or its ordered copy)
@@ -708,7 +770,7 @@ struct Engine final : EngineBase {
   VD* d_vd = nullptr;
   VD* d_vd_init = nullptr;  // vertex data as given at bind time
   VD* d_vd_user = nullptr;  // staging in user order (ordered layout only)
-  ED* d_ed = nullptr;  // CSC order
+  ED* d_ed = nullptr;  // CSC order (edge-list order for programs with a scatter)
   ED* d_ed_csr = nullptr;  // CSR order (delta-gather programs: push relax)
   G* d_cand = nullptr;     // push-relax candidates per vertex (gatherZero between gathers)
   source_value_t<P>* d_srcval = nullptr;  // P::sourceValue of every vertex (programs with one)
@@ -896,10 +958,14 @@ struct Engine final : EngineBase {
           GR_HIP(hipMemcpyAsync(tmp, edata, sizeof(ED) * E, hipMemcpyHostToDevice, stream));
           src = tmp;
         }
-        k_permute_words<<<grid_1d(E, 256), 256, 0, stream>>>(
-            reinterpret_cast<const uint32_t*>(src), g->csc_eid, E, (int)(sizeof(ED) / 4),
-            reinterpret_cast<uint32_t*>(d_ed));
-        GR_HIP(hipGetLastError());
+        if constexpr (kScatter) {
+          GR_HIP(hipMemcpyAsync(d_ed, src, sizeof(ED) * E, hipMemcpyDeviceToDevice, stream));
+        } else {
+          k_permute_words<<<grid_1d(E, 256), 256, 0, stream>>>(
+              reinterpret_cast<const uint32_t*>(src), g->csc_eid, E, (int)(sizeof(ED) / 4),
+              reinterpret_cast<uint32_t*>(d_ed));
+          GR_HIP(hipGetLastError());
+        }
         if constexpr (kDelta) {
           GR_TRY(dalloc(&d_ed_csr, E));
           k_permute_words<<<grid_1d(E, 256), 256, 0, stream>>>(
@@ -1083,7 +1149,7 @@ struct Engine final : EngineBase {
 #define GR_GATHER_LAUNCH(VT)                                                                \
   k_gather_tiles<P, VT><<<n_tiles, kNT, 0, stream>>>(n, w, d_active, d_escan, d_part, d_cbase, \
                                                      g->srcs, d_vd, edge_ptr(), d_gtmp,       \
-                                                     d_carry, d_head, dbits, d_srcval)
+                                                     d_carry, d_head, dbits, d_srcval, g->csc_eid)
     switch (gather_vt) {
       case 4: GR_GATHER_LAUNCH(4); break;
       case 16: GR_GATHER_LAUNCH(16); break;
@@ -1165,9 +1231,7 @@ struct Engine final : EngineBase {
     return (long long)((double)g->nE / (dir_alpha > 0 ? dir_alpha : 1.0));
   }
 
-  int scatter_activate(bool) override {
-    // Program::scatter is a no-op for every compiled program (kHasScatter).
-    static_assert(!P::kHasScatter, "non-trivial scatter is not compiled in");
+  int scatter_activate(bool have_scatter) override {
     it_scatter = true;
     const long long thr = pull_threshold();
     tick();
@@ -1186,6 +1250,17 @@ struct Engine final : EngineBase {
       if (grid < 1) grid = 1;
       k_push_tiles<<<grid, kNT, 0, stream>>>(d_ctr, dir_mode, thr, d_push_part, d_chg_v, d_chg_e,
                                              g->off_out, g->dsts, d_bitmap);
+    }
+    if constexpr (kScatter) {
+      // Program::scatter, as the reference's CPU engine runs it when
+      // haveScatter (refgas.h:192-196); its GPU engine ignores the flag
+      if (have_scatter) {
+        k_push_partition<<<grid_1d((int64_t)max_push_tiles + 1, 256), 256, 0, stream>>>(
+            d_ctr, 1, 0, d_chg_e, d_push_part, max_push_tiles);
+        const int grid = max_push_tiles < 8192 ? max_push_tiles : 8192;
+        k_scatter_tiles<P><<<grid < 1 ? 1 : grid, kNT, 0, stream>>>(
+            d_ctr, d_push_part, d_chg_v, d_chg_e, g->off_out, g->dsts, g->csr_eid, d_vd, d_ed);
+      }
     }
     tock(kPhScatter);
     GR_HIP(hipGetLastError());
@@ -1256,6 +1331,32 @@ struct Engine final : EngineBase {
     return GR_OK;
   }
 
+  int read_edge_data(void* out, size_t eelem) override {
+    if constexpr (kEdgeEmpty) {
+      (void)out;
+      return eelem == 0 ? GR_OK : set_error(GR_ERR_INVALID, "this program has no edge data");
+    } else {
+      if (eelem != sizeof(ED)) return set_error(GR_ERR_INVALID, "edge element size mismatch");
+      const int64_t E = g->nE;
+      if (E == 0) return GR_OK;
+      if constexpr (kScatter) {
+        GR_HIP(hipMemcpyAsync(out, d_ed, sizeof(ED) * E, hipMemcpyDeviceToHost, stream));
+      } else {
+        // CSC order -> edge-list order: slot i holds edge csc_eid[i]
+        ED* tmp = nullptr;
+        GR_TRY(dalloc(&tmp, E));
+        k_unpermute_words<<<grid_1d(E, 256), 256, 0, stream>>>(
+            reinterpret_cast<const uint32_t*>(d_ed), g->csc_eid, E, (int)(sizeof(ED) / 4),
+            reinterpret_cast<uint32_t*>(tmp));
+        GR_HIP(hipMemcpyAsync(out, tmp, sizeof(ED) * E, hipMemcpyDeviceToHost, stream));
+        GR_HIP(hipStreamSynchronize(stream));
+        (void)hipFree(tmp);
+      }
+      GR_HIP(hipStreamSynchronize(stream));
+      return GR_OK;
+    }
+  }
+
   int read_vertex_data(void* out, size_t velem) override {
     if (velem != sizeof(VD)) return set_error(GR_ERR_INVALID, "vertex element size mismatch");
     const VD* src = user_order_vd();
@@ -1307,6 +1408,7 @@ EngineBase* make_engine(int program) {
     case GR_PROGRAM_SSSP: return new Engine<SSSPProgram>();
     case GR_PROGRAM_PAGERANK: return new Engine<PageRankProgram>();
     case GR_PROGRAM_CC: return new Engine<CCProgram>();
+    case GR_PROGRAM_SSSP_CONGESTION: return new Engine<SSSPCongestionProgram>();
     default: return nullptr;
   }
 }
@@ -1434,6 +1536,23 @@ extern "C" int gr_engine_read_vertex_data(gr_engine* e, void* out, size_t vertex
   GR_TRY(check(e));
   if (!out) return set_error(GR_ERR_INVALID, "out is NULL");
   return e->impl->read_vertex_data(out, vertex_elem_size);
+}
+
+extern "C" int gr_engine_read_edge_data(gr_engine* e, void* out, size_t edge_elem_size) {
+  GR_TRY(check(e));
+  if (!out && edge_elem_size) return set_error(GR_ERR_INVALID, "out is NULL");
+  return e->impl->read_edge_data(out, edge_elem_size);
+}
+
+extern "C" int gr_program_has_scatter(int program_id) {
+  switch (program_id) {
+    case GR_PROGRAM_BFS: return BFSProgram::kHasScatter ? 1 : 0;
+    case GR_PROGRAM_SSSP: return SSSPProgram::kHasScatter ? 1 : 0;
+    case GR_PROGRAM_PAGERANK: return PageRankProgram::kHasScatter ? 1 : 0;
+    case GR_PROGRAM_CC: return CCProgram::kHasScatter ? 1 : 0;
+    case GR_PROGRAM_SSSP_CONGESTION: return SSSPCongestionProgram::kHasScatter ? 1 : 0;
+    default: return -1;
+  }
 }
 
 extern "C" int gr_engine_reset_vertex_data(gr_engine* e) {

@@ -182,6 +182,18 @@ constexpr bool source_value_v = source_value<P>::value;
 template <class P>
 using source_value_t = typename source_value<P>::type;
 
+// Position of the edge data of CSC slot c: programs with a scatter keep their
+// edge data in edge-list order (one copy that the CSR-ordered scatter writes
+// and the gather reads through csc_eid, as refgas.h:150-152 and 192-196 do),
+// the others keep a CSC-ordered copy for coalesced reads.
+template <class P>
+__device__ __forceinline__ int edge_pos(const int32_t* __restrict__ ed_index, int c) {
+  if constexpr (P::kHasScatter)
+    return ed_index[c];
+  else
+    return c;
+}
+
 // gatherMap of a source, through its kept source value when there is one.
 template <class P>
 __device__ __forceinline__ typename P::GatherResult map_source(
@@ -219,7 +231,8 @@ __global__ __launch_bounds__(kNT) void k_gather_tiles(
     const typename P::EdgeData* __restrict__ ed, typename P::GatherResult* __restrict__ gtmp,
     Part<typename P::GatherResult>* __restrict__ carry,
     Part<typename P::GatherResult>* __restrict__ head,
-    const uint32_t* __restrict__ delta_bits, const source_value_t<P>* __restrict__ srcval) {
+    const uint32_t* __restrict__ delta_bits, const source_value_t<P>* __restrict__ srcval,
+    const int32_t* __restrict__ ed_index) {
   typedef typename P::GatherResult G;
   typedef Seg<P> S;
   constexpr int TILE = kNT * VT;
@@ -303,7 +316,7 @@ __global__ __launch_bounds__(kNT) void k_gather_tiles(
           int e = tid + m * kNT;
           if (e < n_edges) {
             if ((word[m] >> (src_v[m] & 31)) & 1u) {
-              const typename P::EdgeData ev = stream_load(ed + csc_v[m]);
+              const typename P::EdgeData ev = stream_load(ed + edge_pos<P>(ed_index, csc_v[m]));
               s_val[e] = map_source<P>(vd, vd, srcval, src_v[m], &ev);
             } else {
               s_val[e] = P::gatherZero;
@@ -319,7 +332,7 @@ __global__ __launch_bounds__(kNT) void k_gather_tiles(
         if (e < n_edges) {
           const typename P::VertexData* dstp = vd;
           if constexpr (P::kGatherUsesDst) dstp = vd + active[i0 + s_row[e]];
-          const typename P::EdgeData ev = stream_load(ed + csc_v[m]);
+          const typename P::EdgeData ev = stream_load(ed + edge_pos<P>(ed_index, csc_v[m]));
           s_val[e] = map_source<P>(dstp, vd, srcval, src_v[m], &ev);
         }
       }

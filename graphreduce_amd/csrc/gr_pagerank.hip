@@ -200,7 +200,8 @@ int pagerank_precise(gr_graph* g, int max_iters, double tol, float* ranks, doubl
       k_gather_tiles<P, VT><<<n_tiles, kNT, 0, st>>>(nV, w, nullptr, g->off_in, part.p, cbase.p,
                                                      g->srcs, contrib.p,
                                                      reinterpret_cast<const typename P::EdgeData*>(contrib.p),
-                                                     gsum.p, carry.p, head.p, nullptr, nullptr);
+                                                     gsum.p, carry.p, head.p, nullptr, nullptr,
+                                                     nullptr);
     GR_HIP(hipEventRecord(g1, st));
     k_gather_fixup<P><<<grid_1d(n_tiles, 256), 256, 0, st>>>(n_tiles, carry.p, head.p, gsum.p);
     k_pr_update<G><<<n_ublocks, kPRNT, 0, st>>>(nV, gsum.p, g->off_out, rank.p, contrib.p,

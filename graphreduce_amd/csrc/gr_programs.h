@@ -154,6 +154,40 @@ struct CCProgram {
   __device__ static void scatter(const VertexData*, const VertexData*, EdgeData*) {}
 };
 
+// A program with a non-trivial scatter (the four reference apps have none),
+// for the general Program::scatter of refgas.h:172-196: the SSSP gather and
+// apply of sssp.cu:25-66 on links that congest.  Each time a vertex's
+// distance drops, its out-edges toward vertices still farther away get one
+// unit heavier.  scatter runs for every out-edge of a vertex whose apply
+// returned true, after all applies of the iteration, and only writes that
+// edge's data, so the calls are independent of each other.
+struct SSSPCongestionProgram {
+  typedef int VertexData;
+  typedef int EdgeData;
+  typedef int GatherResult;
+  static constexpr int gatherZero = INT_MAX - 100000;
+  static constexpr bool kHasGather = true;
+  static constexpr bool kHasScatter = true;
+  static constexpr bool kGatherUsesDst = false;
+  static constexpr bool kDeltaGather = false;
+
+  __device__ static int gatherReduce(const int& left, const int& right) {
+    return min(left, right);
+  }
+  __device__ static int gatherMap(const VertexData*, const VertexData* srcDist,
+                                  const EdgeData* edgeLen) {
+    return *srcDist + *edgeLen;
+  }
+  __device__ static bool apply(VertexData* curDist, GatherResult dist, int) {
+    bool changed = dist < *curDist;
+    *curDist = min(*curDist, dist);
+    return changed;
+  }
+  __device__ static void scatter(const VertexData* src, const VertexData* dst, EdgeData* edge) {
+    if (*dst > *src) *edge += 1;
+  }
+};
+
 }  // namespace gr
 
 #endif

@@ -12,7 +12,9 @@
 //    cudaMemcpyToSymbol (bfs.cu:27-28, 83-90); here the engine hands its
 //    iteration number to Program::apply, set with setIterationCount().
 //  - getResults() copies the device vertex data back into the caller's
-//    array (the reference's copy is compiled out, gpugas.h:1778-1787).
+//    array (the reference's copy is compiled out, gpugas.h:1778-1787), and
+//    for a program whose scatter changes edge data also the edge data (the
+//    reference's CPU engine works on the caller's arrays, refgas.h:69-72).
 //  - Errors print the library's message and abort(), like the reference's
 //    CHECK (gpugas.h:158-165).
 #ifndef GRAPHREDUCE_AMD_GASENGINEGPU_H
@@ -60,6 +62,7 @@ class GASEngineGPU {
     check(gr_engine_set_graph(e_, nVertices, vertexData, sizeof(VertexData), nEdges,
                               esz ? edgeData : nullptr, esz, edgeListSrcs, edgeListDsts),
           "setGraph");
+    edgeData_ = esz && gr_program_has_scatter(ProgramId<Program>::value) == 1 ? edgeData : nullptr;
   }
 
   // gpugas.h:1524
@@ -93,7 +96,10 @@ class GASEngineGPU {
   }
 
   // gpugas.h:1778
-  void getResults() { check(gr_engine_get_results(e_), "getResults"); }
+  void getResults() {
+    check(gr_engine_get_results(e_), "getResults");
+    if (edgeData_) check(gr_engine_read_edge_data(e_, edgeData_, sizeof(EdgeData)), "getResults");
+  }
 
   // gpugas.h:1792
   void run() {
@@ -116,6 +122,7 @@ class GASEngineGPU {
 
  private:
   gr_engine* e_ = nullptr;
+  EdgeData* edgeData_ = nullptr;  // written back by getResults (programs with a scatter)
 };
 
 }  // namespace graphreduce_amd

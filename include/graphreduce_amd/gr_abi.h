@@ -44,6 +44,11 @@ extern "C" {
 #define GR_PROGRAM_SSSP 2
 #define GR_PROGRAM_PAGERANK 3
 #define GR_PROGRAM_CC 4
+/* SSSP on congesting links: the gather/apply of sssp.cu with a scatter that
+ * adds 1 to every out-edge weight (int32) of an improved vertex whose target
+ * is still farther away.  An example of a program with a non-trivial
+ * Program::scatter (refgas.h:172-196); its edge data changes during a run. */
+#define GR_PROGRAM_SSSP_CONGESTION 5
 
 typedef struct gr_graph gr_graph;
 typedef struct gr_engine gr_engine;
@@ -193,6 +198,16 @@ int gr_engine_get_iteration_count(gr_engine* e, int32_t* iteration);
 /* Copies device vertex data into out (n_vertices elements) without touching
  * the bound host array. */
 int gr_engine_read_vertex_data(gr_engine* e, void* out, size_t vertex_elem_size);
+
+/* Copies the engine's edge data into out (n_edges elements, the caller's
+ * edge-list order).  Programs with a scatter change it during a run: the
+ * reference's engine works on the caller's array, so its changes are visible
+ * there (refgas.h:69-72, 192-196); GASEngineGPU::getResults copies it back. */
+int gr_engine_read_edge_data(gr_engine* e, void* out, size_t edge_elem_size);
+
+/* 1 if the program's scatter may change edge data, 0 if not, -1 for an
+ * unknown program id. */
+int gr_program_has_scatter(int program_id);
 
 /* Restores the device vertex data to the values given at setGraph/bindGraph
  * with a device-to-device copy (repeated runs without a host round trip;

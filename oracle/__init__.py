@@ -32,6 +32,7 @@ _SIGS = {
     "oracle_bfs": [C.c_int32, C.c_int64, _I32P, _I32P, C.c_int32, _I32P, _I64P],
     "oracle_bfs_csr": [C.c_int32, _I32P, _I32P, _I32P, _I32P, C.c_int32, _I32P, _I64P],
     "oracle_sssp": [C.c_int32, C.c_int64, _I32P, _I32P, _I32P, C.c_int32, _I32P, _I64P],
+    "oracle_sssp_congestion": [C.c_int32, C.c_int64, _I32P, _I32P, _I32P, C.c_int32, _I32P, _I64P],
     "oracle_sssp_csr": [C.c_int32, _I32P, _I32P, _I32P, _I32P, _I32P, C.c_int32, _I32P, _I64P],
     "oracle_cc": [C.c_int32, C.c_int64, _I32P, _I32P, _I32P, _I64P],
     "oracle_cc_csr": [C.c_int32, _I32P, _I32P, _I32P, _I32P, _I32P, _I64P],
@@ -104,6 +105,19 @@ def sssp(n, srcs, dsts, weights, source, L=None):
     st = np.zeros(6, np.int64)
     it = L.oracle_sssp(n, len(s), _p(s), _p(d), _p(w), source, _p(out), _p(st, _I64P))
     return out, it, _stats(st)
+
+
+def sssp_congestion(n, srcs, dsts, weights, source, L=None):
+    """SSSP on congesting links (a program whose scatter changes edge data):
+    returns (dist, trips, stats, weights after the run, edge-list order)."""
+    L = L or lib()
+    s, d = _i32(srcs), _i32(dsts)
+    w = np.array(weights, dtype=np.int32, copy=True)
+    out = np.empty(n, np.int32)
+    st = np.zeros(6, np.int64)
+    it = L.oracle_sssp_congestion(n, len(s), _p(s), _p(d), _p(w), source, _p(out),
+                                  _p(st, _I64P))
+    return out, it, _stats(st), w
 
 
 def sssp_csr(n, off_out, dsts, off_in, srcs, weights_csc, source, L=None):

@@ -279,6 +279,29 @@ struct CC {
   static void scatter(const VertexData*, const VertexData*, EdgeData*) {}
 };
 
+// SSSP on congesting links (the library's GR_PROGRAM_SSSP_CONGESTION): the
+// SSSP gather/apply, and a scatter that adds 1 to an out-edge of an improved
+// vertex whose target is still farther away (called by scatterActivate as
+// refgas.h:192-196 does).
+struct SSSPCongestion {
+  typedef int VertexData;
+  typedef int EdgeData;
+  typedef int GatherResult;
+  static constexpr int gatherZero = INT_MAX - 100000;
+  static int gatherReduce(const int& a, const int& b) { return std::min(a, b); }
+  static int gatherMap(const VertexData*, const VertexData* src, const EdgeData* w) {
+    return *src + *w;
+  }
+  static bool apply(VertexData* cur, GatherResult d, int) {
+    bool changed = d < *cur;
+    *cur = std::min(*cur, d);
+    return changed;
+  }
+  static void scatter(const VertexData* src, const VertexData* dst, EdgeData* w) {
+    if (*dst > *src) *w += 1;
+  }
+};
+
 }  // namespace oracle
 
 #endif

@@ -108,6 +108,20 @@ int oracle_sssp(int32_t nV, int64_t nE, const int32_t* srcs, const int32_t* dsts
   return it;
 }
 
+// weights: in, edge-list order; out: the weights after the run (scatter)
+int oracle_sssp_congestion(int32_t nV, int64_t nE, const int32_t* srcs, const int32_t* dsts,
+                           int32_t* weights, int32_t source, int32_t* dist, int64_t* stats) {
+  std::vector<int> vd((size_t)nV, SSSPCongestion::gatherZero);
+  vd[source] = 0;
+  Engine<SSSPCongestion> eng;
+  eng.setGraph(nV, vd.data(), nE, weights, srcs, dsts);
+  int64_t t0 = now_ns();
+  int32_t it = all_active_loop(eng, nV);
+  put_stats(eng, now_ns() - t0, stats);
+  std::memcpy(dist, vd.data(), sizeof(int32_t) * nV);
+  return it;
+}
+
 // weights_csc in CSC order
 int oracle_sssp_csr(int32_t nV, const int32_t* off_out, const int32_t* dsts, const int32_t* off_in,
                     const int32_t* srcs, const int32_t* weights_csc, int32_t source, int32_t* dist,

@@ -454,3 +454,89 @@ def test_vertex_order_pagerank_within_tolerance(gpu):
     b, tb, _ = _ordered_run(gr, gr.PAGERANK, n, vd, s, d, order=1)
     assert np.array_equal(a["numOutEdges"], b["numOutEdges"])
     assert np.allclose(a["rank"], b["rank"], rtol=1e-5, atol=1e-6)
+
+
+# --------------------------------------------- Program::scatter on edge data
+
+def _congestion_run(gr, n, s, d, w, src, order=1, have_scatter=True, bind=False):
+    vd = np.full(n, gr.SSSP_INF, np.int32)
+    vd[src] = 0
+    eng = gr.GASEngine(gr.SSSP_CONGESTION)
+    eng.setVertexOrder(order)
+    w_io = np.array(w, np.int32)
+    if bind:
+        g = gr.Graph.from_edges(n, s, d)
+        eng.bindGraph(g, vd, w_io)
+    else:
+        eng.setGraph(n, vd, len(s), w_io, s, d)
+    eng.setActive(0, n)
+    trips = 0
+    while eng.countActive():
+        eng.gather()
+        eng.apply()
+        eng.scatterActivate(have_scatter)
+        eng.nextIter()
+        trips += 1
+    eng.getResults()
+    out = eng.vertexData.copy()
+    w_dev = eng.readEdgeData()
+    eng.close()
+    return out, trips, w_io, w_dev
+
+
+@pytest.mark.parametrize("order", [0, 1])
+@pytest.mark.parametrize("case", ["rmat12", "grid40", "multigraph"])
+def test_scatter_program_matches_oracle(gpu, case, order):
+    """A program whose scatter writes edge data (refgas.h:192-196): distances,
+    trips and the final edge data equal the oracle's; getResults writes the
+    edge data back into the caller's array, readEdgeData gives the same."""
+    gr = gpu
+    rng = np.random.default_rng(4)
+    if case == "rmat12":
+        n, s, d = _rmat(gr, 12, seed=5)
+    elif case == "grid40":
+        n, (s, d) = 1600, gr.grid_edges(40, 0.05, seed=2)
+    else:
+        n, m = 300, 2000
+        s = rng.integers(0, n, m).astype(np.int32)
+        d = rng.integers(0, n, m).astype(np.int32)
+        s = np.concatenate([s, s[:40], np.arange(5, dtype=np.int32)])  # duplicates, self-loops
+        d = np.concatenate([d, d[:40], np.arange(5, dtype=np.int32)])
+    w = gr.hash_weights(len(s), seed=6)
+    src = _max_out_src(n, s)
+    ref, rtrips, _, wref = oracle.sssp_congestion(n, s, d, w, src)
+    assert not np.array_equal(wref, w)  # the scatter did change edges
+    got, trips, w_io, w_dev = _congestion_run(gr, n, s, d, w, src, order=order)
+    assert np.array_equal(got, ref) and trips == rtrips
+    assert np.array_equal(w_io, wref) and np.array_equal(w_dev, wref)
+    got, trips, w_io, w_dev = _congestion_run(gr, n, s, d, w, src, order=order, bind=True)
+    assert np.array_equal(got, ref) and trips == rtrips and np.array_equal(w_dev, wref)
+
+
+def test_scatter_flag_off_is_plain_sssp(gpu):
+    """scatterActivate(false) skips Program::scatter, as the reference's CPU
+    engine does (refgas.h:192): the edges stay as given and the distances are
+    plain SSSP's."""
+    gr = gpu
+    n, s, d = _rmat(gr, 11, seed=2)
+    w = gr.hash_weights(len(s), seed=2)
+    src = _max_out_src(n, s)
+    got, trips, w_io, w_dev = _congestion_run(gr, n, s, d, w, src, have_scatter=False)
+    ref, rtrips, _ = oracle.sssp(n, s, d, w, src)
+    assert np.array_equal(got, ref) and trips == rtrips
+    assert np.array_equal(w_io, w) and np.array_equal(w_dev, w)
+
+
+def test_scatter_program_through_cpp_header(gpu, tmp_path):
+    """GASEngineGPU.h: getResults() copies the scattered edge data back into
+    the caller's EdgeData array (hand case: see oracle test)."""
+    import subprocess
+    from pathlib import Path
+    root = Path(__file__).resolve().parents[1]
+    exe = tmp_path / "scatter_header_test"
+    subprocess.run(["g++", "-O1", "-std=c++17", f"-I{root / 'include'}",
+                    str(root / "tests" / "cpp" / "scatter_header_test.cpp"), "-o", str(exe),
+                    f"-L{root / 'graphreduce_amd' / 'lib'}", "-lgraphreduce_amd",
+                    f"-Wl,-rpath,{root / 'graphreduce_amd' / 'lib'}"], check=True, timeout=120)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60, check=True).stdout
+    assert "trips 2" in out and "dist 0 1 5 3" in out and "w 1 5 2 2" in out

@@ -169,10 +169,12 @@ def test_pagerank_fp64_all_active_vs_scipy():
 
 # ------------------------------------------------ brute force, tiny graphs
 
-def _brute_gas(n, s, d, vd, gather_map, reduce_fn, zero, apply_fn, active, it0=0):
+def _brute_gas(n, s, d, vd, gather_map, reduce_fn, zero, apply_fn, active, it0=0,
+               scatter_fn=None):
     """Independent Python statement of the bulk-synchronous loop of
     refgas.h:220-229 (separate gather and apply sweeps, flag-based
-    activation, ascending next active list)."""
+    activation with Program::scatter on every out-edge of a changed vertex,
+    ascending next active list)."""
     it = it0
     trips = 0
     while active:
@@ -189,6 +191,8 @@ def _brute_gas(n, s, d, vd, gather_map, reduce_fn, zero, apply_fn, active, it0=0
             for e in range(len(s)):
                 if s[e] == u:
                     flags.add(int(d[e]))
+                    if scatter_fn is not None:
+                        scatter_fn(vd, u, int(d[e]), e)
         active = sorted(flags)
         it += 1
         trips += 1
@@ -250,6 +254,29 @@ def test_brute_force_tiny(seed):
     got, t, _ = oracle.pagerank(n, s, d)
     assert t == trips
     assert np.array_equal(got, np.array(rk, np.float32))
+
+    # a program whose scatter changes edge data: SSSP on congesting links
+    dist = [INF] * n
+    dist[src] = 0
+    wc = [int(x) for x in w]
+
+    def congest(vd, u, v, e):
+        if vd[v] > vd[u]:
+            wc[e] += 1
+
+    trips = _brute_gas(n, s, d, dist, lambda vd, u, e: vd[u] + wc[e], min, INF, min_apply,
+                       list(range(n)), scatter_fn=congest)
+    got, t, _, wout = oracle.sssp_congestion(n, s, d, w, src)
+    assert got.tolist() == dist and t == trips and wout.tolist() == wc
+
+
+def test_congestion_program_by_hand():
+    """0->1 (1), 0->2 (5), 1->3 (1), 2->3 (1), source 0, all active: trip 1
+    sets d1 = 1, d2 = 5 and both raise their edge to the still-unreached 3
+    to weight 2; trip 2 gives d3 = min(1 + 2, 5 + 2) = 3; 3 has no out-edge."""
+    s, d, w = [0, 0, 1, 2], [1, 2, 3, 3], [1, 5, 1, 1]
+    dist, trips, _, wout = oracle.sssp_congestion(4, s, d, w, 0)
+    assert dist.tolist() == [0, 1, 5, 3] and trips == 2 and wout.tolist() == [1, 5, 2, 2]
 
 
 # ------------------------------------------------------------ golden files
