@@ -16,7 +16,9 @@ reference's TEPS, sssp.cu:195-213).  roofline = the merge-path gather kernel
 active vertex, divided by its measured duration (hipEvents on the engine's
 stream).  cpu_baseline = the CPU oracle (restated refgas.h engine, one
 thread) on the same graph, which also checks the GPU result bit for bit.
-Extra fields report plain BFS, CC and PageRank at the same scale.
+Extra fields report plain BFS, CC and PageRank at the same scale, SSSP with
+the edge arrays streamed from host memory, 64-source batches, the query
+server, CC on the 4096^2 grid and the PageRank precision form at RMAT-22.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -146,8 +148,8 @@ def main() -> int:
     eng.enableTiming(False)
     eng.resetStats()
     # Dominant kernel: the merge-path gather tile kernel, which runs the full
-    # gathers (the all-active first one and the dense iterations); the sparse
-    # iterations are gathered by pushing the changed out-edges (push_*).
+    # gathers (the dense iterations); the first, all-active gather and the
+    # sparse iterations are gathered by pushing out-edges (push_*).
     tile_bytes = 12 * st["tile_gather_edges"] + 16 * st["tile_gather_rows"]
     gather_s = st["gather_ms"] / 1e3
     achieved = tile_bytes / gather_s / 1e9 if gather_s > 0 else 0.0
@@ -316,6 +318,8 @@ def run_extras(gr, g, args, src, outdeg, w_dev):
                      gather_alg_gbs=round((8 * ein + 16 * st["active_vertices"])
                                           / ((st["gather_ms"] + st["push_ms"]) / 1e3) / 1e9, 1))
     log(f"CC: {out['cc']}")
+    out["sssp_out_of_core"] = run_out_of_core(gr, g, args, src, outdeg, w_dev)
+    log(f"SSSP out of core: {out['sssp_out_of_core']}")
     out["batched"] = run_batched(gr, g, args, outdeg)
     out["serving"] = run_serving(gr, g, args, outdeg, w_dev)
     out["cc_grid_4096"] = run_cc_grid(gr)
@@ -472,6 +476,47 @@ def run_serving(gr, g, args, outdeg, w_dev, n_req=384, rate=2000.0, per_kind_sam
     res["sssp_equals_plain_engine"] = bool(np.array_equal(row, plain))
     log(f"serving: {res}")
     return res
+
+
+def run_out_of_core(gr, g, args, src, outdeg, w_dev, shard_edges=1 << 25):
+    """Plain SSSP with the edge arrays in pinned host memory, cut into vertex
+    ranges of <= 32M edges and streamed every iteration (the reference's
+    out-of-core shards, gpugas.h:1398-1619); distances must equal the
+    in-memory run's."""
+    n = g.n_vertices
+    s, d = g.edges()
+    w = gr.hash_weights(len(s), seed=args.seed)
+    vd = np.full(n, gr.SSSP_INF, np.int32)
+    vd[src] = 0
+    eng = gr.GASEngine(gr.SSSP)
+    eng.setStreaming(shard_edges)
+    t0 = time.perf_counter()
+    eng.setGraph(n, vd, len(s), w, s, d)
+    setup = time.perf_counter() - t0
+    del s, d, w
+    trips = run_loop(eng, all_active_n=n)  # warm-up
+    eng.resetStats()
+    eng.enableTiming(True)
+    gr.synchronize()
+    t0 = time.perf_counter()
+    trips = run_loop(eng, all_active_n=n)
+    gr.synchronize()
+    sec = time.perf_counter() - t0
+    st = eng.stats()
+    dist = eng.readVertexData()
+    eng.close()
+    ref = gr.GASEngine(gr.SSSP)
+    ref.bindGraph(g, vd, w_dev)
+    run_loop(ref, all_active_n=n)
+    same = bool(np.array_equal(dist, ref.readVertexData()))
+    ref.close()
+    e_r = int(outdeg[dist < gr.SSSP_INF].sum())
+    return dict(shard_edges=shard_edges, trips=trips, ms=round(sec * 1e3, 2),
+                gteps=round(e_r / sec / 1e9, 3), streamed_gb=round(st["streamed_bytes"] / 1e9, 2),
+                shard_copies=st["streamed_shards"],
+                h2d_gbs=round(st["streamed_bytes"] / sec / 1e9, 1),
+                phase_ms={k: round(st[k + "_ms"], 2) for k in ("gather", "scatter", "apply", "next")},
+                setup_s=round(setup, 1), equals_in_memory=same)
 
 
 def run_cc_grid(gr):

@@ -83,6 +83,8 @@ class _Stats(C.Structure):
         ("tile_gather_edges", C.c_int64),
         ("inert_gathers", C.c_int64),
         ("inert_ms", C.c_double),
+        ("streamed_bytes", C.c_int64),
+        ("streamed_shards", C.c_int64),
     ]
 
 
@@ -169,6 +171,7 @@ _SIGS = {
     "gr_engine_set_gather_tile": (C.c_int, [_P, C.c_int]),
     "gr_engine_set_delta_gather": (C.c_int, [_P, C.c_int]),
     "gr_engine_set_vertex_order": (C.c_int, [_P, C.c_int]),
+    "gr_engine_set_streaming": (C.c_int, [_P, C.c_int64]),
     "gr_engine_enable_timing": (C.c_int, [_P, C.c_int]),
     "gr_engine_get_stats": (C.c_int, [_P, C.POINTER(_Stats)]),
     "gr_engine_reset_stats": (C.c_int, [_P]),
@@ -625,6 +628,12 @@ class GASEngine:
     def setDirection(self, mode: int = 0, alpha: float = 15.0) -> None:
         """0 = automatic push/pull, 1 = push only, 2 = pull only."""
         _check(lib().gr_engine_set_direction(self._h, mode, alpha))
+
+    def setStreaming(self, max_edges_per_shard: int) -> None:
+        """Out-of-core mode for the next setGraph: the edge arrays stay in
+        pinned host memory in shards of at most this many edges and are
+        streamed per iteration (0: the graph lives in device memory)."""
+        _check(lib().gr_engine_set_streaming(self._h, int(max_edges_per_shard)))
 
     def setVertexOrder(self, order: int = 1) -> None:
         """1 (default): run on the degree-ordered copy of the graph; 0: as

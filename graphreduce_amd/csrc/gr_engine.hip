@@ -709,6 +709,91 @@ __global__ void k_permute_words(const uint32_t* __restrict__ in, const int32_t* 
   for (int k = 0; k < words; ++k) out[i * words + k] = in[s * words + k];
 }
 
+// ------------------------------------------------ out-of-core shards
+//
+// With the edge arrays in host memory (gr_engine_set_streaming), vertex-sized
+// state stays on the device and every iteration streams the shards it needs:
+// the CSC slices of the destination ranges that hold active rows (gather) and
+// the CSR slices of the source ranges that hold changed vertices (activation).
+
+// pos[s] = first active-list index with vertex >= bound[s]; wsum[s] = escan
+// there (the active list is ascending).
+__global__ void k_shard_positions(int n_bounds, const int32_t* __restrict__ bound,
+                                  const int32_t* __restrict__ active, int n_active,
+                                  const int32_t* __restrict__ escan, int32_t* __restrict__ pos,
+                                  int32_t* __restrict__ wsum) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_bounds) return;
+  const int b = bound[s];
+  int lo = 0, hi = n_active;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (active[mid] < b)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  pos[s] = lo;
+  wsum[s] = escan[lo];
+}
+
+// The active sub-list [a0, a0 + n] of one shard as a list of its own: its
+// in-degree scan from 0 and CSC bases relative to the shard's first edge e0.
+__global__ void k_shard_rebase(int n, const int32_t* __restrict__ escan,
+                               const int32_t* __restrict__ cbase, int a0, int e0,
+                               int32_t* __restrict__ escan_sub, int32_t* __restrict__ cbase_sub) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  const int base = escan[a0];
+  escan_sub[i] = escan[a0 + i] - base;
+  if (i < n) cbase_sub[i] = cbase[a0 + i] + base - e0;
+}
+
+// cnt[s] = changed vertices (bits set) in [bound[s], bound[s + 1]); one
+// workgroup per shard.
+__global__ void k_shard_changed(int n_shards, const int32_t* __restrict__ bound,
+                                const uint32_t* __restrict__ bits, int32_t* __restrict__ cnt) {
+  __shared__ int s_sum;
+  const int s = blockIdx.x;
+  if (s >= n_shards) return;
+  if (threadIdx.x == 0) s_sum = 0;
+  __syncthreads();
+  const int v0 = bound[s], v1 = bound[s + 1];
+  int c = 0;
+  for (int v = v0 + threadIdx.x; v < v1; v += blockDim.x) c += (bits[v >> 5] >> (v & 31)) & 1u;
+  atomicAdd(&s_sum, c);
+  __syncthreads();
+  if (threadIdx.x == 0) cnt[s] = s_sum;
+}
+
+// Activation from one staged CSR shard: every out-neighbour of a changed
+// vertex in [u0, u1) gets its frontier bit; dsts_stage holds the shard's
+// targets from CSR position e0 on.  One wave per 64 sources.
+__global__ __launch_bounds__(256) void k_shard_push(int u0, int u1, int e0,
+                                                    const int32_t* __restrict__ off_out,
+                                                    const int32_t* __restrict__ dsts_stage,
+                                                    const uint32_t* __restrict__ chg_bits,
+                                                    uint32_t* __restrict__ bitmap) {
+  const int lane = lane_id();
+  const int nw = gridDim.x * blockDim.x / kWave;
+  for (int base = u0 + (blockIdx.x * blockDim.x + threadIdx.x) / kWave * kWave; base < u1;
+       base += nw * kWave) {
+    const int u = base + lane;
+    const bool ch = u < u1 && ((chg_bits[u >> 5] >> (u & 31)) & 1u);
+    uint64_t todo = __ballot(ch);
+    while (todo) {
+      const int j = __builtin_ctzll(todo);
+      todo &= todo - 1;
+      const int b = off_out[base + j] - e0, e = off_out[base + j + 1] - e0;
+      for (int p = b + lane; p < e; p += kWave) {
+        const int v = dsts_stage[p];
+        const uint32_t bit = 1u << (v & 31);
+        if (!(bitmap[v >> 5] & bit)) atomicOr(&bitmap[v >> 5], bit);
+      }
+    }
+  }
+}
+
 // out[eid[i]] = in[i] (the inverse of k_permute_words), n elements of `words` words.
 __global__ void k_unpermute_words(const uint32_t* __restrict__ in, const int32_t* __restrict__ eid,
                                   int64_t n, int words, uint32_t* __restrict__ out) {
@@ -723,7 +808,10 @@ __global__ void k_unpermute_words(const uint32_t* __restrict__ in, const int32_t
 struct EngineBase {
   virtual ~EngineBase() {}
   virtual int bind(gr_graph* g, bool owns, void* vdata, size_t velem, const void* edata,
-                   size_t eelem, bool edata_on_device) = 0;
+                   size_t eelem, bool edata_on_device, bool stream_mode = false) = 0;
+  // setGraph with the edge arrays kept in host memory (stream_max_edges > 0)
+  virtual int bind_streaming(int32_t nV, void* vdata, size_t velem, int32_t nE, const void* edata,
+                             size_t eelem, const int32_t* srcs, const int32_t* dsts) = 0;
   virtual int set_active(int32_t start, int32_t end) = 0;
   virtual int gather() = 0;
   virtual int apply() = 0;
@@ -746,6 +834,7 @@ struct EngineBase {
   double dir_alpha = 15.0;
   int gather_vt = 8;  // merge-path items per thread in the gather tiles (4, 8 or 16)
   int vertex_order = 1;  // 1: engine works on the degree-ordered copy of the graph
+  int64_t stream_max_edges = 0;  // > 0: out-of-core shards of at most this many edges
   bool timing = false;
   gr_engine_stats stats{};
 };
@@ -774,6 +863,27 @@ struct Engine final : EngineBase {
   ED* d_ed_csr = nullptr;  // CSR order (delta-gather programs: push relax)
   G* d_cand = nullptr;     // push-relax candidates per vertex (gatherZero between gathers)
   source_value_t<P>* d_srcval = nullptr;  // P::sourceValue of every vertex (programs with one)
+  // ---- out-of-core mode: edge arrays in pinned host memory, streamed per shard
+  bool streaming = false;
+  int32_t* h_srcs = nullptr;  // CSC sources
+  int32_t* h_dsts = nullptr;  // CSR targets
+  ED* h_ed = nullptr;         // edge data in CSC order
+  std::vector<int32_t> h_csc_eid;         // CSC position -> edge-list index
+  std::vector<int32_t> csc_vb, csr_vb;    // shard vertex bounds
+  int32_t* d_csc_vb = nullptr;
+  int32_t* d_csr_vb = nullptr;
+  int32_t* d_spos = nullptr;   // per CSC shard: first active-list index, escan there
+  int32_t* d_swsum = nullptr;
+  int32_t* d_scnt = nullptr;   // per CSR shard: changed vertices
+  int32_t* h_sbuf = nullptr;   // pinned host copy of those
+  int32_t* d_stage_src[2] = {nullptr, nullptr};
+  ED* d_stage_ed[2] = {nullptr, nullptr};
+  int32_t* d_stage_dst[2] = {nullptr, nullptr};
+  int32_t* d_escan_sub = nullptr;
+  int32_t* d_cbase_sub = nullptr;
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t ev_copied[2] = {nullptr, nullptr};
+  hipEvent_t ev_used[2] = {nullptr, nullptr};
   int32_t* d_active = nullptr;
   int32_t* d_escan = nullptr;
   int32_t* d_cbase = nullptr;  // per active row: CSC offset minus its escan value
@@ -834,6 +944,33 @@ struct Engine final : EngineBase {
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     if (h_ctr) (void)hipHostFree(h_ctr);
+    {
+      void* sptrs[] = {d_csc_vb,       d_csr_vb,       d_spos,         d_swsum,
+                       d_scnt,         d_stage_src[0], d_stage_src[1], d_stage_ed[0],
+                       d_stage_ed[1],  d_stage_dst[0], d_stage_dst[1], d_escan_sub,
+                       d_cbase_sub};
+      for (void* p : sptrs)
+        if (p) (void)hipFree(p);
+      void* hptrs[] = {h_srcs, h_dsts, h_ed, h_sbuf};
+      for (void* p : hptrs)
+        if (p) (void)hipHostFree(p);
+      for (int b = 0; b < 2; ++b) {
+        if (ev_copied[b]) (void)hipEventDestroy(ev_copied[b]);
+        if (ev_used[b]) (void)hipEventDestroy(ev_used[b]);
+        ev_copied[b] = ev_used[b] = nullptr;
+        d_stage_src[b] = d_stage_dst[b] = nullptr;
+        d_stage_ed[b] = nullptr;
+      }
+      if (copy_stream) (void)hipStreamDestroy(copy_stream);
+      copy_stream = nullptr;
+      d_csc_vb = d_csr_vb = d_spos = d_swsum = d_scnt = d_escan_sub = d_cbase_sub = nullptr;
+      h_srcs = h_dsts = h_sbuf = nullptr;
+      h_ed = nullptr;
+      h_csc_eid.clear();
+      csc_vb.clear();
+      csr_vb.clear();
+      streaming = false;
+    }
     for (auto& v : ev) {
       for (hipEvent_t e : v) (void)hipEventDestroy(e);
       v.clear();
@@ -876,7 +1013,7 @@ struct Engine final : EngineBase {
   }
 
   int bind(gr_graph* graph, bool owns, void* vdata, size_t velem, const void* edata,
-           size_t eelem, bool edata_on_device) override {
+           size_t eelem, bool edata_on_device, bool stream_mode = false) override {
     if (velem != sizeof(VD))
       return set_errorf(GR_ERR_INVALID, "vertex element size %zu != program VertexData size %zu",
                         velem, sizeof(VD));
@@ -891,7 +1028,8 @@ struct Engine final : EngineBase {
     g = graph;
     owns_graph = owns;
     host_vdata = vdata;
-    if (vertex_order == 1 && graph->nV > 0) {
+    streaming = stream_mode;  // edge arrays stay in host memory (bind_streaming)
+    if (vertex_order == 1 && graph->nV > 0 && !streaming) {
       GR_TRY(graph_ordered(graph, &g));
       d_perm = graph->perm;
       d_inv = graph->inv;
@@ -949,8 +1087,8 @@ struct Engine final : EngineBase {
       GR_TRY(refresh_source_values());
     }
     if constexpr (!kEdgeEmpty) {
-      GR_TRY(dalloc(&d_ed, E));
-      if (E > 0) {
+      GR_TRY(dalloc(&d_ed, streaming ? 0 : E));
+      if (E > 0 && !streaming) {
         const ED* src = static_cast<const ED*>(edata);
         ED* tmp = nullptr;
         if (!edata_on_device) {
@@ -978,7 +1116,7 @@ struct Engine final : EngineBase {
       }
     }
     inert_ok = false;
-    if constexpr (kInert) {
+    if constexpr (kInert) if (!streaming) {
       unsigned int* d_bad = nullptr;
       unsigned int h_bad = 1;
       GR_TRY(dalloc(&d_bad, 1));
@@ -1004,6 +1142,228 @@ struct Engine final : EngineBase {
     delta_chain = delta_ready = false;
     it_gather = it_apply = it_scatter = false;
     bound = true;
+    return GR_OK;
+  }
+
+  // Stable order of the edges by (primary, secondary) key: a counting sort by
+  // the secondary key, then a stable one by the primary key.
+  static void edge_order(int32_t nV, int64_t nE, const int32_t* primary,
+                         const int32_t* secondary, std::vector<int32_t>& order,
+                         std::vector<int32_t>& offsets) {
+    std::vector<int32_t> tmp((size_t)nE);
+    std::vector<int64_t> next((size_t)nV + 1);
+    auto pass = [&](const int32_t* key, const int32_t* in, int32_t* out) {
+      std::fill(next.begin(), next.end(), 0);
+      for (int64_t i = 0; i < nE; ++i) ++next[(size_t)key[in ? in[i] : i] + 1];
+      for (int32_t v = 0; v < nV; ++v) next[(size_t)v + 1] += next[v];
+      offsets.assign(next.begin(), next.end());
+      for (int64_t i = 0; i < nE; ++i) {
+        const int32_t e = in ? in[i] : (int32_t)i;
+        out[next[key[e]]++] = e;
+      }
+    };
+    order.resize((size_t)nE);
+    pass(secondary, nullptr, tmp.data());
+    pass(primary, tmp.data(), order.data());
+  }
+
+  // Vertex ranges of at most max_edges edges (a vertex with more is a shard
+  // of its own); bounds[s] .. bounds[s + 1].
+  static std::vector<int32_t> shard_bounds(const std::vector<int32_t>& off, int64_t max_edges) {
+    std::vector<int32_t> b{0};
+    const int32_t nV = (int32_t)off.size() - 1;
+    int64_t start = 0;
+    for (int32_t v = 0; v < nV; ++v) {
+      if ((int64_t)off[v + 1] - start > max_edges && v > b.back()) {
+        b.push_back(v);
+        start = off[v];
+      }
+    }
+    if (b.back() != nV || nV == 0) b.push_back(nV);
+    return b;
+  }
+
+  int bind_streaming(int32_t nV, void* vdata, size_t velem, int32_t nE, const void* edata,
+                     size_t eelem, const int32_t* srcs, const int32_t* dsts) override {
+    if constexpr (kScatter) {
+      return set_error(GR_ERR_UNSUPPORTED,
+                       "streaming: programs whose scatter changes edge data are not supported");
+    } else {
+      if (nV < 0 || nE < 0 || (nE > 0 && (!srcs || !dsts)))
+        return set_error(GR_ERR_INVALID, "setGraph: bad sizes or edge arrays");
+      for (int64_t i = 0; i < nE; ++i)
+        if (srcs[i] < 0 || srcs[i] >= nV || dsts[i] < 0 || dsts[i] >= nV)
+          return set_errorf(GR_ERR_INVALID, "setGraph: edge %lld has a vertex outside [0, %d)",
+                            (long long)i, nV);
+      // CSC by (dst, src) and CSR by (src, dst), as the device builder orders them
+      std::vector<int32_t> csc_idx, csr_idx, off_in, off_out;
+      edge_order(nV, nE, dsts, srcs, csc_idx, off_in);
+      edge_order(nV, nE, srcs, dsts, csr_idx, off_out);
+      gr_graph* gg = new (std::nothrow) gr_graph;
+      if (!gg) return set_error(GR_ERR_NOMEM, "streaming: out of host memory");
+      gg->nV = nV;
+      gg->nE = nE;
+      gg->h_off_in = off_in;
+      gg->h_off_out = off_out;
+      if (hipMalloc((void**)&gg->off_in, sizeof(int32_t) * ((size_t)nV + 1)) != hipSuccess ||
+          hipMalloc((void**)&gg->off_out, sizeof(int32_t) * ((size_t)nV + 1)) != hipSuccess) {
+        graph_free(gg);
+        return set_error(GR_ERR_NOMEM, "streaming: offsets");
+      }
+      if (hipMemcpy(gg->off_in, off_in.data(), sizeof(int32_t) * ((size_t)nV + 1),
+                    hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(gg->off_out, off_out.data(), sizeof(int32_t) * ((size_t)nV + 1),
+                    hipMemcpyHostToDevice) != hipSuccess) {
+        graph_free(gg);
+        return set_error(GR_ERR_HIP, "streaming: offsets copy");
+      }
+      GR_TRY(bind(gg, true, vdata, velem, edata, eelem, false, true));
+      // edge arrays into pinned host memory, in CSC (sources, edge data) and
+      // CSR (targets) order
+      const size_t E = (size_t)nE;
+      GR_HIP(hipHostMalloc((void**)&h_srcs, sizeof(int32_t) * (E + 1), hipHostMallocDefault));
+      GR_HIP(hipHostMalloc((void**)&h_dsts, sizeof(int32_t) * (E + 1), hipHostMallocDefault));
+      for (size_t k = 0; k < E; ++k) {
+        h_srcs[k] = srcs[csc_idx[k]];
+        h_dsts[k] = dsts[csr_idx[k]];
+      }
+      if constexpr (!kEdgeEmpty) {
+        GR_HIP(hipHostMalloc((void**)&h_ed, sizeof(ED) * (E + 1), hipHostMallocDefault));
+        const ED* ed = static_cast<const ED*>(edata);
+        for (size_t k = 0; k < E; ++k) h_ed[k] = ed[csc_idx[k]];
+      }
+      h_csc_eid.swap(csc_idx);
+      csc_vb = shard_bounds(off_in, stream_max_edges);
+      csr_vb = shard_bounds(off_out, stream_max_edges);
+      int64_t max_in = 0, max_out = 0;
+      for (size_t s = 0; s + 1 < csc_vb.size(); ++s)
+        max_in = std::max<int64_t>(max_in, off_in[csc_vb[s + 1]] - off_in[csc_vb[s]]);
+      for (size_t s = 0; s + 1 < csr_vb.size(); ++s)
+        max_out = std::max<int64_t>(max_out, off_out[csr_vb[s + 1]] - off_out[csr_vb[s]]);
+      const int n_in = (int)csc_vb.size(), n_out = (int)csr_vb.size();
+      GR_TRY(dalloc(&d_csc_vb, n_in));
+      GR_TRY(dalloc(&d_csr_vb, n_out));
+      GR_TRY(dalloc(&d_spos, n_in));
+      GR_TRY(dalloc(&d_swsum, n_in));
+      GR_TRY(dalloc(&d_scnt, n_out));
+      GR_HIP(hipHostMalloc((void**)&h_sbuf, sizeof(int32_t) * (2 * (size_t)n_in + n_out),
+                           hipHostMallocDefault));
+      GR_HIP(hipMemcpy(d_csc_vb, csc_vb.data(), sizeof(int32_t) * n_in, hipMemcpyHostToDevice));
+      GR_HIP(hipMemcpy(d_csr_vb, csr_vb.data(), sizeof(int32_t) * n_out, hipMemcpyHostToDevice));
+      GR_TRY(dalloc(&d_escan_sub, (int64_t)nV + 1));
+      GR_TRY(dalloc(&d_cbase_sub, (int64_t)nV + 1));
+      GR_HIP(hipStreamCreateWithFlags(&copy_stream, hipStreamNonBlocking));
+      for (int b = 0; b < 2; ++b) {
+        GR_TRY(dalloc(&d_stage_src[b], max_in));
+        if constexpr (!kEdgeEmpty) GR_TRY(dalloc(&d_stage_ed[b], max_in));
+        GR_TRY(dalloc(&d_stage_dst[b], max_out));
+        GR_HIP(hipEventCreateWithFlags(&ev_copied[b], hipEventDisableTiming));
+        GR_HIP(hipEventCreateWithFlags(&ev_used[b], hipEventDisableTiming));
+        GR_HIP(hipEventRecord(ev_used[b], stream));
+      }
+      GR_HIP(hipStreamSynchronize(stream));
+      return GR_OK;
+    }
+  }
+
+  // Stages host edge range [e0, e1) of `src` into `dst` on the copy stream
+  // once buffer b is free, and makes the compute stream wait for it.
+  template <class T>
+  int stage(int b, T* dst, const T* src, int64_t e0, int64_t e1) {
+    GR_HIP(hipStreamWaitEvent(copy_stream, ev_used[b], 0));
+    if (e1 > e0)
+      GR_HIP(hipMemcpyAsync(dst, src + e0, sizeof(T) * (size_t)(e1 - e0), hipMemcpyHostToDevice,
+                            copy_stream));
+    GR_HIP(hipEventRecord(ev_copied[b], copy_stream));
+    GR_HIP(hipStreamWaitEvent(stream, ev_copied[b], 0));
+    return GR_OK;
+  }
+
+  // Gather over the CSC shards that hold active rows; shard k + 1 is copied
+  // while shard k is gathered (two staging buffers).
+  int gather_streamed() {
+    const int n = n_active;
+    const int S = (int)csc_vb.size() - 1;
+    k_shard_positions<<<grid_1d((int64_t)S + 1, 256), 256, 0, stream>>>(
+        S + 1, d_csc_vb, d_active, n, d_escan, d_spos, d_swsum);
+    GR_HIP(hipGetLastError());
+    GR_HIP(hipMemcpyAsync(h_sbuf, d_spos, sizeof(int32_t) * (S + 1), hipMemcpyDeviceToHost,
+                          stream));
+    GR_HIP(hipMemcpyAsync(h_sbuf + S + 1, d_swsum, sizeof(int32_t) * (S + 1),
+                          hipMemcpyDeviceToHost, stream));
+    GR_HIP(hipStreamSynchronize(stream));
+    std::vector<int32_t> pos(h_sbuf, h_sbuf + S + 1), wsum(h_sbuf + S + 1, h_sbuf + 2 * S + 2);
+    tick();
+    int used = 0;
+    const int tile = kNT * gather_vt;
+    for (int s = 0; s < S; ++s) {
+      const int a0 = pos[s], a1 = pos[s + 1];
+      if (a1 == a0) continue;
+      const int ns = a1 - a0, ws = wsum[s + 1] - wsum[s];
+      const int64_t e0 = g->h_off_in[csc_vb[s]], e1 = g->h_off_in[csc_vb[s + 1]];
+      const int b = used++ & 1;
+      GR_TRY(stage(b, d_stage_src[b], h_srcs, e0, ws ? e1 : e0));
+      if constexpr (!kEdgeEmpty) GR_TRY(stage(b, d_stage_ed[b], h_ed, e0, ws ? e1 : e0));
+      k_shard_rebase<<<grid_1d((int64_t)ns + 1, 256), 256, 0, stream>>>(
+          ns, d_escan, d_cbase, a0, (int)e0, d_escan_sub, d_cbase_sub);
+      const int n_tiles = (int)(((int64_t)ns + ws + tile - 1) / tile);
+      k_merge_partition<<<grid_1d((int64_t)n_tiles + 1, 256), 256, 0, stream>>>(
+          d_escan_sub, ns, ws, n_tiles, tile, d_part);
+      const ED* sed = kEdgeEmpty ? reinterpret_cast<const ED*>(d_gtmp) : d_stage_ed[b];
+#define GR_STREAM_GATHER(VT)                                                                   \
+  k_gather_tiles<P, VT><<<n_tiles, kNT, 0, stream>>>(ns, ws, d_active + a0, d_escan_sub, d_part, \
+                                                     d_cbase_sub, d_stage_src[b], d_vd, sed,    \
+                                                     d_gtmp + a0, d_carry, d_head, nullptr,     \
+                                                     d_srcval, nullptr)
+      switch (gather_vt) {
+        case 4: GR_STREAM_GATHER(4); break;
+        case 16: GR_STREAM_GATHER(16); break;
+        default: GR_STREAM_GATHER(8); break;
+      }
+#undef GR_STREAM_GATHER
+      k_gather_fixup<P><<<grid_1d(n_tiles, 256), 256, 0, stream>>>(n_tiles, d_carry, d_head,
+                                                                   d_gtmp + a0);
+      GR_HIP(hipGetLastError());
+      GR_HIP(hipEventRecord(ev_used[b], stream));
+      if (ws) {
+        stats.streamed_bytes += (e1 - e0) * (int64_t)(4 + (kEdgeEmpty ? 0 : sizeof(ED)));
+        stats.streamed_shards += 1;
+      }
+    }
+    tock(kPhGather);
+    stats.active_vertices += n;
+    stats.gathered_edges += w_edges;
+    return GR_OK;
+  }
+
+  // Activation over the CSR shards that hold changed vertices.
+  int scatter_streamed() {
+    const int S = (int)csr_vb.size() - 1;
+    if (S > 0) {
+      k_shard_changed<<<S, 256, 0, stream>>>(S, d_csr_vb, d_chg_bits, d_scnt);
+      GR_HIP(hipGetLastError());
+      GR_HIP(hipMemcpyAsync(h_sbuf, d_scnt, sizeof(int32_t) * S, hipMemcpyDeviceToHost, stream));
+      GR_HIP(hipStreamSynchronize(stream));
+    }
+    std::vector<int32_t> cnt(h_sbuf, h_sbuf + S);
+    tick();
+    int used = 0;
+    for (int s = 0; s < S; ++s) {
+      if (!cnt[s]) continue;
+      const int u0 = csr_vb[s], u1 = csr_vb[s + 1];
+      const int64_t e0 = g->h_off_out[u0], e1 = g->h_off_out[u1];
+      if (e1 == e0) continue;
+      const int b = used++ & 1;
+      GR_TRY(stage(b, d_stage_dst[b], h_dsts, e0, e1));
+      const unsigned grid = std::min<unsigned>(grid_1d((int64_t)u1 - u0, 256), 8192);
+      k_shard_push<<<grid, 256, 0, stream>>>(u0, u1, (int)e0, g->off_out, d_stage_dst[b],
+                                             d_chg_bits, d_bitmap);
+      GR_HIP(hipGetLastError());
+      GR_HIP(hipEventRecord(ev_used[b], stream));
+      stats.streamed_bytes += (e1 - e0) * 4;
+      stats.streamed_shards += 1;
+    }
+    tock(kPhScatter);
     return GR_OK;
   }
 
@@ -1121,6 +1481,7 @@ struct Engine final : EngineBase {
   int gather() override {
     it_gather = true;
     if (!P::kHasGather || n_active == 0) return GR_OK;
+    if (streaming) return gather_streamed();
     const int n = n_active, w = w_edges;
     const uint32_t* dbits = nullptr;
     if constexpr (kDelta) {
@@ -1213,7 +1574,7 @@ struct Engine final : EngineBase {
   }
 
   // The changed bitmap feeds pull activation and the next delta gather.
-  bool changed_bits_on() const { return dir_mode != 1 || kDelta; }
+  bool changed_bits_on() const { return dir_mode != 1 || kDelta || streaming; }
 
   int apply() override {
     it_apply = true;
@@ -1233,6 +1594,7 @@ struct Engine final : EngineBase {
 
   int scatter_activate(bool have_scatter) override {
     it_scatter = true;
+    if (streaming) return scatter_streamed();
     const long long thr = pull_threshold();
     tick();
     if (dir_mode != 1) {
@@ -1339,6 +1701,11 @@ struct Engine final : EngineBase {
       if (eelem != sizeof(ED)) return set_error(GR_ERR_INVALID, "edge element size mismatch");
       const int64_t E = g->nE;
       if (E == 0) return GR_OK;
+      if (streaming) {  // host copy in CSC order
+        ED* o = static_cast<ED*>(out);
+        for (int64_t k = 0; k < E; ++k) o[h_csc_eid[(size_t)k]] = h_ed[k];
+        return GR_OK;
+      }
       if constexpr (kScatter) {
         GR_HIP(hipMemcpyAsync(out, d_ed, sizeof(ED) * E, hipMemcpyDeviceToHost, stream));
       } else {
@@ -1451,6 +1818,9 @@ extern "C" int gr_engine_set_graph(gr_engine* e, int32_t n_vertices, void* verte
     return set_errorf(GR_ERR_INVALID, "program needs edge data of %zu-byte elements",
                       e->impl->edge_size());
   if (!vertex_data) return set_error(GR_ERR_INVALID, "vertex data is NULL");
+  if (e->impl->stream_max_edges > 0)
+    return e->impl->bind_streaming(n_vertices, vertex_data, vertex_elem_size, n_edges, edge_data,
+                                   edge_elem_size, srcs, dsts);
   gr_graph* g = nullptr;
   GR_TRY(gr_graph_from_edges(n_vertices, n_edges, srcs, dsts, &g));
   // bind() takes ownership of g once its argument checks (repeated above) pass
@@ -1565,6 +1935,13 @@ extern "C" int gr_engine_set_direction(gr_engine* e, int mode, double alpha) {
   if (mode < 0 || mode > 2) return set_error(GR_ERR_INVALID, "direction mode must be 0, 1 or 2");
   e->impl->dir_mode = mode;
   if (alpha > 0) e->impl->dir_alpha = alpha;
+  return GR_OK;
+}
+
+extern "C" int gr_engine_set_streaming(gr_engine* e, int64_t max_edges_per_shard) {
+  GR_TRY(check(e, false));
+  if (max_edges_per_shard < 0) return set_error(GR_ERR_INVALID, "max_edges_per_shard < 0");
+  e->impl->stream_max_edges = max_edges_per_shard;
   return GR_OK;
 }
 

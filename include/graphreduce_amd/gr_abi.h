@@ -234,6 +234,20 @@ int gr_engine_set_gather_tile(gr_engine* e, int items_per_thread);
  * the next setGraph / bindGraph. */
 int gr_engine_set_vertex_order(gr_engine* e, int order);
 
+/* Out-of-core mode, the reference's shard streaming (gpugas.h:1398-1503,
+ * 1553-1619): max_edges_per_shard > 0 makes the next gr_engine_set_graph keep
+ * the edge arrays (CSC sources, CSR targets, edge data) in pinned host memory,
+ * cut into vertex ranges of at most that many edges (a vertex with more is a
+ * shard of its own); vertex-sized state stays on the device.  Every gather
+ * streams the CSC shards that hold active rows, every activation the CSR
+ * shards that hold changed vertices, each copy overlapping the previous
+ * shard's kernels (two staging buffers, a copy stream).  Results equal the
+ * in-memory engine's (FP32 sums may differ in the last place: other tiles).
+ * The degree order, the delta/inert gathers and programs whose scatter
+ * changes edge data are not used in this mode (the latter: GR_ERR_UNSUPPORTED);
+ * gr_engine_bind_graph ignores it.  0 (default): the graph lives in HBM. */
+int gr_engine_set_streaming(gr_engine* e, int64_t max_edges_per_shard);
+
 /* Delta gather (no reference counterpart).  For SSSP and CC, once an
  * iteration with every vertex active has run (setActive(0, V)) and every
  * later active set came from scatterActivate, gather may read only the
@@ -279,6 +293,8 @@ typedef struct gr_engine_stats {
                                 unreached vertices, weights in [0, 100000]) */
   double inert_ms;           /* their vertex scans (the gather itself is in
                                 gather_ms or push_ms) */
+  int64_t streamed_bytes;    /* out-of-core mode: edge bytes copied host -> device */
+  int64_t streamed_shards;   /* ... in this many shard copies */
 } gr_engine_stats;
 
 /* Enables hipEvent timing of the engine phases on the engine's stream. */

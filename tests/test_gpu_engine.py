@@ -540,3 +540,84 @@ def test_scatter_program_through_cpp_header(gpu, tmp_path):
                     f"-Wl,-rpath,{root / 'graphreduce_amd' / 'lib'}"], check=True, timeout=120)
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60, check=True).stdout
     assert "trips 2" in out and "dist 0 1 5 3" in out and "w 1 5 2 2" in out
+
+
+# ------------------------------------------------------- out-of-core shards
+
+def _stream_run(gr, prog, n, vd, s, d, w=None, start=None, max_edges=0, order=1):
+    eng = gr.GASEngine(prog)
+    eng.setVertexOrder(order)
+    eng.setStreaming(max_edges)
+    eng.setGraph(n, vd.copy(), len(s), w, s, d)
+    if start is None:
+        eng.setActive(0, n)
+    else:
+        eng.setActive(start, start + 1)
+    trips = 0
+    while eng.countActive():
+        eng.gather()
+        eng.apply()
+        eng.scatterActivate()
+        eng.nextIter()
+        trips += 1
+        eng.setIterationCount(trips)
+    eng.getResults()
+    out = eng.vertexData.copy()
+    st = eng.stats()
+    eng.close()
+    return out, trips, st
+
+
+@pytest.mark.parametrize("prog", ["bfs", "sssp", "cc", "pagerank"])
+@pytest.mark.parametrize("max_edges", [3000, 40000])
+def test_streaming_equals_in_memory(gpu, prog, max_edges):
+    """Out-of-core shards (gpugas.h:1398-1619): the edge arrays stay in host
+    memory, cut into vertex ranges of <= max_edges edges (hubs exceed 3000:
+    shards of their own) and are streamed every iteration; values, trips and
+    work counters equal the in-memory engine's (PageRank within FP32 noise:
+    other tiles)."""
+    gr = gpu
+    n, s, d = _rmat(gr, 13, seed=7, sym=(prog == "cc"))
+    src = _max_out_src(n, s)
+    w, start = None, None
+    if prog == "bfs":
+        p = gr.BFS
+        vd = np.zeros(n, gr.PROGRAMS[gr.BFS]["vertex"])
+        vd["depth"] = -1
+        start = src
+    elif prog == "sssp":
+        p = gr.SSSP
+        w = gr.hash_weights(len(s), seed=7)
+        vd = np.full(n, gr.SSSP_INF, np.int32)
+        vd[src] = 0
+    elif prog == "cc":
+        p, vd = gr.CC, np.arange(n, dtype=np.int32)
+    else:
+        p = gr.PAGERANK
+        vd = np.zeros(n, gr.PROGRAMS[gr.PAGERANK]["vertex"])
+        vd["rank"] = 0.15
+        vd["numOutEdges"] = np.bincount(s, minlength=n)
+    ref, rt, rst = _stream_run(gr, p, n, vd, s, d, w, start, max_edges=0)
+    got, t, st = _stream_run(gr, p, n, vd, s, d, w, start, max_edges=max_edges)
+    assert t == rt
+    for k in ("active_vertices", "changed_vertices"):
+        assert st[k] == rst[k], k
+    if prog == "pagerank":
+        assert np.allclose(got["rank"], ref["rank"], rtol=1e-5, atol=1e-6)
+    else:
+        assert np.array_equal(got, ref)
+    assert st["streamed_shards"] > t and st["streamed_bytes"] > 0
+    assert rst["streamed_shards"] == 0
+    if prog == "sssp":
+        oref, otrips, _ = oracle.sssp(n, s, d, w, src)
+        assert np.array_equal(got, oref) and t == otrips
+
+
+def test_streaming_refuses_scatter_programs(gpu):
+    gr = gpu
+    n, s, d = _rmat(gr, 8, seed=1)
+    eng = gr.GASEngine(gr.SSSP_CONGESTION)
+    eng.setStreaming(1000)
+    with pytest.raises(gr.GRError, match="scatter"):
+        eng.setGraph(n, np.zeros(n, np.int32), len(s), np.ones(len(s), np.int32), s, d)
+    eng.close()
