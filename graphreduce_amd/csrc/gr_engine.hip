@@ -1116,19 +1116,21 @@ struct Engine final : EngineBase {
       }
     }
     inert_ok = false;
-    if constexpr (kInert) if (!streaming) {
-      unsigned int* d_bad = nullptr;
-      unsigned int h_bad = 1;
-      GR_TRY(dalloc(&d_bad, 1));
-      GR_HIP(hipMemsetAsync(d_bad, 0, sizeof(unsigned int), stream));
-      if (E > 0)
-        k_check_inert_edges<P><<<grid_1d(E, 256 * 16) < 8192 ? grid_1d(E, 256 * 16) : 8192, 256, 0,
-                                 stream>>>(E, d_ed, d_bad);
-      GR_HIP(hipGetLastError());
-      GR_HIP(hipMemcpyAsync(&h_bad, d_bad, sizeof(unsigned int), hipMemcpyDeviceToHost, stream));
-      GR_HIP(hipStreamSynchronize(stream));
-      (void)hipFree(d_bad);
-      inert_ok = h_bad == 0;
+    if constexpr (kInert) {
+      if (!streaming) {  // the weights are checked on the device copy
+        unsigned int* d_bad = nullptr;
+        unsigned int h_bad = 1;
+        GR_TRY(dalloc(&d_bad, 1));
+        GR_HIP(hipMemsetAsync(d_bad, 0, sizeof(unsigned int), stream));
+        if (E > 0)
+          k_check_inert_edges<P><<<grid_1d(E, 256 * 16) < 8192 ? grid_1d(E, 256 * 16) : 8192, 256,
+                                   0, stream>>>(E, d_ed, d_bad);
+        GR_HIP(hipGetLastError());
+        GR_HIP(hipMemcpyAsync(&h_bad, d_bad, sizeof(unsigned int), hipMemcpyDeviceToHost, stream));
+        GR_HIP(hipStreamSynchronize(stream));
+        (void)hipFree(d_bad);
+        inert_ok = h_bad == 0;
+      }
     }
     if constexpr (kDelta) {
       GR_TRY(dalloc(&d_cand, V));
