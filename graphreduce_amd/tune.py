@@ -23,7 +23,7 @@ import graphreduce_amd as gr
 
 DEFAULT_FILE = Path(__file__).resolve().parent / "tuned.json"
 TILES = (4, 8, 16)
-ALPHAS = (4.0, 8.0, 15.0, 30.0, 60.0)
+ALPHAS = (1.0, 2.0, 4.0, 8.0, 15.0, 30.0, 60.0)
 
 
 def _loop(eng, n, source=None):
