@@ -163,6 +163,12 @@ def main() -> int:
     step_alg_gbs = ref_bytes / sec / 1e9
     log(f"  phases ms {phases} launches {launches}; tile kernel {achieved:.1f} GB/s algorithmic")
 
+    # The random-read floor of a full gather: one bare pass of dist[srcs[e]]
+    # over every CSC slot of the same degree-ordered layout (a full gather of
+    # this run covers > 99 % of the slots).
+    floor_ms = g.source_read_ms(ordered=True)
+    avg_launch = st["gather_ms"] / max(1, launches["gather"])
+
     # HBM traffic per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     # summary (calibrated against a plain copy kernel)
     prof_traffic = None
@@ -224,6 +230,12 @@ def main() -> int:
             "reference_bytes": "12 B per in-edge of every active vertex + 16 B per active "
                                "vertex over all 9 gathers, as the reference's full gathers "
                                "would move them",
+            "random_read_floor_ms": round(floor_ms, 4),
+            "kernel_over_floor": round(avg_launch / floor_ms, 3) if floor_ms > 0 else None,
+            "random_read_floor": "one bare pass of value[srcs[e]] over all CSC slots of the "
+                                 "same layout (gr_graph_source_read_probe): the random 4-byte "
+                                 "reads a full gather cannot avoid, without edge data, "
+                                 "reduction or writes",
         },
         "phase_ms_per_step": phases,
         "tuning": "graphreduce_amd/tuned.json (autotuner)" if tuned else "defaults",

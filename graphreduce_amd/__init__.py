@@ -130,6 +130,7 @@ _SIGS = {
     "gr_memcpy_h2d": (C.c_int, [_P, _P, C.c_size_t]),
     "gr_memcpy_d2h": (C.c_int, [_P, _P, C.c_size_t]),
     "gr_device_copy": (C.c_int, [_P, _P, C.c_size_t, C.POINTER(C.c_float)]),
+    "gr_graph_source_read_probe": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(C.c_float)]),
     "gr_host_rmat_edges": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                       C.c_uint64, C.c_int, _I32P, _I32P, C.c_int64, _I64P]),
     "gr_host_grid_edges": (C.c_int, [C.c_int, C.c_double, C.c_uint64, _I32P, _I32P,
@@ -444,6 +445,13 @@ class Graph:
         b = np.empty(self.n_edges, np.int32)
         _check(lib().gr_graph_copy_perm(self._h, _p32(a), _p32(b)))
         return a, b
+
+    def source_read_ms(self, ordered: bool = True, repeats: int = 5) -> float:
+        """Time (ms) of one bare pass of value[srcs[e]] reads over the CSC
+        (the random reads a full gather cannot avoid)."""
+        t = C.c_float(0)
+        _check(lib().gr_graph_source_read_probe(self._h, int(ordered), repeats, C.byref(t)))
+        return t.value
 
     def max_out_degree_vertex(self):
         v = C.c_int32(0)

@@ -438,6 +438,50 @@ extern "C" int gr_device_copy(void* dst, const void* src, size_t bytes, float* m
   return GR_OK;
 }
 
+namespace {
+// One read of dist[srcs[e]] per CSC slot (source ids streamed non-temporally,
+// as the gather loads them); each thread keeps a minimum so the reads count.
+__global__ __launch_bounds__(256) void k_source_reads(const int32_t* __restrict__ srcs,
+                                                      int64_t n, const int32_t* __restrict__ dist,
+                                                      int32_t* __restrict__ out) {
+  constexpr int64_t kPer = 4096;
+  const int64_t b0 = (int64_t)blockIdx.x * kPer;
+  const int64_t b1 = b0 + kPer < n ? b0 + kPer : n;
+  int32_t acc = INT_MAX;
+  for (int64_t e = b0 + threadIdx.x; e < b1; e += blockDim.x)
+    acc = min(acc, dist[__builtin_nontemporal_load(srcs + e)]);
+  out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = acc;
+}
+}  // namespace
+
+extern "C" int gr_graph_source_read_probe(gr_graph* g, int ordered, int repeats, float* ms) {
+  if (!g || repeats < 1) return set_error(GR_ERR_INVALID, "gr_graph_source_read_probe: bad arguments");
+  gr_graph* lay = g;
+  if (ordered) GR_TRY(gr::graph_ordered(g, &lay));
+  const int64_t E = lay->nE;
+  const unsigned blocks = (unsigned)((E + 4095) / 4096 > 0 ? (E + 4095) / 4096 : 1);
+  DevBuf<int32_t> dist, out;
+  GR_TRY(dist.alloc(lay->nV));
+  GR_TRY(out.alloc((int64_t)blocks * 256));
+  GR_HIP(hipMemset(dist.p, 0, sizeof(int32_t) * (size_t)(lay->nV > 0 ? lay->nV : 1)));
+  k_source_reads<<<blocks, 256>>>(lay->srcs, E, dist.p, out.p);  // warm
+  GR_HIP(hipGetLastError());
+  hipEvent_t e0, e1;
+  GR_HIP(hipEventCreate(&e0));
+  GR_HIP(hipEventCreate(&e1));
+  GR_HIP(hipEventRecord(e0, 0));
+  for (int r = 0; r < repeats; ++r) k_source_reads<<<blocks, 256>>>(lay->srcs, E, dist.p, out.p);
+  GR_HIP(hipGetLastError());
+  GR_HIP(hipEventRecord(e1, 0));
+  GR_HIP(hipEventSynchronize(e1));
+  float t = 0;
+  GR_HIP(hipEventElapsedTime(&t, e0, e1));
+  if (ms) *ms = t / repeats;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return GR_OK;
+}
+
 extern "C" int gr_memcpy_h2d(void* dst, const void* src, size_t bytes) {
   if (bytes) GR_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
   return GR_OK;

@@ -80,6 +80,13 @@ int gr_memcpy_d2h(void* host_dst, const void* device_src, size_t bytes);
  * with hipEvents: the reference point for bandwidth and HBM-counter scaling. */
 int gr_device_copy(void* device_dst, const void* device_src, size_t bytes, float* ms);
 
+/* The bare random-read pattern of a full gather, timed (average of `repeats`
+ * launches after a warm-up): one 4-byte read of a vertex value per CSC slot,
+ * value[srcs[e]], with the source ids streamed, over the graph's CSC (of its
+ * degree-ordered copy when ordered != 0) — the reads a full gather cannot
+ * avoid, without its edge data, reduction or writes. */
+int gr_graph_source_read_probe(gr_graph* g, int ordered, int repeats, float* ms);
+
 /* ------------------------------------------------------- host generators */
 /* Deterministic synthetic inputs.  All draws are counter-based, so host and
  * device generators give identical edge lists. */
