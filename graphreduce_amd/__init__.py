@@ -501,6 +501,7 @@ class GASEngine:
         self._vdata = None
         self._edata = None
         self._graph = None
+        self._n_edges = 0
         self._scatter = lib().gr_program_has_scatter(program) == 1
 
     def close(self) -> None:
@@ -542,6 +543,7 @@ class GASEngine:
         # by getResults (the reference engine works on the caller's array)
         self._edata = keep if self._scatter else None
         self._graph = None
+        self._n_edges = nEdges
 
     def bindGraph(self, graph: Graph, vertexData, edgeData=None) -> None:
         """setGraph with a prebuilt device graph; edgeData may be a DeviceArray."""
@@ -559,6 +561,7 @@ class GASEngine:
         self._vdata = vd
         self._edata = _keep if (self._scatter and not on_dev) else None
         self._graph = graph
+        self._n_edges = graph.n_edges
 
     @property
     def vertexData(self) -> np.ndarray:
@@ -603,12 +606,9 @@ class GASEngine:
         """The engine's edge data in edge-list order (a copy)."""
         if self.edge_dtype is None:
             raise GRError(1, f"{PROGRAMS[self.program]['name']} has no edge data")
-        if self._graph is not None:
-            m = self._graph.n_edges
-        elif self._edata is not None:
-            m = len(self._edata)
-        else:
-            raise GRError(4, "readEdgeData: edge count unknown for this binding")
+        if self._vdata is None:
+            raise GRError(4, "readEdgeData: no graph bound")
+        m = self._n_edges
         out = np.empty(m, self.edge_dtype)
         if m:
             _check(lib().gr_engine_read_edge_data(self._h, out.ctypes.data_as(_P),

@@ -1220,6 +1220,22 @@ struct Engine final : EngineBase {
         return set_error(GR_ERR_HIP, "streaming: offsets copy");
       }
       GR_TRY(bind(gg, true, vdata, velem, edata, eelem, false, true));
+      const int rc = stream_buffers(nE, edata, srcs, dsts, csc_idx, csr_idx, off_in, off_out);
+      if (rc != GR_OK) {  // leave no half-built engine behind
+        release();
+        bound = false;
+      }
+      return rc;
+    }
+  }
+
+  // Pinned host edge arrays, shard tables and device staging of the streaming
+  // mode (after bind() has set up the vertex-sized state).
+  int stream_buffers(int32_t nE, const void* edata, const int32_t* srcs, const int32_t* dsts,
+                     std::vector<int32_t>& csc_idx, const std::vector<int32_t>& csr_idx,
+                     const std::vector<int32_t>& off_in, const std::vector<int32_t>& off_out) {
+    {
+      const int32_t nV = g->nV;
       // edge arrays into pinned host memory, in CSC (sources, edge data) and
       // CSR (targets) order
       const size_t E = (size_t)nE;

@@ -621,3 +621,26 @@ def test_streaming_refuses_scatter_programs(gpu):
     with pytest.raises(gr.GRError, match="scatter"):
         eng.setGraph(n, np.zeros(n, np.int32), len(s), np.ones(len(s), np.int32), s, d)
     eng.close()
+
+
+def test_streaming_edge_cases(gpu):
+    """Streaming mode on an edgeless graph, isolated vertices and a graph whose
+    every vertex is its own shard; edge data reads back in edge-list order."""
+    gr = gpu
+    e = np.zeros(0, np.int32)
+    got, t, st = _stream_run(gr, gr.CC, 5, np.arange(5, dtype=np.int32), e, e, max_edges=10)
+    assert got.tolist() == list(range(5)) and t == 1 and st["streamed_shards"] == 0
+    s = np.array([0, 1, 2, 2, 5], np.int32)
+    d = np.array([1, 2, 0, 3, 4], np.int32)
+    w = np.array([3, 1, 4, 1, 5], np.int32)
+    vd = np.full(8, gr.SSSP_INF, np.int32)
+    vd[0] = 0
+    for max_edges in (1, 2, 100):
+        got, t, _ = _stream_run(gr, gr.SSSP, 8, vd, s, d, w, max_edges=max_edges)
+        ref, rt, _ = oracle.sssp(8, s, d, w, 0)
+        assert np.array_equal(got, ref) and t == rt
+    eng = gr.GASEngine(gr.SSSP)
+    eng.setStreaming(2)
+    eng.setGraph(8, vd, len(s), w, s, d)
+    assert eng.readEdgeData().tolist() == w.tolist()
+    eng.close()
