@@ -85,6 +85,8 @@ class _Stats(C.Structure):
         ("inert_ms", C.c_double),
         ("streamed_bytes", C.c_int64),
         ("streamed_shards", C.c_int64),
+        ("fused_gathers", C.c_int64),
+        ("fused_edges", C.c_int64),
     ]
 
 

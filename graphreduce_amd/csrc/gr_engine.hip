@@ -236,6 +236,22 @@ __device__ __forceinline__ bool use_pull(unsigned long long chg, int mode, long 
   return (long long)(uint32_t)chg > threshold;
 }
 
+// Fused activation + push gather (delta-gather programs): when the changed
+// vertices' out-edges number at most fuse_max (>= 0), ONE pass over them sets
+// the frontier bits and lowers the next gather's candidates, replacing the
+// pull or push activation and the next iteration's push relax.
+__device__ __forceinline__ bool fused_scatter(unsigned long long chg, long long fuse_max) {
+  return fuse_max >= 0 && (chg >> 32) != 0 && (long long)(uint32_t)chg <= fuse_max;
+}
+
+// A fused pass over more than kTouchMinEdges out-edges marks its targets
+// with plain byte stores in `touch` (no atomics); k_touch_to_bits then folds
+// the bytes into the frontier bitmap.
+constexpr long long kTouchMinEdges = 1 << 20;
+__device__ __forceinline__ bool fused_touch(unsigned long long chg, long long fuse_max) {
+  return fused_scatter(chg, fuse_max) && (long long)(uint32_t)chg > kTouchMinEdges;
+}
+
 __device__ __forceinline__ int upper_bound_dev(const int32_t* __restrict__ a, int n, int x) {
   int lo = 0, hi = n;
   while (lo < hi) {
@@ -249,11 +265,14 @@ __device__ __forceinline__ int upper_bound_dev(const int32_t* __restrict__ a, in
 }
 
 // part[t] = changed-list entry holding out-edge t * kTile (t <= n_tiles).
-__global__ void k_push_partition(const DevCounters* __restrict__ ctr, int mode,
-                                 long long threshold, const int32_t* __restrict__ chg_e,
-                                 int32_t* __restrict__ part, int max_tiles) {
+// (also for the fused pass; write_fused: record in ctr->fused whether it runs)
+__global__ void k_push_partition(DevCounters* __restrict__ ctr, int mode, long long threshold,
+                                 const int32_t* __restrict__ chg_e, int32_t* __restrict__ part,
+                                 int max_tiles, long long fuse_max, int write_fused) {
   const unsigned long long c = ctr->chg;
-  if (use_pull(c, mode, threshold)) return;
+  const bool fused = fused_scatter(c, fuse_max);
+  if (write_fused && blockIdx.x == 0 && threadIdx.x == 0) ctr->fused = fused ? 1 : 0;
+  if (use_pull(c, mode, threshold) && !fused) return;
   const int n_c = (int)(c >> 32);
   const int w_c = (int)(uint32_t)c;
   const int n_tiles = (w_c + kTile - 1) / kTile;
@@ -266,6 +285,7 @@ __global__ void k_push_partition(const DevCounters* __restrict__ ctr, int mode,
 
 __global__ __launch_bounds__(kNT) void k_push_tiles(const DevCounters* __restrict__ ctr,
                                                     int mode, long long threshold,
+                                                    long long fuse_max,
                                                     const int32_t* __restrict__ part,
                                                     const int32_t* __restrict__ chg_v,
                                                     const int32_t* __restrict__ chg_e,
@@ -275,7 +295,7 @@ __global__ __launch_bounds__(kNT) void k_push_tiles(const DevCounters* __restric
   __shared__ int32_t s_start[kTile + 2];
   __shared__ int32_t s_csr[kTile + 2];
   const unsigned long long c = ctr->chg;
-  if (use_pull(c, mode, threshold)) return;
+  if (use_pull(c, mode, threshold) || fused_scatter(c, fuse_max)) return;
   const int w_c = (int)(uint32_t)c;
   const int n_tiles = (w_c + kTile - 1) / kTile;
   const int tid = threadIdx.x;
@@ -379,15 +399,27 @@ __global__ void k_relax_partition(int n_c, int w_c, const int32_t* __restrict__ 
 }
 
 template <class P>
+// With ctr (the fused pass of scatterActivate): runs only when fused_scatter
+// holds, takes the edge count from ctr->chg and also sets the targets'
+// frontier bits in `bitmap`.
 __global__ __launch_bounds__(kNT) void k_relax_tiles(
-    int w_c, const int32_t* __restrict__ part, const int32_t* __restrict__ chg_v,
+    int w_c, const DevCounters* __restrict__ ctr, long long fuse_max,
+    const int32_t* __restrict__ part, const int32_t* __restrict__ chg_v,
     const int32_t* __restrict__ chg_e, const int32_t* __restrict__ off_out,
     const int32_t* __restrict__ dsts, const typename P::VertexData* __restrict__ vd,
-    const typename P::EdgeData* __restrict__ ed_csr, typename P::GatherResult* __restrict__ cand) {
+    const typename P::EdgeData* __restrict__ ed_csr, typename P::GatherResult* __restrict__ cand,
+    uint32_t* __restrict__ bitmap, uint8_t* __restrict__ touch) {
   static_assert(!P::kGatherUsesDst, "push relax needs a gatherMap without the destination");
   __shared__ int32_t s_start[kTile + 2];
   __shared__ int32_t s_csr[kTile + 2];
   __shared__ typename P::VertexData s_vd[kTile + 2];
+  bool bytes = false;
+  if (ctr) {
+    const unsigned long long c = ctr->chg;
+    if (!fused_scatter(c, fuse_max)) return;
+    w_c = (int)(uint32_t)c;
+    bytes = touch && fused_touch(c, fuse_max);
+  }
   const int n_tiles = (w_c + kTile - 1) / kTile;
   const int tid = threadIdx.x;
   for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
@@ -418,9 +450,39 @@ __global__ __launch_bounds__(kNT) void k_relax_tiles(
         const typename P::EdgeData ev = stream_load(ed_csr + pos);
         const typename P::GatherResult val = P::gatherMap(nullptr, &s_vd[lo], &ev);
         if (val < cand[v]) atomicMin(&cand[v], val);
+        if (bytes) {
+          touch[v] = 1;
+        } else if (bitmap) {
+          const uint32_t bit = 1u << (v & 31);
+          if (!(bitmap[v >> 5] & bit)) atomicOr(&bitmap[v >> 5], bit);
+        }
       }
     }
     __syncthreads();
+  }
+}
+
+// After a fused pass that marked bytes: bitmap word w |= the 32 touch bytes of
+// its vertices, which are cleared (only this kernel writes those words then:
+// the other activation kernels stood down).
+__global__ void k_touch_to_bits(const DevCounters* __restrict__ ctr, long long fuse_max,
+                                int n_words, uint8_t* __restrict__ touch,
+                                uint32_t* __restrict__ bitmap) {
+  if (!fused_touch(ctr->chg, fuse_max)) return;
+  for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < n_words; w += gridDim.x * blockDim.x) {
+    uint4* t = reinterpret_cast<uint4*>(touch + (size_t)w * 32);
+    const uint4 a = t[0], b = t[1];
+    const uint32_t q[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    uint32_t m = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) m |= ((q[i] >> (8 * j)) & 1u) << (4 * i + j);
+    if (m) {
+      bitmap[w] |= m;
+      t[0] = make_uint4(0, 0, 0, 0);
+      t[1] = make_uint4(0, 0, 0, 0);
+    }
   }
 }
 
@@ -465,10 +527,10 @@ __device__ __forceinline__ uint64_t lane_range_mask(int lo, int hi) {  // bits [
 }
 
 __global__ __launch_bounds__(256) void k_pull_stream(
-    const DevCounters* __restrict__ ctr, int mode, long long threshold, int nV,
-    const int32_t* __restrict__ off_in, const int32_t* __restrict__ srcs,
+    const DevCounters* __restrict__ ctr, int mode, long long threshold, long long fuse_max,
+    int nV, const int32_t* __restrict__ off_in, const int32_t* __restrict__ srcs,
     const uint32_t* __restrict__ chg_bits, uint32_t* __restrict__ bitmap) {
-  if (!use_pull(ctr->chg, mode, threshold)) return;
+  if (!use_pull(ctr->chg, mode, threshold) || fused_scatter(ctr->chg, fuse_max)) return;
   const int lane = lane_id();
   const int wpb = blockDim.x / kWave;
   const int nw = gridDim.x * wpb;
@@ -516,11 +578,11 @@ __global__ __launch_bounds__(256) void k_pull_stream(
 }
 
 __global__ __launch_bounds__(256) void k_pull_heavy(
-    const DevCounters* __restrict__ ctr, int mode, long long threshold,
+    const DevCounters* __restrict__ ctr, int mode, long long threshold, long long fuse_max,
     const int32_t* __restrict__ heavy, int n_heavy, const int32_t* __restrict__ off_in,
     const int32_t* __restrict__ srcs, const uint32_t* __restrict__ chg_bits,
     uint32_t* __restrict__ bitmap) {
-  if (!use_pull(ctr->chg, mode, threshold)) return;
+  if (!use_pull(ctr->chg, mode, threshold) || fused_scatter(ctr->chg, fuse_max)) return;
   for (int h = blockIdx.x; h < n_heavy; h += gridDim.x) {
     const int v = heavy[h];
     const int b = off_in[v], e = off_in[v + 1];
@@ -626,7 +688,7 @@ __global__ __launch_bounds__(1024) void k_compact_scan(int2* __restrict__ blk, i
     ctr->w_edges = run_b;
     if (escan) escan[run_a] = run_b;
     ctr->chg_last = ctr->chg;
-    ctr->pull = use_pull(ctr->chg, mode, threshold) ? 1 : 0;
+    ctr->pull = use_pull(ctr->chg, mode, threshold) && !ctr->fused ? 1 : 0;
     ctr->chg = 0;
   }
 }
@@ -862,6 +924,7 @@ struct Engine final : EngineBase {
   ED* d_ed = nullptr;  // CSC order (edge-list order for programs with a scatter)
   ED* d_ed_csr = nullptr;  // CSR order (delta-gather programs: push relax)
   G* d_cand = nullptr;     // push-relax candidates per vertex (gatherZero between gathers)
+  uint8_t* d_touch = nullptr;  // fused activation bytes (all zero between scatters)
   source_value_t<P>* d_srcval = nullptr;  // P::sourceValue of every vertex (programs with one)
   // ---- out-of-core mode: edge arrays in pinned host memory, streamed per shard
   bool streaming = false;
@@ -920,6 +983,13 @@ struct Engine final : EngineBase {
   // 2.5 at 0.94); the pull-delta gather is never the fastest there.
   double changed_share = 1.0;
   static constexpr double kPushMaxChangedShare = 0.6;
+  // Fused activation + push gather when the changed vertices' out-edges are at
+  // most this share of E (adaptive mode): the separate activation (pull scans
+  // of whole rows, or a push) plus the next push relax cost more than one
+  // relax pass that also sets the frontier bits.
+  static constexpr double kFuseMaxShare = 0.4;
+  bool fuse_launched = false;  // scatterActivate enqueued the fused pass
+  bool fused_pending = false;  // d_cand holds the next gather's values
   // Inert sources (programs with kInertSources): with every bound edge passing
   // P::edgeKeepsInert, a gather without a previous changed set reads only the
   // non-inert sources (SSSP's first all-active gather: the start vertex).
@@ -937,7 +1007,7 @@ struct Engine final : EngineBase {
   size_t edge_size() const override { return kEdgeEmpty ? 0 : sizeof(ED); }
 
   void release() {
-    void* ptrs[] = {d_srcval, d_cbase, d_heavy,   d_vd,       d_vd_init, d_vd_user, d_ed,   d_ed_csr, d_cand, d_active,
+    void* ptrs[] = {d_touch, d_srcval, d_cbase, d_heavy,   d_vd,       d_vd_init, d_vd_user, d_ed,   d_ed_csr, d_cand, d_active,
                     d_escan, d_gtmp,    d_chg_v,    d_chg_e,   d_bitmap, d_chg_bits,
                     d_chg_prev, d_blk,  d_part,     d_push_part, d_carry, d_head,
                     d_ctr};
@@ -987,6 +1057,7 @@ struct Engine final : EngineBase {
     d_ed = nullptr;
     d_ed_csr = nullptr;
     d_cand = nullptr;
+    d_touch = nullptr;
     d_srcval = nullptr;
     d_cbase = nullptr;
     d_active = d_escan = d_chg_v = d_chg_e = d_part = d_push_part = nullptr;
@@ -1136,12 +1207,16 @@ struct Engine final : EngineBase {
       GR_TRY(dalloc(&d_cand, V));
       k_fill<G><<<1024, 256, 0, stream>>>(V, P::gatherZero, d_cand);
       GR_HIP(hipGetLastError());
+      // whole 32-byte groups for every bitmap word
+      GR_TRY(dalloc(&d_touch, (int64_t)n_words * 32 + 32));
+      GR_HIP(hipMemsetAsync(d_touch, 0, (size_t)n_words * 32 + 32, stream));
     }
     GR_HIP(hipStreamSynchronize(stream));
     n_active = 0;
     w_edges = 0;
     iteration = 0;
     delta_chain = delta_ready = false;
+    fuse_launched = fused_pending = false;
     it_gather = it_apply = it_scatter = false;
     bound = true;
     return GR_OK;
@@ -1420,6 +1495,7 @@ struct Engine final : EngineBase {
     if (start < 0 || end < start || end > g->nV)
       return set_errorf(GR_ERR_INVALID, "setActive range [%d, %d) outside [0, %d)", start, end,
                         g->nV);
+    GR_TRY(drop_candidates());
     const int n = end - start;
     if (d_inv && n < g->nV) return set_active_list(start, end);
     GR_HIP(hipMemsetAsync(d_bitmap, 0, sizeof(uint32_t) * (n_words + 1), stream));
@@ -1443,6 +1519,7 @@ struct Engine final : EngineBase {
   // setActive on a sub-range of user ids with the ordered layout: the layout
   // ids of the range, ascending, with their in-edge scan built on the host.
   int set_active_list(int32_t start, int32_t end) {
+    GR_TRY(drop_candidates());
     const int n = end - start;
     std::vector<int32_t> ids((size_t)n), escan((size_t)n + 1), cbase((size_t)n + 1);
     if (n > 0)
@@ -1503,6 +1580,10 @@ struct Engine final : EngineBase {
     const int n = n_active, w = w_edges;
     const uint32_t* dbits = nullptr;
     if constexpr (kDelta) {
+      if (fused_pending && delta_ready) {
+        fused_pending = false;
+        return gather_collect();
+      }
       if (delta_ready) {
         if (delta_mode == 3 || (delta_mode == 1 && changed_share < kPushMaxChangedShare))
           return gather_push();
@@ -1546,6 +1627,36 @@ struct Engine final : EngineBase {
     return GR_OK;
   }
 
+  // The next gather after a fused scatterActivate: the candidates are already
+  // relaxed; read and reset them for the active rows.
+  int gather_collect() {
+    const int n = n_active;
+    tick();
+    k_relax_collect<P><<<grid_1d(n, 256), 256, 0, stream>>>(n, d_active, d_cand, d_gtmp);
+    tock(kPhPush);
+    GR_HIP(hipGetLastError());
+    stats.active_vertices += n;
+    stats.gathered_edges += w_edges;
+    stats.delta_gathers += 1;
+    stats.push_gathers += 1;
+    stats.fused_gathers += 1;
+    stats.fused_edges += chg_edges;
+    return GR_OK;
+  }
+
+  // Candidates relaxed by a fused scatterActivate whose gather will not run
+  // (the active set was replaced or the vertex data reset): back to gatherZero.
+  int drop_candidates() {
+    if constexpr (kDelta) {
+      if (fused_pending) {
+        k_fill<G><<<1024, 256, 0, stream>>>(g->nV, P::gatherZero, d_cand);
+        GR_HIP(hipGetLastError());
+      }
+    }
+    fused_pending = false;
+    return GR_OK;
+  }
+
   // Bitmap (in d_chg_prev) and list (d_chg_v / d_chg_e, chg_count / chg_edges)
   // of the non-inert vertices.
   int inert_scan() {
@@ -1577,8 +1688,9 @@ struct Engine final : EngineBase {
       k_relax_partition<<<grid_1d((int64_t)n_tiles + 1, 256), 256, 0, stream>>>(n_c, w_c, d_chg_e,
                                                                                 d_push_part);
       const int grid = n_tiles < 8192 ? n_tiles : 8192;
-      k_relax_tiles<P><<<grid, kNT, 0, stream>>>(w_c, d_push_part, d_chg_v, d_chg_e, g->off_out,
-                                                 g->dsts, d_vd, edge_csr_ptr(), d_cand);
+      k_relax_tiles<P><<<grid, kNT, 0, stream>>>(w_c, nullptr, -1, d_push_part, d_chg_v, d_chg_e,
+                                                 g->off_out, g->dsts, d_vd, edge_csr_ptr(),
+                                                 d_cand, nullptr, nullptr);
     }
     k_relax_collect<P><<<grid_1d(n, 256), 256, 0, stream>>>(n, d_active, d_cand, d_gtmp);
     tock(kPhPush);
@@ -1614,29 +1726,53 @@ struct Engine final : EngineBase {
     it_scatter = true;
     if (streaming) return scatter_streamed();
     const long long thr = pull_threshold();
+    // Fused activation + next push gather, decided on the device from the
+    // changed out-edges: only while the next gather may use the delta rule
+    // (the same conditions next_iter checks) and not in forced-pull mode.
+    long long fuse_max = -1;
+    if constexpr (kDelta) {
+      if (delta_chain && it_gather && it_apply && dir_mode != 2 &&
+          (delta_mode == 1 || delta_mode == 3))
+        fuse_max = delta_mode == 3 ? (long long)g->nE
+                                   : (long long)(kFuseMaxShare * (double)g->nE);
+    }
+    fuse_launched = fuse_max >= 0;
     tick();
     if (dir_mode != 1) {
-      k_pull_stream<<<grid_1d(g->nV, 256), 256, 0, stream>>>(d_ctr, dir_mode, thr, g->nV,
-                                                              g->off_in, g->srcs, d_chg_bits,
-                                                              d_bitmap);
+      k_pull_stream<<<grid_1d(g->nV, 256), 256, 0, stream>>>(d_ctr, dir_mode, thr, fuse_max,
+                                                              g->nV, g->off_in, g->srcs,
+                                                              d_chg_bits, d_bitmap);
       if (n_heavy > 0)
         k_pull_heavy<<<n_heavy < 4096 ? n_heavy : 4096, 256, 0, stream>>>(
-            d_ctr, dir_mode, thr, d_heavy, n_heavy, g->off_in, g->srcs, d_chg_bits, d_bitmap);
+            d_ctr, dir_mode, thr, fuse_max, d_heavy, n_heavy, g->off_in, g->srcs, d_chg_bits,
+            d_bitmap);
     }
-    if (dir_mode != 2) {
+    {
+      // partition of the changed out-edges (push or fused pass); also
+      // records in ctr->fused whether the fused pass runs
       k_push_partition<<<grid_1d((int64_t)max_push_tiles + 1, 256), 256, 0, stream>>>(
-          d_ctr, dir_mode, thr, d_chg_e, d_push_part, max_push_tiles);
+          d_ctr, dir_mode, thr, d_chg_e, d_push_part, max_push_tiles, fuse_max, 1);
       int grid = max_push_tiles < 8192 ? max_push_tiles : 8192;
       if (grid < 1) grid = 1;
-      k_push_tiles<<<grid, kNT, 0, stream>>>(d_ctr, dir_mode, thr, d_push_part, d_chg_v, d_chg_e,
-                                             g->off_out, g->dsts, d_bitmap);
+      if (dir_mode != 2)
+        k_push_tiles<<<grid, kNT, 0, stream>>>(d_ctr, dir_mode, thr, fuse_max, d_push_part,
+                                               d_chg_v, d_chg_e, g->off_out, g->dsts, d_bitmap);
+      if constexpr (kDelta) {
+        if (fuse_launched) {
+          k_relax_tiles<P><<<grid, kNT, 0, stream>>>(0, d_ctr, fuse_max, d_push_part, d_chg_v,
+                                                     d_chg_e, g->off_out, g->dsts, d_vd,
+                                                     edge_csr_ptr(), d_cand, d_bitmap, d_touch);
+          k_touch_to_bits<<<grid_1d(n_words, 256) < 4096 ? grid_1d(n_words, 256) : 4096, 256, 0,
+                            stream>>>(d_ctr, fuse_max, n_words, d_touch, d_bitmap);
+        }
+      }
     }
     if constexpr (kScatter) {
       // Program::scatter, as the reference's CPU engine runs it when
       // haveScatter (refgas.h:192-196); its GPU engine ignores the flag
       if (have_scatter) {
         k_push_partition<<<grid_1d((int64_t)max_push_tiles + 1, 256), 256, 0, stream>>>(
-            d_ctr, 1, 0, d_chg_e, d_push_part, max_push_tiles);
+            d_ctr, 1, 0, d_chg_e, d_push_part, max_push_tiles, -1, 0);
         const int grid = max_push_tiles < 8192 ? max_push_tiles : 8192;
         k_scatter_tiles<P><<<grid < 1 ? 1 : grid, kNT, 0, stream>>>(
             d_ctr, d_push_part, d_chg_v, d_chg_e, g->off_out, g->dsts, g->csr_eid, d_vd, d_ed);
@@ -1668,6 +1804,12 @@ struct Engine final : EngineBase {
     it_gather = it_apply = it_scatter = false;
     GR_HIP(hipMemcpyAsync(h_ctr, d_ctr, sizeof(DevCounters), hipMemcpyDeviceToHost, stream));
     GR_HIP(hipStreamSynchronize(stream));
+    if constexpr (kDelta) {
+      const bool fused = fuse_launched && h_ctr->fused;
+      fuse_launched = false;
+      fused_pending = fused;
+      if (fused && !delta_ready) GR_TRY(drop_candidates());  // not reached: same conditions
+    }
     const unsigned long long c = h_ctr->chg_last;
     stats.changed_vertices += (int64_t)(c >> 32);
     stats.pushed_edges += (int64_t)(uint32_t)c;
@@ -1700,6 +1842,7 @@ struct Engine final : EngineBase {
   }
 
   int reset_vertex_data() override {
+    GR_TRY(drop_candidates());
     GR_HIP(hipMemcpyAsync(d_vd, d_vd_init, sizeof(VD) * g->nV, hipMemcpyDeviceToDevice, stream));
     GR_TRY(refresh_source_values());
     delta_chain = delta_ready = false;

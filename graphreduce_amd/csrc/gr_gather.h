@@ -33,7 +33,7 @@ struct DevCounters {
   unsigned long long chg;         // (#changed << 32) | out-edges, appended by apply
   unsigned long long chg_last;    // chg as seen by the last nextIter
   int32_t pull;                   // last scatterActivate ran in pull mode
-  int32_t pad;
+  int32_t fused;                  // last scatterActivate also pushed the next gather
   unsigned long long inert;       // (#non-inert << 32) | out-edges, from k_inert_scan
 };
 

@@ -644,3 +644,44 @@ def test_streaming_edge_cases(gpu):
     eng.setGraph(8, vd, len(s), w, s, d)
     assert eng.readEdgeData().tolist() == w.tolist()
     eng.close()
+
+
+@pytest.mark.parametrize("delta", [1, 3])
+def test_fused_scatter_gather(gpu, delta):
+    """When the changed out-edges are few, scatterActivate also relaxes the
+    next gather's candidates (one pass); values, trips and counters stay the
+    oracle's.  Restarting right after such a scatter (resetVertexData +
+    setActive) must not leak the pending candidates into the new run."""
+    gr = gpu
+    n, s, d = _rmat(gr, 13, seed=9)
+    w = gr.hash_weights(len(s), seed=9)
+    src = _max_out_src(n, s)
+    ref, rtrips, rst = oracle.sssp(n, s, d, w, src)
+    vd = np.full(n, gr.SSSP_INF, np.int32)
+    vd[src] = 0
+    eng = gr.GASEngine(gr.SSSP)
+    eng.setDeltaGather(delta)
+    eng.setGraph(n, vd, len(s), w, s, d)
+
+    def step():
+        eng.gather()
+        eng.apply()
+        eng.scatterActivate()
+        eng.nextIter()
+
+    eng.setActive(0, n)
+    for _ in range(rtrips - 1):  # stop with a fused gather still pending
+        step()
+    eng.resetVertexData()
+    eng.setActive(0, n)
+    eng.resetStats()
+    trips = 0
+    while eng.countActive():
+        step()
+        trips += 1
+    eng.getResults()
+    st = eng.stats()
+    assert np.array_equal(eng.vertexData, ref) and trips == rtrips
+    assert st["fused_gathers"] > 0 and st["fused_edges"] > 0
+    assert st["active_vertices"] == rst["active"] and st["gathered_edges"] == rst["in_edges"]
+    eng.close()
