@@ -218,6 +218,7 @@ def main() -> int:
                                  "distance) + 16 B per active vertex",
             "push_gathers": {
                 "launches": st["push_gathers"],
+                "fused_with_scatter": st["fused_gathers"],
                 "edges": st["push_edges"],
                 "ms": round(st["push_ms"], 4),
                 "alg_gbs": round(12 * st["push_edges"] / (st["push_ms"] / 1e3) / 1e9, 1)

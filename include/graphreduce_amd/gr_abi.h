@@ -304,8 +304,8 @@ typedef struct gr_engine_stats {
   int64_t streamed_shards;   /* ... in this many shard copies */
   int64_t fused_gathers;     /* push gathers whose relax ran fused with the previous
                                 scatterActivate (counted in push_gathers too) */
-  int64_t fused_edges;       /* out-edges relaxed by those fused passes (their time is
-                                in scatter_ms; not in push_edges) */
+  int64_t fused_edges;       /* out-edges relaxed by those fused passes (also in
+                                push_edges; their time is in push_ms) */
 } gr_engine_stats;
 
 /* Enables hipEvent timing of the engine phases on the engine's stream. */
