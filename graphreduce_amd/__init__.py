@@ -175,6 +175,7 @@ _SIGS = {
     "gr_engine_set_delta_gather": (C.c_int, [_P, C.c_int]),
     "gr_engine_set_vertex_order": (C.c_int, [_P, C.c_int]),
     "gr_engine_set_streaming": (C.c_int, [_P, C.c_int64]),
+    "gr_engine_set_fuse_share": (C.c_int, [_P, C.c_double]),
     "gr_engine_enable_timing": (C.c_int, [_P, C.c_int]),
     "gr_engine_get_stats": (C.c_int, [_P, C.POINTER(_Stats)]),
     "gr_engine_reset_stats": (C.c_int, [_P]),
@@ -638,6 +639,11 @@ class GASEngine:
     def setDirection(self, mode: int = 0, alpha: float = 15.0) -> None:
         """0 = automatic push/pull, 1 = push only, 2 = pull only."""
         _check(lib().gr_engine_set_direction(self._h, mode, alpha))
+
+    def setFuseShare(self, share: float) -> None:
+        """SSSP/CC: fuse activation with the next push gather when the changed
+        out-edges are at most share * E (0: off; same results)."""
+        _check(lib().gr_engine_set_fuse_share(self._h, float(share)))
 
     def setStreaming(self, max_edges_per_shard: int) -> None:
         """Out-of-core mode for the next setGraph: the edge arrays stay in

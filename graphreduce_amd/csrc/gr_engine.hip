@@ -897,6 +897,7 @@ struct EngineBase {
   int gather_vt = 8;  // merge-path items per thread in the gather tiles (4, 8 or 16)
   int vertex_order = 1;  // 1: engine works on the degree-ordered copy of the graph
   int64_t stream_max_edges = 0;  // > 0: out-of-core shards of at most this many edges
+  double fuse_share = 0.4;        // fused activation + push gather up to this share of E
   bool timing = false;
   gr_engine_stats stats{};
 };
@@ -984,10 +985,9 @@ struct Engine final : EngineBase {
   double changed_share = 1.0;
   static constexpr double kPushMaxChangedShare = 0.6;
   // Fused activation + push gather when the changed vertices' out-edges are at
-  // most this share of E (adaptive mode): the separate activation (pull scans
-  // of whole rows, or a push) plus the next push relax cost more than one
-  // relax pass that also sets the frontier bits.
-  static constexpr double kFuseMaxShare = 0.4;
+  // most fuse_share of E (adaptive mode; gr_engine_set_fuse_share): below it,
+  // the separate activation (pull scans of whole rows, or a push) plus the
+  // next push relax cost more than one relax pass that also marks the targets.
   bool fuse_launched = false;  // scatterActivate enqueued the fused pass
   bool fused_pending = false;  // d_cand holds the next gather's values
   // Inert sources (programs with kInertSources): with every bound edge passing
@@ -1734,8 +1734,9 @@ struct Engine final : EngineBase {
     if constexpr (kDelta) {
       if (delta_chain && it_gather && it_apply && dir_mode != 2 &&
           (delta_mode == 1 || delta_mode == 3))
-        fuse_max = delta_mode == 3 ? (long long)g->nE
-                                   : (long long)(kFuseMaxShare * (double)g->nE);
+        fuse_max = delta_mode == 3 || fuse_share >= 1.0
+                       ? (long long)g->nE : (long long)(fuse_share * (double)g->nE);
+      if (fuse_share <= 0 && delta_mode != 3) fuse_max = -1;
     }
     fuse_launched = fuse_max >= 0;
     tick();
@@ -2101,6 +2102,13 @@ extern "C" int gr_engine_set_direction(gr_engine* e, int mode, double alpha) {
   if (mode < 0 || mode > 2) return set_error(GR_ERR_INVALID, "direction mode must be 0, 1 or 2");
   e->impl->dir_mode = mode;
   if (alpha > 0) e->impl->dir_alpha = alpha;
+  return GR_OK;
+}
+
+extern "C" int gr_engine_set_fuse_share(gr_engine* e, double share) {
+  GR_TRY(check(e, false));
+  if (!(share >= 0.0)) return set_error(GR_ERR_INVALID, "fuse share must be >= 0");
+  e->impl->fuse_share = share;
   return GR_OK;
 }
 

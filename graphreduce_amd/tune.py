@@ -1,10 +1,11 @@
 """Autotuner of the engine's launch parameters.
 
 Chooses, per vertex program and graph, the gather tile shape (merge-path
-items per thread: 4, 8 or 16 in a 256-thread tile) and the push/pull switch
+items per thread: 4, 8 or 16 in a 256-thread tile), the push/pull switch
 point alpha of scatterActivate (pull when the out-edges of the changed
-vertices exceed E / alpha), by timing complete runs of the reference loop on
-the device graph.  Results are written as JSON and applied with
+vertices exceed E / alpha) and, for the delta-gather programs, the share of E
+up to which activation and the next push gather are fused, by timing complete
+runs of the reference loop on the device graph.  Results are written as JSON and applied with
 :func:`apply_tuning`.
 
 Usage:  python -m graphreduce_amd.tune --scale 24 --seed 6 --out graphreduce_amd/tuned.json
@@ -24,6 +25,7 @@ import graphreduce_amd as gr
 DEFAULT_FILE = Path(__file__).resolve().parent / "tuned.json"
 TILES = (4, 8, 16)
 ALPHAS = (1.0, 2.0, 4.0, 8.0, 15.0, 30.0, 60.0)
+FUSE_SHARES = (0.0, 0.2, 0.4, 0.6)  # SSSP and CC only
 
 
 def _loop(eng, n, source=None):
@@ -63,7 +65,8 @@ def _engine(program, g, weights=None):
 
 
 def tune_program(program, g, source, weights=None, repeats=2, log=print):
-    """Returns {"gather_tile": vt, "alpha": a, "ms": best} for one program."""
+    """Returns {"gather_tile": vt, "alpha": a, ["fuse_share": f,] "ms": best}
+    for one program."""
     eng, vd = _engine(program, g)
     if program == gr.SSSP:
         vd[source] = 0
@@ -72,9 +75,12 @@ def tune_program(program, g, source, weights=None, repeats=2, log=print):
     _loop(eng, g.n_vertices, src)  # warm
     best = None
     tiles = TILES if program != gr.BFS else (8,)
-    for vt, alpha in itertools.product(tiles, ALPHAS):
+    fuses = FUSE_SHARES if program in (gr.SSSP, gr.CC) else (None,)
+    for vt, alpha, fuse in itertools.product(tiles, ALPHAS, fuses):
         eng.setGatherTile(vt)
         eng.setDirection(0, alpha)
+        if fuse is not None:
+            eng.setFuseShare(fuse)
         ts = []
         for _ in range(repeats):
             gr.synchronize()
@@ -83,9 +89,11 @@ def tune_program(program, g, source, weights=None, repeats=2, log=print):
             gr.synchronize()
             ts.append(time.perf_counter() - t0)
         ms = min(ts) * 1e3
-        log(f"  {gr.PROGRAMS[program]['name']} tile={vt} alpha={alpha}: {ms:.3f} ms")
+        log(f"  {gr.PROGRAMS[program]['name']} tile={vt} alpha={alpha} fuse={fuse}: {ms:.3f} ms")
         if best is None or ms < best["ms"]:
             best = {"gather_tile": vt, "alpha": alpha, "ms": round(ms, 4)}
+            if fuse is not None:
+                best["fuse_share"] = fuse
     eng.close()
     return best
 
@@ -101,6 +109,8 @@ def apply_tuning(eng: "gr.GASEngine", program: int, path: Path | str = DEFAULT_F
         return False
     eng.setGatherTile(int(entry["gather_tile"]))
     eng.setDirection(0, float(entry["alpha"]))
+    if "fuse_share" in entry:
+        eng.setFuseShare(float(entry["fuse_share"]))
     return True
 
 

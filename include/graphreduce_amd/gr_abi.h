@@ -255,6 +255,14 @@ int gr_engine_set_vertex_order(gr_engine* e, int order);
  * gr_engine_bind_graph ignores it.  0 (default): the graph lives in HBM. */
 int gr_engine_set_streaming(gr_engine* e, int64_t max_edges_per_shard);
 
+/* Delta-gather programs, adaptive mode: scatterActivate also relaxes the next
+ * gather's candidates (one pass over the changed out-edges that marks the
+ * next frontier too) when those edges are at most share * E; 0 turns the
+ * fusion off, 1 or more fuses every such scatter.  Default 0.4
+ * (graphreduce_amd.tune sweeps it per program).  Results are the same either
+ * way.  Negative or NaN shares: GR_ERR_INVALID. */
+int gr_engine_set_fuse_share(gr_engine* e, double share);
+
 /* Delta gather (no reference counterpart).  For SSSP and CC, once an
  * iteration with every vertex active has run (setActive(0, V)) and every
  * later active set came from scatterActivate, gather may read only the

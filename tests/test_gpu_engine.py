@@ -685,3 +685,36 @@ def test_fused_scatter_gather(gpu, delta):
     assert st["fused_gathers"] > 0 and st["fused_edges"] > 0
     assert st["active_vertices"] == rst["active"] and st["gathered_edges"] == rst["in_edges"]
     eng.close()
+
+
+@pytest.mark.parametrize("prog_name", ["SSSP", "CC"])
+def test_fuse_share_keeps_results(gpu, prog_name):
+    """setFuseShare moves work between the scatter and the next gather only:
+    off (0), the default and always (1) give the oracle's values, trips and
+    work counters; 0 fuses nothing, 1 fuses every eligible scatter."""
+    gr = gpu
+    n, s, d = _rmat(gr, 14, seed=12, sym=prog_name == "CC")
+    if prog_name == "SSSP":
+        prog, w = gr.SSSP, gr.hash_weights(len(s), seed=12)
+        src = _max_out_src(n, s)
+        vd = np.full(n, gr.SSSP_INF, np.int32)
+        vd[src] = 0
+        ref, rtrips, rst = oracle.sssp(n, s, d, w, src)
+    else:
+        prog, w, vd = gr.CC, None, np.arange(n, dtype=np.int32)
+        ref, rtrips, rst = oracle.cc(n, s, d)
+    fused = {}
+    for share in (0.0, 0.4, 1.0):
+        eng = gr.GASEngine(prog)
+        eng.setFuseShare(share)
+        eng.setGraph(n, vd.copy(), len(s), w, s, d)  # getResults writes back into it
+        eng.setActive(0, n)
+        trips = eng.run()
+        eng.getResults()
+        st = eng.stats()
+        assert np.array_equal(eng.vertexData, ref), share
+        assert trips == rtrips, share
+        assert st["active_vertices"] == rst["active"] and st["gathered_edges"] == rst["in_edges"]
+        fused[share] = st["fused_gathers"]
+        eng.close()
+    assert fused[0.0] == 0 and fused[1.0] >= fused[0.4] and fused[1.0] > 0
