@@ -244,12 +244,13 @@ __device__ __forceinline__ bool fused_scatter(unsigned long long chg, long long 
   return fuse_max >= 0 && (chg >> 32) != 0 && (long long)(uint32_t)chg <= fuse_max;
 }
 
-// A fused pass over more than kTouchMinEdges out-edges marks its targets
-// with plain byte stores in `touch` (no atomics); k_touch_to_bits then folds
-// the bytes into the frontier bitmap.
-constexpr long long kTouchMinEdges = 1 << 20;
-__device__ __forceinline__ bool fused_touch(unsigned long long chg, long long fuse_max) {
-  return fused_scatter(chg, fuse_max) && (long long)(uint32_t)chg > kTouchMinEdges;
+// A fused pass over more than kScanMinEdges out-edges leaves the marking of
+// most targets to k_cand_to_bits: a target whose candidate ends below
+// gatherZero was relaxed, so one sequential scan of the candidates replaces a
+// random bitmap update per edge.
+constexpr long long kScanMinEdges = 1 << 20;
+__device__ __forceinline__ bool fused_scan(unsigned long long chg, long long fuse_max) {
+  return fused_scatter(chg, fuse_max) && (long long)(uint32_t)chg > kScanMinEdges;
 }
 
 __device__ __forceinline__ int upper_bound_dev(const int32_t* __restrict__ a, int n, int x) {
@@ -400,25 +401,26 @@ __global__ void k_relax_partition(int n_c, int w_c, const int32_t* __restrict__ 
 
 template <class P>
 // With ctr (the fused pass of scatterActivate): runs only when fused_scatter
-// holds, takes the edge count from ctr->chg and also sets the targets'
-// frontier bits in `bitmap`.
+// holds, takes the edge count from ctr->chg and also marks the targets in the
+// frontier `bitmap`; with scan_marks above kScanMinEdges edges only those
+// whose relaxed value is not below gatherZero (k_cand_to_bits finds the rest).
 __global__ __launch_bounds__(kNT) void k_relax_tiles(
     int w_c, const DevCounters* __restrict__ ctr, long long fuse_max,
     const int32_t* __restrict__ part, const int32_t* __restrict__ chg_v,
     const int32_t* __restrict__ chg_e, const int32_t* __restrict__ off_out,
     const int32_t* __restrict__ dsts, const typename P::VertexData* __restrict__ vd,
     const typename P::EdgeData* __restrict__ ed_csr, typename P::GatherResult* __restrict__ cand,
-    uint32_t* __restrict__ bitmap, uint8_t* __restrict__ touch) {
+    uint32_t* __restrict__ bitmap, int scan_marks) {
   static_assert(!P::kGatherUsesDst, "push relax needs a gatherMap without the destination");
   __shared__ int32_t s_start[kTile + 2];
   __shared__ int32_t s_csr[kTile + 2];
   __shared__ typename P::VertexData s_vd[kTile + 2];
-  bool bytes = false;
+  bool scan = false;
   if (ctr) {
     const unsigned long long c = ctr->chg;
     if (!fused_scatter(c, fuse_max)) return;
     w_c = (int)(uint32_t)c;
-    bytes = touch && fused_touch(c, fuse_max);
+    scan = scan_marks && fused_scan(c, fuse_max);
   }
   const int n_tiles = (w_c + kTile - 1) / kTile;
   const int tid = threadIdx.x;
@@ -450,9 +452,7 @@ __global__ __launch_bounds__(kNT) void k_relax_tiles(
         const typename P::EdgeData ev = stream_load(ed_csr + pos);
         const typename P::GatherResult val = P::gatherMap(nullptr, &s_vd[lo], &ev);
         if (val < cand[v]) atomicMin(&cand[v], val);
-        if (bytes) {
-          touch[v] = 1;
-        } else if (bitmap) {
+        if (bitmap && !(scan && val < P::gatherZero)) {
           const uint32_t bit = 1u << (v & 31);
           if (!(bitmap[v >> 5] & bit)) atomicOr(&bitmap[v >> 5], bit);
         }
@@ -462,26 +462,30 @@ __global__ __launch_bounds__(kNT) void k_relax_tiles(
   }
 }
 
-// After a fused pass that marked bytes: bitmap word w |= the 32 touch bytes of
-// its vertices, which are cleared (only this kernel writes those words then:
-// the other activation kernels stood down).
-__global__ void k_touch_to_bits(const DevCounters* __restrict__ ctr, long long fuse_max,
-                                int n_words, uint8_t* __restrict__ touch,
-                                uint32_t* __restrict__ bitmap) {
-  if (!fused_touch(ctr->chg, fuse_max)) return;
-  for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < n_words; w += gridDim.x * blockDim.x) {
-    uint4* t = reinterpret_cast<uint4*>(touch + (size_t)w * 32);
-    const uint4 a = t[0], b = t[1];
-    const uint32_t q[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-    uint32_t m = 0;
-#pragma unroll
-    for (int i = 0; i < 8; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) m |= ((q[i] >> (8 * j)) & 1u) << (4 * i + j);
-    if (m) {
-      bitmap[w] |= m;
-      t[0] = make_uint4(0, 0, 0, 0);
-      t[1] = make_uint4(0, 0, 0, 0);
+// After a fused pass with scan marks: bitmap |= {v : cand[v] < gatherZero}.
+// The candidates were all gatherZero before the pass and only fall during it,
+// and every relaxed value below gatherZero leaves its target below it, so the
+// set is exactly those targets (the others were marked by the pass itself).
+// One wave reads 64 consecutive candidates and writes their two bitmap words;
+// cand holds whole 32-entry groups (the padding stays gatherZero).  Only this
+// kernel writes the bitmap then: the other activation kernels stood down.
+template <class P>
+__global__ void k_cand_to_bits(const DevCounters* __restrict__ ctr, long long fuse_max,
+                               int n_words, const typename P::GatherResult* __restrict__ cand,
+                               uint32_t* __restrict__ bitmap) {
+  if (!fused_scan(ctr->chg, fuse_max)) return;
+  const long long total = (long long)n_words * 32;
+  const int lane = threadIdx.x & 63;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long base = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < total;
+       base += stride) {
+    const long long i = base + lane;
+    const bool hit = i < total && cand[i] < P::gatherZero;
+    const unsigned long long m = __ballot(hit);
+    if (lane == 0 && m) {
+      const long long w = base >> 5;
+      if ((uint32_t)m) bitmap[w] |= (uint32_t)m;
+      if ((m >> 32) && w + 1 < n_words) bitmap[w + 1] |= (uint32_t)(m >> 32);
     }
   }
 }
@@ -925,7 +929,6 @@ struct Engine final : EngineBase {
   ED* d_ed = nullptr;  // CSC order (edge-list order for programs with a scatter)
   ED* d_ed_csr = nullptr;  // CSR order (delta-gather programs: push relax)
   G* d_cand = nullptr;     // push-relax candidates per vertex (gatherZero between gathers)
-  uint8_t* d_touch = nullptr;  // fused activation bytes (all zero between scatters)
   source_value_t<P>* d_srcval = nullptr;  // P::sourceValue of every vertex (programs with one)
   // ---- out-of-core mode: edge arrays in pinned host memory, streamed per shard
   bool streaming = false;
@@ -1007,7 +1010,7 @@ struct Engine final : EngineBase {
   size_t edge_size() const override { return kEdgeEmpty ? 0 : sizeof(ED); }
 
   void release() {
-    void* ptrs[] = {d_touch, d_srcval, d_cbase, d_heavy,   d_vd,       d_vd_init, d_vd_user, d_ed,   d_ed_csr, d_cand, d_active,
+    void* ptrs[] = {d_srcval, d_cbase, d_heavy,   d_vd,       d_vd_init, d_vd_user, d_ed,   d_ed_csr, d_cand, d_active,
                     d_escan, d_gtmp,    d_chg_v,    d_chg_e,   d_bitmap, d_chg_bits,
                     d_chg_prev, d_blk,  d_part,     d_push_part, d_carry, d_head,
                     d_ctr};
@@ -1057,7 +1060,6 @@ struct Engine final : EngineBase {
     d_ed = nullptr;
     d_ed_csr = nullptr;
     d_cand = nullptr;
-    d_touch = nullptr;
     d_srcval = nullptr;
     d_cbase = nullptr;
     d_active = d_escan = d_chg_v = d_chg_e = d_part = d_push_part = nullptr;
@@ -1204,12 +1206,10 @@ struct Engine final : EngineBase {
       }
     }
     if constexpr (kDelta) {
-      GR_TRY(dalloc(&d_cand, V));
-      k_fill<G><<<1024, 256, 0, stream>>>(V, P::gatherZero, d_cand);
+      // whole 32-entry groups for every bitmap word (k_cand_to_bits)
+      GR_TRY(dalloc(&d_cand, (int64_t)n_words * 32));
+      k_fill<G><<<1024, 256, 0, stream>>>((int64_t)n_words * 32, P::gatherZero, d_cand);
       GR_HIP(hipGetLastError());
-      // whole 32-byte groups for every bitmap word
-      GR_TRY(dalloc(&d_touch, (int64_t)n_words * 32 + 32));
-      GR_HIP(hipMemsetAsync(d_touch, 0, (size_t)n_words * 32 + 32, stream));
     }
     GR_HIP(hipStreamSynchronize(stream));
     n_active = 0;
@@ -1691,7 +1691,7 @@ struct Engine final : EngineBase {
       const int grid = n_tiles < 8192 ? n_tiles : 8192;
       k_relax_tiles<P><<<grid, kNT, 0, stream>>>(w_c, nullptr, -1, d_push_part, d_chg_v, d_chg_e,
                                                  g->off_out, g->dsts, d_vd, edge_csr_ptr(),
-                                                 d_cand, nullptr, nullptr);
+                                                 d_cand, nullptr, 0);
     }
     k_relax_collect<P><<<grid_1d(n, 256), 256, 0, stream>>>(n, d_active, d_cand, d_gtmp);
     tock(kPhPush);
@@ -1779,9 +1779,10 @@ struct Engine final : EngineBase {
         const int grid = max_push_tiles < 8192 ? (max_push_tiles < 1 ? 1 : max_push_tiles) : 8192;
         k_relax_tiles<P><<<grid, kNT, 0, stream>>>(0, d_ctr, fuse_max, d_push_part, d_chg_v,
                                                    d_chg_e, g->off_out, g->dsts, d_vd,
-                                                   edge_csr_ptr(), d_cand, d_bitmap, d_touch);
-        k_touch_to_bits<<<grid_1d(n_words, 256) < 4096 ? grid_1d(n_words, 256) : 4096, 256, 0,
-                          stream>>>(d_ctr, fuse_max, n_words, d_touch, d_bitmap);
+                                                   edge_csr_ptr(), d_cand, d_bitmap, 1);
+        const int64_t n_scan = (int64_t)n_words * 32;
+        k_cand_to_bits<P><<<grid_1d(n_scan, 256) < 4096 ? grid_1d(n_scan, 256) : 4096, 256, 0,
+                            stream>>>(d_ctr, fuse_max, n_words, d_cand, d_bitmap);
         tock(kPhPush);
       }
     }

@@ -718,3 +718,40 @@ def test_fuse_share_keeps_results(gpu, prog_name):
         fused[share] = st["fused_gathers"]
         eng.close()
     assert fused[0.0] == 0 and fused[1.0] >= fused[0.4] and fused[1.0] > 0
+
+
+@pytest.mark.parametrize("prog_name", ["SSSP", "CC"])
+def test_fused_pass_marks_by_candidate_scan(gpu, prog_name):
+    """A fused pass over more than 2^20 out-edges leaves the frontier marks to
+    a scan of the candidates (a relaxed target ends below gatherZero): the
+    next active lists, values, trips and counters stay the oracle's."""
+    gr = gpu
+    n, s, d = _rmat(gr, 18, seed=21, sym=prog_name == "CC")
+    if prog_name == "SSSP":
+        prog, w = gr.SSSP, gr.hash_weights(len(s), seed=21)
+        src = _max_out_src(n, s)
+        vd = np.full(n, gr.SSSP_INF, np.int32)
+        vd[src] = 0
+        ref, rtrips, rst = oracle.sssp(n, s, d, w, src)
+    else:
+        prog, w, vd = gr.CC, None, np.arange(n, dtype=np.int32)
+        ref, rtrips, rst = oracle.cc(n, s, d)
+    eng = gr.GASEngine(prog)
+    eng.setFuseShare(1.0)
+    eng.setGraph(n, vd, len(s), w, s, d)
+    eng.setActive(0, n)
+    trips, biggest, prev = 0, 0, 0
+    while eng.countActive():
+        eng.gather()
+        eng.apply()
+        eng.scatterActivate()
+        eng.nextIter()
+        trips += 1
+        fe = eng.stats()["fused_edges"]
+        biggest, prev = max(biggest, fe - prev), fe
+    eng.getResults()
+    st = eng.stats()
+    assert biggest > 1 << 20, biggest
+    assert np.array_equal(eng.vertexData, ref) and trips == rtrips
+    assert st["active_vertices"] == rst["active"] and st["gathered_edges"] == rst["in_edges"]
+    eng.close()
