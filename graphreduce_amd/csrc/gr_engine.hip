@@ -85,35 +85,61 @@ __global__ __launch_bounds__(kApplyNT) void k_apply(
   __shared__ unsigned long long s_base;
   const int tid = threadIdx.x;
   for (int chunk = blockIdx.x * kApplyChunk; chunk < n; chunk += gridDim.x * kApplyChunk) {
+    // In phases over the lane's kApplyVT entries (ids and gather values,
+    // vertex values, apply, out-degrees of the changed ones, then the stores
+    // and bit atomics), so that the loads of all entries are in flight
+    // together: a store or atomic between them would make later waits for
+    // loaded data wait for it too.
+    typedef typename P::VertexData VD;
     int dvs[kApplyVT], degs[kApplyVT];
-    unsigned want_mask = 0;
+    typename P::GatherResult gs[kApplyVT];
+    VD vs[kApplyVT];
+    unsigned valid_mask = 0, r_mask = 0, chg_mask = 0, want_mask = 0;
     int my_cnt = 0, my_edges = 0;
 #pragma unroll
     for (int m = 0; m < kApplyVT; ++m) {
       const int i = chunk + m * kApplyNT + tid;
-      const bool valid = i < n;
-      bool r = false;
-      int dv = 0, deg = 0;
-      if (valid) {
-        dv = active[i];
-        typename P::GatherResult g = use_gather ? gtmp[i] : P::gatherZero;
-        typename P::VertexData v = vd[dv];
-        const typename P::VertexData old = v;
-        r = P::apply(&v, g, iteration);
-        if (!pod_equal(v, old)) {
-          vd[dv] = v;
-          if constexpr (source_value_v<P>) srcval[dv] = P::sourceValue(v);
-        }
-        if (r) deg = off_out[dv + 1] - off_out[dv];
+      dvs[m] = 0;
+      gs[m] = P::gatherZero;
+      if (i < n) {
+        valid_mask |= 1u << m;
+        dvs[m] = active[i];
+        if (use_gather) gs[m] = gtmp[i];
       }
-      const bool want = r && deg > 0;
-      if (chg_bits) wave_set_bits_sorted(valid, want, dv, chg_bits);
-      dvs[m] = dv;
-      degs[m] = deg;
+    }
+#pragma unroll
+    for (int m = 0; m < kApplyVT; ++m)
+      if (valid_mask & (1u << m)) vs[m] = vd[dvs[m]];
+#pragma unroll
+    for (int m = 0; m < kApplyVT; ++m) {
+      if (valid_mask & (1u << m)) {
+        const VD old = vs[m];
+        if (P::apply(&vs[m], gs[m], iteration)) r_mask |= 1u << m;
+        if (!pod_equal(vs[m], old)) chg_mask |= 1u << m;
+      }
+    }
+    // unconditional offset reads (vertex 0's for the others) keep them batched
+    int o0[kApplyVT], o1[kApplyVT];
+#pragma unroll
+    for (int m = 0; m < kApplyVT; ++m) {
+      const int32_t* o = off_out + ((r_mask & (1u << m)) ? dvs[m] : 0);
+      o0[m] = o[0];
+      o1[m] = o[1];
+    }
+#pragma unroll
+    for (int m = 0; m < kApplyVT; ++m) degs[m] = (r_mask & (1u << m)) ? o1[m] - o0[m] : 0;
+#pragma unroll
+    for (int m = 0; m < kApplyVT; ++m) {
+      if (chg_mask & (1u << m)) {
+        vd[dvs[m]] = vs[m];
+        if constexpr (source_value_v<P>) srcval[dvs[m]] = P::sourceValue(vs[m]);
+      }
+      const bool want = degs[m] > 0;  // apply returned true and out-edges exist
+      if (chg_bits) wave_set_bits_sorted((valid_mask >> m) & 1u, want, dvs[m], chg_bits);
       if (want) {
         want_mask |= 1u << m;
         my_cnt += 1;
-        my_edges += deg;
+        my_edges += degs[m];
       }
     }
     int tc, te;
@@ -309,9 +335,14 @@ __global__ __launch_bounds__(kNT) void k_push_tiles(const DevCounters* __restric
       s_csr[k] = off_out[chg_v[k0 + k]];
     }
     __syncthreads();
+    // The lane's kVT targets, then their bitmap words, are all read before its
+    // first atomic: the compiler keeps later loads behind an atomic that may
+    // alias them, which would make the items one memory round trip each.
+    int pos[kVT];
 #pragma unroll
     for (int m = 0; m < kVT; ++m) {
       const int e = e0 + tid + m * kNT;
+      pos[m] = -1;
       if (e < e1) {
         int lo = 0, hi = nr - 1;
         while (lo < hi) {
@@ -321,13 +352,27 @@ __global__ __launch_bounds__(kNT) void k_push_tiles(const DevCounters* __restric
           else
             hi = mid - 1;
         }
-        const int dst = __builtin_nontemporal_load(dsts + s_csr[lo] + (e - s_start[lo]));
-        // many changed vertices share a few hub targets: an atomic on a word
-        // whose bit is already visible set would only serialise behind others
-        const uint32_t bit = 1u << (dst & 31);
-        if (!(bitmap[dst >> 5] & bit)) atomicOr(&bitmap[dst >> 5], bit);
+        pos[m] = s_csr[lo] + (e - s_start[lo]);
       }
     }
+    int dst[kVT];
+#pragma unroll
+    for (int m = 0; m < kVT; ++m)
+      dst[m] = pos[m] >= 0 ? __builtin_nontemporal_load(dsts + pos[m]) : -1;
+    uint32_t word[kVT];
+#pragma unroll
+    for (int m = 0; m < kVT; ++m) word[m] = dst[m] >= 0 ? bitmap[dst[m] >> 5] : ~0u;
+    // many changed vertices share a few hub targets: an atomic on a word
+    // whose bit is already visible set would only serialise behind others.
+    // The decisions come first, so that no wait for loaded data (which would
+    // also wait for the atomics already issued) falls between the atomics.
+    uint32_t need = 0;
+#pragma unroll
+    for (int m = 0; m < kVT; ++m)
+      if (dst[m] >= 0 && !(word[m] & (1u << (dst[m] & 31)))) need |= 1u << m;
+#pragma unroll
+    for (int m = 0; m < kVT; ++m)
+      if (need & (1u << m)) atomicOr(&bitmap[dst[m] >> 5], 1u << (dst[m] & 31));
     __syncthreads();
   }
 }
@@ -435,9 +480,17 @@ __global__ __launch_bounds__(kNT) void k_relax_tiles(
       s_vd[k] = vd[u];
     }
     __syncthreads();
+    // Targets and values of the lane's kVT edges, then every candidate (and
+    // frontier word) read, then the atomics: the compiler keeps later loads
+    // behind an atomic that may alias them, which would make the items one
+    // memory round trip each.
+    typedef typename P::GatherResult G;
+    int pos[kVT], row[kVT];
 #pragma unroll
     for (int m = 0; m < kVT; ++m) {
       const int e = e0 + tid + m * kNT;
+      pos[m] = -1;
+      row[m] = 0;
       if (e < e1) {
         int lo = 0, hi = nr - 1;
         while (lo < hi) {
@@ -447,16 +500,43 @@ __global__ __launch_bounds__(kNT) void k_relax_tiles(
           else
             hi = mid - 1;
         }
-        const int pos = s_csr[lo] + (e - s_start[lo]);
-        const int v = __builtin_nontemporal_load(dsts + pos);
-        const typename P::EdgeData ev = stream_load(ed_csr + pos);
-        const typename P::GatherResult val = P::gatherMap(nullptr, &s_vd[lo], &ev);
-        if (val < cand[v]) atomicMin(&cand[v], val);
-        if (bitmap && !(scan && val < P::gatherZero)) {
-          const uint32_t bit = 1u << (v & 31);
-          if (!(bitmap[v >> 5] & bit)) atomicOr(&bitmap[v >> 5], bit);
-        }
+        pos[m] = s_csr[lo] + (e - s_start[lo]);
+        row[m] = lo;
       }
+    }
+    int vv[kVT];
+    typename P::EdgeData ev[kVT];
+#pragma unroll
+    for (int m = 0; m < kVT; ++m) {
+      vv[m] = -1;
+      if (pos[m] >= 0) {
+        vv[m] = __builtin_nontemporal_load(dsts + pos[m]);
+        ev[m] = stream_load(ed_csr + pos[m]);
+      }
+    }
+    G val[kVT];
+#pragma unroll
+    for (int m = 0; m < kVT; ++m)
+      val[m] = pos[m] >= 0 ? P::gatherMap(nullptr, &s_vd[row[m]], &ev[m]) : P::gatherZero;
+    G cur[kVT];
+    uint32_t word[kVT];
+#pragma unroll
+    for (int m = 0; m < kVT; ++m) {
+      cur[m] = vv[m] >= 0 ? cand[vv[m]] : P::gatherZero;
+      const bool mark = vv[m] >= 0 && bitmap && !(scan && val[m] < P::gatherZero);
+      word[m] = mark ? bitmap[vv[m] >> 5] : ~0u;
+    }
+    // decisions first: no wait for loaded data between the atomics
+    uint32_t need_min = 0, need_or = 0;
+#pragma unroll
+    for (int m = 0; m < kVT; ++m) {
+      if (vv[m] >= 0 && val[m] < cur[m]) need_min |= 1u << m;
+      if (vv[m] >= 0 && !(word[m] & (1u << (vv[m] & 31)))) need_or |= 1u << m;
+    }
+#pragma unroll
+    for (int m = 0; m < kVT; ++m) {
+      if (need_min & (1u << m)) atomicMin(&cand[vv[m]], val[m]);
+      if (need_or & (1u << m)) atomicOr(&bitmap[vv[m] >> 5], 1u << (vv[m] & 31));
     }
     __syncthreads();
   }
