@@ -601,8 +601,12 @@ __global__ void k_fill(int64_t n, T value, T* __restrict__ out) {
 // needed.  The row holding the chunk start is found from a ballot over the
 // lanes' row offsets; rows already hit, and heavy rows (in-degree above
 // kPullHeavy, done by k_pull_heavy with a whole workgroup and early exit),
-// are jumped over.
+// are jumped over.  Before that, each lane tests the first kPullFirst
+// in-edges of its own row with all loads in flight at once: after a large
+// change most rows hit there, rows with at most kPullFirst in-edges are
+// settled either way, and the segment walk starts at the first open row.
 constexpr int kPullHeavy = 1024;
+constexpr int kPullFirst = 4;
 
 __device__ __forceinline__ uint64_t lane_range_mask(int lo, int hi) {  // bits [lo, hi)
   const uint64_t upto = hi >= 64 ? ~0ull : ((1ull << hi) - 1);
@@ -623,18 +627,33 @@ __global__ __launch_bounds__(256) void k_pull_stream(
     const bool mine = lane < cnt;
     const int my_off = mine ? off_in[base + lane] : INT_MAX;
     const int my_end = mine ? off_in[base + lane + 1] : INT_MAX;
-    const bool heavy = mine && (my_end - my_off) > kPullHeavy;
-    const uint64_t need = __ballot(mine && my_end > my_off && !heavy);
-    const uint64_t heavy_mask = __ballot(heavy);
-    const int seg0 = __shfl(my_off, 0, kWave);
+    const int deg = mine ? my_end - my_off : 0;
+    const bool heavy = deg > kPullHeavy;
+    // the first kPullFirst in-edges of the lane's own row
+    bool first_hit = false;
+    if (!heavy && deg > 0) {
+      int u[kPullFirst];
+#pragma unroll
+      for (int k = 0; k < kPullFirst; ++k) u[k] = k < deg ? srcs[my_off + k] : -1;
+      uint32_t wd[kPullFirst];
+#pragma unroll
+      for (int k = 0; k < kPullFirst; ++k) wd[k] = u[k] >= 0 ? chg_bits[u[k] >> 5] : 0u;
+#pragma unroll
+      for (int k = 0; k < kPullFirst; ++k)
+        first_hit |= u[k] >= 0 && ((wd[k] >> (u[k] & 31)) & 1u);
+    }
+    uint64_t hits = __ballot(first_hit);
+    // rows still open: more than kPullFirst in-edges, not heavy, no hit yet
+    const uint64_t need = __ballot(deg > kPullFirst && !heavy && !first_hit);
+    const uint64_t settled = ~need;  // for the walk: nothing left to find there
     const int seg1 = __shfl(my_end, cnt - 1, kWave);
-    uint64_t hits = 0;
-    int pb = seg0;
+    int pb = seg1;
+    if (need) pb = __shfl(my_off, __builtin_ctzll(need), kWave) + kPullFirst;
     while (pb < seg1 && (hits & need) != need) {
       // row holding edge pb: the last lane whose row starts at or before pb
       const uint64_t started = __ballot(mine && my_off <= pb);
       const int r0 = 63 - __builtin_clzll(started);
-      if (((heavy_mask | hits) >> r0) & 1ull) {
+      if (((settled | hits) >> r0) & 1ull) {
         pb = __shfl(my_end, r0, kWave);
         continue;
       }
