@@ -310,6 +310,12 @@ __global__ void k_push_partition(DevCounters* __restrict__ ctr, int mode, long l
   }
 }
 
+// Push activation.  Targets below n_hub (the highest-degree vertices in the
+// degree-ordered layout) are marked with a plain byte store in hub_flags,
+// which k_hub_fold folds into the bitmap: many changed vertices share a few
+// hub targets, and atomics on the same word are serialised where they are
+// performed, while stores to one line merge in each L2.  The other targets
+// are marked with an atomicOr unless their bit is already visible.
 __global__ __launch_bounds__(kNT) void k_push_tiles(const DevCounters* __restrict__ ctr,
                                                     int mode, long long threshold,
                                                     long long fuse_max,
@@ -318,7 +324,8 @@ __global__ __launch_bounds__(kNT) void k_push_tiles(const DevCounters* __restric
                                                     const int32_t* __restrict__ chg_e,
                                                     const int32_t* __restrict__ off_out,
                                                     const int32_t* __restrict__ dsts,
-                                                    uint32_t* __restrict__ bitmap) {
+                                                    uint32_t* __restrict__ bitmap,
+                                                    uint8_t* __restrict__ hub_flags, int n_hub) {
   __shared__ int32_t s_start[kTile + 2];
   __shared__ int32_t s_csr[kTile + 2];
   const unsigned long long c = ctr->chg;
@@ -361,19 +368,45 @@ __global__ __launch_bounds__(kNT) void k_push_tiles(const DevCounters* __restric
       dst[m] = pos[m] >= 0 ? __builtin_nontemporal_load(dsts + pos[m]) : -1;
     uint32_t word[kVT];
 #pragma unroll
-    for (int m = 0; m < kVT; ++m) word[m] = dst[m] >= 0 ? bitmap[dst[m] >> 5] : ~0u;
-    // many changed vertices share a few hub targets: an atomic on a word
-    // whose bit is already visible set would only serialise behind others.
+    for (int m = 0; m < kVT; ++m) word[m] = dst[m] >= n_hub ? bitmap[dst[m] >> 5] : ~0u;
     // The decisions come first, so that no wait for loaded data (which would
     // also wait for the atomics already issued) falls between the atomics.
     uint32_t need = 0;
 #pragma unroll
     for (int m = 0; m < kVT; ++m)
-      if (dst[m] >= 0 && !(word[m] & (1u << (dst[m] & 31)))) need |= 1u << m;
+      if (dst[m] >= n_hub && !(word[m] & (1u << (dst[m] & 31)))) need |= 1u << m;
 #pragma unroll
-    for (int m = 0; m < kVT; ++m)
+    for (int m = 0; m < kVT; ++m) {
+      if (dst[m] >= 0 && dst[m] < n_hub) hub_flags[dst[m]] = 1;
       if (need & (1u << m)) atomicOr(&bitmap[dst[m] >> 5], 1u << (dst[m] & 31));
+    }
     __syncthreads();
+  }
+}
+
+// After a push activation: bitmap word w |= the 32 hub flag bytes of its
+// vertices, which are cleared (n_hub is a multiple of 32; the push kernel
+// marked no other vertex of these words, and no other kernel writes them now).
+__global__ void k_hub_fold(const DevCounters* __restrict__ ctr, int mode, long long threshold,
+                           long long fuse_max, int n_hub, uint8_t* __restrict__ hub_flags,
+                           uint32_t* __restrict__ bitmap) {
+  const unsigned long long c = ctr->chg;
+  if (use_pull(c, mode, threshold) || fused_scatter(c, fuse_max)) return;
+  for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < n_hub / 32;
+       w += gridDim.x * blockDim.x) {
+    uint4* t = reinterpret_cast<uint4*>(hub_flags + (size_t)w * 32);
+    const uint4 a = t[0], b = t[1];
+    const uint32_t q[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    uint32_t m = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) m |= ((q[i] >> (8 * j)) & 1u) << (4 * i + j);
+    if (m) {
+      bitmap[w] |= m;
+      t[0] = make_uint4(0, 0, 0, 0);
+      t[1] = make_uint4(0, 0, 0, 0);
+    }
   }
 }
 
@@ -607,6 +640,7 @@ __global__ void k_fill(int64_t n, T value, T* __restrict__ out) {
 // settled either way, and the segment walk starts at the first open row.
 constexpr int kPullHeavy = 1024;
 constexpr int kPullFirst = 4;
+constexpr int kHubVertices = 1 << 20;  // push marks by byte store below this id
 
 __device__ __forceinline__ uint64_t lane_range_mask(int lo, int hi) {  // bits [lo, hi)
   const uint64_t upto = hi >= 64 ? ~0ull : ((1ull << hi) - 1);
@@ -1068,6 +1102,8 @@ struct Engine final : EngineBase {
   DevCounters* h_ctr = nullptr;  // pinned
   int n_words = 0, n_cblocks = 0, max_tiles = 0, max_push_tiles = 0;
   int32_t* d_heavy = nullptr;  // vertices with in-degree > kPullHeavy (pull mode)
+  uint8_t* d_hub_flags = nullptr;  // push marks of vertices below n_hub (zero between scatters)
+  int n_hub = 0;                   // multiple of 32, at most kHubVertices
   int n_heavy = 0;
   // Delta gather (programs with kDeltaGather): valid once an all-active
   // iteration has run and every later active set came from scatterActivate.
@@ -1109,7 +1145,7 @@ struct Engine final : EngineBase {
   size_t edge_size() const override { return kEdgeEmpty ? 0 : sizeof(ED); }
 
   void release() {
-    void* ptrs[] = {d_srcval, d_cbase, d_heavy,   d_vd,       d_vd_init, d_vd_user, d_ed,   d_ed_csr, d_cand, d_active,
+    void* ptrs[] = {d_hub_flags, d_srcval, d_cbase, d_heavy,   d_vd,       d_vd_init, d_vd_user, d_ed,   d_ed_csr, d_cand, d_active,
                     d_escan, d_gtmp,    d_chg_v,    d_chg_e,   d_bitmap, d_chg_bits,
                     d_chg_prev, d_blk,  d_part,     d_push_part, d_carry, d_head,
                     d_ctr};
@@ -1155,6 +1191,8 @@ struct Engine final : EngineBase {
     d_vd_init = nullptr;
     d_vd_user = nullptr;
     d_heavy = nullptr;
+    d_hub_flags = nullptr;
+    n_hub = 0;
     n_heavy = 0;
     d_ed = nullptr;
     d_ed_csr = nullptr;
@@ -1243,6 +1281,11 @@ struct Engine final : EngineBase {
         if (g->h_off_in[v + 1] - g->h_off_in[v] > kPullHeavy) hv.push_back(v);
       n_heavy = (int)hv.size();
       GR_TRY(dalloc(&d_heavy, n_heavy));
+      n_hub = (int)std::min<int64_t>(V / 32 * 32, kHubVertices);
+      if (n_hub > 0) {
+        GR_TRY(dalloc(&d_hub_flags, n_hub));
+        GR_HIP(hipMemsetAsync(d_hub_flags, 0, (size_t)n_hub, stream));
+      }
       if (n_heavy)
         GR_HIP(hipMemcpy(d_heavy, hv.data(), sizeof(int32_t) * n_heavy, hipMemcpyHostToDevice));
     }
@@ -1855,9 +1898,14 @@ struct Engine final : EngineBase {
           d_ctr, dir_mode, thr, d_chg_e, d_push_part, max_push_tiles, fuse_max, 1);
       int grid = max_push_tiles < 8192 ? max_push_tiles : 8192;
       if (grid < 1) grid = 1;
-      if (dir_mode != 2)
+      if (dir_mode != 2) {
         k_push_tiles<<<grid, kNT, 0, stream>>>(d_ctr, dir_mode, thr, fuse_max, d_push_part,
-                                               d_chg_v, d_chg_e, g->off_out, g->dsts, d_bitmap);
+                                               d_chg_v, d_chg_e, g->off_out, g->dsts, d_bitmap,
+                                               d_hub_flags, n_hub);
+        if (n_hub > 0)
+          k_hub_fold<<<grid_1d(n_hub / 32, 256), 256, 0, stream>>>(d_ctr, dir_mode, thr, fuse_max,
+                                                                   n_hub, d_hub_flags, d_bitmap);
+      }
     }
     if constexpr (kScatter) {
       // Program::scatter, as the reference's CPU engine runs it when
