@@ -634,10 +634,12 @@ __global__ void k_fill(int64_t n, T value, T* __restrict__ out) {
 // needed.  The row holding the chunk start is found from a ballot over the
 // lanes' row offsets; rows already hit, and heavy rows (in-degree above
 // kPullHeavy, done by k_pull_heavy with a whole workgroup and early exit),
-// are jumped over.  Before that, each lane tests the first kPullFirst
-// in-edges of its own row with all loads in flight at once: after a large
-// change most rows hit there, rows with at most kPullFirst in-edges are
-// settled either way, and the segment walk starts at the first open row.
+// are jumped over.  Before that, each lane tests kPullFirst in-edges of its
+// own row, the first and the last half of them, with all loads in flight at
+// once: rows list their sources by id, which the degree order makes hubs
+// first, so a change among the hubs shows at the front of a row and one among
+// the low-degree vertices at its back.  Rows with at most kPullFirst in-edges
+// are settled either way, and the segment walk starts at the first open row.
 constexpr int kPullHeavy = 1024;
 constexpr int kPullFirst = 4;
 constexpr int kHubVertices = 1 << 20;  // push marks by byte store below this id
@@ -668,7 +670,11 @@ __global__ __launch_bounds__(256) void k_pull_stream(
     if (!heavy && deg > 0) {
       int u[kPullFirst];
 #pragma unroll
-      for (int k = 0; k < kPullFirst; ++k) u[k] = k < deg ? srcs[my_off + k] : -1;
+      for (int k = 0; k < kPullFirst; ++k) {
+        const int p = deg <= kPullFirst ? my_off + k
+                      : k < kPullFirst / 2 ? my_off + k : my_end - (kPullFirst - k);
+        u[k] = k < deg ? srcs[p] : -1;
+      }
       uint32_t wd[kPullFirst];
 #pragma unroll
       for (int k = 0; k < kPullFirst; ++k) wd[k] = u[k] >= 0 ? chg_bits[u[k] >> 5] : 0u;
@@ -682,7 +688,7 @@ __global__ __launch_bounds__(256) void k_pull_stream(
     const uint64_t settled = ~need;  // for the walk: nothing left to find there
     const int seg1 = __shfl(my_end, cnt - 1, kWave);
     int pb = seg1;
-    if (need) pb = __shfl(my_off, __builtin_ctzll(need), kWave) + kPullFirst;
+    if (need) pb = __shfl(my_off, __builtin_ctzll(need), kWave) + kPullFirst / 2;
     while (pb < seg1 && (hits & need) != need) {
       // row holding edge pb: the last lane whose row starts at or before pb
       const uint64_t started = __ballot(mine && my_off <= pb);
