@@ -1109,6 +1109,11 @@ struct Engine final : EngineBase {
   int n_words = 0, n_cblocks = 0, max_tiles = 0, max_push_tiles = 0;
   int32_t* d_heavy = nullptr;  // vertices with in-degree > kPullHeavy (pull mode)
   uint8_t* d_hub_flags = nullptr;  // push marks of vertices below n_hub (zero between scatters)
+  // Out-degree histogram by log2 bucket (vertices, out-edges): bounds the
+  // out-edges of any k vertices, so that scatterActivate does not launch the
+  // kernels whose device-side condition the changed vertices cannot meet.
+  int64_t deg_cnt[32] = {}, deg_sum[32] = {};
+  int n_applies = 0;  // apply calls since the active set was last built
   int n_hub = 0;                   // multiple of 32, at most kHubVertices
   int n_heavy = 0;
   // Delta gather (programs with kDeltaGather): valid once an all-active
@@ -1287,6 +1292,14 @@ struct Engine final : EngineBase {
         if (g->h_off_in[v + 1] - g->h_off_in[v] > kPullHeavy) hv.push_back(v);
       n_heavy = (int)hv.size();
       GR_TRY(dalloc(&d_heavy, n_heavy));
+      for (int b = 0; b < 32; ++b) deg_cnt[b] = deg_sum[b] = 0;
+      for (int32_t v = 0; v < g->nV; ++v) {
+        const int32_t d = g->h_off_out[v + 1] - g->h_off_out[v];
+        if (d <= 0) continue;
+        const int b = 31 - __builtin_clz((unsigned)d);
+        deg_cnt[b] += 1;
+        deg_sum[b] += d;
+      }
       n_hub = (int)std::min<int64_t>(V / 32 * 32, kHubVertices);
       if (n_hub > 0) {
         GR_TRY(dalloc(&d_hub_flags, n_hub));
@@ -1362,6 +1375,7 @@ struct Engine final : EngineBase {
     GR_HIP(hipStreamSynchronize(stream));
     n_active = 0;
     w_edges = 0;
+    n_applies = 0;
     iteration = 0;
     delta_chain = delta_ready = false;
     fuse_launched = fused_pending = false;
@@ -1660,6 +1674,7 @@ struct Engine final : EngineBase {
     GR_HIP(hipGetLastError());
     n_active = n;
     w_edges = g->h_off_in[end] - g->h_off_in[start];
+    n_applies = 0;
     iteration = 0;
     return GR_OK;
   }
@@ -1697,6 +1712,7 @@ struct Engine final : EngineBase {
     GR_HIP(hipStreamSynchronize(stream));  // the host vectors go out of scope
     n_active = n;
     w_edges = (int32_t)acc;
+    n_applies = 0;
     iteration = 0;
     delta_chain = false;
     delta_ready = false;
@@ -1858,6 +1874,7 @@ struct Engine final : EngineBase {
   int apply() override {
     it_apply = true;
     if (n_active == 0) return GR_OK;
+    ++n_applies;
     tick();
     k_apply<P><<<grid_1d(n_active, kApplyChunk), kApplyNT, 0, stream>>>(
         n_active, d_active, d_gtmp, P::kHasGather ? 1 : 0, d_vd, g->off_out, iteration, d_chg_v,
@@ -1865,6 +1882,22 @@ struct Engine final : EngineBase {
     tock(kPhApply);
     GR_HIP(hipGetLastError());
     return GR_OK;
+  }
+
+  // At least the out-edges of any k vertices: the largest buckets first, the
+  // last one taken at its bucket's largest degree.
+  int64_t top_out_edges(int64_t k) const {
+    int64_t sum = 0;
+    for (int b = 31; b >= 0 && k > 0; --b) {
+      if (k >= deg_cnt[b]) {
+        sum += deg_sum[b];
+        k -= deg_cnt[b];
+      } else {
+        sum += k * (((int64_t)2 << b) - 1);
+        k = 0;
+      }
+    }
+    return sum;
   }
 
   long long pull_threshold() const {
@@ -1887,8 +1920,14 @@ struct Engine final : EngineBase {
       if (fuse_share <= 0 && delta_mode != 3) fuse_max = -1;
     }
     fuse_launched = fuse_max >= 0;
+    // The changed vertices are among the active ones (once per apply call
+    // since the active set was built): their out-edges are at most `bound`.
+    // Kernels whose device-side condition that rules out are not launched.
+    const long long bound = (long long)(top_out_edges(n_active) * (n_applies > 1 ? n_applies : 1));
+    const bool may_pull = dir_mode == 2 || (dir_mode == 0 && bound > thr);
+    const bool may_push = dir_mode != 2 && bound > 0 && !(fuse_max >= 0 && bound <= fuse_max);
     tick();
-    if (dir_mode != 1) {
+    if (may_pull) {
       k_pull_stream<<<grid_1d(g->nV, 256), 256, 0, stream>>>(d_ctr, dir_mode, thr, fuse_max,
                                                               g->nV, g->off_in, g->srcs,
                                                               d_chg_bits, d_bitmap);
@@ -1904,7 +1943,7 @@ struct Engine final : EngineBase {
           d_ctr, dir_mode, thr, d_chg_e, d_push_part, max_push_tiles, fuse_max, 1);
       int grid = max_push_tiles < 8192 ? max_push_tiles : 8192;
       if (grid < 1) grid = 1;
-      if (dir_mode != 2) {
+      if (may_push) {
         k_push_tiles<<<grid, kNT, 0, stream>>>(d_ctr, dir_mode, thr, fuse_max, d_push_part,
                                                d_chg_v, d_chg_e, g->off_out, g->dsts, d_bitmap,
                                                d_hub_flags, n_hub);
@@ -1927,15 +1966,16 @@ struct Engine final : EngineBase {
     tock(kPhScatter);
     if constexpr (kDelta) {
       // the fused pass is the next gather's relax: timed with the push gathers
-      if (fuse_launched) {
+      if (fuse_launched && bound > 0) {
         tick();
         const int grid = max_push_tiles < 8192 ? (max_push_tiles < 1 ? 1 : max_push_tiles) : 8192;
         k_relax_tiles<P><<<grid, kNT, 0, stream>>>(0, d_ctr, fuse_max, d_push_part, d_chg_v,
                                                    d_chg_e, g->off_out, g->dsts, d_vd,
                                                    edge_csr_ptr(), d_cand, d_bitmap, 1);
         const int64_t n_scan = (int64_t)n_words * 32;
-        k_cand_to_bits<P><<<grid_1d(n_scan, 256) < 4096 ? grid_1d(n_scan, 256) : 4096, 256, 0,
-                            stream>>>(d_ctr, fuse_max, n_words, d_cand, d_bitmap);
+        if (bound > kScanMinEdges)
+          k_cand_to_bits<P><<<grid_1d(n_scan, 256) < 4096 ? grid_1d(n_scan, 256) : 4096, 256, 0,
+                              stream>>>(d_ctr, fuse_max, n_words, d_cand, d_bitmap);
         tock(kPhPush);
       }
     }
@@ -1977,6 +2017,7 @@ struct Engine final : EngineBase {
     if (n_active > 0) stats.iterations += 1;
     n_active = h_ctr->n_active;
     w_edges = h_ctr->w_edges;
+    n_applies = 0;
     changed_share = w_edges > 0 ? (double)(uint32_t)c / (double)w_edges : 1.0;
     chg_count = (int64_t)(c >> 32);
     chg_edges = (int64_t)(uint32_t)c;
