@@ -755,3 +755,26 @@ def test_fused_pass_marks_by_candidate_scan(gpu, prog_name):
     assert np.array_equal(eng.vertexData, ref) and trips == rtrips
     assert st["active_vertices"] == rst["active"] and st["gathered_edges"] == rst["in_edges"]
     eng.close()
+
+
+@pytest.mark.parametrize("direction", [0, 1])
+def test_push_marks_hubs_and_tail(gpu, direction):
+    """Push activation marks targets below 2^20 (the hubs of the degree order)
+    with byte flags folded into the bitmap and the others with atomics: on a
+    skewed graph of 2^20 + 37 vertices (a partial last bitmap word) BFS
+    depths and trips equal the oracle's, pushing only or automatic."""
+    gr = gpu
+    n = (1 << 20) + 37
+    rng = np.random.default_rng(31)
+    m = 4_000_000
+    s = rng.integers(0, n, m)
+    d = np.minimum((n * rng.random(m) ** 3).astype(np.int64), n - 1)
+    keep = s != d
+    pairs = np.unique(np.stack([s[keep], d[keep]], 1), axis=0)
+    s, d = pairs[:, 0].astype(np.int32), pairs[:, 1].astype(np.int32)
+    src = _max_out_src(n, s)
+    ref, ref_trips, _ = oracle.bfs(n, s, d, src)
+    depth, trips = gr.run_bfs((n, s, d), src, direction=direction)
+    assert np.array_equal(depth, ref), f"{(depth != ref).sum()} depth mismatches"
+    assert trips == ref_trips
+    assert (ref >= 0).sum() > n // 2  # most vertices reached, through both kinds of marks
