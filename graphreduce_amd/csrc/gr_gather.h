@@ -101,8 +101,10 @@ __device__ __forceinline__ int merge_path_search(int d, const int32_t* __restric
   return lo;
 }
 
-static __global__ void k_merge_partition(const int32_t* __restrict__ escan, int n, int w, int n_tiles,
-                                  int tile, int32_t* __restrict__ part) {
+// (static: every translation unit that launches it has its own copy)
+[[maybe_unused]] static __global__ void k_merge_partition(const int32_t* __restrict__ escan, int n,
+                                                          int w, int n_tiles, int tile,
+                                                          int32_t* __restrict__ part) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t > n_tiles) return;
   int64_t d = min((int64_t)t * tile, (int64_t)n + w);
