@@ -483,27 +483,36 @@ __global__ __launch_bounds__(kBatchNT) void k_ms_update(
 
 // depth_vs ([V][64], layout ids; valid where the visit bit is set) ->
 // out[s * nV + u] in the caller's ids, -1 where unreached.  A workgroup moves
-// 64 consecutive caller ids through an LDS tile: four waves read 16 vertex
-// rows of 256 B each, then write 64 consecutive depths per source.
+// 64 consecutive caller ids through an LDS tile: their layout ids go to LDS
+// first, then each of the four waves reads its 16 vertex rows of 256 B (and
+// visit masks) with all loads in flight at once, then writes 64 consecutive
+// depths per source.
 __global__ __launch_bounds__(256) void k_ms_depth_out(int nV, int ns,
                                                       const int32_t* __restrict__ depth_vs,
                                                       const uint64_t* __restrict__ visit,
                                                       const int32_t* __restrict__ inv,
                                                       int32_t* __restrict__ out) {
+  constexpr int kRows = kWave / 4;  // rows per wave
   __shared__ int32_t tile[kWave][kWave + 1];
+  __shared__ int32_t s_v[kWave];
   const int lane = lane_id();
   const int w = threadIdx.x / kWave;
   for (int u0 = blockIdx.x * kWave; u0 < nV; u0 += gridDim.x * kWave) {
-    for (int r = w; r < kWave; r += 4) {
-      const int u = u0 + r;
-      int32_t d = -1;
-      if (u < nV) {
-        const int v = inv[u];
-        const uint64_t vis = visit[v];
-        if ((vis >> lane) & 1ull) d = depth_vs[(size_t)v * kWave + lane];
-      }
-      tile[r][lane] = d;
+    if (threadIdx.x < kWave) {
+      const int u = u0 + threadIdx.x;
+      s_v[threadIdx.x] = u < nV ? inv[u] : -1;
     }
+    __syncthreads();
+    int32_t d[kRows];
+    uint64_t vis[kRows];
+#pragma unroll
+    for (int k = 0; k < kRows; ++k) {
+      const int v = s_v[w + 4 * k];
+      vis[k] = v >= 0 ? visit[v] : 0ull;
+      d[k] = v >= 0 ? depth_vs[(size_t)v * kWave + lane] : -1;
+    }
+#pragma unroll
+    for (int k = 0; k < kRows; ++k) tile[w + 4 * k][lane] = ((vis[k] >> lane) & 1ull) ? d[k] : -1;
     __syncthreads();
     for (int s = w; s < ns; s += 4) {
       const int u = u0 + lane;
@@ -750,20 +759,32 @@ __global__ void k_mssp_sources(int ns, const int32_t* __restrict__ sources,
 
 // dist (vertex-major [V][64], layout ids) -> out[s * nV + u] in the caller's
 // ids (u's layout id is inv[u]): a workgroup moves 64 consecutive caller ids
-// through an LDS tile, reading 256-B vertex rows and writing 64 consecutive
-// values per source.
+// through an LDS tile (layout ids to LDS first, then each wave's 16 rows of
+// 256 B read with all loads in flight) and writes 64 consecutive values per
+// source.
 __global__ __launch_bounds__(256) void k_mssp_transpose(int nV, int ns,
                                                         const int32_t* __restrict__ dist,
                                                         const int32_t* __restrict__ inv,
                                                         int32_t* __restrict__ out) {
+  constexpr int kRows = kWave / 4;  // rows per wave
   __shared__ int32_t tile[kWave][kWave + 1];
+  __shared__ int32_t s_v[kWave];
   const int lane = lane_id();
   const int w = threadIdx.x / kWave;
   for (int u0 = blockIdx.x * kWave; u0 < nV; u0 += gridDim.x * kWave) {
-    for (int r = w; r < kWave; r += 4) {
-      const int u = u0 + r;
-      if (u < nV) tile[r][lane] = dist[(size_t)(inv ? inv[u] : u) * kWave + lane];
+    if (threadIdx.x < kWave) {
+      const int u = u0 + threadIdx.x;
+      s_v[threadIdx.x] = u < nV ? (inv ? inv[u] : u) : -1;
     }
+    __syncthreads();
+    int32_t d[kRows];
+#pragma unroll
+    for (int k = 0; k < kRows; ++k) {
+      const int v = s_v[w + 4 * k];
+      d[k] = v >= 0 ? dist[(size_t)v * kWave + lane] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kRows; ++k) tile[w + 4 * k][lane] = d[k];
     __syncthreads();
     for (int s = w; s < ns; s += 4) {
       const int u = u0 + lane;
