@@ -269,6 +269,8 @@ def main() -> int:
 
 
 def run_extras(gr, g, args, src, outdeg, w_dev):
+    from graphreduce_amd import tune
+
     out = {}
     n = g.n_vertices
     # plain BFS (bfs.cu loop)
@@ -276,6 +278,7 @@ def run_extras(gr, g, args, src, outdeg, w_dev):
     vd["depth"] = -1
     eng = gr.GASEngine(gr.BFS)
     eng.bindGraph(g, vd)
+    tune.apply_tuning(eng, gr.BFS)
     step = lambda: run_loop(eng, source=src)  # noqa: E731
     trips = step()
     depth = eng.readVertexData()["depth"]
@@ -298,14 +301,14 @@ def run_extras(gr, g, args, src, outdeg, w_dev):
     vd["numOutEdges"] = outdeg.astype(np.int32)
     eng = gr.GASEngine(gr.PAGERANK)
     eng.bindGraph(g, vd)
+    tune.apply_tuning(eng, gr.PAGERANK)
     step = lambda: run_loop(eng, all_active_n=n)  # noqa: E731
     trips = step()
+    sec = timed(step, 2, gr.synchronize)
+    # one more run with the phase events for the gather figures
     eng.resetStats()
     eng.enableTiming(True)
-    t0 = time.perf_counter()
     step()
-    gr.synchronize()
-    sec = time.perf_counter() - t0
     st = eng.stats()
     eng.close()
     ein = st["gathered_edges"]
@@ -319,14 +322,13 @@ def run_extras(gr, g, args, src, outdeg, w_dev):
     vd = np.arange(n, dtype=np.int32)
     eng = gr.GASEngine(gr.CC)
     eng.bindGraph(gs, vd)
+    tune.apply_tuning(eng, gr.CC)
     step = lambda: run_loop(eng, all_active_n=n)  # noqa: E731
     trips = step()
+    sec = timed(step, max(3, args.steps), gr.synchronize)
     eng.resetStats()
     eng.enableTiming(True)
-    t0 = time.perf_counter()
     step()
-    gr.synchronize()
-    sec = time.perf_counter() - t0
     st = eng.stats()
     eng.close()
     sym_edges = int(gs.n_edges)
