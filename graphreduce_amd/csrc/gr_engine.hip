@@ -1156,10 +1156,10 @@ struct Engine final : EngineBase {
   size_t edge_size() const override { return kEdgeEmpty ? 0 : sizeof(ED); }
 
   void release() {
-    void* ptrs[] = {d_hub_flags, d_srcval, d_cbase, d_heavy,   d_vd,       d_vd_init, d_vd_user, d_ed,   d_ed_csr, d_cand, d_active,
-                    d_escan, d_gtmp,    d_chg_v,    d_chg_e,   d_bitmap, d_chg_bits,
-                    d_chg_prev, d_blk,  d_part,     d_push_part, d_carry, d_head,
-                    d_ctr};
+    void* ptrs[] = {d_hub_flags, d_srcval, d_cbase,    d_heavy,  d_vd,     d_vd_init,
+                    d_vd_user,   d_ed,     d_ed_csr,   d_cand,   d_active, d_escan,
+                    d_gtmp,      d_chg_v,  d_chg_e,    d_bitmap, d_chg_bits, d_chg_prev,
+                    d_blk,       d_part,   d_push_part, d_carry, d_head,   d_ctr};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     if (h_ctr) (void)hipHostFree(h_ctr);
@@ -1920,10 +1920,12 @@ struct Engine final : EngineBase {
       if (fuse_share <= 0 && delta_mode != 3) fuse_max = -1;
     }
     fuse_launched = fuse_max >= 0;
-    // The changed vertices are among the active ones (once per apply call
-    // since the active set was built): their out-edges are at most `bound`.
-    // Kernels whose device-side condition that rules out are not launched.
-    const long long bound = (long long)(top_out_edges(n_active) * (n_applies > 1 ? n_applies : 1));
+    // The changed list holds active vertices only, each apply call since the
+    // active set was built appending at most all of them: their out-edges
+    // are at most `bound`.  Kernels whose device-side condition that rules
+    // out are not launched.
+    const long long bound =
+        (long long)(top_out_edges(n_active) * (n_applies > 1 ? n_applies : 1));
     const bool may_pull = dir_mode == 2 || (dir_mode == 0 && bound > thr);
     const bool may_push = dir_mode != 2 && bound > 0 && !(fuse_max >= 0 && bound <= fuse_max);
     tick();
