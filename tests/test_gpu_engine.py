@@ -778,3 +778,45 @@ def test_push_marks_hubs_and_tail(gpu, direction):
     assert np.array_equal(depth, ref), f"{(depth != ref).sum()} depth mismatches"
     assert trips == ref_trips
     assert (ref >= 0).sum() > n // 2  # most vertices reached, through both kinds of marks
+
+
+def test_scan_marks_values_not_below_gather_zero(gpu):
+    """In a fused pass above 2^20 edges, targets are found by scanning the
+    candidates for values below gatherZero; an edge whose relaxed value is not
+    below it is marked in the pass itself.  Vertices that start above
+    gatherZero fall to it in the first apply, so their out-edges relax to
+    gatherZero + w: the fused run (share 1) and the unfused one (share 0)
+    must agree in values, trips and work counters."""
+    gr = gpu
+    n, s, d = _rmat(gr, 18, seed=41)
+    w = gr.hash_weights(len(s), seed=41)
+    src = _max_out_src(n, s)
+    rng = np.random.default_rng(41)
+    vd = np.full(n, gr.SSSP_INF, np.int32)
+    high = rng.random(n) < 0.5
+    vd[high] = gr.SSSP_INF + 50000  # above gatherZero, no overflow with w <= 64
+    vd[src] = 0
+    runs = []
+    for share in (0.0, 1.0):
+        eng = gr.GASEngine(gr.SSSP)
+        eng.setFuseShare(share)
+        eng.setGraph(n, vd.copy(), len(s), w, s, d)
+        eng.setActive(0, n)
+        trips, biggest, prev = 0, 0, 0
+        while eng.countActive():
+            eng.gather()
+            eng.apply()
+            eng.scatterActivate()
+            eng.nextIter()
+            trips += 1
+            fe = eng.stats()["fused_edges"]
+            biggest, prev = max(biggest, fe - prev), fe
+        eng.getResults()
+        st = eng.stats()
+        runs.append((eng.vertexData.copy(), trips, st["active_vertices"], st["gathered_edges"],
+                     biggest))
+        eng.close()
+    (a, ta, aa, ga, _), (b, tb, ab, gb, big) = runs
+    assert big > 1 << 20, big
+    assert np.array_equal(a, b) and ta == tb and aa == ab and ga == gb
+    assert (a == gr.SSSP_INF).any()  # some vertices settle at gatherZero itself
