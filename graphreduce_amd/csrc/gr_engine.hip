@@ -841,10 +841,20 @@ __global__ __launch_bounds__(1024) void k_compact_scan(int2* __restrict__ blk, i
 // changed bitmap (the host swaps it in as the buffer the next apply fills).
 __global__ __launch_bounds__(kCompactNT) void k_compact_write(
     uint32_t* __restrict__ bitmap, int n_words, int nV, const int32_t* __restrict__ off_in,
-    int need_deg, const int2* __restrict__ blk, int32_t* __restrict__ active,
-    int32_t* __restrict__ escan, int32_t* __restrict__ cbase, uint32_t* __restrict__ chg_bits) {
+    int need_deg, const int2* __restrict__ blk, const DevCounters* __restrict__ ctr,
+    int32_t* __restrict__ active, int32_t* __restrict__ escan, int32_t* __restrict__ cbase,
+    uint32_t* __restrict__ chg_bits) {
   __shared__ unsigned long long s_scan[kCompactNT / kWave];
   const int2 base = blk[blockIdx.x];
+  // a block without frontier bits (its count is the gap to the next block's
+  // offset) only clears its words of the changed bitmap
+  const int next = blockIdx.x + 1 < (int)gridDim.x ? blk[blockIdx.x + 1].x : ctr->n_active;
+  if (next == base.x) {
+    const int wi = blockIdx.x * (kCompactRounds * kCompactNT / 4) + threadIdx.x;
+    if (chg_bits && threadIdx.x < kCompactRounds * kCompactNT / 4 && wi < n_words)
+      chg_bits[wi] = 0u;
+    return;
+  }
   int pos_run = base.x, deg_run = base.y;
   for (int r = 0; r < kCompactRounds; ++r) {
     const int bi = (blockIdx.x * kCompactRounds + r) * kCompactNT + threadIdx.x;
@@ -1993,7 +2003,7 @@ struct Engine final : EngineBase {
     k_compact_scan<<<1, 1024, 0, stream>>>(d_blk, n_cblocks, d_ctr, need_deg ? d_escan : nullptr,
                                            dir_mode, pull_threshold());
     k_compact_write<<<n_cblocks, kCompactNT, 0, stream>>>(d_bitmap, n_words, g->nV, g->off_in,
-                                                          need_deg, d_blk, d_active, d_escan,
+                                                          need_deg, d_blk, d_ctr, d_active, d_escan,
                                                           d_cbase,
                                                           changed_bits_on() ? d_chg_prev : nullptr);
     tock(kPhNext);
