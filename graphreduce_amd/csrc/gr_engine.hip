@@ -1938,6 +1938,10 @@ struct Engine final : EngineBase {
         (long long)(top_out_edges(n_active) * (n_applies > 1 ? n_applies : 1));
     const bool may_pull = dir_mode == 2 || (dir_mode == 0 && bound > thr);
     const bool may_push = dir_mode != 2 && bound > 0 && !(fuse_max >= 0 && bound <= fuse_max);
+    // tiles the changed out-edges can fill: the push kernels' grids (they
+    // stride over the tiles, so any grid of at least one workgroup is whole)
+    const int push_tiles = (int)std::min<long long>(max_push_tiles, bound / kTile + 1);
+    const int push_grid = push_tiles < 8192 ? (push_tiles < 1 ? 1 : push_tiles) : 8192;
     tick();
     if (may_pull) {
       k_pull_stream<<<grid_1d(g->nV, 256), 256, 0, stream>>>(d_ctr, dir_mode, thr, fuse_max,
@@ -1951,12 +1955,10 @@ struct Engine final : EngineBase {
     {
       // partition of the changed out-edges (push or fused pass); also
       // records in ctr->fused whether the fused pass runs
-      k_push_partition<<<grid_1d((int64_t)max_push_tiles + 1, 256), 256, 0, stream>>>(
+      k_push_partition<<<grid_1d((int64_t)push_tiles + 1, 256), 256, 0, stream>>>(
           d_ctr, dir_mode, thr, d_chg_e, d_push_part, max_push_tiles, fuse_max, 1);
-      int grid = max_push_tiles < 8192 ? max_push_tiles : 8192;
-      if (grid < 1) grid = 1;
       if (may_push) {
-        k_push_tiles<<<grid, kNT, 0, stream>>>(d_ctr, dir_mode, thr, fuse_max, d_push_part,
+        k_push_tiles<<<push_grid, kNT, 0, stream>>>(d_ctr, dir_mode, thr, fuse_max, d_push_part,
                                                d_chg_v, d_chg_e, g->off_out, g->dsts, d_bitmap,
                                                d_hub_flags, n_hub);
         if (n_hub > 0)
@@ -1980,8 +1982,7 @@ struct Engine final : EngineBase {
       // the fused pass is the next gather's relax: timed with the push gathers
       if (fuse_launched && bound > 0) {
         tick();
-        const int grid = max_push_tiles < 8192 ? (max_push_tiles < 1 ? 1 : max_push_tiles) : 8192;
-        k_relax_tiles<P><<<grid, kNT, 0, stream>>>(0, d_ctr, fuse_max, d_push_part, d_chg_v,
+        k_relax_tiles<P><<<push_grid, kNT, 0, stream>>>(0, d_ctr, fuse_max, d_push_part, d_chg_v,
                                                    d_chg_e, g->off_out, g->dsts, d_vd,
                                                    edge_csr_ptr(), d_cand, d_bitmap, 1);
         const int64_t n_scan = (int64_t)n_words * 32;
