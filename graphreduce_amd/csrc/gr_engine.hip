@@ -809,11 +809,14 @@ __global__ __launch_bounds__(kCompactNT) void k_compact_count(const uint32_t* __
 }
 
 // Single workgroup: exclusive scan of the per-block counts; publishes the
-// totals and resets the changed-list counter for the next apply.
+// totals, resets the changed-list counter for the next apply and writes the
+// counters to the host's mapped copy (`host`, read after the stream is
+// synchronised; no separate device-to-host copy).
 __global__ __launch_bounds__(1024) void k_compact_scan(int2* __restrict__ blk, int nb,
                                                        DevCounters* __restrict__ ctr,
                                                        int32_t* __restrict__ escan, int mode,
-                                                       long long threshold) {
+                                                       long long threshold,
+                                                       DevCounters* __restrict__ host) {
   __shared__ int s_a[1024 / kWave], s_b[1024 / kWave];
   int run_a = 0, run_b = 0;
   for (int base = 0; base < nb; base += 1024) {
@@ -833,6 +836,8 @@ __global__ __launch_bounds__(1024) void k_compact_scan(int2* __restrict__ blk, i
     ctr->chg_last = ctr->chg;
     ctr->pull = use_pull(ctr->chg, mode, threshold) && !ctr->fused ? 1 : 0;
     ctr->chg = 0;
+    *host = *ctr;
+    __threadfence_system();
   }
 }
 
@@ -1115,7 +1120,8 @@ struct Engine final : EngineBase {
   Part<G>* d_carry = nullptr;
   Part<G>* d_head = nullptr;
   DevCounters* d_ctr = nullptr;
-  DevCounters* h_ctr = nullptr;  // pinned
+  DevCounters* h_ctr = nullptr;  // pinned, mapped
+  DevCounters* d_hctr = nullptr; // h_ctr as the device addresses it
   int n_words = 0, n_cblocks = 0, max_tiles = 0, max_push_tiles = 0;
   int32_t* d_heavy = nullptr;  // vertices with in-degree > kPullHeavy (pull mode)
   uint8_t* d_hub_flags = nullptr;  // push marks of vertices below n_hub (zero between scatters)
@@ -1225,7 +1231,7 @@ struct Engine final : EngineBase {
     d_bitmap = d_chg_bits = d_chg_prev = nullptr;
     d_blk = nullptr;
     d_carry = d_head = nullptr;
-    d_ctr = h_ctr = nullptr;
+    d_ctr = h_ctr = d_hctr = nullptr;
     stream = nullptr;
     g = nullptr;
     bound = false;
@@ -1291,7 +1297,8 @@ struct Engine final : EngineBase {
     GR_TRY(dalloc(&d_carry, max_tiles + 1));
     GR_TRY(dalloc(&d_head, max_tiles + 1));
     GR_TRY(dalloc(&d_ctr, 1));
-    GR_HIP(hipHostMalloc((void**)&h_ctr, sizeof(DevCounters), hipHostMallocDefault));
+    GR_HIP(hipHostMalloc((void**)&h_ctr, sizeof(DevCounters), hipHostMallocMapped));
+    GR_HIP(hipHostGetDevicePointer((void**)&d_hctr, h_ctr, 0));
     GR_HIP(hipMemsetAsync(d_bitmap, 0, sizeof(uint32_t) * (n_words + 1), stream));
     GR_HIP(hipMemsetAsync(d_chg_bits, 0, sizeof(uint32_t) * (n_words + 1), stream));
     GR_HIP(hipMemsetAsync(d_chg_prev, 0, sizeof(uint32_t) * (n_words + 1), stream));
@@ -2002,7 +2009,7 @@ struct Engine final : EngineBase {
     k_compact_count<<<n_cblocks, kCompactNT, 0, stream>>>(d_bitmap, n_words, g->nV, g->off_in,
                                                           need_deg, d_blk);
     k_compact_scan<<<1, 1024, 0, stream>>>(d_blk, n_cblocks, d_ctr, need_deg ? d_escan : nullptr,
-                                           dir_mode, pull_threshold());
+                                           dir_mode, pull_threshold(), d_hctr);
     k_compact_write<<<n_cblocks, kCompactNT, 0, stream>>>(d_bitmap, n_words, g->nV, g->off_in,
                                                           need_deg, d_blk, d_ctr, d_active, d_escan,
                                                           d_cbase,
@@ -2015,8 +2022,7 @@ struct Engine final : EngineBase {
     delta_chain = delta_chain && it_gather && it_apply && it_scatter;
     delta_ready = kDelta && delta_chain && changed_bits_on();
     it_gather = it_apply = it_scatter = false;
-    GR_HIP(hipMemcpyAsync(h_ctr, d_ctr, sizeof(DevCounters), hipMemcpyDeviceToHost, stream));
-    GR_HIP(hipStreamSynchronize(stream));
+    GR_HIP(hipStreamSynchronize(stream));  // k_compact_scan wrote h_ctr
     if constexpr (kDelta) {
       const bool fused = fuse_launched && h_ctr->fused;
       fuse_launched = false;
