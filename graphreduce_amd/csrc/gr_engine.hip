@@ -80,7 +80,7 @@ __global__ __launch_bounds__(kApplyNT) void k_apply(
     int use_gather, typename P::VertexData* __restrict__ vd, const int32_t* __restrict__ off_out,
     int iteration, int32_t* __restrict__ chg_v, int32_t* __restrict__ chg_e,
     uint32_t* __restrict__ chg_bits, DevCounters* __restrict__ ctr,
-    source_value_t<P>* __restrict__ srcval) {
+    source_value_t<P>* __restrict__ srcval, typename P::GatherResult* __restrict__ cand) {
   __shared__ int s_a[kApplyNT / kWave], s_b[kApplyNT / kWave];
   __shared__ unsigned long long s_base;
   const int tid = threadIdx.x;
@@ -104,12 +104,22 @@ __global__ __launch_bounds__(kApplyNT) void k_apply(
       if (i < n) {
         valid_mask |= 1u << m;
         dvs[m] = active[i];
-        if (use_gather) gs[m] = gtmp[i];
+        if (use_gather == 1) gs[m] = gtmp[i];
       }
     }
 #pragma unroll
     for (int m = 0; m < kApplyVT; ++m)
       if (valid_mask & (1u << m)) vs[m] = vd[dvs[m]];
+    // use_gather 2 (push gathers of delta programs): the relaxed candidate of
+    // each active row is its gather value; read it and reset it to gatherZero
+    if (use_gather == 2) {
+#pragma unroll
+      for (int m = 0; m < kApplyVT; ++m)
+        if (valid_mask & (1u << m)) gs[m] = cand[dvs[m]];
+#pragma unroll
+      for (int m = 0; m < kApplyVT; ++m)
+        if (valid_mask & (1u << m)) cand[dvs[m]] = P::gatherZero;
+    }
 #pragma unroll
     for (int m = 0; m < kApplyVT; ++m) {
       if (valid_mask & (1u << m)) {
@@ -1155,6 +1165,9 @@ struct Engine final : EngineBase {
   // next push relax cost more than one relax pass that also marks the targets.
   bool fuse_launched = false;  // scatterActivate enqueued the fused pass
   bool fused_pending = false;  // d_cand holds the next gather's values
+  // A push gather leaves the active rows' values in d_cand: apply reads and
+  // resets them; any other phase first resets them (collect_flush).
+  bool collect_pending = false;
   // Inert sources (programs with kInertSources): with every bound edge passing
   // P::edgeKeepsInert, a gather without a previous changed set reads only the
   // non-inert sources (SSSP's first all-active gather: the start vertex).
@@ -1395,7 +1408,7 @@ struct Engine final : EngineBase {
     n_applies = 0;
     iteration = 0;
     delta_chain = delta_ready = false;
-    fuse_launched = fused_pending = false;
+    fuse_launched = fused_pending = collect_pending = false;
     it_gather = it_apply = it_scatter = false;
     bound = true;
     return GR_OK;
@@ -1756,6 +1769,7 @@ struct Engine final : EngineBase {
 
   int gather() override {
     it_gather = true;
+    GR_TRY(collect_flush());  // a previous gather whose apply did not run
     if (!P::kHasGather || n_active == 0) return GR_OK;
     if (streaming) return gather_streamed();
     const int n = n_active, w = w_edges;
@@ -1812,10 +1826,7 @@ struct Engine final : EngineBase {
   // relaxed; read and reset them for the active rows.
   int gather_collect() {
     const int n = n_active;
-    tick();
-    k_relax_collect<P><<<grid_1d(n, 256), 256, 0, stream>>>(n, d_active, d_cand, d_gtmp);
-    tock(kPhPush);
-    GR_HIP(hipGetLastError());
+    collect_pending = true;  // apply reads the candidates
     stats.active_vertices += n;
     stats.gathered_edges += w_edges;
     stats.delta_gathers += 1;
@@ -1826,9 +1837,24 @@ struct Engine final : EngineBase {
     return GR_OK;
   }
 
+  // Candidates of a push gather whose apply did not run: reset them (the
+  // values go to d_gtmp, unused).
+  int collect_flush() {
+    if constexpr (kDelta) {
+      if (collect_pending) {
+        k_relax_collect<P><<<grid_1d(n_active, 256), 256, 0, stream>>>(n_active, d_active, d_cand,
+                                                                       d_gtmp);
+        GR_HIP(hipGetLastError());
+      }
+    }
+    collect_pending = false;
+    return GR_OK;
+  }
+
   // Candidates relaxed by a fused scatterActivate whose gather will not run
   // (the active set was replaced or the vertex data reset): back to gatherZero.
   int drop_candidates() {
+    GR_TRY(collect_flush());
     if constexpr (kDelta) {
       if (fused_pending) {
         k_fill<G><<<1024, 256, 0, stream>>>(g->nV, P::gatherZero, d_cand);
@@ -1874,9 +1900,9 @@ struct Engine final : EngineBase {
                                                  g->off_out, g->dsts, d_vd, edge_csr_ptr(),
                                                  d_cand, nullptr, 0);
     }
-    k_relax_collect<P><<<grid_1d(n, 256), 256, 0, stream>>>(n, d_active, d_cand, d_gtmp);
     tock(kPhPush);
     GR_HIP(hipGetLastError());
+    collect_pending = true;  // apply reads the candidates
     stats.active_vertices += n;
     stats.gathered_edges += w_edges;
     if (delta_ready) stats.delta_gathers += 1;  // else the list holds the non-inert vertices
@@ -1892,10 +1918,12 @@ struct Engine final : EngineBase {
     it_apply = true;
     if (n_active == 0) return GR_OK;
     ++n_applies;
+    const int use_gather = !P::kHasGather ? 0 : collect_pending ? 2 : 1;
+    collect_pending = false;
     tick();
     k_apply<P><<<grid_1d(n_active, kApplyChunk), kApplyNT, 0, stream>>>(
-        n_active, d_active, d_gtmp, P::kHasGather ? 1 : 0, d_vd, g->off_out, iteration, d_chg_v,
-        d_chg_e, changed_bits_on() ? d_chg_bits : nullptr, d_ctr, d_srcval);
+        n_active, d_active, d_gtmp, use_gather, d_vd, g->off_out, iteration, d_chg_v, d_chg_e,
+        changed_bits_on() ? d_chg_bits : nullptr, d_ctr, d_srcval, d_cand);
     tock(kPhApply);
     GR_HIP(hipGetLastError());
     return GR_OK;
@@ -1923,6 +1951,7 @@ struct Engine final : EngineBase {
 
   int scatter_activate(bool have_scatter) override {
     it_scatter = true;
+    GR_TRY(collect_flush());  // a push gather whose apply did not run
     if (streaming) return scatter_streamed();
     const long long thr = pull_threshold();
     // Fused activation + next push gather, decided on the device from the
@@ -2004,6 +2033,7 @@ struct Engine final : EngineBase {
   }
 
   int next_iter() override {
+    GR_TRY(collect_flush());
     const int need_deg = P::kHasGather ? 1 : 0;
     tick();
     k_compact_count<<<n_cblocks, kCompactNT, 0, stream>>>(d_bitmap, n_words, g->nV, g->off_in,

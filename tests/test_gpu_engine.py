@@ -820,3 +820,47 @@ def test_scan_marks_values_not_below_gather_zero(gpu):
     assert big > 1 << 20, big
     assert np.array_equal(a, b) and ta == tb and aa == ab and ga == gb
     assert (a == gr.SSSP_INF).any()  # some vertices settle at gatherZero itself
+
+
+@pytest.mark.parametrize("prog_name", ["SSSP", "CC"])
+def test_push_gather_values_reach_apply_once(gpu, prog_name):
+    """A push gather leaves the active rows' values in the candidates, which
+    apply reads and resets.  Calling gather twice per iteration (the second
+    call first resets the first one's candidates) gives the oracle's values,
+    trips and work counters; so does a restart right after a gather."""
+    gr = gpu
+    n, s, d = _rmat(gr, 13, seed=51, sym=prog_name == "CC")
+    if prog_name == "SSSP":
+        prog, w = gr.SSSP, gr.hash_weights(len(s), seed=51)
+        src = _max_out_src(n, s)
+        vd = np.full(n, gr.SSSP_INF, np.int32)
+        vd[src] = 0
+        ref, rtrips, rst = oracle.sssp(n, s, d, w, src)
+    else:
+        prog, w, vd = gr.CC, None, np.arange(n, dtype=np.int32)
+        ref, rtrips, rst = oracle.cc(n, s, d)
+    eng = gr.GASEngine(prog)
+    eng.setGraph(n, vd.copy(), len(s), w, s, d)
+    eng.setActive(0, n)
+    for _ in range(3):  # a few iterations, then a restart right after a gather
+        eng.gather()
+        eng.apply()
+        eng.scatterActivate()
+        eng.nextIter()
+    eng.gather()
+    eng.resetVertexData()
+    eng.setActive(0, n)
+    eng.resetStats()
+    trips = 0
+    while eng.countActive():
+        eng.gather()
+        eng.gather()
+        eng.apply()
+        eng.scatterActivate()
+        eng.nextIter()
+        trips += 1
+    eng.getResults()
+    st = eng.stats()
+    assert np.array_equal(eng.vertexData, ref) and trips == rtrips
+    assert st["push_gathers"] > 0
+    eng.close()
