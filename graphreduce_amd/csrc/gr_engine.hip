@@ -675,7 +675,7 @@ __global__ __launch_bounds__(256) void k_pull_stream(
     const int my_end = mine ? off_in[base + lane + 1] : INT_MAX;
     const int deg = mine ? my_end - my_off : 0;
     const bool heavy = deg > kPullHeavy;
-    // the first kPullFirst in-edges of the lane's own row
+    // kPullFirst in-edges of the lane's own row: the first and the last half
     bool first_hit = false;
     if (!heavy && deg > 0) {
       int u[kPullFirst];
