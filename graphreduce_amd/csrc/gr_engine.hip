@@ -859,7 +859,11 @@ __global__ __launch_bounds__(kCompactNT) void k_compact_write(
     int need_deg, const int2* __restrict__ blk, const DevCounters* __restrict__ ctr,
     int32_t* __restrict__ active, int32_t* __restrict__ escan, int32_t* __restrict__ cbase,
     uint32_t* __restrict__ chg_bits) {
+  // a round's entries (at most 8 per thread) are staged here and written out
+  // coalesced: each thread's own entries are consecutive, a wave's are not
+  constexpr int kRoundMax = 8 * kCompactNT;
   __shared__ unsigned long long s_scan[kCompactNT / kWave];
+  __shared__ int32_t s_act[kRoundMax], s_esc[kRoundMax], s_cb[kRoundMax];
   const int2 base = blk[blockIdx.x];
   // a block without frontier bits (its count is the gap to the next block's
   // offset) only clears its words of the changed bitmap
@@ -878,9 +882,10 @@ __global__ __launch_bounds__(kCompactNT) void k_compact_write(
     unsigned long long tot;
     const unsigned long long ex = block_excl_scan_add<kCompactNT>(
         ((unsigned long long)(uint32_t)b.deg << 16) | (unsigned long long)b.cnt, s_scan, &tot);
-    int pos = pos_run + (int)(ex & 0xFFFFu);
+    const int round_start = pos_run, round_cnt = (int)(tot & 0xFFFFu);
+    int pos = (int)(ex & 0xFFFFu);  // within the round
     int dpos = deg_run + (int)(ex >> 16);
-    pos_run += (int)(tot & 0xFFFFu);
+    pos_run += round_cnt;
     deg_run += (int)(tot >> 16);
     // every thread has read its byte before the scan's barriers
     const int wi = bi >> 2;
@@ -888,18 +893,28 @@ __global__ __launch_bounds__(kCompactNT) void k_compact_write(
       if (chg_bits) chg_bits[wi] = 0u;
       bitmap[wi] = 0u;
     }
+    if (round_cnt == 0) continue;  // block-uniform
     uint32_t bits = b.bits;
     while (bits) {
       const int j = __builtin_ctz(bits);
       bits &= bits - 1;
-      active[pos] = b.v0 + j;
+      s_act[pos] = b.v0 + j;
       if (need_deg) {
-        escan[pos] = dpos;
-        cbase[pos] = b.o[j] - dpos;
+        s_esc[pos] = dpos;
+        s_cb[pos] = b.o[j] - dpos;
         dpos += b.o[j + 1] - b.o[j];
       }
       ++pos;
     }
+    __syncthreads();
+    for (int i = threadIdx.x; i < round_cnt; i += kCompactNT) {
+      active[round_start + i] = s_act[i];
+      if (need_deg) {
+        escan[round_start + i] = s_esc[i];
+        cbase[round_start + i] = s_cb[i];
+      }
+    }
+    __syncthreads();  // the next round stages into the same arrays
   }
 }
 
