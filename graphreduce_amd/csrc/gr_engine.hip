@@ -503,6 +503,7 @@ __global__ __launch_bounds__(kNT) void k_relax_tiles(
   __shared__ int32_t s_start[kTile + 2];
   __shared__ int32_t s_csr[kTile + 2];
   __shared__ typename P::VertexData s_vd[kTile + 2];
+  __shared__ uint16_t s_row[kTile];
   bool scan = false;
   if (ctr) {
     const unsigned long long c = ctr->chg;
@@ -523,6 +524,32 @@ __global__ __launch_bounds__(kNT) void k_relax_tiles(
       s_vd[k] = vd[u];
     }
     __syncthreads();
+    // Row of every edge of the tile: each lane searches the row of its first
+    // of kVT consecutive edges and walks the rest (changed entries all have
+    // out-edges, so the starts rise strictly and nr <= kTile + 1).
+    {
+      const int l0 = tid * kVT, n_e = e1 - e0;
+      if (l0 < n_e) {
+        const int e = e0 + l0;
+        int lo = 0, hi = nr - 1;
+        while (lo < hi) {
+          int mid = (lo + hi + 1) >> 1;
+          if (s_start[mid] <= e)
+            lo = mid;
+          else
+            hi = mid - 1;
+        }
+        int next = lo + 1 < nr ? s_start[lo + 1] : INT_MAX;
+        for (int q = 0; q < kVT && l0 + q < n_e; ++q) {
+          while (next <= e + q) {
+            ++lo;
+            next = lo + 1 < nr ? s_start[lo + 1] : INT_MAX;
+          }
+          s_row[l0 + q] = (uint16_t)lo;
+        }
+      }
+    }
+    __syncthreads();
     // Targets and values of the lane's kVT edges, then every candidate (and
     // frontier word) read, then the atomics: the compiler keeps later loads
     // behind an atomic that may alias them, which would make the items one
@@ -535,16 +562,9 @@ __global__ __launch_bounds__(kNT) void k_relax_tiles(
       pos[m] = -1;
       row[m] = 0;
       if (e < e1) {
-        int lo = 0, hi = nr - 1;
-        while (lo < hi) {
-          int mid = (lo + hi + 1) >> 1;
-          if (s_start[mid] <= e)
-            lo = mid;
-          else
-            hi = mid - 1;
-        }
-        pos[m] = s_csr[lo] + (e - s_start[lo]);
-        row[m] = lo;
+        const int r = s_row[e - e0];
+        pos[m] = s_csr[r] + (e - s_start[r]);
+        row[m] = r;
       }
     }
     int vv[kVT];
