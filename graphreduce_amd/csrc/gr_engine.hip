@@ -22,6 +22,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <type_traits>
 #include <vector>
 
@@ -840,13 +841,16 @@ __global__ __launch_bounds__(kCompactNT) void k_compact_count(const uint32_t* __
 
 // Single workgroup: exclusive scan of the per-block counts; publishes the
 // totals, resets the changed-list counter for the next apply and writes the
-// counters to the host's mapped copy (`host`, read after the stream is
-// synchronised; no separate device-to-host copy).
+// counters to the host's mapped copy (`host`; no separate device-to-host
+// copy), then `seq` to `host_seq` once they are visible to the host, so that
+// the host can go on before the rest of the stream has finished.
 __global__ __launch_bounds__(1024) void k_compact_scan(int2* __restrict__ blk, int nb,
                                                        DevCounters* __restrict__ ctr,
                                                        int32_t* __restrict__ escan, int mode,
                                                        long long threshold,
-                                                       DevCounters* __restrict__ host) {
+                                                       DevCounters* __restrict__ host,
+                                                       uint32_t* __restrict__ host_seq,
+                                                       uint32_t seq) {
   __shared__ int s_a[1024 / kWave], s_b[1024 / kWave];
   int run_a = 0, run_b = 0;
   for (int base = 0; base < nb; base += 1024) {
@@ -868,6 +872,7 @@ __global__ __launch_bounds__(1024) void k_compact_scan(int2* __restrict__ blk, i
     ctr->chg = 0;
     *host = *ctr;
     __threadfence_system();
+    __hip_atomic_store(host_seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
@@ -1165,8 +1170,11 @@ struct Engine final : EngineBase {
   Part<G>* d_carry = nullptr;
   Part<G>* d_head = nullptr;
   DevCounters* d_ctr = nullptr;
-  DevCounters* h_ctr = nullptr;  // pinned, mapped
+  DevCounters* h_ctr = nullptr;  // pinned, mapped, coherent
   DevCounters* d_hctr = nullptr; // h_ctr as the device addresses it
+  uint32_t* h_seq = nullptr;     // last counters' sequence number (pinned, mapped, coherent)
+  uint32_t* d_hseq = nullptr;
+  uint32_t ctr_seq = 0;
   int n_words = 0, n_cblocks = 0, max_tiles = 0, max_push_tiles = 0;
   int32_t* d_heavy = nullptr;  // vertices with in-degree > kPullHeavy (pull mode)
   uint8_t* d_hub_flags = nullptr;  // push marks of vertices below n_hub (zero between scatters)
@@ -1227,6 +1235,7 @@ struct Engine final : EngineBase {
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     if (h_ctr) (void)hipHostFree(h_ctr);
+    if (h_seq) (void)hipHostFree(h_seq);
     {
       void* sptrs[] = {d_csc_vb,       d_csr_vb,       d_spos,         d_swsum,
                        d_scnt,         d_stage_src[0], d_stage_src[1], d_stage_ed[0],
@@ -1280,6 +1289,7 @@ struct Engine final : EngineBase {
     d_blk = nullptr;
     d_carry = d_head = nullptr;
     d_ctr = h_ctr = d_hctr = nullptr;
+    h_seq = d_hseq = nullptr;
     stream = nullptr;
     g = nullptr;
     bound = false;
@@ -1345,8 +1355,13 @@ struct Engine final : EngineBase {
     GR_TRY(dalloc(&d_carry, max_tiles + 1));
     GR_TRY(dalloc(&d_head, max_tiles + 1));
     GR_TRY(dalloc(&d_ctr, 1));
-    GR_HIP(hipHostMalloc((void**)&h_ctr, sizeof(DevCounters), hipHostMallocMapped));
+    GR_HIP(hipHostMalloc((void**)&h_ctr, sizeof(DevCounters),
+                         hipHostMallocMapped | hipHostMallocCoherent));
     GR_HIP(hipHostGetDevicePointer((void**)&d_hctr, h_ctr, 0));
+    GR_HIP(hipHostMalloc((void**)&h_seq, sizeof(uint32_t),
+                         hipHostMallocMapped | hipHostMallocCoherent));
+    GR_HIP(hipHostGetDevicePointer((void**)&d_hseq, h_seq, 0));
+    __atomic_store_n(h_seq, ctr_seq, __ATOMIC_RELEASE);
     GR_HIP(hipMemsetAsync(d_bitmap, 0, sizeof(uint32_t) * (n_words + 1), stream));
     GR_HIP(hipMemsetAsync(d_chg_bits, 0, sizeof(uint32_t) * (n_words + 1), stream));
     GR_HIP(hipMemsetAsync(d_chg_prev, 0, sizeof(uint32_t) * (n_words + 1), stream));
@@ -2073,8 +2088,9 @@ struct Engine final : EngineBase {
     tick();
     k_compact_count<<<n_cblocks, kCompactNT, 0, stream>>>(d_bitmap, n_words, g->nV, g->off_in,
                                                           need_deg, d_blk);
+    ++ctr_seq;
     k_compact_scan<<<1, 1024, 0, stream>>>(d_blk, n_cblocks, d_ctr, need_deg ? d_escan : nullptr,
-                                           dir_mode, pull_threshold(), d_hctr);
+                                           dir_mode, pull_threshold(), d_hctr, d_hseq, ctr_seq);
     k_compact_write<<<n_cblocks, kCompactNT, 0, stream>>>(d_bitmap, n_words, g->nV, g->off_in,
                                                           need_deg, d_blk, d_ctr, d_active, d_escan,
                                                           d_cbase,
@@ -2087,7 +2103,7 @@ struct Engine final : EngineBase {
     delta_chain = delta_chain && it_gather && it_apply && it_scatter;
     delta_ready = kDelta && delta_chain && changed_bits_on();
     it_gather = it_apply = it_scatter = false;
-    GR_HIP(hipStreamSynchronize(stream));  // k_compact_scan wrote h_ctr
+    GR_TRY(wait_counters(ctr_seq));
     if constexpr (kDelta) {
       const bool fused = fuse_launched && h_ctr->fused;
       fuse_launched = false;
@@ -2106,6 +2122,26 @@ struct Engine final : EngineBase {
     chg_count = (int64_t)(c >> 32);
     chg_edges = (int64_t)(uint32_t)c;
     ++iteration;
+    return GR_OK;
+  }
+
+  // Waits until k_compact_scan has published counters number `want` to h_ctr.
+  // The host polls the sequence number instead of synchronising the stream:
+  // it learns the next frontier's size before k_compact_write has finished
+  // and queues the next iteration's kernels behind it.  The polling is
+  // bounded; after kSpinLimit the stream is synchronised, after which the
+  // number must be there.
+  int wait_counters(uint32_t want) {
+    constexpr auto kSpinLimit = std::chrono::milliseconds(20);
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t polls = 1;; ++polls) {
+      if (__atomic_load_n(h_seq, __ATOMIC_ACQUIRE) == want) return GR_OK;
+      __builtin_ia32_pause();
+      if ((polls & 1023u) == 0 && std::chrono::steady_clock::now() - t0 > kSpinLimit) break;
+    }
+    GR_HIP(hipStreamSynchronize(stream));
+    if (__atomic_load_n(h_seq, __ATOMIC_ACQUIRE) != want)
+      return set_error(GR_ERR_HIP, "frontier counters were not published");
     return GR_OK;
   }
 
