@@ -321,6 +321,35 @@ __global__ void k_push_partition(DevCounters* __restrict__ ctr, int mode, long l
   }
 }
 
+// Row of each of a push tile's n_e edges (index into the tile's changed-list
+// entries s_start[0..nr)): each lane searches the row of its first of kVT
+// consecutive edges and walks the rest.  The changed list holds only vertices
+// with out-edges, so the starts rise strictly and nr <= kTile + 1.  The
+// callers then read the row of a strided edge from s_row instead of searching
+// for it.
+__device__ __forceinline__ void label_push_rows(const int32_t* s_start, int nr, int e0, int n_e,
+                                                uint16_t* s_row) {
+  const int l0 = threadIdx.x * kVT;
+  if (l0 >= n_e) return;
+  const int e = e0 + l0;
+  int lo = 0, hi = nr - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (s_start[mid] <= e)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  int next = lo + 1 < nr ? s_start[lo + 1] : INT_MAX;
+  for (int q = 0; q < kVT && l0 + q < n_e; ++q) {
+    while (next <= e + q) {
+      ++lo;
+      next = lo + 1 < nr ? s_start[lo + 1] : INT_MAX;
+    }
+    s_row[l0 + q] = (uint16_t)lo;
+  }
+}
+
 // Push activation.  Targets below n_hub (the highest-degree vertices in the
 // degree-ordered layout) are marked with a plain byte store in hub_flags,
 // which k_hub_fold folds into the bitmap: many changed vertices share a few
@@ -339,6 +368,7 @@ __global__ __launch_bounds__(kNT) void k_push_tiles(const DevCounters* __restric
                                                     uint8_t* __restrict__ hub_flags, int n_hub) {
   __shared__ int32_t s_start[kTile + 2];
   __shared__ int32_t s_csr[kTile + 2];
+  __shared__ uint16_t s_row[kTile];
   const unsigned long long c = ctr->chg;
   if (use_pull(c, mode, threshold) || fused_scatter(c, fuse_max)) return;
   const int w_c = (int)(uint32_t)c;
@@ -353,6 +383,8 @@ __global__ __launch_bounds__(kNT) void k_push_tiles(const DevCounters* __restric
       s_csr[k] = off_out[chg_v[k0 + k]];
     }
     __syncthreads();
+    label_push_rows(s_start, nr, e0, e1 - e0, s_row);
+    __syncthreads();
     // The lane's kVT targets, then their bitmap words, are all read before its
     // first atomic: the compiler keeps later loads behind an atomic that may
     // alias them, which would make the items one memory round trip each.
@@ -362,15 +394,8 @@ __global__ __launch_bounds__(kNT) void k_push_tiles(const DevCounters* __restric
       const int e = e0 + tid + m * kNT;
       pos[m] = -1;
       if (e < e1) {
-        int lo = 0, hi = nr - 1;
-        while (lo < hi) {
-          int mid = (lo + hi + 1) >> 1;
-          if (s_start[mid] <= e)
-            lo = mid;
-          else
-            hi = mid - 1;
-        }
-        pos[m] = s_csr[lo] + (e - s_start[lo]);
+        const int r = s_row[e - e0];
+        pos[m] = s_csr[r] + (e - s_start[r]);
       }
     }
     int dst[kVT];
@@ -525,31 +550,7 @@ __global__ __launch_bounds__(kNT) void k_relax_tiles(
       s_vd[k] = vd[u];
     }
     __syncthreads();
-    // Row of every edge of the tile: each lane searches the row of its first
-    // of kVT consecutive edges and walks the rest (changed entries all have
-    // out-edges, so the starts rise strictly and nr <= kTile + 1).
-    {
-      const int l0 = tid * kVT, n_e = e1 - e0;
-      if (l0 < n_e) {
-        const int e = e0 + l0;
-        int lo = 0, hi = nr - 1;
-        while (lo < hi) {
-          int mid = (lo + hi + 1) >> 1;
-          if (s_start[mid] <= e)
-            lo = mid;
-          else
-            hi = mid - 1;
-        }
-        int next = lo + 1 < nr ? s_start[lo + 1] : INT_MAX;
-        for (int q = 0; q < kVT && l0 + q < n_e; ++q) {
-          while (next <= e + q) {
-            ++lo;
-            next = lo + 1 < nr ? s_start[lo + 1] : INT_MAX;
-          }
-          s_row[l0 + q] = (uint16_t)lo;
-        }
-      }
-    }
+    label_push_rows(s_start, nr, e0, e1 - e0, s_row);
     __syncthreads();
     // Targets and values of the lane's kVT edges, then every candidate (and
     // frontier word) read, then the atomics: the compiler keeps later loads
