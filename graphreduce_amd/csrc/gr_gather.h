@@ -437,13 +437,32 @@ __global__ void k_gather_fixup(int n_tiles, const Part<typename P::GatherResult>
   if (r < 0 || (t > 0 && carry[t - 1].row == r)) return;
   int has = carry[t].has;
   G acc = carry[t].val;
+  // A hub row spans many tiles: their carries are read kBatch at a time, so
+  // that the walk costs one memory round trip per kBatch tiles, not per tile.
+  // u ends at the first tile past t whose carry belongs to another row.
+  constexpr int kBatch = 16;
   int u = t + 1;
-  while (u < n_tiles && carry[u].row == r) {
-    if (carry[u].has) {
-      acc = has ? P::gatherReduce(acc, carry[u].val) : carry[u].val;
-      has = 1;
+  bool open = true;
+  while (open && u < n_tiles) {
+    const int cnt = min(kBatch, n_tiles - u);
+    Part<G> c[kBatch];
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k)
+      if (k < cnt) c[k] = carry[u + k];
+    int step = cnt;
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k) {
+      if (open && k < cnt) {
+        if (c[k].row != r) {
+          open = false;
+          step = k;
+        } else if (c[k].has) {
+          acc = has ? P::gatherReduce(acc, c[k].val) : c[k].val;
+          has = 1;
+        }
+      }
     }
-    ++u;
+    u += step;
   }
   if (u < n_tiles && head[u].row == r && head[u].has) {
     acc = has ? P::gatherReduce(acc, head[u].val) : head[u].val;
