@@ -1229,6 +1229,9 @@ struct Engine final : EngineBase {
   size_t edge_size() const override { return kEdgeEmpty ? 0 : sizeof(ED); }
 
   void release() {
+    // nextIter returns while k_compact_write may still run: drain the stream
+    // before its buffers go
+    if (stream) (void)hipStreamSynchronize(stream);
     void* ptrs[] = {d_hub_flags, d_srcval, d_cbase,    d_heavy,  d_vd,     d_vd_init,
                     d_vd_user,   d_ed,     d_ed_csr,   d_cand,   d_active, d_escan,
                     d_gtmp,      d_chg_v,  d_chg_e,    d_bitmap, d_chg_bits, d_chg_prev,
