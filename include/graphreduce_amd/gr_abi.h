@@ -188,7 +188,10 @@ int gr_engine_apply(gr_engine* e);
 /* scatterActivate (gpugas.h:1650); Program::scatter runs when
  * have_scatter != 0 (refgas.h:192-196). */
 int gr_engine_scatter_activate(gr_engine* e, int have_scatter);
-/* nextIter (gpugas.h:1761): builds the next active list; returns its size. */
+/* nextIter (gpugas.h:1761): builds the next active list; returns its size.
+ * Returns as soon as the size is known: the rest of the list's construction
+ * may still run on the engine's stream, and every later call on the engine
+ * is ordered behind it. */
 int gr_engine_next_iter(gr_engine* e, int32_t* n_active);
 /* getResults (gpugas.h:1778): copies device vertex data back into the
  * caller's vertexData (the reference's version is compiled out). */
