@@ -263,7 +263,10 @@ def main() -> int:
             result["cpu_baseline"] = cpu_baseline(gr, g, args, src, off_out, dsts, dist, e_reached)
         except Exception as exc:
             result["cpu_baseline"] = {"error": repr(exc)}
+    # release device objects while the runtime is up (no finalizer at exit)
     eng.close()
+    w_dev.free()
+    g.close()
     print(json.dumps(result), flush=True)
     return 0
 
@@ -428,6 +431,7 @@ def run_batched(gr, g, args, outdeg):
                        speedup=round(t_plain64 / t_b, 2), rounds=info["levels"],
                        identical_to_plain_for_timed_rows=same)
     log(f"batched SSSP: {res['sssp']}")
+    w.free()
     return res
 
 

@@ -12,8 +12,10 @@ the package fails loudly when the library has not been built.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 from pathlib import Path
 
 import numpy as np
@@ -206,6 +208,28 @@ _SIGS = {
 }
 
 _lib = None
+
+# Interpreter shutdown: once the atexit hook below has run, finalizers make no
+# library call.  Objects still alive then are collected during module teardown
+# or not at all, possibly after the HIP runtime (a C-level exit handler) has
+# been torn down, where a hipFree or stream call can fault; the process exit
+# returns their memory anyway.  Live servers are stopped by the hook itself,
+# while the runtime is still up, so that no worker thread outlives it.
+_SHUTDOWN = False
+_LIVE_SERVERS = weakref.WeakSet()
+
+
+def _at_exit() -> None:
+    global _SHUTDOWN
+    for srv in list(_LIVE_SERVERS):
+        try:
+            srv.close()
+        except Exception:
+            pass
+    _SHUTDOWN = True
+
+
+atexit.register(_at_exit)
 
 
 def lib() -> C.CDLL:
@@ -415,6 +439,8 @@ class Graph:
             self._h = None
 
     def __del__(self):
+        if _SHUTDOWN:
+            return
         try:
             self.close()
         except Exception:
@@ -481,6 +507,8 @@ class DeviceArray:
             self.ptr = None
 
     def __del__(self):
+        if _SHUTDOWN:
+            return
         try:
             self.free()
         except Exception:
@@ -513,6 +541,8 @@ class GASEngine:
             self._h = None
 
     def __del__(self):
+        if _SHUTDOWN:
+            return
         try:
             self.close()
         except Exception:
@@ -880,6 +910,7 @@ class Server:
         self._h = h
         self.graph = graph
         self.n = graph.n_vertices
+        _LIVE_SERVERS.add(self)
 
     def close(self):
         if self._h:
@@ -887,6 +918,8 @@ class Server:
             self._h = None
 
     def __del__(self):
+        if _SHUTDOWN:
+            return
         try:
             self.close()
         except Exception:

@@ -843,10 +843,12 @@ using namespace gr;
 // The streaming-pull kernel (k_msbfs_stream) stays off: its cooperative
 // launch did not finish on an MI355X (RMAT-14, 64 sources, automatic
 // direction) although a lane-level CPU emulation of the same code terminates
-// with exact depths on that graph.  Unlike k_msbfs it needs 86 SGPRs, so one
-// open question is whether all of the 8 workgroups per CU that the occupancy
-// query reports are resident at once (grid.sync waits for every workgroup).
+// with exact depths on that graph.  Residency is not the difference from
+// k_msbfs: the compiler reports 8 waves per SIMD for both kernels (8
+// workgroups per CU, as the occupancy query says).  The cause is not
+// established.
 constexpr bool kStreamPullEnabled = false;
+constexpr int kCoopBlocksPerCu = 4;  // cooperative MS-BFS grid: workgroups per CU
 
 extern "C" int gr_msbfs(gr_graph* g, const int32_t* sources, int n_sources, int32_t* depth_device,
                         int direction, double alpha, gr_batch_result* result) {
@@ -962,7 +964,14 @@ extern "C" int gr_msbfs(gr_graph* g, const int32_t* sources, int n_sources, int3
   GR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   GR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBatchNT, 0));
   if (per_cu < 1) return set_error(GR_ERR_HIP, "gr_msbfs: kernel cannot be resident");
-  const int grid = cus * per_cu;
+  // Every workgroup must be resident at once (grid.sync waits for all of
+  // them).  The compiler's resource report for gfx950 (k_msbfs: 66 SGPRs,
+  // 36 VGPRs, no LDS; k_msbfs_stream: 92 SGPRs, 48 VGPRs, no LDS) gives
+  // 8 waves per SIMD for both, i.e. 8 workgroups of 4 waves per CU, which is
+  // what the occupancy query returns.  The grid takes at most half of that,
+  // so the launch stays resident even when other work holds part of every
+  // CU; the level loops are grid-strided and do not depend on the grid size.
+  const int grid = cus * std::max(1, std::min(per_cu, kCoopBlocksPerCu));
   hipEvent_t e0, e1;
   GR_HIP(hipEventCreate(&e0));
   GR_HIP(hipEventCreate(&e1));

@@ -74,15 +74,24 @@ def test_cc_grid(gpu):
 @pytest.mark.parametrize("scale", [10, 14])
 def test_pagerank_rmat(gpu, scale):
     """Activation-driven PageRank of pagerank.cu.  Sum order differs from the
-    sequential CPU sum, so ranks agree to FP32 rounding: tolerance 1e-5
-    relative per vertex (and the activation schedule must match: same
-    trips)."""
+    sequential CPU sum, so ranks agree to FP32 rounding.  The reference's own
+    test rule (pagerank.cu:154-166) is 1e-6 absolute per vertex; it is applied
+    to every vertex whose rank is below 1, where FP32 spacing is at most
+    1.2e-7 and a few roundings of a reordered sum stay inside it.  Above that
+    the spacing itself grows past 1e-6 (rank 8 and up: 9.5e-7 per ulp, hubs
+    reach ranks in the hundreds), so one different rounding already breaks an
+    absolute bound, and those vertices are held to 1e-5 relative.  The
+    activation schedule must match: same trips."""
     gr = gpu
     n, s, d = _rmat(gr, scale, seed=3)
     outdeg = np.bincount(s, minlength=n).astype(np.int32)
     ref, ref_trips, _ = oracle.pagerank(n, s, d)
     rank, trips = gr.run_pagerank((n, s, d), outdeg)
-    rel = np.abs(rank.astype(np.float64) - ref) / np.abs(ref.astype(np.float64))
+    diff = np.abs(rank.astype(np.float64) - ref.astype(np.float64))
+    small = ref < 1.0
+    assert small.sum() > n // 2
+    assert diff[small].max() <= 1e-6, f"max abs err {diff[small].max():.3g} below rank 1"
+    rel = diff / np.abs(ref.astype(np.float64))
     assert rel.max() <= 1e-5, f"max rel err {rel.max():.3g}"
     assert trips == ref_trips
 
@@ -104,7 +113,9 @@ def test_work_counters_match_oracle(gpu):
     assert trips == st["iterations"]
     assert got["active_vertices"] == st["active"]
     assert got["gathered_edges"] == st["in_edges"]
-    assert got["changed_vertices"] == st["changed"] or got["changed_vertices"] <= st["changed"]
+    # sum #C and sum E_out(C): the changed vertices and their out-edges
+    assert got["changed_vertices"] == st["changed"]
+    assert got["pushed_edges"] == st["out_edges"]
     eng.close()
 
 

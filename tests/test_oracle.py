@@ -316,3 +316,137 @@ def test_build_csr_vs_scipy():
     order = np.argsort(s, kind="stable")
     assert np.array_equal(adj, d[order])
     assert np.array_equal(off, np.concatenate([[0], np.cumsum(np.bincount(s, minlength=n))]))
+
+
+# ------------------------------------------------- CSR-input entry points
+
+def _min_weight_adj(n, s, d, w):
+    """Weighted adjacency keeping the lightest of parallel arcs (scipy would
+    add duplicate entries together)."""
+    key = s.astype(np.int64) * n + d
+    order = np.lexsort((w, key))
+    first = np.ones(len(order), bool)
+    first[1:] = key[order][1:] != key[order][:-1]
+    keep = order[first]
+    return _adj(n, s[keep], d[keep], w[keep])
+
+
+def _random_multigraph(seed):
+    """Random graph with duplicate arcs, self-loops and isolated vertices."""
+    rng = np.random.default_rng(500 + seed)
+    n = int(rng.integers(2, 300))
+    m = int(rng.integers(0, 6 * n))
+    s = rng.integers(0, n, m).astype(np.int32)
+    d = rng.integers(0, n, m).astype(np.int32)
+    if m:
+        k = max(1, m // 10)
+        s[:k] = d[:k]                       # self-loops
+        s[k:2 * k], d[k:2 * k] = s[2 * k:3 * k], d[2 * k:3 * k]  # duplicates
+    w = rng.integers(1, 65, m).astype(np.int32)
+    return n, s, d, w
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_csr_entry_points_equal_edge_list_twins(seed):
+    """oracle_*_csr (prebuilt CSR + CSC, edge data in CSC order) give the same
+    values, trip counts and work counters as the edge-list drivers, which
+    tests above pin against scipy.  The CSC is the oracle builder's, stable in
+    edge-list order within a row, so PageRank's FP32 sums run in the same
+    order and must agree bit for bit."""
+    n, s, d, w = _random_multigraph(seed)
+    off_out, dsts = oracle.build_csr(n, s, d)
+    off_in, srcs = oracle.build_csr(n, d, s)
+    w_csc = w[np.argsort(d, kind="stable")]
+    src = int(np.argmax(np.bincount(s, minlength=n))) if len(s) else 0
+
+    a, ta, sa = oracle.bfs(n, s, d, src)
+    b, tb, sb = oracle.bfs_csr(n, off_out, dsts, off_in, srcs, src)
+    assert np.array_equal(a, b) and ta == tb
+    assert {k: sa[k] for k in sa if k != "seconds"} == {k: sb[k] for k in sb if k != "seconds"}
+
+    a, ta, sa = oracle.sssp(n, s, d, w, src)
+    b, tb, sb = oracle.sssp_csr(n, off_out, dsts, off_in, srcs, w_csc, src)
+    assert np.array_equal(a, b) and ta == tb and sa["in_edges"] == sb["in_edges"]
+    ref = cg.dijkstra(_min_weight_adj(n, s, d, w), indices=src)
+    assert np.array_equal(b, np.where(np.isinf(ref), INF, ref).astype(np.int64))
+
+    # CC on the symmetric closure
+    ss, dd = np.concatenate([s, d]), np.concatenate([d, s])
+    so, sd_ = oracle.build_csr(n, ss, dd)
+    si, sc = oracle.build_csr(n, dd, ss)
+    a, ta, sa = oracle.cc(n, ss, dd)
+    b, tb, sb = oracle.cc_csr(n, so, sd_, si, sc)
+    assert np.array_equal(a, b) and ta == tb and sa["in_edges"] == sb["in_edges"]
+    assert np.array_equal(b, _cc_min_labels(n, ss, dd))
+
+    a, ta, sa = oracle.pagerank(n, s, d)
+    b, tb, sb = oracle.pagerank_csr(n, off_out, dsts, off_in, srcs)
+    assert np.array_equal(a, b) and ta == tb and sa["in_edges"] == sb["in_edges"]
+
+
+def test_csr_entry_points_on_rmat():
+    n = 1 << 12
+    s, d = gr.rmat_edges(12, 16, seed=2)
+    w = gr.hash_weights(len(s), seed=2)
+    off_out, dsts = oracle.build_csr(n, s, d)
+    off_in, srcs = oracle.build_csr(n, d, s)
+    src = int(np.argmax(np.bincount(s, minlength=n)))
+    b, _, _ = oracle.sssp_csr(n, off_out, dsts, off_in, srcs, w[np.argsort(d, kind="stable")], src)
+    ref = cg.dijkstra(_adj(n, s, d, w), indices=src)
+    assert np.array_equal(b, np.where(np.isinf(ref), INF, ref).astype(np.int64))
+    b, _, _ = oracle.bfs_csr(n, off_out, dsts, off_in, srcs, src)
+    assert np.array_equal(b, _bfs_levels(n, s, d, src))
+
+
+def _pagerank_activation_fp64(n, s, d):
+    """pagerank.cu's activation-driven loop (all active first; rank starts at
+    0.15f; gather = sum over in-edges in CSC order of rank/outdeg; apply sets
+    rank = 0.15 + 0.85 g and activates on |change| >= 0.01) written in plain
+    Python floats (IEEE double), one in-edge at a time."""
+    c015 = float(np.float32(0.15))
+    tol = float(np.float32(0.01))
+    outdeg = np.bincount(s, minlength=n).tolist()
+    order = np.argsort(d, kind="stable")
+    ins = [[] for _ in range(n)]
+    for e in order.tolist():
+        ins[int(d[e])].append(int(s[e]))
+    outs = [[] for _ in range(n)]
+    for e in range(len(s)):
+        outs[int(s[e])].append(int(d[e]))
+    rank = [c015] * n
+    active = list(range(n))
+    trips = 0
+    while active:
+        g = {}
+        for v in active:
+            acc = 0.0
+            for u in ins[v]:
+                acc = acc + rank[u] / outdeg[u]
+            g[v] = acc
+        changed = []
+        for v in active:
+            nr = c015 + (1.0 - c015) * g[v]
+            if abs(nr - rank[v]) >= tol:
+                changed.append(v)
+            rank[v] = nr
+        nxt = set()
+        for u in changed:
+            nxt.update(outs[u])
+        active = sorted(nxt)
+        trips += 1
+    return np.array(rank), trips
+
+
+@pytest.mark.parametrize("scale,seed", [(8, 3), (10, 3)])
+def test_pagerank64_equals_python_double_loop(scale, seed):
+    """oracle_pagerank64 computes in double: it equals a plain-Python double
+    statement of the same loop bit for bit (a float accumulation or a float
+    division would differ by ~1e-8 relative and fail)."""
+    n = 1 << scale
+    s, d = gr.rmat_edges(scale, 16, seed=seed)
+    r64, t64, _ = oracle.pagerank64(n, s, d)
+    ref, trips = _pagerank_activation_fp64(n, s, d)
+    assert t64 == trips
+    assert np.array_equal(r64, ref)
+    r32, _, _ = oracle.pagerank(n, s, d)
+    assert not np.array_equal(r32.astype(np.float64), r64)
