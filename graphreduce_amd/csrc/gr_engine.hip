@@ -81,10 +81,14 @@ __global__ __launch_bounds__(kApplyNT) void k_apply(
     int use_gather, typename P::VertexData* __restrict__ vd, const int32_t* __restrict__ off_out,
     int iteration, int32_t* __restrict__ chg_v, int32_t* __restrict__ chg_e,
     uint32_t* __restrict__ chg_bits, DevCounters* __restrict__ ctr,
-    source_value_t<P>* __restrict__ srcval, typename P::GatherResult* __restrict__ cand) {
+    source_value_t<P>* __restrict__ srcval, typename P::GatherResult* __restrict__ cand,
+    unsigned long long* __restrict__ applied) {
   __shared__ int s_a[kApplyNT / kWave], s_b[kApplyNT / kWave];
   __shared__ unsigned long long s_base;
+  __shared__ unsigned s_applied;
   const int tid = threadIdx.x;
+  if (tid == 0) s_applied = 0;
+  unsigned my_applied = 0;  // applies that returned true, this thread
   for (int chunk = blockIdx.x * kApplyChunk; chunk < n; chunk += gridDim.x * kApplyChunk) {
     // In phases over the lane's kApplyVT entries (ids and gather values,
     // vertex values, apply, out-degrees of the changed ones, then the stores
@@ -153,6 +157,7 @@ __global__ __launch_bounds__(kApplyNT) void k_apply(
         my_edges += degs[m];
       }
     }
+    my_applied += __popc(r_mask);
     int tc, te;
     const int ec = block_excl_scan_add<kApplyNT>(my_cnt, s_a, &tc);
     const int ee = block_excl_scan_add<kApplyNT>(my_edges, s_b, &te);
@@ -176,6 +181,15 @@ __global__ __launch_bounds__(kApplyNT) void k_apply(
       __syncthreads();
     }
   }
+  // every apply that returned true counts toward sum #C, also those of
+  // vertices without out-edges, which join no changed list
+  const unsigned wsum = wave_reduce_add(my_applied);
+  __syncthreads();
+  if (lane_id() == 0 && wsum) atomicAdd(&s_applied, wsum);
+  __syncthreads();
+  if (tid == 0 && s_applied)
+    atomicAdd(&applied[(blockIdx.x % kAppliedSlots) * kAppliedStride],
+              (unsigned long long)s_applied);
 }
 
 // srcval[v] = P::sourceValue(vd[v]) for every vertex.
@@ -851,8 +865,17 @@ __global__ __launch_bounds__(1024) void k_compact_scan(int2* __restrict__ blk, i
                                                        long long threshold,
                                                        DevCounters* __restrict__ host,
                                                        uint32_t* __restrict__ host_seq,
-                                                       uint32_t seq) {
+                                                       uint32_t seq,
+                                                       unsigned long long* __restrict__ applied) {
   __shared__ int s_a[1024 / kWave], s_b[1024 / kWave];
+  __shared__ unsigned long long s_applied;
+  unsigned long long my_applied = 0;
+  if (threadIdx.x < kAppliedSlots) {
+    my_applied = applied[threadIdx.x * kAppliedStride];
+    applied[threadIdx.x * kAppliedStride] = 0;
+  }
+  my_applied = wave_reduce_add(my_applied);  // slots 0..63: wave 0
+  if (threadIdx.x == 0) s_applied = my_applied;
   int run_a = 0, run_b = 0;
   for (int base = 0; base < nb; base += 1024) {
     const int i = base + threadIdx.x;
@@ -871,6 +894,7 @@ __global__ __launch_bounds__(1024) void k_compact_scan(int2* __restrict__ blk, i
     ctr->chg_last = ctr->chg;
     ctr->pull = use_pull(ctr->chg, mode, threshold) && !ctr->fused ? 1 : 0;
     ctr->chg = 0;
+    ctr->applied_last = s_applied;
     *host = *ctr;
     __threadfence_system();
     __hip_atomic_store(host_seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -1171,6 +1195,7 @@ struct Engine final : EngineBase {
   Part<G>* d_carry = nullptr;
   Part<G>* d_head = nullptr;
   DevCounters* d_ctr = nullptr;
+  unsigned long long* d_applied = nullptr;  // kAppliedSlots spread counters
   DevCounters* h_ctr = nullptr;  // pinned, mapped, coherent
   DevCounters* d_hctr = nullptr; // h_ctr as the device addresses it
   uint32_t* h_seq = nullptr;     // last counters' sequence number (pinned, mapped, coherent)
@@ -1235,7 +1260,8 @@ struct Engine final : EngineBase {
     void* ptrs[] = {d_hub_flags, d_srcval, d_cbase,    d_heavy,  d_vd,     d_vd_init,
                     d_vd_user,   d_ed,     d_ed_csr,   d_cand,   d_active, d_escan,
                     d_gtmp,      d_chg_v,  d_chg_e,    d_bitmap, d_chg_bits, d_chg_prev,
-                    d_blk,       d_part,   d_push_part, d_carry, d_head,   d_ctr};
+                    d_blk,       d_part,   d_push_part, d_carry, d_head,   d_ctr,
+                    d_applied};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     if (h_ctr) (void)hipHostFree(h_ctr);
@@ -1293,6 +1319,7 @@ struct Engine final : EngineBase {
     d_blk = nullptr;
     d_carry = d_head = nullptr;
     d_ctr = h_ctr = d_hctr = nullptr;
+    d_applied = nullptr;
     h_seq = d_hseq = nullptr;
     stream = nullptr;
     g = nullptr;
@@ -1359,6 +1386,7 @@ struct Engine final : EngineBase {
     GR_TRY(dalloc(&d_carry, max_tiles + 1));
     GR_TRY(dalloc(&d_head, max_tiles + 1));
     GR_TRY(dalloc(&d_ctr, 1));
+    GR_TRY(dalloc(&d_applied, kAppliedSlots * kAppliedStride));
     GR_HIP(hipHostMalloc((void**)&h_ctr, sizeof(DevCounters),
                          hipHostMallocMapped | hipHostMallocCoherent));
     GR_HIP(hipHostGetDevicePointer((void**)&d_hctr, h_ctr, 0));
@@ -1370,6 +1398,8 @@ struct Engine final : EngineBase {
     GR_HIP(hipMemsetAsync(d_chg_bits, 0, sizeof(uint32_t) * (n_words + 1), stream));
     GR_HIP(hipMemsetAsync(d_chg_prev, 0, sizeof(uint32_t) * (n_words + 1), stream));
     GR_HIP(hipMemsetAsync(d_ctr, 0, sizeof(DevCounters), stream));
+    GR_HIP(hipMemsetAsync(d_applied, 0,
+                          sizeof(unsigned long long) * kAppliedSlots * kAppliedStride, stream));
     {
       std::vector<int32_t> hv;
       for (int32_t v = 0; v < g->nV; ++v)
@@ -1747,6 +1777,9 @@ struct Engine final : EngineBase {
     GR_HIP(hipMemsetAsync(d_bitmap, 0, sizeof(uint32_t) * (n_words + 1), stream));
     GR_HIP(hipMemsetAsync(d_chg_bits, 0, sizeof(uint32_t) * (n_words + 1), stream));
     GR_HIP(hipMemsetAsync(d_chg_prev, 0, sizeof(uint32_t) * (n_words + 1), stream));
+    // applies since the last nextIter belong to the replaced state
+    GR_HIP(hipMemsetAsync(d_applied, 0,
+                          sizeof(unsigned long long) * kAppliedSlots * kAppliedStride, stream));
     // An all-active iteration gathers every in-edge of every vertex, which
     // establishes the delta-gather invariant; a partial range does not.
     delta_chain = (start == 0 && end == g->nV);
@@ -1791,6 +1824,8 @@ struct Engine final : EngineBase {
     }
     GR_HIP(hipMemcpyAsync(d_escan, escan.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice,
                           stream));
+    GR_HIP(hipMemsetAsync(d_applied, 0,
+                          sizeof(unsigned long long) * kAppliedSlots * kAppliedStride, stream));
     k_set_counters<<<1, 1, 0, stream>>>(d_ctr, n, (int32_t)acc);
     GR_HIP(hipGetLastError());
     GR_HIP(hipStreamSynchronize(stream));  // the host vectors go out of scope
@@ -1977,7 +2012,7 @@ struct Engine final : EngineBase {
     tick();
     k_apply<P><<<grid_1d(n_active, kApplyChunk), kApplyNT, 0, stream>>>(
         n_active, d_active, d_gtmp, use_gather, d_vd, g->off_out, iteration, d_chg_v, d_chg_e,
-        changed_bits_on() ? d_chg_bits : nullptr, d_ctr, d_srcval, d_cand);
+        changed_bits_on() ? d_chg_bits : nullptr, d_ctr, d_srcval, d_cand, d_applied);
     tock(kPhApply);
     GR_HIP(hipGetLastError());
     return GR_OK;
@@ -2094,7 +2129,8 @@ struct Engine final : EngineBase {
                                                           need_deg, d_blk);
     ++ctr_seq;
     k_compact_scan<<<1, 1024, 0, stream>>>(d_blk, n_cblocks, d_ctr, need_deg ? d_escan : nullptr,
-                                           dir_mode, pull_threshold(), d_hctr, d_hseq, ctr_seq);
+                                           dir_mode, pull_threshold(), d_hctr, d_hseq, ctr_seq,
+                                           d_applied);
     k_compact_write<<<n_cblocks, kCompactNT, 0, stream>>>(d_bitmap, n_words, g->nV, g->off_in,
                                                           need_deg, d_blk, d_ctr, d_active, d_escan,
                                                           d_cbase,
@@ -2115,7 +2151,7 @@ struct Engine final : EngineBase {
       if (fused && !delta_ready) GR_TRY(drop_candidates());  // not reached: same conditions
     }
     const unsigned long long c = h_ctr->chg_last;
-    stats.changed_vertices += (int64_t)(c >> 32);
+    stats.changed_vertices += (int64_t)h_ctr->applied_last;
     stats.pushed_edges += (int64_t)(uint32_t)c;
     stats.pull_iterations += h_ctr->pull ? 1 : 0;
     if (n_active > 0) stats.iterations += 1;

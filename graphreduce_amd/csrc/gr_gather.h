@@ -35,7 +35,17 @@ struct DevCounters {
   int32_t pull;                   // last scatterActivate ran in pull mode
   int32_t fused;                  // last scatterActivate also pushed the next gather
   unsigned long long inert;       // (#non-inert << 32) | out-edges, from k_inert_scan
+  unsigned long long applied_last;  // vertices whose apply returned true since
+                                    // the nextIter before the last one
 };
+
+// Vertices whose apply returned true (with or without out-edges) are counted
+// by apply per workgroup into kAppliedSlots counters 128 bytes apart (slot =
+// workgroup id mod kAppliedSlots), so that the adds do not contend on one
+// address; nextIter sums and clears them.
+constexpr int kAppliedSlots = 64;
+constexpr int kAppliedStride = 16;  // unsigned long longs between slots
+static_assert(kAppliedSlots <= kWave, "nextIter sums the slots in one wave");
 
 // Read-once operands (CSC sources, edge data) are loaded non-temporally so
 // that streaming them does not displace the randomly re-read vertex values
