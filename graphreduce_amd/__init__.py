@@ -194,6 +194,9 @@ _SIGS = {
     "gr_pagerank_precise": (C.c_int, [_P, C.c_int, C.c_double, C.c_int,
                                        C.POINTER(C.c_float), C.POINTER(C.c_double),
                                        C.POINTER(_PRResult)]),
+    "gr_pagerank_precise_noalloc": (C.c_int, [_P, C.c_int, C.c_double, C.c_int,
+                                               C.POINTER(C.c_float), C.POINTER(C.c_double),
+                                               C.POINTER(_PRResult)]),
     # gr_graphio.h
     "gr_graphfile_load": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(_P)]),
     "gr_graphfile_n_vertices": (C.c_int32, [_P]),
@@ -810,17 +813,20 @@ def run_pagerank(graph_or_edges, out_degree, direction: int = 0):
 
 
 def pagerank_precise(graph: Graph, max_iters: int = 50, tol: float = 1e-7,
-                     compensated: bool = True):
+                     compensated: bool = True, noalloc: bool = False):
     """All-active PageRank (FP32, compensated sums optional, FP64 L1 residual).
 
+    noalloc=True calls gr_pagerank_precise_noalloc (no device allocation; its
+    buffers live in a static 80 MiB device array per GPU).
     Returns (ranks float32[V], info) with info = iterations, residual (last
     step), residuals (per step), ms, gather_ms, gathered_edges."""
     ranks = np.empty(graph.n_vertices, np.float32)
     resid = np.zeros(max(1, max_iters), np.float64)
     r = _PRResult()
-    _check(lib().gr_pagerank_precise(graph.handle, max_iters, tol, int(compensated),
-                                     ranks.ctypes.data_as(C.POINTER(C.c_float)),
-                                     resid.ctypes.data_as(C.POINTER(C.c_double)), C.byref(r)))
+    fn = lib().gr_pagerank_precise_noalloc if noalloc else lib().gr_pagerank_precise
+    _check(fn(graph.handle, max_iters, tol, int(compensated),
+              ranks.ctypes.data_as(C.POINTER(C.c_float)),
+              resid.ctypes.data_as(C.POINTER(C.c_double)), C.byref(r)))
     info = {name: getattr(r, name) for name, _ in _PRResult._fields_}
     info["residuals"] = resid[: r.iterations].tolist()
     return ranks, info

@@ -266,7 +266,9 @@ __global__ __launch_bounds__(kNT) void k_gather_tiles(
   const int n_edges = j1 - j0;
 
   for (int k = tid; k <= n_rows; k += kNT) s_scan[k] = escan[i0 + k];
-  for (int k = tid; k < n_ref; k += kNT) s_base[k] = cbase[i0 + k];
+  // cbase == nullptr: every row's CSC base is 0 (the active list is every
+  // vertex in order and escan is the CSC offset array itself)
+  for (int k = tid; k < n_ref; k += kNT) s_base[k] = cbase ? cbase[i0 + k] : 0;
   __syncthreads();
 
   // Phase 1: each lane walks its VT path items and labels edges with rows.

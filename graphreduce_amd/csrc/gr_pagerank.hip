@@ -12,6 +12,8 @@
 // The L1 residual sum_v |r_{t+1}[v] - r_t[v]| is accumulated in FP64 with a
 // fixed reduction order and decides convergence (stop when it drops below
 // tol or after max_iters steps).
+#include <chrono>
+#include <mutex>
 #include <vector>
 
 #include "gr_device.h"
@@ -72,6 +74,7 @@ __device__ __forceinline__ float gather_value(const KahanF& g) { return g.s + g.
 __device__ __forceinline__ float gather_value(float g) { return g; }
 
 constexpr int kPRNT = 256;
+constexpr int kPRVT = 8;  // merge-path items per thread of the gather tiles
 
 __global__ void k_pr_init(int nV, const int32_t* __restrict__ off_out, float* __restrict__ rank,
                           float* __restrict__ contrib) {
@@ -149,83 +152,99 @@ struct Buf {
   }
 };
 
+// Device buffers of one precision PageRank run.
 template <class P>
-int pagerank_precise(gr_graph* g, int max_iters, double tol, float* ranks, double* residuals,
-                     gr_pagerank_result* res) {
+struct PRWork {
   typedef typename P::GatherResult G;
-  const int nV = g->nV;
-  const int w = (int)g->nE;
-  constexpr int VT = 8;
-  const int tile = kNT * VT;
-  const int n_tiles = (int)(((int64_t)nV + w + tile - 1) / tile);
-  const int n_ublocks = (nV + kPRNT - 1) / kPRNT;
-  Buf<float> rank, contrib;
-  Buf<G> gsum;
-  Buf<int32_t> cbase, part;
-  Buf<Part<G>> carry, head;
-  Buf<double> partial, total;
-  GR_TRY(rank.alloc(nV));
-  GR_TRY(contrib.alloc(nV));
-  GR_TRY(gsum.alloc(nV));
-  GR_TRY(cbase.alloc((int64_t)nV + 1));
-  GR_TRY(part.alloc((int64_t)n_tiles + 1));
-  GR_TRY(carry.alloc(n_tiles));
-  GR_TRY(head.alloc(n_tiles));
-  GR_TRY(partial.alloc(n_ublocks));
-  GR_TRY(total.alloc(1));
-  hipStream_t st = nullptr;
-  GR_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-  struct StreamGuard {
-    hipStream_t s;
-    ~StreamGuard() { (void)hipStreamDestroy(s); }
-  } guard{st};
-  hipEvent_t e0, e1, g0, g1;
-  GR_HIP(hipEventCreate(&e0));
-  GR_HIP(hipEventCreate(&e1));
-  GR_HIP(hipEventCreate(&g0));
-  GR_HIP(hipEventCreate(&g1));
+  float* rank = nullptr;
+  float* contrib = nullptr;
+  G* gsum = nullptr;
+  int32_t* part = nullptr;
+  Part<G>* carry = nullptr;
+  Part<G>* head = nullptr;
+  double* partial = nullptr;
+  double* total = nullptr;
+};
+
+struct PRShape {
+  int nV, w, n_tiles, n_ublocks;
+  explicit PRShape(const gr_graph* g)
+      : nV(g->nV),
+        w((int)g->nE),
+        n_tiles((int)(((int64_t)g->nV + g->nE + kNT * kPRVT - 1) / (kNT * kPRVT))),
+        n_ublocks((g->nV + kPRNT - 1) / kPRNT) {}
+};
+
+// The iteration loop on `st`.  timed: device time from events (ms, gather_ms);
+// otherwise the host clock around the loop gives ms and gather_ms is 0.
+// Every buffer in `wk` is written before it is read, so none needs clearing.
+template <class P>
+int pagerank_loop(gr_graph* g, const PRWork<P>& wk, hipStream_t st, bool timed, int max_iters,
+                  double tol, float* ranks, double* residuals, gr_pagerank_result* res) {
+  const PRShape sh(g);
+  const int tile = kNT * kPRVT;
+  hipEvent_t e0 = nullptr, e1 = nullptr, g0 = nullptr, g1 = nullptr;
+  struct EventGuard {
+    hipEvent_t* ev[4];
+    ~EventGuard() {
+      for (hipEvent_t* e : ev)
+        if (*e) (void)hipEventDestroy(*e);
+    }
+  } eguard{{&e0, &e1, &g0, &g1}};
+  if (timed) {
+    GR_HIP(hipEventCreate(&e0));
+    GR_HIP(hipEventCreate(&e1));
+    GR_HIP(hipEventCreate(&g0));
+    GR_HIP(hipEventCreate(&g1));
+  }
   // identity active list: escan = CSC offsets, CSC base 0 for every row
-  GR_HIP(hipMemsetAsync(cbase.p, 0, sizeof(int32_t) * ((size_t)nV + 1), st));
-  k_merge_partition<<<grid_1d((int64_t)n_tiles + 1, 256), 256, 0, st>>>(g->off_in, nV, w, n_tiles,
-                                                                       tile, part.p);
-  k_pr_init<<<n_ublocks, kPRNT, 0, st>>>(nV, g->off_out, rank.p, contrib.p);
+  k_merge_partition<<<grid_1d((int64_t)sh.n_tiles + 1, 256), 256, 0, st>>>(
+      g->off_in, sh.nV, sh.w, sh.n_tiles, tile, wk.part);
+  k_pr_init<<<sh.n_ublocks, kPRNT, 0, st>>>(sh.nV, g->off_out, wk.rank, wk.contrib);
   GR_HIP(hipGetLastError());
   GR_HIP(hipStreamSynchronize(st));
   double gather_ms = 0.0, resid = 0.0;
   int it = 0;
-  GR_HIP(hipEventRecord(e0, st));
+  const auto t0 = std::chrono::steady_clock::now();
+  if (timed) GR_HIP(hipEventRecord(e0, st));
   while (it < max_iters) {
-    GR_HIP(hipEventRecord(g0, st));
-    if (n_tiles > 0)
-      k_gather_tiles<P, VT><<<n_tiles, kNT, 0, st>>>(nV, w, nullptr, g->off_in, part.p, cbase.p,
-                                                     g->srcs, contrib.p,
-                                                     reinterpret_cast<const typename P::EdgeData*>(contrib.p),
-                                                     gsum.p, carry.p, head.p, nullptr, nullptr,
-                                                     nullptr);
-    GR_HIP(hipEventRecord(g1, st));
-    k_gather_fixup<P><<<grid_1d(n_tiles, 256), 256, 0, st>>>(n_tiles, carry.p, head.p, gsum.p);
-    k_pr_update<G><<<n_ublocks, kPRNT, 0, st>>>(nV, gsum.p, g->off_out, rank.p, contrib.p,
-                                                partial.p);
-    k_sum_partials<<<1, 1024, 0, st>>>(partial.p, n_ublocks, total.p);
+    if (timed) GR_HIP(hipEventRecord(g0, st));
+    if (sh.n_tiles > 0)
+      k_gather_tiles<P, kPRVT><<<sh.n_tiles, kNT, 0, st>>>(
+          sh.nV, sh.w, nullptr, g->off_in, wk.part, nullptr, g->srcs, wk.contrib,
+          reinterpret_cast<const typename P::EdgeData*>(wk.contrib), wk.gsum, wk.carry, wk.head,
+          nullptr, nullptr, nullptr);
+    if (timed) GR_HIP(hipEventRecord(g1, st));
+    k_gather_fixup<P><<<grid_1d(sh.n_tiles, 256), 256, 0, st>>>(sh.n_tiles, wk.carry, wk.head,
+                                                               wk.gsum);
+    k_pr_update<typename P::GatherResult><<<sh.n_ublocks, kPRNT, 0, st>>>(
+        sh.nV, wk.gsum, g->off_out, wk.rank, wk.contrib, wk.partial);
+    k_sum_partials<<<1, 1024, 0, st>>>(wk.partial, sh.n_ublocks, wk.total);
     GR_HIP(hipGetLastError());
-    GR_HIP(hipMemcpyAsync(&resid, total.p, sizeof(double), hipMemcpyDeviceToHost, st));
+    GR_HIP(hipMemcpyAsync(&resid, wk.total, sizeof(double), hipMemcpyDeviceToHost, st));
     GR_HIP(hipStreamSynchronize(st));
-    float gm = 0.0f;
-    GR_HIP(hipEventElapsedTime(&gm, g0, g1));
-    gather_ms += gm;
+    if (timed) {
+      float gm = 0.0f;
+      GR_HIP(hipEventElapsedTime(&gm, g0, g1));
+      gather_ms += gm;
+    }
     if (residuals) residuals[it] = resid;
     ++it;
     if (resid < tol) break;
   }
-  GR_HIP(hipEventRecord(e1, st));
-  GR_HIP(hipEventSynchronize(e1));
-  float ms = 0.0f;
-  GR_HIP(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipEventDestroy(g0);
-  (void)hipEventDestroy(g1);
-  if (ranks) GR_HIP(hipMemcpy(ranks, rank.p, sizeof(float) * nV, hipMemcpyDeviceToHost));
+  double ms = 0.0;
+  if (timed) {
+    GR_HIP(hipEventRecord(e1, st));
+    GR_HIP(hipEventSynchronize(e1));
+    float fms = 0.0f;
+    GR_HIP(hipEventElapsedTime(&fms, e0, e1));
+    ms = fms;
+  } else {
+    ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  if (ranks)
+    GR_HIP(hipMemcpyAsync(ranks, wk.rank, sizeof(float) * sh.nV, hipMemcpyDeviceToHost, st));
+  GR_HIP(hipStreamSynchronize(st));
   if (res) {
     res->iterations = it;
     res->residual = resid;
@@ -234,6 +253,116 @@ int pagerank_precise(gr_graph* g, int max_iters, double tol, float* ranks, doubl
     res->gathered_edges = (int64_t)it * g->nE;
   }
   return GR_OK;
+}
+
+template <class P>
+int pagerank_precise(gr_graph* g, int max_iters, double tol, float* ranks, double* residuals,
+                     gr_pagerank_result* res) {
+  typedef typename P::GatherResult G;
+  const PRShape sh(g);
+  Buf<float> rank, contrib;
+  Buf<G> gsum;
+  Buf<int32_t> part;
+  Buf<Part<G>> carry, head;
+  Buf<double> partial, total;
+  GR_TRY(rank.alloc(sh.nV));
+  GR_TRY(contrib.alloc(sh.nV));
+  GR_TRY(gsum.alloc(sh.nV));
+  GR_TRY(part.alloc((int64_t)sh.n_tiles + 1));
+  GR_TRY(carry.alloc(sh.n_tiles));
+  GR_TRY(head.alloc(sh.n_tiles));
+  GR_TRY(partial.alloc(sh.n_ublocks));
+  GR_TRY(total.alloc(1));
+  hipStream_t st = nullptr;
+  GR_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  struct StreamGuard {
+    hipStream_t s;
+    ~StreamGuard() { (void)hipStreamDestroy(s); }
+  } guard{st};
+  PRWork<P> wk;
+  wk.rank = rank.p;
+  wk.contrib = contrib.p;
+  wk.gsum = gsum.p;
+  wk.part = part.p;
+  wk.carry = carry.p;
+  wk.head = head.p;
+  wk.partial = partial.p;
+  wk.total = total.p;
+  return pagerank_loop<P>(g, wk, st, true, max_iters, tol, ranks, residuals, res);
+}
+
+// ------------------------------------------------- allocation-free variant
+//
+// The buffers come from `scratch`, a static device array that every GPU the
+// library runs on holds (its own copy) from the first use of this file's
+// kernels until the process ends.  Host code never passes the array's name
+// to a kernel: the device address of each GPU's copy is looked up once with
+// hipGetSymbolAddress and kept per device.
+
+constexpr size_t kPRScratchBytes = (size_t)80 << 20;
+static __device__ __align__(256) unsigned char scratch[kPRScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+static int scratch_address(unsigned char** out) {
+  int dev = 0;
+  GR_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    return set_errorf(GR_ERR_INVALID, "device %d beyond the %d the scratch table holds", dev,
+                      kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_errorf(GR_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                        hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return GR_OK;
+}
+
+// Carves the run's buffers out of [base, base + kPRScratchBytes) at 256-byte
+// boundaries; with base == nullptr only counts.  Returns the bytes needed.
+template <class P>
+size_t carve(unsigned char* base, const PRShape& sh, PRWork<P>* wk) {
+  typedef typename P::GatherResult G;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    unsigned char* p = base ? base + off : nullptr;
+    off += (bytes + 255) & ~(size_t)255;
+    return p;
+  };
+  wk->rank = reinterpret_cast<float*>(take(sizeof(float) * (size_t)sh.nV));
+  wk->contrib = reinterpret_cast<float*>(take(sizeof(float) * (size_t)sh.nV));
+  wk->gsum = reinterpret_cast<G*>(take(sizeof(G) * (size_t)sh.nV));
+  wk->part = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * ((size_t)sh.n_tiles + 1)));
+  wk->carry = reinterpret_cast<Part<G>*>(take(sizeof(Part<G>) * (size_t)sh.n_tiles));
+  wk->head = reinterpret_cast<Part<G>*>(take(sizeof(Part<G>) * (size_t)sh.n_tiles));
+  wk->partial = reinterpret_cast<double*>(take(sizeof(double) * (size_t)sh.n_ublocks));
+  wk->total = reinterpret_cast<double*>(take(sizeof(double)));
+  return off;
+}
+
+template <class P>
+int pagerank_precise_noalloc(gr_graph* g, int max_iters, double tol, float* ranks,
+                             double* residuals, gr_pagerank_result* res) {
+  const PRShape sh(g);
+  PRWork<P> wk;
+  const size_t need = carve<P>(nullptr, sh, &wk);
+  if (need > kPRScratchBytes)
+    return set_errorf(GR_ERR_INVALID,
+                      "gr_pagerank_precise_noalloc: this graph needs %zu bytes of scratch, the "
+                      "maximum is %zu (16 B per vertex + 36 B per 2048 vertices and edges)",
+                      need, kPRScratchBytes);
+  unsigned char* base = nullptr;
+  GR_TRY(scratch_address(&base));
+  carve<P>(base, sh, &wk);
+  // the null stream: no stream or event is created
+  return pagerank_loop<P>(g, wk, nullptr, false, max_iters, tol, ranks, residuals, res);
 }
 
 }  // namespace gr
@@ -247,4 +376,16 @@ extern "C" int gr_pagerank_precise(gr_graph* g, int max_iters, double tol, int c
     return set_error(GR_ERR_INVALID, "graph too large for int32 merge-path coordinates");
   if (compensated) return pagerank_precise<PRKahanProgram>(g, max_iters, tol, ranks, residuals, result);
   return pagerank_precise<PRPlainProgram>(g, max_iters, tol, ranks, residuals, result);
+}
+
+extern "C" int gr_pagerank_precise_noalloc(gr_graph* g, int max_iters, double tol,
+                                           int compensated, float* ranks, double* residuals,
+                                           gr_pagerank_result* result) {
+  if (!g || max_iters < 0)
+    return set_error(GR_ERR_INVALID, "gr_pagerank_precise_noalloc: bad arguments");
+  if ((int64_t)g->nV + g->nE >= ((int64_t)1 << 31) - kNT * 16)
+    return set_error(GR_ERR_INVALID, "graph too large for int32 merge-path coordinates");
+  if (compensated)
+    return pagerank_precise_noalloc<PRKahanProgram>(g, max_iters, tol, ranks, residuals, result);
+  return pagerank_precise_noalloc<PRPlainProgram>(g, max_iters, tol, ranks, residuals, result);
 }

@@ -343,6 +343,20 @@ typedef struct gr_pagerank_result {
 int gr_pagerank_precise(gr_graph* g, int max_iters, double tol, int compensated, float* ranks,
                         double* residuals, gr_pagerank_result* result);
 
+/* gr_pagerank_precise without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, results and
+ * errors as gr_pagerank_precise (identical ranks and residuals); its buffers
+ * come from one static device array per GPU of 80 MiB (83,886,080 bytes),
+ * held from the first use until the process ends.  A graph needing more
+ * (16 B per vertex + 36 B per 2048 vertices and edges, each buffer rounded
+ * up to 256 B; RMAT-22 needs about 69 MB) is refused with GR_ERR_INVALID.
+ * Runs on the null stream, creates no stream or event; result->ms is the host
+ * clock around the loop and result->gather_ms is 0.  Calls that overlap on
+ * one GPU share the array and race: serialise them. */
+int gr_pagerank_precise_noalloc(gr_graph* g, int max_iters, double tol, int compensated,
+                                float* ranks, double* residuals, gr_pagerank_result* result);
+
 /* ----------------------------------------------- batched multi-source runs */
 /* Up to 64 sources share one traversal (no reference counterpart: the
  * reference runs one source per engine, bfs.cu:93-134, sssp.cu:69-111).
