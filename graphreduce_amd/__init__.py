@@ -89,6 +89,7 @@ class _Stats(C.Structure):
         ("streamed_shards", C.c_int64),
         ("fused_gathers", C.c_int64),
         ("fused_edges", C.c_int64),
+        ("blocked_gathers", C.c_int64),
     ]
 
 
@@ -178,6 +179,7 @@ _SIGS = {
     "gr_engine_set_vertex_order": (C.c_int, [_P, C.c_int]),
     "gr_engine_set_streaming": (C.c_int, [_P, C.c_int64]),
     "gr_engine_set_fuse_share": (C.c_int, [_P, C.c_double]),
+    "gr_engine_set_blocked_gather": (C.c_int, [_P, C.c_double]),
     "gr_engine_enable_timing": (C.c_int, [_P, C.c_int]),
     "gr_engine_get_stats": (C.c_int, [_P, C.POINTER(_Stats)]),
     "gr_engine_reset_stats": (C.c_int, [_P]),
@@ -672,6 +674,11 @@ class GASEngine:
     def setDirection(self, mode: int = 0, alpha: float = 15.0) -> None:
         """0 = automatic push/pull, 1 = push only, 2 = pull only."""
         _check(lib().gr_engine_set_direction(self._h, mode, alpha))
+
+    def setBlockedGather(self, min_share: float = 0.5) -> None:
+        """Gathers over at least min_share * E in-edges run on the graph's
+        source-listed row blocks (SSSP, CC); <= 0 turns that off."""
+        _check(lib().gr_engine_set_blocked_gather(self._h, float(min_share)))
 
     def setFuseShare(self, share: float) -> None:
         """SSSP/CC: fuse activation with the next push gather when the changed

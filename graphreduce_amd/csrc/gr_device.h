@@ -82,6 +82,23 @@ __device__ __forceinline__ T block_excl_scan_add(T v, T* scratch, T* total) {
 
 }  // namespace gr
 
+// Row blocks of the CSC whose in-edges are listed by source (for the blocked
+// gather): block b holds rows [brow[b], brow[b] + bnr[b]) -- at most
+// gr::kBlockRows rows and, unless one row alone has more, gr::kBlockEdges
+// in-edges -- and their in-edges are positions [boff[b], boff[b + 1]) of
+// bsrc / brl / bpos, sorted by source (stable).  Rows from rows_end on have
+// no in-edges and belong to no block.
+struct gr_blocked {
+  int32_t n_blocks = 0;
+  int32_t rows_end = 0;
+  int32_t* brow = nullptr;  // [n_blocks]
+  int32_t* bnr = nullptr;   // [n_blocks]
+  int32_t* boff = nullptr;  // [n_blocks + 1]
+  int32_t* bsrc = nullptr;  // [E] source of the edge
+  uint16_t* brl = nullptr;  // [E] its row - brow of the block
+  int32_t* bpos = nullptr;  // [E] its CSC position
+};
+
 struct gr_graph {
   int32_t nV = 0;
   int64_t nE = 0;
@@ -99,12 +116,17 @@ struct gr_graph {
   gr_graph* ordered = nullptr;
   int32_t* perm = nullptr;  // ordered id -> id   [V], device
   int32_t* inv = nullptr;   // id -> ordered id   [V], device
+  gr_blocked* blocked = nullptr;  // built on first use (graph_blocked)
 };
 
 namespace gr {
+constexpr int kBlockRows = 16384;       // rows per gather block (LDS: 64 KB of int)
+constexpr int kBlockEdges = 1 << 18;    // in-edges per gather block (target)
 int graph_free(gr_graph* g);
 // The degree-ordered copy of g (cached in g; freed with g).
 int graph_ordered(gr_graph* g, gr_graph** out);
+// The row blocks of g's CSC with in-edges listed by source (cached in g).
+int graph_blocked(gr_graph* g, const gr_blocked** out);
 }
 
 #endif

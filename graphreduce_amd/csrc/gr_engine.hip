@@ -125,6 +125,12 @@ __global__ __launch_bounds__(kApplyNT) void k_apply(
       for (int m = 0; m < kApplyVT; ++m)
         if (valid_mask & (1u << m)) cand[dvs[m]] = P::gatherZero;
     }
+    // use_gather 3 (blocked gather): cand holds every vertex's gather value
+    if (use_gather == 3) {
+#pragma unroll
+      for (int m = 0; m < kApplyVT; ++m)
+        if (valid_mask & (1u << m)) gs[m] = cand[dvs[m]];
+    }
 #pragma unroll
     for (int m = 0; m < kApplyVT; ++m) {
       if (valid_mask & (1u << m)) {
@@ -1131,6 +1137,9 @@ struct EngineBase {
   int vertex_order = 1;  // 1: engine works on the degree-ordered copy of the graph
   int64_t stream_max_edges = 0;  // > 0: out-of-core shards of at most this many edges
   double fuse_share = 0.4;        // fused activation + push gather up to this share of E
+  // blocked gather (programs that qualify) for gathers over at least this
+  // share of E; <= 0: never (gr_engine_set_blocked_gather)
+  double blocked_share = 0.5;
   bool timing = false;
   gr_engine_stats stats{};
 };
@@ -1158,6 +1167,14 @@ struct Engine final : EngineBase {
   ED* d_ed = nullptr;  // CSC order (edge-list order for programs with a scatter)
   ED* d_ed_csr = nullptr;  // CSR order (delta-gather programs: push relax)
   G* d_cand = nullptr;     // push-relax candidates per vertex (gatherZero between gathers)
+  // Blocked gather: qualifying programs, the graph's source-listed row blocks,
+  // the edge data in their order, and the vertex-indexed gather values.
+  static constexpr bool kBlocked = std::is_same<G, int>::value && delta_gather_v<P> &&
+                                   !P::kGatherUsesDst && !P::kHasScatter;
+  const gr_blocked* blk = nullptr;
+  ED* d_ed_blk = nullptr;
+  G* d_gdense = nullptr;   // rows from blk->rows_end on stay gatherZero
+  bool dense_pending = false;  // apply reads d_gdense[v] (the last gather was blocked)
   source_value_t<P>* d_srcval = nullptr;  // P::sourceValue of every vertex (programs with one)
   // ---- out-of-core mode: edge arrays in pinned host memory, streamed per shard
   bool streaming = false;
@@ -1257,6 +1274,12 @@ struct Engine final : EngineBase {
     // nextIter returns while k_compact_write may still run: drain the stream
     // before its buffers go
     if (stream) (void)hipStreamSynchronize(stream);
+    for (void* p : {(void*)d_ed_blk, (void*)d_gdense})
+      if (p) (void)hipFree(p);
+    d_ed_blk = nullptr;
+    d_gdense = nullptr;
+    blk = nullptr;
+    dense_pending = false;
     void* ptrs[] = {d_hub_flags, d_srcval, d_cbase,    d_heavy,  d_vd,     d_vd_init,
                     d_vd_user,   d_ed,     d_ed_csr,   d_cand,   d_active, d_escan,
                     d_gtmp,      d_chg_v,  d_chg_e,    d_bitmap, d_chg_bits, d_chg_prev,
@@ -1492,7 +1515,7 @@ struct Engine final : EngineBase {
     n_applies = 0;
     iteration = 0;
     delta_chain = delta_ready = false;
-    fuse_launched = fused_pending = collect_pending = false;
+    fuse_launched = fused_pending = collect_pending = dense_pending = false;
     it_gather = it_apply = it_scatter = false;
     bound = true;
     return GR_OK;
@@ -1859,6 +1882,7 @@ struct Engine final : EngineBase {
   int gather() override {
     it_gather = true;
     GR_TRY(collect_flush());  // a previous gather whose apply did not run
+    dense_pending = false;
     if (!P::kHasGather || n_active == 0) return GR_OK;
     if (streaming) return gather_streamed();
     const int n = n_active, w = w_edges;
@@ -1885,6 +1909,10 @@ struct Engine final : EngineBase {
         if (delta_mode != 1 || share < kPushMaxChangedShare) dbits = d_chg_prev;
       }
     }
+    if constexpr (kBlocked) {
+      if (blocked_share > 0 && (double)w >= blocked_share * (double)g->nE && g->nE > 0)
+        return gather_blocked(n, w, dbits);
+    }
     const int tile = kNT * gather_vt;
     const int n_tiles = (int)(((int64_t)n + w + tile - 1) / tile);
     k_merge_partition<<<grid_1d((int64_t)n_tiles + 1, 256), 256, 0, stream>>>(d_escan, n, w,
@@ -1908,6 +1936,48 @@ struct Engine final : EngineBase {
     stats.tile_gather_rows += n;
     stats.tile_gather_edges += w;
     if (dbits && delta_ready) stats.delta_gathers += 1;
+    return GR_OK;
+  }
+
+  // Gather of every row through the graph's source-listed row blocks
+  // (k_gather_blocked); apply reads the active rows' values from d_gdense.
+  // The blocks, the edge data in their order and d_gdense are set up on the
+  // first such gather of a binding.
+  int gather_blocked(int n, int w, const uint32_t* dbits) {
+    if constexpr (kBlocked) {
+      if (!blk) {
+        GR_TRY(graph_blocked(g, &blk));
+        GR_TRY(dalloc(&d_gdense, g->nV));
+        k_fill<G><<<1024, 256, 0, stream>>>(g->nV, P::gatherZero, d_gdense);
+        GR_HIP(hipGetLastError());
+        if constexpr (!kEdgeEmpty) {
+          static_assert(sizeof(ED) % 4 == 0, "edge data is moved as 4-byte words");
+          GR_TRY(dalloc(&d_ed_blk, g->nE));
+          k_permute_words<<<grid_1d(g->nE, 256), 256, 0, stream>>>(
+              reinterpret_cast<const uint32_t*>(d_ed), blk->bpos, g->nE, (int)(sizeof(ED) / 4),
+              reinterpret_cast<uint32_t*>(d_ed_blk));
+          GR_HIP(hipGetLastError());
+        }
+      }
+      tick();
+      if (blk->n_blocks > 0)
+        k_gather_blocked<P><<<blk->n_blocks, kBlkNT, 0, stream>>>(
+            blk->brow, blk->bnr, blk->boff, blk->bsrc, blk->brl, d_ed_blk, d_vd, d_srcval, dbits,
+            d_gdense);
+      tock(kPhGather);
+      GR_HIP(hipGetLastError());
+      dense_pending = true;
+      stats.active_vertices += n;
+      stats.gathered_edges += w;
+      stats.tile_gather_rows += n;
+      stats.tile_gather_edges += w;
+      stats.blocked_gathers += 1;
+      if (dbits && delta_ready) stats.delta_gathers += 1;
+    } else {
+      (void)n;
+      (void)w;
+      (void)dbits;
+    }
     return GR_OK;
   }
 
@@ -2007,12 +2077,14 @@ struct Engine final : EngineBase {
     it_apply = true;
     if (n_active == 0) return GR_OK;
     ++n_applies;
-    const int use_gather = !P::kHasGather ? 0 : collect_pending ? 2 : 1;
+    const int use_gather = !P::kHasGather ? 0 : collect_pending ? 2 : dense_pending ? 3 : 1;
     collect_pending = false;
+    dense_pending = false;
     tick();
     k_apply<P><<<grid_1d(n_active, kApplyChunk), kApplyNT, 0, stream>>>(
         n_active, d_active, d_gtmp, use_gather, d_vd, g->off_out, iteration, d_chg_v, d_chg_e,
-        changed_bits_on() ? d_chg_bits : nullptr, d_ctr, d_srcval, d_cand, d_applied);
+        changed_bits_on() ? d_chg_bits : nullptr, d_ctr, d_srcval,
+        use_gather == 3 ? d_gdense : d_cand, d_applied);
     tock(kPhApply);
     GR_HIP(hipGetLastError());
     return GR_OK;
@@ -2457,6 +2529,13 @@ extern "C" int gr_engine_set_direction(gr_engine* e, int mode, double alpha) {
   if (mode < 0 || mode > 2) return set_error(GR_ERR_INVALID, "direction mode must be 0, 1 or 2");
   e->impl->dir_mode = mode;
   if (alpha > 0) e->impl->dir_alpha = alpha;
+  return GR_OK;
+}
+
+extern "C" int gr_engine_set_blocked_gather(gr_engine* e, double min_share) {
+  GR_TRY(check(e, false));
+  if (!(min_share == min_share)) return set_error(GR_ERR_INVALID, "blocked-gather share is NaN");
+  e->impl->blocked_share = min_share;
   return GR_OK;
 }
 

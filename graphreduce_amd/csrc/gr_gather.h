@@ -483,6 +483,74 @@ __global__ void k_gather_fixup(int n_tiles, const Part<typename P::GatherResult>
   gtmp[r] = finalize<P>(has, acc);
 }
 
+// ------------------------------------------------------- blocked gather
+//
+// Gather of every row by row blocks (gr_blocked): one workgroup per block
+// keeps the block's row values in LDS and streams the block's in-edges, which
+// are listed by source, so that the random source reads of all workgroups
+// advance through the vertex ids together and mostly hit the same cache
+// lines.  For programs whose gatherReduce is an int minimum that gatherMap
+// only feeds from the source and the edge (the delta-gather programs): the
+// row value starts at gatherZero and falls by LDS atomicMin, which equals the
+// reference's fold of gatherReduce from gatherZero over the in-edges
+// (refgas.h:141-160) in any order.  delta_bits (optional) skips the sources
+// outside the bitmap as k_gather_tiles does.  Writes gdense[row] for every row
+// of every block (vertex-indexed).
+constexpr int kBlkNT = 256;
+constexpr int kBlkVT = 8;
+
+template <class P>
+__global__ __launch_bounds__(kBlkNT) void k_gather_blocked(
+    const int32_t* __restrict__ brow, const int32_t* __restrict__ bnr,
+    const int32_t* __restrict__ boff, const int32_t* __restrict__ bsrc,
+    const uint16_t* __restrict__ brl, const typename P::EdgeData* __restrict__ ed,
+    const typename P::VertexData* __restrict__ vd, const source_value_t<P>* __restrict__ srcval,
+    const uint32_t* __restrict__ delta_bits, typename P::GatherResult* __restrict__ gdense) {
+  typedef typename P::GatherResult G;
+  static_assert(std::is_same<G, int>::value && delta_gather_v<P> && !P::kGatherUsesDst,
+                "the blocked gather folds int minima of source-only gatherMaps");
+  __shared__ int acc[kBlockRows];
+  const int b = blockIdx.x;
+  const int r0 = brow[b], nr = bnr[b];
+  const int e0 = boff[b], e1 = boff[b + 1];
+  const int tid = threadIdx.x;
+  for (int r = tid; r < nr; r += kBlkNT) acc[r] = P::gatherZero;
+  __syncthreads();
+  for (int base = e0; base < e1; base += kBlkNT * kBlkVT) {
+    int src[kBlkVT], row[kBlkVT];
+    typename P::EdgeData ev[kBlkVT];
+#pragma unroll
+    for (int m = 0; m < kBlkVT; ++m) {
+      const int e = base + m * kBlkNT + tid;
+      src[m] = -1;
+      row[m] = 0;
+      if (e < e1) {
+        src[m] = __builtin_nontemporal_load(bsrc + e);
+        row[m] = brl[e];
+        ev[m] = stream_load(ed + e);
+      }
+    }
+    if (delta_bits) {
+      uint32_t word[kBlkVT];
+#pragma unroll
+      for (int m = 0; m < kBlkVT; ++m) word[m] = src[m] >= 0 ? delta_bits[src[m] >> 5] : 0u;
+#pragma unroll
+      for (int m = 0; m < kBlkVT; ++m)
+        if (!((word[m] >> (src[m] & 31)) & 1u)) src[m] = -1;
+    }
+    G val[kBlkVT];
+#pragma unroll
+    for (int m = 0; m < kBlkVT; ++m)
+      val[m] = src[m] >= 0 ? map_source<P>(vd, vd, srcval, src[m], &ev[m]) : P::gatherZero;
+    // a value not below gatherZero cannot lower a row that starts there
+#pragma unroll
+    for (int m = 0; m < kBlkVT; ++m)
+      if (src[m] >= 0 && val[m] < P::gatherZero) atomicMin(&acc[row[m]], val[m]);
+  }
+  __syncthreads();
+  for (int r = tid; r < nr; r += kBlkNT) gdense[r0 + r] = acc[r];
+}
+
 }  // namespace gr
 
 #endif

@@ -182,6 +182,13 @@ int finish_graph(gr_graph* g) {
 int graph_free(gr_graph* g) {
   if (!g) return GR_OK;
   if (g->ordered) graph_free(g->ordered);
+  if (g->blocked) {
+    gr_blocked* b = g->blocked;
+    for (void* p : {(void*)b->brow, (void*)b->bnr, (void*)b->boff, (void*)b->bsrc,
+                    (void*)b->brl, (void*)b->bpos})
+      if (p) (void)hipFree(p);
+    delete b;
+  }
   for (int32_t* p : {g->off_out, g->dsts, g->off_in, g->srcs, g->csr_eid, g->csc_eid, g->perm,
                      g->inv})
     if (p) (void)hipFree(p);
@@ -296,6 +303,120 @@ int graph_ordered(gr_graph* g, gr_graph** out) {
   g->perm = perm.release();
   g->inv = inv.release();
   *out = o;
+  return GR_OK;
+}
+
+namespace {
+
+// key[j] = block of CSC position j's row << shift | its source; val[j] = j.
+__global__ void k_block_keys(int64_t nE, const int32_t* __restrict__ rowid,
+                             const int32_t* __restrict__ row_block,
+                             const int32_t* __restrict__ srcs, int shift,
+                             uint64_t* __restrict__ keys, int32_t* __restrict__ vals) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nE) return;
+  keys[j] = ((uint64_t)(uint32_t)row_block[rowid[j]] << shift) | (uint32_t)srcs[j];
+  vals[j] = (int32_t)j;
+}
+
+// Sorted keys -> source, row within its block, CSC position.
+__global__ void k_block_split(int64_t nE, const uint64_t* __restrict__ keys,
+                              const int32_t* __restrict__ vals, int shift,
+                              const int32_t* __restrict__ rowid, const int32_t* __restrict__ brow,
+                              int32_t* __restrict__ bsrc, uint16_t* __restrict__ brl,
+                              int32_t* __restrict__ bpos) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nE) return;
+  const uint64_t key = keys[k];
+  const int32_t j = vals[k];
+  bsrc[k] = (int32_t)(key & ((1ull << shift) - 1));
+  brl[k] = (uint16_t)(rowid[j] - brow[(int32_t)(key >> shift)]);
+  bpos[k] = j;
+}
+
+std::mutex g_block_mu;
+
+}  // namespace
+
+int graph_blocked(gr_graph* g, const gr_blocked** out) {
+  std::lock_guard<std::mutex> lk(g_block_mu);
+  if (g->blocked) {
+    *out = g->blocked;
+    return GR_OK;
+  }
+  const int32_t nV = g->nV;
+  const int64_t nE = g->nE;
+  const std::vector<int32_t>& off = g->h_off_in;
+  // blocks of consecutive rows: at most kBlockRows rows, at most kBlockEdges
+  // in-edges unless a single row has more
+  int32_t last = nV;
+  while (last > 0 && off[last] == off[last - 1]) --last;
+  std::vector<int32_t> brow, bnr, boff, row_block((size_t)std::max(nV, 1), 0);
+  for (int32_t r = 0; r < last;) {
+    const int64_t target = (int64_t)off[r] + kBlockEdges;
+    int32_t r1 = (int32_t)(std::upper_bound(off.begin() + r + 1, off.begin() + last + 1,
+                                            (int32_t)std::min<int64_t>(target, INT32_MAX)) -
+                           off.begin()) - 1;
+    r1 = std::min(std::min(r1, r + kBlockRows), last);
+    if (r1 <= r) r1 = r + 1;
+    for (int32_t v = r; v < r1; ++v) row_block[v] = (int32_t)brow.size();
+    brow.push_back(r);
+    bnr.push_back(r1 - r);
+    boff.push_back(off[r]);
+    r = r1;
+  }
+  boff.push_back(off[last]);
+  const int32_t nb = (int32_t)brow.size();
+  gr_blocked* b = new gr_blocked();
+  b->n_blocks = nb;
+  b->rows_end = last;
+  DevBuf<int32_t> d_brow, d_bnr, d_boff, d_bsrc, d_bpos, d_rowblk;
+  DevBuf<uint16_t> d_brl;
+  int rc = GR_OK;
+  auto fail = [&](int code) {
+    delete b;
+    return code;
+  };
+  if ((rc = d_brow.alloc(nb)) || (rc = d_bnr.alloc(nb)) || (rc = d_boff.alloc(nb + 1)) ||
+      (rc = d_bsrc.alloc(nE)) || (rc = d_brl.alloc(nE)) || (rc = d_bpos.alloc(nE)) ||
+      (rc = d_rowblk.alloc(nV)))
+    return fail(rc);
+  if (nb > 0) {
+    if (hipMemcpy(d_brow.p, brow.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice) ||
+        hipMemcpy(d_bnr.p, bnr.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice))
+      return fail(set_error(GR_ERR_HIP, "graph_blocked: block table upload"));
+  }
+  if (hipMemcpy(d_boff.p, boff.data(), sizeof(int32_t) * (nb + 1), hipMemcpyHostToDevice) ||
+      (nV > 0 && hipMemcpy(d_rowblk.p, row_block.data(), sizeof(int32_t) * nV,
+                           hipMemcpyHostToDevice)))
+    return fail(set_error(GR_ERR_HIP, "graph_blocked: block table upload"));
+  if (nE > 0) {
+    const int shift = bits_for(nV);
+    DevBuf<uint64_t> keys, ksorted;
+    DevBuf<int32_t> vals, vsorted, rowid;
+    if ((rc = keys.alloc(nE)) || (rc = ksorted.alloc(nE)) || (rc = vals.alloc(nE)) ||
+        (rc = vsorted.alloc(nE)) || (rc = rowid.alloc(nE)))
+      return fail(rc);
+    if ((rc = row_ids(nV, nE, g->off_in, rowid.p))) return fail(rc);
+    k_block_keys<<<grid_for(nE), kBlock>>>(nE, rowid.p, d_rowblk.p, g->srcs, shift, keys.p,
+                                           vals.p);
+    if (hipGetLastError() != hipSuccess) return fail(set_error(GR_ERR_HIP, "k_block_keys"));
+    if ((rc = radix_sort_pairs(keys.p, ksorted.p, vals.p, vsorted.p, nE, shift + bits_for(nb))))
+      return fail(rc);
+    k_block_split<<<grid_for(nE), kBlock>>>(nE, ksorted.p, vsorted.p, shift, rowid.p, d_brow.p,
+                                            d_bsrc.p, d_brl.p, d_bpos.p);
+    if (hipGetLastError() != hipSuccess) return fail(set_error(GR_ERR_HIP, "k_block_split"));
+    if (hipDeviceSynchronize() != hipSuccess)
+      return fail(set_error(GR_ERR_HIP, "blocked CSC build failed"));
+  }
+  b->brow = d_brow.release();
+  b->bnr = d_bnr.release();
+  b->boff = d_boff.release();
+  b->bsrc = d_bsrc.release();
+  b->brl = d_brl.release();
+  b->bpos = d_bpos.release();
+  g->blocked = b;
+  *out = b;
   return GR_OK;
 }
 

@@ -266,6 +266,16 @@ int gr_engine_set_streaming(gr_engine* e, int64_t max_edges_per_shard);
  * way.  Negative or NaN shares: GR_ERR_INVALID. */
 int gr_engine_set_fuse_share(gr_engine* e, double share);
 
+/* Blocked gather (programs whose gather folds int minima of the source value
+ * and the edge: SSSP, CC): a gather over at least min_share * E in-edges is
+ * done over row blocks of the CSC whose in-edges are listed by source
+ * (every row of the graph, one workgroup per block of <= 16384 rows and about
+ * 2^18 in-edges, the rows' values kept in LDS), replacing the merge-path tile
+ * kernel over the active rows (gpugas.h:714-786, gpugas_kernels.cuh:109-197).
+ * The minima are the same; the blocks are built once per graph on first use.
+ * Default 0.5; <= 0 turns it off. */
+int gr_engine_set_blocked_gather(gr_engine* e, double min_share);
+
 /* Delta gather (no reference counterpart).  For SSSP and CC, once an
  * iteration with every vertex active has run (setActive(0, V)) and every
  * later active set came from scatterActivate, gather may read only the
@@ -317,6 +327,9 @@ typedef struct gr_engine_stats {
                                 scatterActivate (counted in push_gathers too) */
   int64_t fused_edges;       /* out-edges relaxed by those fused passes (also in
                                 push_edges; their time is in push_ms) */
+  int64_t blocked_gathers;   /* gathers done by the blocked kernel (counted in
+                                tile_gather_rows / tile_gather_edges, timed in
+                                gather_ms) */
 } gr_engine_stats;
 
 /* Enables hipEvent timing of the engine phases on the engine's stream. */

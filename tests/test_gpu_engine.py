@@ -875,3 +875,87 @@ def test_push_gather_values_reach_apply_once(gpu, prog_name):
     assert np.array_equal(eng.vertexData, ref) and trips == rtrips
     assert st["push_gathers"] > 0
     eng.close()
+
+
+# ------------------------------------------------------------ blocked gather
+
+def _run_min_program(gr, prog, graph, n, w_dev, src, share, delta, order=1):
+    if prog == gr.SSSP:
+        vd = np.full(n, gr.SSSP_INF, np.int32)
+        vd[src] = 0
+    else:
+        vd = np.arange(n, dtype=np.int32)
+    eng = gr.GASEngine(prog)
+    eng.setVertexOrder(order)
+    eng.setDeltaGather(delta)
+    eng.setBlockedGather(share)
+    eng.bindGraph(graph, vd, w_dev if prog == gr.SSSP else None)
+    eng.setActive(0, n)
+    trips = eng.run()
+    st = eng.stats()
+    out = eng.readVertexData().copy()
+    eng.close()
+    return out, trips, st
+
+
+@pytest.mark.parametrize("delta", [0, 1, 2])
+@pytest.mark.parametrize("order", [0, 1])
+def test_blocked_gather_equals_oracle(gpu, delta, order):
+    """The source-listed row-block gather gives the oracle's distances,
+    labels, trip counts and work counters: off (share 0), for the dense
+    gathers (0.5) and for every gather that reaches the tile path (1e-9)."""
+    gr = gpu
+    n, s, d = _rmat(gr, 15, seed=2)
+    w = gr.hash_weights(len(s), seed=2)
+    src = _max_out_src(n, s)
+    ref, rtrips, rst = oracle.sssp(n, s, d, w, src)
+    g = gr.Graph.from_edges(n, s, d)
+    wd = gr.upload_i32(w)
+    for share in (0.0, 0.5, 1e-9):
+        dist, trips, st = _run_min_program(gr, gr.SSSP, g, n, wd, src, share, delta, order)
+        assert np.array_equal(dist, ref) and trips == rtrips, f"share {share}"
+        assert st["gathered_edges"] == rst["in_edges"]
+        if share == 0.0:
+            assert st["blocked_gathers"] == 0
+        elif delta == 0 and share < 0.01:
+            assert st["blocked_gathers"] == trips  # every gather is a full one
+        else:
+            assert st["blocked_gathers"] > 0
+    g.close()
+    ns, ss, ds = _rmat(gr, 14, seed=4, sym=True)
+    lref, ltrips, _ = oracle.cc(ns, ss, ds)
+    gs = gr.Graph.from_edges(ns, ss, ds)
+    for share in (0.0, 0.5, 1e-9):
+        lab, trips, st = _run_min_program(gr, gr.CC, gs, ns, None, 0, share, delta, order)
+        assert np.array_equal(lab, lref) and trips == ltrips, f"CC share {share}"
+        assert (st["blocked_gathers"] > 0) == (share > 0)
+    gs.close()
+
+
+def test_blocked_gather_giant_row_and_sparse_tail(gpu):
+    """A row with more in-edges than one block holds (2^18), parallel arcs,
+    self-loops, vertices without in-edges in the middle and at the end."""
+    gr = gpu
+    rng = np.random.default_rng(8)
+    n = 400_000
+    hub_in = rng.integers(1, n, 300_000).astype(np.int32)  # 300k arcs into vertex 0
+    s = np.concatenate([hub_in, rng.integers(0, n // 2, 600_000).astype(np.int32),
+                        np.arange(0, n // 2, 7, dtype=np.int32)])
+    d = np.concatenate([np.zeros(len(hub_in), np.int32),
+                        rng.integers(0, n // 2, 600_000).astype(np.int32),
+                        np.arange(0, n // 2, 7, dtype=np.int32)])
+    w = rng.integers(1, 65, len(s)).astype(np.int32)
+    src = int(hub_in[0])
+    ref, rtrips, _ = oracle.sssp(n, s, d, w, src)
+    g = gr.Graph.from_edges(n, s, d)
+    wd = gr.upload_i32(w)
+    for order in (0, 1):
+        dist, trips, st = _run_min_program(gr, gr.SSSP, g, n, wd, src, 1e-9, 0, order)
+        assert np.array_equal(dist, ref) and trips == rtrips
+        assert st["blocked_gathers"] == trips
+    lref, ltrips, _ = oracle.cc(n, np.concatenate([s, d]), np.concatenate([d, s]))
+    gs = gr.Graph.from_edges(n, np.concatenate([s, d]), np.concatenate([d, s]))
+    lab, trips, _ = _run_min_program(gr, gr.CC, gs, n, None, 0, 1e-9, 1, 1)
+    assert np.array_equal(lab, lref) and trips == ltrips
+    g.close()
+    gs.close()
