@@ -180,6 +180,7 @@ _SIGS = {
     "gr_engine_set_streaming": (C.c_int, [_P, C.c_int64]),
     "gr_engine_set_fuse_share": (C.c_int, [_P, C.c_double]),
     "gr_engine_set_blocked_gather": (C.c_int, [_P, C.c_double]),
+    "gr_engine_set_push_share": (C.c_int, [_P, C.c_double]),
     "gr_engine_enable_timing": (C.c_int, [_P, C.c_int]),
     "gr_engine_get_stats": (C.c_int, [_P, C.POINTER(_Stats)]),
     "gr_engine_reset_stats": (C.c_int, [_P]),
@@ -674,6 +675,10 @@ class GASEngine:
     def setDirection(self, mode: int = 0, alpha: float = 15.0) -> None:
         """0 = automatic push/pull, 1 = push only, 2 = pull only."""
         _check(lib().gr_engine_set_direction(self._h, mode, alpha))
+
+    def setPushShare(self, share: float = 0.6) -> None:
+        """Adaptive delta gather: push below this share of changed in-edges."""
+        _check(lib().gr_engine_set_push_share(self._h, float(share)))
 
     def setBlockedGather(self, min_share: float = 0.5) -> None:
         """Gathers over at least min_share * E in-edges run on the graph's

@@ -84,10 +84,11 @@ __device__ __forceinline__ T block_excl_scan_add(T v, T* scratch, T* total) {
 
 // Row blocks of the CSC whose in-edges are listed by source (for the blocked
 // gather): block b holds rows [brow[b], brow[b] + bnr[b]) -- at most
-// gr::kBlockRows rows and, unless one row alone has more, gr::kBlockEdges
-// in-edges -- and their in-edges are positions [boff[b], boff[b + 1]) of
+// gr::kBlockRows rows and, unless one row alone has more, the edge target
+// below -- and their in-edges are positions [boff[b], boff[b + 1]) of
 // bsrc / brl / bpos, sorted by source (stable).  Rows from rows_end on have
-// no in-edges and belong to no block.
+// no in-edges and belong to no block.  The edge target per block is
+// E / (2 * compute units), at most kBlockEdges and at least 4096.
 struct gr_blocked {
   int32_t n_blocks = 0;
   int32_t rows_end = 0;
@@ -120,8 +121,8 @@ struct gr_graph {
 };
 
 namespace gr {
-constexpr int kBlockRows = 16384;       // rows per gather block (LDS: 64 KB of int)
-constexpr int kBlockEdges = 1 << 18;    // in-edges per gather block (target)
+constexpr int kBlockRows = 32768;       // rows per gather block (LDS: 128 KB of int)
+constexpr int kBlockEdges = 1 << 19;    // in-edges per gather block (largest target)
 int graph_free(gr_graph* g);
 // The degree-ordered copy of g (cached in g; freed with g).
 int graph_ordered(gr_graph* g, gr_graph** out);

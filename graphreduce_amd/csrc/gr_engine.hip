@@ -1140,6 +1140,9 @@ struct EngineBase {
   // blocked gather (programs that qualify) for gathers over at least this
   // share of E; <= 0: never (gr_engine_set_blocked_gather)
   double blocked_share = 0.5;
+  // adaptive delta gather: push the changed out-edges below this share of the
+  // gather's in-edges (gr_engine_set_push_share)
+  double push_share = 0.6;
   bool timing = false;
   gr_engine_stats stats{};
 };
@@ -1231,7 +1234,7 @@ struct Engine final : EngineBase {
   // Delta gather (programs with kDeltaGather): valid once an all-active
   // iteration has run and every later active set came from scatterActivate.
   static constexpr bool kDelta = delta_gather_v<P>;
-  // 0 off; 1 adaptive (push below kPushMaxChangedShare, else full); 2 delta
+  // 0 off; 1 adaptive (push below push_share, else full); 2 delta
   // pull whenever valid; 3 push whenever valid
   int delta_mode = 1;
   int64_t chg_count = 0, chg_edges = 0;  // changed list of the previous iteration
@@ -1244,7 +1247,6 @@ struct Engine final : EngineBase {
   // above it (1.8 ms vs 1.85 push and 2.1 pull-delta at 0.65; 1.8 vs 2.4 and
   // 2.5 at 0.94); the pull-delta gather is never the fastest there.
   double changed_share = 1.0;
-  static constexpr double kPushMaxChangedShare = 0.6;
   // Fused activation + push gather when the changed vertices' out-edges are at
   // most fuse_share of E (adaptive mode; gr_engine_set_fuse_share): below it,
   // the separate activation (pull scans of whole rows, or a push) plus the
@@ -1893,9 +1895,12 @@ struct Engine final : EngineBase {
         return gather_collect();
       }
       if (delta_ready) {
-        if (delta_mode == 3 || (delta_mode == 1 && changed_share < kPushMaxChangedShare))
+        if (delta_mode == 3 || (delta_mode == 1 && changed_share < push_share))
           return gather_push();
         if (delta_mode == 2) dbits = d_chg_prev;
+        // the blocked gather reads the changed bitmap in source order, at
+        // little cost, and skips the unchanged sources' values and edge data
+        if (delta_mode == 1 && blocked_gather_applies(w)) dbits = d_chg_prev;
       } else if (kInert && inert_ok && delta_mode != 0) {
         // The non-inert vertices stand in for the changed set: the gather's
         // values are those of the full gather, so an all-active start still
@@ -1903,16 +1908,13 @@ struct Engine final : EngineBase {
         // (a pushed candidate is collected and reset by its active row only).
         GR_TRY(inert_scan());
         const double share = w > 0 ? (double)chg_edges / (double)w : 1.0;
-        if (n == g->nV &&
-            (delta_mode == 3 || (delta_mode == 1 && share < kPushMaxChangedShare)))
+        if (n == g->nV && (delta_mode == 3 || (delta_mode == 1 && share < push_share)))
           return gather_push();
-        if (delta_mode != 1 || share < kPushMaxChangedShare) dbits = d_chg_prev;
+        if (delta_mode != 1 || share < push_share || blocked_gather_applies(w))
+          dbits = d_chg_prev;
       }
     }
-    if constexpr (kBlocked) {
-      if (blocked_share > 0 && (double)w >= blocked_share * (double)g->nE && g->nE > 0)
-        return gather_blocked(n, w, dbits);
-    }
+    if (blocked_gather_applies(w)) return gather_blocked(n, w, dbits);
     const int tile = kNT * gather_vt;
     const int n_tiles = (int)(((int64_t)n + w + tile - 1) / tile);
     k_merge_partition<<<grid_1d((int64_t)n_tiles + 1, 256), 256, 0, stream>>>(d_escan, n, w,
@@ -1937,6 +1939,11 @@ struct Engine final : EngineBase {
     stats.tile_gather_edges += w;
     if (dbits && delta_ready) stats.delta_gathers += 1;
     return GR_OK;
+  }
+
+  bool blocked_gather_applies(int w) const {
+    return kBlocked && blocked_share > 0 && g->nE > 0 &&
+           (double)w >= blocked_share * (double)g->nE;
   }
 
   // Gather of every row through the graph's source-listed row blocks
@@ -2529,6 +2536,13 @@ extern "C" int gr_engine_set_direction(gr_engine* e, int mode, double alpha) {
   if (mode < 0 || mode > 2) return set_error(GR_ERR_INVALID, "direction mode must be 0, 1 or 2");
   e->impl->dir_mode = mode;
   if (alpha > 0) e->impl->dir_alpha = alpha;
+  return GR_OK;
+}
+
+extern "C" int gr_engine_set_push_share(gr_engine* e, double share) {
+  GR_TRY(check(e, false));
+  if (!(share >= 0.0)) return set_error(GR_ERR_INVALID, "push share must be >= 0");
+  e->impl->push_share = share;
   return GR_OK;
 }
 

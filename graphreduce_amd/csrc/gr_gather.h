@@ -496,7 +496,7 @@ __global__ void k_gather_fixup(int n_tiles, const Part<typename P::GatherResult>
 // (refgas.h:141-160) in any order.  delta_bits (optional) skips the sources
 // outside the bitmap as k_gather_tiles does.  Writes gdense[row] for every row
 // of every block (vertex-indexed).
-constexpr int kBlkNT = 256;
+constexpr int kBlkNT = 1024;  // one workgroup per CU (128 KB of LDS): 16 waves
 constexpr int kBlkVT = 8;
 
 template <class P>
@@ -527,7 +527,6 @@ __global__ __launch_bounds__(kBlkNT) void k_gather_blocked(
       if (e < e1) {
         src[m] = __builtin_nontemporal_load(bsrc + e);
         row[m] = brl[e];
-        ev[m] = stream_load(ed + e);
       }
     }
     if (delta_bits) {
@@ -538,6 +537,10 @@ __global__ __launch_bounds__(kBlkNT) void k_gather_blocked(
       for (int m = 0; m < kBlkVT; ++m)
         if (!((word[m] >> (src[m] & 31)) & 1u)) src[m] = -1;
     }
+    // edge data only for the edges that count
+#pragma unroll
+    for (int m = 0; m < kBlkVT; ++m)
+      if (src[m] >= 0) ev[m] = stream_load(ed + base + m * kBlkNT + tid);
     G val[kBlkVT];
 #pragma unroll
     for (int m = 0; m < kBlkVT; ++m)

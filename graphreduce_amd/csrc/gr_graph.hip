@@ -347,13 +347,23 @@ int graph_blocked(gr_graph* g, const gr_blocked** out) {
   const int32_t nV = g->nV;
   const int64_t nE = g->nE;
   const std::vector<int32_t>& off = g->h_off_in;
-  // blocks of consecutive rows: at most kBlockRows rows, at most kBlockEdges
+  // blocks of consecutive rows: at most kBlockRows rows, at most edge_target
   // in-edges unless a single row has more
   int32_t last = nV;
   while (last > 0 && off[last] == off[last - 1]) --last;
+  // about two blocks per compute unit: the blocks of one launch then share the
+  // sweep through the source ids (measured at RMAT-24: 686 blocks of <= 2^19
+  // in-edges 0.74 ms, 1376 of <= 2^18 0.91 ms, 459 of <= 2^20 0.84 ms)
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus < 1)
+    cus = 256;
+  const int64_t edge_target =
+      std::max<int64_t>(4096, std::min<int64_t>(kBlockEdges, nE / (2 * (int64_t)cus)));
   std::vector<int32_t> brow, bnr, boff, row_block((size_t)std::max(nV, 1), 0);
   for (int32_t r = 0; r < last;) {
-    const int64_t target = (int64_t)off[r] + kBlockEdges;
+    const int64_t target = (int64_t)off[r] + edge_target;
     int32_t r1 = (int32_t)(std::upper_bound(off.begin() + r + 1, off.begin() + last + 1,
                                             (int32_t)std::min<int64_t>(target, INT32_MAX)) -
                            off.begin()) - 1;

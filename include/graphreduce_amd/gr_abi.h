@@ -269,8 +269,9 @@ int gr_engine_set_fuse_share(gr_engine* e, double share);
 /* Blocked gather (programs whose gather folds int minima of the source value
  * and the edge: SSSP, CC): a gather over at least min_share * E in-edges is
  * done over row blocks of the CSC whose in-edges are listed by source
- * (every row of the graph, one workgroup per block of <= 16384 rows and about
- * 2^18 in-edges, the rows' values kept in LDS), replacing the merge-path tile
+ * (every row of the graph, one workgroup per block of <= 32768 rows and about
+ * E / (2 * compute units) in-edges, at most 2^19, the rows' values kept in
+ * LDS), replacing the merge-path tile
  * kernel over the active rows (gpugas.h:714-786, gpugas_kernels.cuh:109-197).
  * The minima are the same; the blocks are built once per graph on first use.
  * Default 0.5; <= 0 turns it off. */
@@ -289,9 +290,14 @@ int gr_engine_set_blocked_gather(gr_engine* e, double min_share);
  * vertices lower a per-vertex candidate with atomicMin, read back per active
  * row (the same minimum over the same edges).
  * mode 0: always gather every in-edge; 1 (default): push when under 60 % of
- * the in-edges come from changed sources, else the full gather; 2: pull-delta
- * whenever valid; 3: push whenever valid. */
+ * the in-edges come from changed sources (gr_engine_set_push_share), else
+ * the full gather (the blocked gather, when it applies, skipping the
+ * unchanged sources); 2: pull-delta whenever valid; 3: push whenever valid. */
 int gr_engine_set_delta_gather(gr_engine* e, int mode);
+
+/* Mode 1 of gr_engine_set_delta_gather pushes when under `share` of the
+ * gather's in-edges come from changed sources (default 0.6). */
+int gr_engine_set_push_share(gr_engine* e, double share);
 
 /* Work and timing counters, summed since the last reset. */
 typedef struct gr_engine_stats {
