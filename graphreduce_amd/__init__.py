@@ -855,12 +855,15 @@ MSBFS_STREAM_PULL, MSBFS_PERSISTENT, MSBFS_FILTER = 4, 8, 16
 
 
 def msbfs(graph: Graph, sources, direction: int = 0, alpha: float = 20.0,
-          stream_pull: bool = False, persistent: bool = False, pull_filter: bool = False):
+          stream_pull: bool = False, persistent: bool = False, pull_filter: bool = False,
+          out: "DeviceArray | None" = None):
     """BFS from up to 64 sources in one batch (one expand + one update launch
     per level; persistent=True: every level in one cooperative launch).
 
     stream_pull asks for the streaming pull kernel, which the library
-    currently refuses (GRError).  Returns (depth int32[len(sources), V], info)."""
+    currently refuses (GRError).  Returns (depth int32[len(sources), V], info);
+    with `out` (a device array of len(sources) * V int32) the depths stay
+    there and None is returned in their place."""
     if stream_pull:
         direction |= MSBFS_STREAM_PULL
     if persistent:
@@ -870,6 +873,13 @@ def msbfs(graph: Graph, sources, direction: int = 0, alpha: float = 20.0,
     src = _batch_sources(sources)
     n = graph.n_vertices
     nbytes = 4 * n * len(src)
+    if out is not None:
+        if out.nbytes < nbytes:
+            raise GRError(1, "msbfs: output array too small")
+        r = _BatchResult()
+        _check(lib().gr_msbfs(graph.handle, _p32(src), len(src), out.ptr, direction, alpha,
+                              C.byref(r)))
+        return None, {name: getattr(r, name) for name, _ in _BatchResult._fields_}
     d = C.c_void_p()
     _check(lib().gr_device_alloc(nbytes, C.byref(d)))
     try:
@@ -882,14 +892,22 @@ def msbfs(graph: Graph, sources, direction: int = 0, alpha: float = 20.0,
     return out, {name: getattr(r, name) for name, _ in _BatchResult._fields_}
 
 
-def mssp(graph: Graph, weights: "DeviceArray", sources):
+def mssp(graph: Graph, weights: "DeviceArray", sources, out: "DeviceArray | None" = None):
     """SSSP from up to 64 sources (64 lanes per wave, hipGraph-replayed rounds).
 
     weights: DeviceArray of int32 per edge in the graph's edge-list order.
-    Returns (dist int32[len(sources), V], info)."""
+    Returns (dist int32[len(sources), V], info); with `out` the distances stay
+    in that device array and None is returned in their place."""
     src = _batch_sources(sources)
     n = graph.n_vertices
     nbytes = 4 * n * len(src)
+    if out is not None:
+        if out.nbytes < nbytes:
+            raise GRError(1, "mssp: output array too small")
+        r = _BatchResult()
+        _check(lib().gr_mssp(graph.handle, weights.ptr, _p32(src), len(src), out.ptr,
+                             C.byref(r)))
+        return None, {name: getattr(r, name) for name, _ in _BatchResult._fields_}
     d = C.c_void_p()
     _check(lib().gr_device_alloc(nbytes, C.byref(d)))
     try:
@@ -900,6 +918,22 @@ def mssp(graph: Graph, weights: "DeviceArray", sources):
     finally:
         lib().gr_device_free(d)
     return out, {name: getattr(r, name) for name, _ in _BatchResult._fields_}
+
+
+def device_empty(nbytes: int) -> DeviceArray:
+    """An uninitialised device allocation of nbytes."""
+    d = C.c_void_p()
+    _check(lib().gr_device_alloc(max(1, nbytes), C.byref(d)))
+    return DeviceArray(d, nbytes)
+
+
+def download(arr: DeviceArray, dtype, shape) -> np.ndarray:
+    """Copies a device array to a new host array of the given dtype / shape."""
+    out = np.empty(shape, dtype)
+    if out.nbytes > arr.nbytes:
+        raise GRError(1, "download: host array larger than the device array")
+    _check(lib().gr_memcpy_d2h(out.ctypes.data, arr.ptr, out.nbytes))
+    return out
 
 
 def upload_i32(a) -> DeviceArray:
