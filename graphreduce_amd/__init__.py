@@ -155,6 +155,8 @@ _SIGS = {
     "gr_graph_hash_weights_device": (C.c_int, [_P, C.c_uint64, C.c_int32, C.c_int,
                                                 C.POINTER(_P)]),
     "gr_engine_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    # for C++ callers of gas_program.hpp (an engine object from the headers)
+    "gr_engine_adopt": (C.c_int, [_P, C.POINTER(_P)]),
     "gr_engine_destroy": (C.c_int, [_P]),
     "gr_engine_set_graph": (C.c_int, [_P, C.c_int32, _P, C.c_size_t, C.c_int32, _P, C.c_size_t,
                                        _I32P, _I32P]),

@@ -122,6 +122,32 @@ def build_apps(verbose: bool = False) -> list[Path]:
     return outs
 
 
+HIP_APP_SOURCES = ["widest_path.hip"]
+
+
+def build_hip_apps(verbose: bool = False) -> list[Path]:
+    """Apps with a vertex program of their own: compiled with hipcc against
+    include/graphreduce_amd/gas_program.hpp, which instantiates the engine's
+    kernels for the program in the app itself."""
+    BINDIR.mkdir(parents=True, exist_ok=True)
+    cc = hipcc()
+    outs, cmds = [], []
+    for src in HIP_APP_SOURCES:
+        if not (APPS / src).exists():
+            continue
+        exe = BINDIR / Path(src).stem
+        outs.append(exe)
+        cmds.append([cc, *HIP_FLAGS, f"-I{ROOT / 'include'}", f"-I{ROOT / 'include' / 'graphreduce_amd'}",
+                     "-x", "hip", str(APPS / src), "-o", str(exe), f"-L{LIBDIR}",
+                     "-lgraphreduce_amd", "-Wl,-rpath,$ORIGIN/../lib"])
+    for c in cmds:
+        _run(c)
+    if verbose:
+        for o in outs:
+            print(f"built {o}")
+    return outs
+
+
 def build_oracle(verbose: bool = False) -> Path:
     """CPU oracle (test infrastructure, built here so that tests can run)."""
     _run(["make", "-s", "-B", "-C", str(ROOT / "oracle")])
@@ -134,6 +160,7 @@ def build_oracle(verbose: bool = False) -> Path:
 def build(verbose: bool = True) -> None:
     build_library(verbose)
     build_apps(verbose)
+    build_hip_apps(verbose)
     build_oracle(verbose)
 
 

@@ -5,9 +5,14 @@
 // their loops unchanged.
 //
 // Differences a maintainer will meet:
-//  - The device side of a Program is compiled into libgraphreduce_amd (the
-//    four reference programs).  A host Program struct is tied to its device
-//    instantiation once, with GR_REGISTER_PROGRAM(BFS, GR_PROGRAM_BFS).
+//  - The device side of the four reference programs is compiled into
+//    libgraphreduce_amd; a host Program struct is tied to that device
+//    instantiation once, with GR_REGISTER_PROGRAM(BFS, GR_PROGRAM_BFS), which
+//    checks at compile time that its VertexData / EdgeData sizes and
+//    gatherZero are those of the device twin.  Any other program runs when
+//    the file including it is compiled with hipcc and includes
+//    graphreduce_amd/gas_program.hpp instead: the engine is then instantiated
+//    for the program itself, as the reference's template argument is.
 //  - BFS's level reached the device through a __constant__ variable and
 //    cudaMemcpyToSymbol (bfs.cu:27-28, 83-90); here the engine hands its
 //    iteration number to Program::apply, set with setIterationCount().
@@ -20,6 +25,7 @@
 #ifndef GRAPHREDUCE_AMD_GASENGINEGPU_H
 #define GRAPHREDUCE_AMD_GASENGINEGPU_H
 
+#include <limits.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -33,6 +39,53 @@ namespace graphreduce_amd {
 // Maps a host Program struct to the device program compiled into the library.
 template <class Program>
 struct ProgramId;
+
+// Layout and identity of the library's device programs (gr_programs.h), for
+// the compile-time checks of GR_REGISTER_PROGRAM.
+template <int ID>
+struct DeviceTwin;
+template <>
+struct DeviceTwin<GR_PROGRAM_BFS> {  // bfs.cu:31-80
+  static constexpr size_t vertex_bytes = 4, edge_bytes = 0;
+  static constexpr double gather_zero = INT_MAX - 1;
+};
+template <>
+struct DeviceTwin<GR_PROGRAM_SSSP> {  // sssp.cu:25-66
+  static constexpr size_t vertex_bytes = 4, edge_bytes = 4;
+  static constexpr double gather_zero = INT_MAX - 100000;
+};
+template <>
+struct DeviceTwin<GR_PROGRAM_PAGERANK> {  // pagerank.cu:27-73
+  static constexpr size_t vertex_bytes = 8, edge_bytes = 0;
+  static constexpr double gather_zero = 0.0;
+};
+template <>
+struct DeviceTwin<GR_PROGRAM_CC> {  // connected_component.cu:25-63
+  static constexpr size_t vertex_bytes = 4, edge_bytes = 0;
+  static constexpr double gather_zero = INT_MAX;
+};
+template <>
+struct DeviceTwin<GR_PROGRAM_SSSP_CONGESTION> {
+  static constexpr size_t vertex_bytes = 4, edge_bytes = 4;
+  static constexpr double gather_zero = INT_MAX - 100000;
+};
+
+template <class Program, class = void>
+struct has_program_id : std::false_type {};
+template <class Program>
+struct has_program_id<Program, std::void_t<decltype(ProgramId<Program>::value)>>
+    : std::true_type {};
+
+// kHasScatter of a program of the caller's (default true, as gas_program.hpp)
+template <class P, class = void>
+struct program_has_scatter : std::true_type {};
+template <class P>
+struct program_has_scatter<P, std::void_t<decltype(P::kHasScatter)>>
+    : std::integral_constant<bool, P::kHasScatter> {};
+
+// Defined in gas_program.hpp (hipcc): the engine instantiated for Program.
+template <class Program>
+int create_engine(gr_engine** out);
 
 inline void check(int rc, const char* what) {
   if (rc != GR_OK) {
@@ -50,7 +103,19 @@ class GASEngineGPU {
   typedef typename Program::EdgeData EdgeData;
   typedef typename Program::GatherResult GatherResult;
 
-  GASEngineGPU() { check(gr_engine_create(ProgramId<Program>::value, &e_), "create engine"); }
+  GASEngineGPU() {
+    if constexpr (has_program_id<Program>::value) {
+      check(gr_engine_create(ProgramId<Program>::value, &e_), "create engine");
+    } else {
+#ifdef GR_CUSTOM_PROGRAMS
+      check(create_engine<Program>(&e_), "create engine");
+#else
+      static_assert(has_program_id<Program>::value,
+                    "register the program with GR_REGISTER_PROGRAM, or include "
+                    "graphreduce_amd/gas_program.hpp and compile with hipcc");
+#endif
+    }
+  }
   ~GASEngineGPU() { gr_engine_destroy(e_); }
   GASEngineGPU(const GASEngineGPU&) = delete;
   GASEngineGPU& operator=(const GASEngineGPU&) = delete;
@@ -62,7 +127,12 @@ class GASEngineGPU {
     check(gr_engine_set_graph(e_, nVertices, vertexData, sizeof(VertexData), nEdges,
                               esz ? edgeData : nullptr, esz, edgeListSrcs, edgeListDsts),
           "setGraph");
-    edgeData_ = esz && gr_program_has_scatter(ProgramId<Program>::value) == 1 ? edgeData : nullptr;
+    bool scatters = false;
+    if constexpr (has_program_id<Program>::value)
+      scatters = gr_program_has_scatter(ProgramId<Program>::value) == 1;
+    else
+      scatters = program_has_scatter<Program>::value;
+    edgeData_ = esz && scatters ? edgeData : nullptr;
   }
 
   // gpugas.h:1524
@@ -127,12 +197,19 @@ class GASEngineGPU {
 
 }  // namespace graphreduce_amd
 
-#define GR_REGISTER_PROGRAM(PROGRAM, ID)                    \
-  namespace graphreduce_amd {                               \
-  template <>                                               \
-  struct ProgramId<PROGRAM> {                               \
-    static constexpr int value = ID;                        \
-  };                                                        \
+#define GR_REGISTER_PROGRAM(PROGRAM, ID)                                                    \
+  namespace graphreduce_amd {                                                               \
+  template <>                                                                               \
+  struct ProgramId<PROGRAM> {                                                               \
+    static constexpr int value = ID;                                                        \
+  };                                                                                        \
+  static_assert(sizeof(PROGRAM::VertexData) == DeviceTwin<ID>::vertex_bytes,                \
+                #PROGRAM ": VertexData differs in size from the library's device program"); \
+  static_assert((std::is_empty<PROGRAM::EdgeData>::value ? 0 : sizeof(PROGRAM::EdgeData)) == \
+                    DeviceTwin<ID>::edge_bytes,                                             \
+                #PROGRAM ": EdgeData differs in size from the library's device program");   \
+  static_assert((double)PROGRAM::gatherZero == DeviceTwin<ID>::gather_zero,                 \
+                #PROGRAM ": gatherZero differs from the library's device program");         \
   }
 
 #endif

@@ -158,6 +158,15 @@ int gr_graph_hash_weights_device(const gr_graph* g, uint64_t seed, int32_t lo,
  * returned true, and an ascending active list for the next iteration. */
 
 int gr_engine_create(int program_id, gr_engine** out);
+/* Wraps an engine object that code of the caller's instantiated from the
+ * library's engine headers for a program of its own
+ * (graphreduce_amd::create_engine<Program> in gas_program.hpp, compiled
+ * with hipcc against the same headers); the handle then takes every
+ * gr_engine_* call and gr_engine_destroy deletes the object.  Replaces the
+ * reference's template argument GASEngineGPU<Program> (gpugas.h:1025-1027).
+ * Its program id is GR_PROGRAM_CUSTOM. */
+#define GR_PROGRAM_CUSTOM 100
+int gr_engine_adopt(void* engine_impl, gr_engine** out);
 int gr_engine_destroy(gr_engine* e);
 
 /* setGraph (gpugas.h:1314-1319): copies vertexData[n_vertices] (elements of

@@ -35,6 +35,7 @@ _SIGS = {
     "oracle_sssp_congestion": [C.c_int32, C.c_int64, _I32P, _I32P, _I32P, C.c_int32, _I32P, _I64P],
     "oracle_sssp_csr": [C.c_int32, _I32P, _I32P, _I32P, _I32P, _I32P, C.c_int32, _I32P, _I64P],
     "oracle_cc": [C.c_int32, C.c_int64, _I32P, _I32P, _I32P, _I64P],
+    "oracle_widest": [C.c_int32, C.c_int64, _I32P, _I32P, _I32P, C.c_int32, _I32P, _I64P],
     "oracle_cc_csr": [C.c_int32, _I32P, _I32P, _I32P, _I32P, _I32P, _I64P],
     "oracle_pagerank": [C.c_int32, C.c_int64, _I32P, _I32P, _F32P, _I64P],
     "oracle_pagerank64": [C.c_int32, C.c_int64, _I32P, _I32P, _F64P, _I64P],
@@ -127,6 +128,17 @@ def sssp_csr(n, off_out, dsts, off_in, srcs, weights_csc, source, L=None):
     st = np.zeros(6, np.int64)
     it = L.oracle_sssp_csr(n, _p(a), _p(b), _p(c), _p(d), _p(w), source, _p(out),
                            _p(st, _I64P))
+    return out, it, _stats(st)
+
+
+def widest(n, srcs, dsts, caps, source, L=None):
+    """Widest-path program (a program outside the reference's four): width of
+    the best bottleneck path from `source` (INT_MAX there, 0 if unreached)."""
+    L = L or lib()
+    s, d, c = _i32(srcs), _i32(dsts), _i32(caps)
+    out = np.empty(n, np.int32)
+    st = np.zeros(6, np.int64)
+    it = L.oracle_widest(n, len(s), _p(s), _p(d), _p(c), source, _p(out), _p(st, _I64P))
     return out, it, _stats(st)
 
 

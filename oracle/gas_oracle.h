@@ -234,6 +234,31 @@ struct SSSP {
   static void scatter(const VertexData*, const VertexData*, EdgeData*) {}
 };
 
+// A program outside the reference's four, for the open Program interface
+// (gpugas.h:1025-1027, refgas.h:30-31): widest (maximum-bottleneck) paths.
+// A vertex holds the largest capacity of a path from the source (INT_MAX at
+// the source, 0 unreached); gatherMap takes the smaller of the source's width
+// and the edge's capacity, gatherReduce the larger of two, apply keeps an
+// increase.
+struct WidestPath {
+  typedef int VertexData;
+  typedef int EdgeData;
+  typedef int GatherResult;
+  static constexpr int gatherZero = 0;
+  static int gatherReduce(const int& a, const int& b) { return std::max(a, b); }
+  static int gatherMap(const VertexData*, const VertexData* srcWidth, const EdgeData* cap) {
+    return std::min(*srcWidth, *cap);
+  }
+  static bool apply(VertexData* cur, GatherResult width, int) {
+    if (width > *cur) {
+      *cur = width;
+      return true;
+    }
+    return false;
+  }
+  static void scatter(const VertexData*, const VertexData*, EdgeData*) {}
+};
+
 // pagerank.cu:27-73.  Real = float is the reference program; Real = double
 // runs the same activation-driven loop in FP64 (used to bound the FP32 sum
 // error on vertices with very large in-degree).

@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include <chrono>
+#include <climits>
 #include <cstring>
 #include <vector>
 
@@ -135,6 +136,22 @@ int oracle_sssp_csr(int32_t nV, const int32_t* off_out, const int32_t* dsts, con
   int32_t it = all_active_loop(eng, nV);
   put_stats(eng, now_ns() - t0, stats);
   std::memcpy(dist, vd.data(), sizeof(int32_t) * nV);
+  return it;
+}
+
+// Widest paths from `source` (INT_MAX there, 0 elsewhere), all vertices
+// active first, as sssp.cu's loop; capacities in edge-list order.
+int oracle_widest(int32_t nV, int64_t nE, const int32_t* srcs, const int32_t* dsts,
+                  const int32_t* caps, int32_t source, int32_t* width, int64_t* stats) {
+  std::vector<int> vd((size_t)nV, WidestPath::gatherZero);
+  vd[source] = INT_MAX;
+  std::vector<int> ed(caps, caps + nE);
+  Engine<WidestPath> eng;
+  eng.setGraph(nV, vd.data(), nE, ed.data(), srcs, dsts);
+  int64_t t0 = now_ns();
+  int32_t it = all_active_loop(eng, nV);
+  put_stats(eng, now_ns() - t0, stats);
+  std::memcpy(width, vd.data(), sizeof(int32_t) * nV);
   return it;
 }
 

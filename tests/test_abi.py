@@ -155,3 +155,33 @@ def test_hash_weights_equal_numpy_statement():
     ref = (np.uint64(1) + (_draw(base, np.arange(1000, dtype=np.uint64)) >> np.uint64(58))).astype(np.int32)
     assert np.array_equal(w, ref)
     assert w.min() >= 1 and w.max() <= 64
+
+
+def _compile_registered(tmp_path, body):
+    import shutil
+    import subprocess
+
+    src = tmp_path / "reg.cpp"
+    src.write_text('#include <climits>\n#include "graphreduce_amd/GASEngineGPU.h"\n' + body)
+    cxx = shutil.which("g++") or "g++"
+    return subprocess.run([cxx, "-std=c++17", "-fsyntax-only", f"-I{ROOT / 'include'}", str(src)],
+                          capture_output=True, text=True)
+
+
+def test_register_program_checks_the_device_twin(tmp_path):
+    """GR_REGISTER_PROGRAM refuses at compile time a host struct whose layout
+    or gatherZero differs from the library's device program."""
+    good = """struct SSSP { typedef int VertexData; typedef int EdgeData; typedef int GatherResult;
+               static const int gatherZero = INT_MAX - 100000; };
+               GR_REGISTER_PROGRAM(SSSP, GR_PROGRAM_SSSP)"""
+    assert _compile_registered(tmp_path, good).returncode == 0
+    bad_zero = good.replace("INT_MAX - 100000", "INT_MAX")
+    r = _compile_registered(tmp_path, bad_zero)
+    assert r.returncode != 0 and "gatherZero differs" in r.stderr
+    bad_vd = good.replace("typedef int VertexData;", "typedef long VertexData;")
+    r = _compile_registered(tmp_path, bad_vd)
+    assert r.returncode != 0 and "VertexData differs" in r.stderr
+    unregistered = """struct X { typedef int VertexData; typedef int EdgeData; typedef int GatherResult; };
+                      graphreduce_amd::GASEngineGPU<X>* make() { return new graphreduce_amd::GASEngineGPU<X>(); }"""
+    r = _compile_registered(tmp_path, unregistered)
+    assert r.returncode != 0 and "gas_program.hpp" in r.stderr

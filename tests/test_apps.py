@@ -117,3 +117,20 @@ def test_regression_checkers_accept_app_output(gpu, tmp_path):
     pgold.write_text("".join(f"{v} {x:.9f}\n" for v, x in enumerate(pr.tolist())))
     ok, msg = checkers.check_pagerank(tmp_path / "pagerank.txt", pgold)
     assert ok, msg
+
+
+def test_widest_path_app_runs_its_own_program(gpu, tmp_path):
+    """A vertex program outside the library (widest paths, reference-style
+    two-argument apply) compiled into the app with hipcc: its results, trips
+    and reached count equal the oracle's Engine<WidestPath>."""
+    for graph, seed in (("rmat:12:5", 5), ("rmat:14:3", 3)):
+        scale = int(graph.split(":")[1])
+        stdout, width = _run("widest_path", ["-m", graph, "0"], tmp_path)
+        s, d = gr.rmat_edges(scale, 16, seed=seed)
+        caps = gr.hash_weights(len(s), seed=seed)
+        src = int(np.argmax(np.bincount(s, minlength=1 << scale)))
+        assert f"using vertex {src}" in stdout
+        ref, trips, _ = oracle.widest(1 << scale, s, d, caps, src)
+        assert np.array_equal(width.astype(np.int64), ref.astype(np.int64)), graph
+        assert f"number of iterations: {trips}" in stdout
+        assert f"vertices reached: {int((ref > 0).sum())}" in stdout

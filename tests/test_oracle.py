@@ -450,3 +450,60 @@ def test_pagerank64_equals_python_double_loop(scale, seed):
     assert np.array_equal(r64, ref)
     r32, _, _ = oracle.pagerank(n, s, d)
     assert not np.array_equal(r32.astype(np.float64), r64)
+
+
+# ------------------------------------------------ widest paths (open Program)
+
+def _widest_dijkstra(n, s, d, cap, src):
+    """Maximum-bottleneck path widths by a max-heap Dijkstra (independent of
+    the GAS loop): INT_MAX at the source, 0 where unreachable."""
+    import heapq
+
+    adj = [[] for _ in range(n)]
+    for u, v, c in zip(s.tolist(), d.tolist(), cap.tolist()):
+        adj[u].append((v, c))
+    best = [0] * n
+    best[src] = 2**31 - 1
+    heap = [(-best[src], src)]
+    done = [False] * n
+    while heap:
+        negw, u = heapq.heappop(heap)
+        if done[u]:
+            continue
+        done[u] = True
+        for v, c in adj[u]:
+            w = min(-negw, c)
+            if w > best[v]:
+                best[v] = w
+                heapq.heappush(heap, (-w, v))
+    return np.array(best, np.int64)
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_widest_vs_bottleneck_dijkstra(seed):
+    n, s, d, w = _random_multigraph(seed)
+    src = int(np.argmax(np.bincount(s, minlength=n))) if len(s) else 0
+    got, trips, _ = oracle.widest(n, s, d, w, src)
+    assert np.array_equal(got, _widest_dijkstra(n, s, d, w, src))
+    # brute force of the loop: same trips
+    width = [0] * n
+    width[src] = 2**31 - 1
+
+    def wapply(vd, v, g, it):
+        if g > vd[v]:
+            vd[v] = g
+            return True
+        return False
+
+    btrips = _brute_gas(n, s, d, width, lambda vd, u, e: min(vd[u], int(w[e])), max, 0, wapply,
+                        list(range(n))) if n * len(s) <= 200_000 else None
+    assert btrips is None or (trips == btrips and got.tolist() == width)
+
+
+def test_widest_on_rmat():
+    n = 1 << 12
+    s, d = gr.rmat_edges(12, 16, seed=5)
+    cap = gr.hash_weights(len(s), seed=5)
+    src = int(np.argmax(np.bincount(s, minlength=n)))
+    got, _, _ = oracle.widest(n, s, d, cap, src)
+    assert np.array_equal(got, _widest_dijkstra(n, s, d, cap, src))
