@@ -84,7 +84,7 @@ __device__ __forceinline__ T block_excl_scan_add(T v, T* scratch, T* total) {
 
 // Row blocks of the CSC whose in-edges are listed by source (for the blocked
 // gather): block b holds rows [brow[b], brow[b] + bnr[b]) -- at most
-// gr::kBlockRows rows and, unless one row alone has more, the edge target
+// gr::kBlockRows (or gr::kBlockRowsWide) rows and, unless one row alone has more, the edge target
 // below -- and their in-edges are positions [boff[b], boff[b + 1]) of
 // bsrc / brl / bpos, sorted by source (stable).  Rows from rows_end on have
 // no in-edges and belong to no block.  The edge target per block is
@@ -117,17 +117,22 @@ struct gr_graph {
   gr_graph* ordered = nullptr;
   int32_t* perm = nullptr;  // ordered id -> id   [V], device
   int32_t* inv = nullptr;   // id -> ordered id   [V], device
-  gr_blocked* blocked = nullptr;  // built on first use (graph_blocked)
+  // row blocks of at most kBlockRows / kBlockRowsWide rows, built on first
+  // use (graph_blocked)
+  gr_blocked* blocked = nullptr;
+  gr_blocked* blocked_half = nullptr;
 };
 
 namespace gr {
 constexpr int kBlockRows = 32768;       // rows per gather block (LDS: 128 KB of int)
+constexpr int kBlockRowsWide = 16384;   // ... for 8-byte accumulators (LDS: 128 KB)
 constexpr int kBlockEdges = 1 << 19;    // in-edges per gather block (largest target)
 int graph_free(gr_graph* g);
 // The degree-ordered copy of g (cached in g; freed with g).
 int graph_ordered(gr_graph* g, gr_graph** out);
-// The row blocks of g's CSC with in-edges listed by source (cached in g).
-int graph_blocked(gr_graph* g, const gr_blocked** out);
+// The row blocks of g's CSC with in-edges listed by source, at most max_rows
+// (kBlockRows or kBlockRowsWide) rows each (cached in g).
+int graph_blocked(gr_graph* g, int max_rows, const gr_blocked** out);
 }
 
 #endif

@@ -1178,8 +1178,11 @@ struct Engine final : EngineBase {
   G* d_cand = nullptr;     // push-relax candidates per vertex (gatherZero between gathers)
   // Blocked gather: qualifying programs, the graph's source-listed row blocks,
   // the edge data in their order, and the vertex-indexed gather values.
-  static constexpr bool kBlocked = std::is_same<G, int>::value && delta_gather_v<P> &&
-                                   !P::kGatherUsesDst && !P::kHasScatter;
+  static constexpr bool kBlockedMin = std::is_same<G, int>::value && delta_gather_v<P> &&
+                                      !P::kGatherUsesDst && !P::kHasScatter;
+  static constexpr bool kBlockedSum = std::is_same<G, float>::value && fixed_point_sum_v<P> &&
+                                      !P::kGatherUsesDst && !P::kHasScatter;
+  static constexpr bool kBlocked = kBlockedMin || kBlockedSum;
   const gr_blocked* blk = nullptr;
   ED* d_ed_blk = nullptr;
   G* d_gdense = nullptr;   // rows from blk->rows_end on stay gatherZero
@@ -1959,7 +1962,7 @@ struct Engine final : EngineBase {
   int gather_blocked(int n, int w, const uint32_t* dbits) {
     if constexpr (kBlocked) {
       if (!blk) {
-        GR_TRY(graph_blocked(g, &blk));
+        GR_TRY(graph_blocked(g, kBlockedSum ? kBlockRowsWide : kBlockRows, &blk));
         GR_TRY(dalloc(&d_gdense, g->nV));
         k_fill<G><<<1024, 256, 0, stream>>>(g->nV, P::gatherZero, d_gdense);
         GR_HIP(hipGetLastError());
@@ -1973,10 +1976,16 @@ struct Engine final : EngineBase {
         }
       }
       tick();
-      if (blk->n_blocks > 0)
-        k_gather_blocked<P><<<blk->n_blocks, kBlkNT, 0, stream>>>(
-            blk->brow, blk->bnr, blk->boff, blk->bsrc, blk->brl, d_ed_blk, d_vd, d_srcval, dbits,
-            d_gdense);
+      if (blk->n_blocks > 0) {
+        if constexpr (kBlockedSum)
+          k_gather_blocked_sum<P><<<blk->n_blocks, kBlkNT, 0, stream>>>(
+              blk->brow, blk->bnr, blk->boff, blk->bsrc, blk->brl, d_ed_blk, d_vd, d_srcval,
+              d_gdense);
+        else
+          k_gather_blocked<P><<<blk->n_blocks, kBlkNT, 0, stream>>>(
+              blk->brow, blk->bnr, blk->boff, blk->bsrc, blk->brl, d_ed_blk, d_vd, d_srcval,
+              dbits, d_gdense);
+      }
       tock(kPhGather);
       GR_HIP(hipGetLastError());
       dense_pending = true;

@@ -554,6 +554,74 @@ __global__ __launch_bounds__(kBlkNT) void k_gather_blocked(
   for (int r = tid; r < nr; r += kBlkNT) gdense[r0 + r] = acc[r];
 }
 
+// Programs whose gather is a sum of non-negative floats may declare
+// kFixedPointSum: the blocked gather then adds their terms exactly in 64-bit
+// fixed point (each term rounded once to a multiple of 2^-32) and rounds the
+// row's sum once to float, so the result does not depend on the order in
+// which the LDS additions land (determinism) and is at least as close to the
+// exact sum as a float tree sum.
+template <class P, class = void>
+struct fixed_point_sum : std::false_type {};
+template <class P>
+struct fixed_point_sum<P, std::void_t<decltype(P::kFixedPointSum)>>
+    : std::integral_constant<bool, P::kFixedPointSum> {};
+template <class P>
+constexpr bool fixed_point_sum_v = fixed_point_sum<P>::value;
+
+constexpr float kFixScale = 4294967296.0f;             // 2^32
+constexpr double kFixUnscale = 1.0 / 4294967296.0;     // 2^-32
+
+// Blocked gather of a fixed-point sum program: as k_gather_blocked, with the
+// rows' sums in 64-bit integers (kBlockRowsWide rows of 8 bytes in LDS).
+template <class P>
+__global__ __launch_bounds__(kBlkNT) void k_gather_blocked_sum(
+    const int32_t* __restrict__ brow, const int32_t* __restrict__ bnr,
+    const int32_t* __restrict__ boff, const int32_t* __restrict__ bsrc,
+    const uint16_t* __restrict__ brl, const typename P::EdgeData* __restrict__ ed,
+    const typename P::VertexData* __restrict__ vd, const source_value_t<P>* __restrict__ srcval,
+    typename P::GatherResult* __restrict__ gdense) {
+  typedef typename P::GatherResult G;
+  static_assert(std::is_same<G, float>::value && fixed_point_sum_v<P> && !P::kGatherUsesDst,
+                "the fixed-point blocked gather sums non-negative floats");
+  __shared__ unsigned long long acc[kBlockRowsWide];
+  const int b = blockIdx.x;
+  const int r0 = brow[b], nr = bnr[b];
+  const int e0 = boff[b], e1 = boff[b + 1];
+  const int tid = threadIdx.x;
+  for (int r = tid; r < nr; r += kBlkNT) acc[r] = 0ull;
+  __syncthreads();
+  for (int base = e0; base < e1; base += kBlkNT * kBlkVT) {
+    int src[kBlkVT], row[kBlkVT];
+    typename P::EdgeData ev[kBlkVT];
+#pragma unroll
+    for (int m = 0; m < kBlkVT; ++m) {
+      const int e = base + m * kBlkNT + tid;
+      src[m] = -1;
+      row[m] = 0;
+      if (e < e1) {
+        src[m] = __builtin_nontemporal_load(bsrc + e);
+        row[m] = brl[e];
+        ev[m] = stream_load(ed + e);
+      }
+    }
+    unsigned long long fix[kBlkVT];
+#pragma unroll
+    for (int m = 0; m < kBlkVT; ++m) {
+      fix[m] = 0ull;
+      if (src[m] >= 0) {
+        const G v = map_source<P>(vd, vd, srcval, src[m], &ev[m]);
+        fix[m] = (unsigned long long)__float2ll_rn(__fmul_rn(v, kFixScale));
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < kBlkVT; ++m)
+      if (fix[m]) atomicAdd(&acc[row[m]], fix[m]);
+  }
+  __syncthreads();
+  for (int r = tid; r < nr; r += kBlkNT)
+    gdense[r0 + r] = (G)((double)(long long)acc[r] * kFixUnscale);
+}
+
 }  // namespace gr
 
 #endif

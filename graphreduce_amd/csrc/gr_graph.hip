@@ -182,8 +182,8 @@ int finish_graph(gr_graph* g) {
 int graph_free(gr_graph* g) {
   if (!g) return GR_OK;
   if (g->ordered) graph_free(g->ordered);
-  if (g->blocked) {
-    gr_blocked* b = g->blocked;
+  for (gr_blocked* b : {g->blocked, g->blocked_half}) {
+    if (!b) continue;
     for (void* p : {(void*)b->brow, (void*)b->bnr, (void*)b->boff, (void*)b->bsrc,
                     (void*)b->brl, (void*)b->bpos})
       if (p) (void)hipFree(p);
@@ -338,16 +338,19 @@ std::mutex g_block_mu;
 
 }  // namespace
 
-int graph_blocked(gr_graph* g, const gr_blocked** out) {
+int graph_blocked(gr_graph* g, int max_rows, const gr_blocked** out) {
   std::lock_guard<std::mutex> lk(g_block_mu);
-  if (g->blocked) {
-    *out = g->blocked;
+  if (max_rows != kBlockRows && max_rows != kBlockRowsWide)
+    return set_errorf(GR_ERR_INVALID, "graph_blocked: %d rows per block", max_rows);
+  gr_blocked*& slot = max_rows == kBlockRows ? g->blocked : g->blocked_half;
+  if (slot) {
+    *out = slot;
     return GR_OK;
   }
   const int32_t nV = g->nV;
   const int64_t nE = g->nE;
   const std::vector<int32_t>& off = g->h_off_in;
-  // blocks of consecutive rows: at most kBlockRows rows, at most edge_target
+  // blocks of consecutive rows: at most max_rows rows, at most edge_target
   // in-edges unless a single row has more
   int32_t last = nV;
   while (last > 0 && off[last] == off[last - 1]) --last;
@@ -367,7 +370,7 @@ int graph_blocked(gr_graph* g, const gr_blocked** out) {
     int32_t r1 = (int32_t)(std::upper_bound(off.begin() + r + 1, off.begin() + last + 1,
                                             (int32_t)std::min<int64_t>(target, INT32_MAX)) -
                            off.begin()) - 1;
-    r1 = std::min(std::min(r1, r + kBlockRows), last);
+    r1 = std::min(std::min(r1, r + max_rows), last);
     if (r1 <= r) r1 = r + 1;
     for (int32_t v = r; v < r1; ++v) row_block[v] = (int32_t)brow.size();
     brow.push_back(r);
@@ -425,7 +428,7 @@ int graph_blocked(gr_graph* g, const gr_blocked** out) {
   b->bsrc = d_bsrc.release();
   b->brl = d_brl.release();
   b->bpos = d_bpos.release();
-  g->blocked = b;
+  slot = b;
   *out = b;
   return GR_OK;
 }

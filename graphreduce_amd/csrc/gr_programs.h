@@ -119,6 +119,10 @@ struct PageRankProgram {
   __device__ static float gatherReduce(const float& left, const float& right) {
     return __fadd_rn(left, right);
   }
+  // the gather is a sum of non-negative terms: the blocked gather may add
+  // them exactly in fixed point and round once (a deterministic order-free
+  // sum; the reference's GPU sums in tree order, its CPU engine sequentially)
+  static constexpr bool kFixedPointSum = true;
   __device__ static bool apply(VertexData* vertexData, const float& gatherResult, int) {
     const float damp = 1.0f - pageConst;
     float newRank = __fadd_rn(pageConst, __fmul_rn(damp, gatherResult));

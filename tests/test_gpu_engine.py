@@ -959,3 +959,42 @@ def test_blocked_gather_giant_row_and_sparse_tail(gpu):
     assert np.array_equal(lab, lref) and trips == ltrips
     g.close()
     gs.close()
+
+
+def test_blocked_pagerank_fixed_point_sum(gpu):
+    """PageRank's dense gathers on the row blocks, summed exactly in fixed
+    point: the reference's tolerance vs the oracle (as test_pagerank_rmat), the
+    oracle's trip count, and the same ranks on a second run (the sum does not
+    depend on the order of the LDS additions)."""
+    gr = gpu
+    n, s, d = _rmat(gr, 14, seed=3)
+    outdeg = np.bincount(s, minlength=n).astype(np.int32)
+    ref, ref_trips, _ = oracle.pagerank(n, s, d)
+    r64, _, _ = oracle.pagerank64(n, s, d)
+    g = gr.Graph.from_edges(n, s, d)
+    runs = []
+    for share in (0.5, 0.5, 0.0):
+        vd = np.zeros(n, gr.PROGRAMS[gr.PAGERANK]["vertex"])
+        vd["rank"] = np.float32(0.15)
+        vd["numOutEdges"] = outdeg
+        eng = gr.GASEngine(gr.PAGERANK)
+        eng.setBlockedGather(share)
+        eng.bindGraph(g, vd)
+        eng.setActive(0, n)
+        trips = eng.run()
+        st = eng.stats()
+        runs.append((eng.readVertexData()["rank"].copy(), trips, st["blocked_gathers"]))
+        eng.close()
+    g.close()
+    (r1, t1, b1), (r2, t2, _), (r0, t0, b0) = runs
+    assert b1 > 0 and b0 == 0
+    assert np.array_equal(r1, r2)
+    for rank, trips in ((r1, t1), (r0, t0)):
+        assert trips == ref_trips
+        diff = np.abs(rank.astype(np.float64) - ref.astype(np.float64))
+        small = ref < 1.0
+        assert diff[small].max() <= 1e-6
+        assert (diff / ref).max() <= 1e-5
+        # and against the same loop in FP64 (the FP32 oracle's own sequential
+        # sums drift from it by up to 2.4e-4 on hub ranks at RMAT-15)
+        assert (np.abs(rank - r64) / r64).max() <= 1e-5
