@@ -442,7 +442,25 @@ def run_batched(gr, g, args, outdeg, w_dev):
                       speedup=round(t_plain / t_b, 2), levels=info["levels"],
                       rows_compared=64, identical_to_plain=bool(same))
     log(f"batched BFS: {res['bfs']}")
-    del plain_rows, depth
+    plain_bfs = plain_rows
+    del depth
+    # the same batch as ONE persistent cooperative launch of every level
+    # (GR_MSBFS_PERSISTENT): measured beside the default per-level launches
+    gr.msbfs(g, srcs, persistent=True, out=out)  # warm
+    gr.synchronize()
+    t0 = time.perf_counter()
+    _, pinfo = gr.msbfs(g, srcs, persistent=True, out=out)
+    gr.synchronize()
+    t_p = time.perf_counter() - t0
+    pdepth = gr.download(out, np.int32, (64, n))
+    res["bfs"]["persistent_launch"] = dict(ms=round(t_p * 1e3, 3),
+                                           device_ms=round(pinfo["ms"], 3), grid=pinfo["grid"],
+                                           speedup=round(t_plain / t_p, 2),
+                                           identical_to_plain=bool(all(
+                                               np.array_equal(pdepth[i], plain_bfs[i])
+                                               for i in range(64))))
+    del pdepth, plain_bfs
+    log(f"batched BFS persistent: {res['bfs']['persistent_launch']}")
     # --- SSSP: a tuned engine per source (the start vertex is part of the
     # bound vertex data), all 64 timed
     t_plain, er_tot, rows = 0.0, 0, []
@@ -552,6 +570,32 @@ def run_serving(gr, g, args, outdeg, w_dev, n_req=384, rates=(250.0, 500.0, 1000
         log(f"serving @ {rate}: {row}")
     res["configured"] = res["by_rate"][-1]
     res["sustained"] = sustained
+    # service time without other traffic (one request on an idle server) and
+    # each pool's throughput with 64 requests of one kind at once: the
+    # latencies of the mixed streams above come from the three pools' kernels
+    # sharing the device, and the pools' throughputs bound the mix's capacity
+    solo, pool = {}, {}
+    kinds_ = (gr.QUERY_BFS, gr.QUERY_SSSP, gr.QUERY_PPR)
+    srv = gr.Server(g, w_dev)
+    for k in kinds_:
+        lat_k = []
+        for v in verts[:3]:
+            rid = srv.submit(k, int(v), keep=False)
+            lat_k.append(srv.wait(rid, want_result=False)[2])
+        solo[names[k]] = round(float(np.median(lat_k)), 2)
+    srv.close()
+    for k in kinds_:
+        srv = gr.Server(g, w_dev)
+        t0 = time.perf_counter()
+        ids = srv.load_test(1e9, verts[:64], np.full(64, k, np.int32), keep=False)
+        for rid in ids:
+            srv.wait(int(rid), want_result=False)
+        pool[names[k]] = round(64 / (time.perf_counter() - t0), 1)
+        srv.close()
+    mix_capacity = 3.0 / sum(1.0 / pool[names[k]] for k in kinds_)
+    res["solo_latency_ms"] = solo
+    res["pool_throughput_rps_64_at_once"] = pool
+    res["mix_capacity_bound_rps"] = round(mix_capacity, 1)
     res["alone_equals_mix"] = bool(equal)
     res["alone_vs_mix_sample"] = len(sample)
     v, row = alone_rows[gr.QUERY_BFS]
