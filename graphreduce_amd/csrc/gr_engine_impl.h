@@ -1187,6 +1187,12 @@ struct Engine final : EngineBase {
   ED* d_ed_blk = nullptr;
   G* d_gdense = nullptr;   // rows from blk->rows_end on stay gatherZero
   bool dense_pending = false;  // apply reads d_gdense[v] (the last gather was blocked)
+  // Fused activation (fixed-point sum programs): after a blocked gather,
+  // scatterActivate runs the next blocked gather over every row, which also
+  // marks the rows with a changed in-neighbour (the pull); the next gather
+  // only collects its values.
+  bool last_gather_blocked = false;
+  bool fab_pending = false;
   source_value_t<P>* d_srcval = nullptr;  // P::sourceValue of every vertex (programs with one)
   // ---- out-of-core mode: edge arrays in pinned host memory, streamed per shard
   bool streaming = false;
@@ -1894,6 +1900,20 @@ struct Engine final : EngineBase {
     it_gather = true;
     GR_TRY(collect_flush());  // a previous gather whose apply did not run
     dense_pending = false;
+    last_gather_blocked = false;
+    if (fab_pending) {  // computed by the previous scatterActivate
+      fab_pending = false;
+      if (P::kHasGather && n_active > 0) {
+        dense_pending = true;
+        last_gather_blocked = true;
+        stats.active_vertices += n_active;
+        stats.gathered_edges += w_edges;
+        stats.tile_gather_rows += n_active;
+        stats.tile_gather_edges += w_edges;
+        stats.blocked_gathers += 1;
+      }
+      return GR_OK;
+    }
     if (!P::kHasGather || n_active == 0) return GR_OK;
     if (streaming) return gather_streamed();
     const int n = n_active, w = w_edges;
@@ -1980,7 +2000,7 @@ struct Engine final : EngineBase {
         if constexpr (kBlockedSum)
           k_gather_blocked_sum<P><<<blk->n_blocks, kBlkNT, 0, stream>>>(
               blk->brow, blk->bnr, blk->boff, blk->bsrc, blk->brl, d_ed_blk, d_vd, d_srcval,
-              d_gdense);
+              d_gdense, nullptr, nullptr);
         else
           k_gather_blocked<P><<<blk->n_blocks, kBlkNT, 0, stream>>>(
               blk->brow, blk->bnr, blk->boff, blk->bsrc, blk->brl, d_ed_blk, d_vd, d_srcval,
@@ -1989,6 +2009,7 @@ struct Engine final : EngineBase {
       tock(kPhGather);
       GR_HIP(hipGetLastError());
       dense_pending = true;
+      last_gather_blocked = true;
       stats.active_vertices += n;
       stats.gathered_edges += w;
       stats.tile_gather_rows += n;
@@ -2036,6 +2057,8 @@ struct Engine final : EngineBase {
   // (the active set was replaced or the vertex data reset): back to gatherZero.
   int drop_candidates() {
     GR_TRY(collect_flush());
+    fab_pending = false;
+    last_gather_blocked = false;
     if constexpr (kDelta) {
       if (fused_pending) {
         k_fill<G><<<1024, 256, 0, stream>>>(g->nV, P::gatherZero, d_cand);
@@ -2136,6 +2159,23 @@ struct Engine final : EngineBase {
     it_scatter = true;
     GR_TRY(collect_flush());  // a push gather whose apply did not run
     if (streaming) return scatter_streamed();
+    if constexpr (kBlockedSum) {
+      // after a dense (blocked) gather the next one is most likely dense too:
+      // gather every row now and mark the pull's activations in the same pass
+      if (last_gather_blocked && blk && it_gather && it_apply && changed_bits_on() &&
+          blocked_share > 0) {
+        tick();
+        if (blk->n_blocks > 0)
+          k_gather_blocked_sum<P><<<blk->n_blocks, kBlkNT, 0, stream>>>(
+              blk->brow, blk->bnr, blk->boff, blk->bsrc, blk->brl, d_ed_blk, d_vd, d_srcval,
+              d_gdense, d_chg_bits, d_bitmap);
+        tock(kPhGather);
+        GR_HIP(hipGetLastError());
+        fab_pending = true;
+        last_gather_blocked = false;
+        return GR_OK;
+      }
+    }
     const long long thr = pull_threshold();
     // Fused activation + next push gather, decided on the device from the
     // changed out-edges: only while the next gather may use the delta rule

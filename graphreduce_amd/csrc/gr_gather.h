@@ -573,22 +573,30 @@ constexpr double kFixUnscale = 1.0 / 4294967296.0;     // 2^-32
 
 // Blocked gather of a fixed-point sum program: as k_gather_blocked, with the
 // rows' sums in 64-bit integers (kBlockRowsWide rows of 8 bytes in LDS).
+// With chg_bits (the vertices whose apply just returned true), the same pass
+// is also scatterActivate's pull: a row with an in-edge from such a vertex is
+// marked in the `frontier` bitmap (which holds no bit of these rows before).
 template <class P>
 __global__ __launch_bounds__(kBlkNT) void k_gather_blocked_sum(
     const int32_t* __restrict__ brow, const int32_t* __restrict__ bnr,
     const int32_t* __restrict__ boff, const int32_t* __restrict__ bsrc,
     const uint16_t* __restrict__ brl, const typename P::EdgeData* __restrict__ ed,
     const typename P::VertexData* __restrict__ vd, const source_value_t<P>* __restrict__ srcval,
-    typename P::GatherResult* __restrict__ gdense) {
+    typename P::GatherResult* __restrict__ gdense, const uint32_t* __restrict__ chg_bits,
+    uint32_t* __restrict__ frontier) {
   typedef typename P::GatherResult G;
   static_assert(std::is_same<G, float>::value && fixed_point_sum_v<P> && !P::kGatherUsesDst,
                 "the fixed-point blocked gather sums non-negative floats");
   __shared__ unsigned long long acc[kBlockRowsWide];
+  __shared__ uint8_t act[kBlockRowsWide];
   const int b = blockIdx.x;
   const int r0 = brow[b], nr = bnr[b];
   const int e0 = boff[b], e1 = boff[b + 1];
   const int tid = threadIdx.x;
-  for (int r = tid; r < nr; r += kBlkNT) acc[r] = 0ull;
+  for (int r = tid; r < nr; r += kBlkNT) {
+    acc[r] = 0ull;
+    act[r] = 0;
+  }
   __syncthreads();
   for (int base = e0; base < e1; base += kBlkNT * kBlkVT) {
     int src[kBlkVT], row[kBlkVT];
@@ -605,21 +613,39 @@ __global__ __launch_bounds__(kBlkNT) void k_gather_blocked_sum(
       }
     }
     unsigned long long fix[kBlkVT];
+    uint32_t word[kBlkVT];
 #pragma unroll
     for (int m = 0; m < kBlkVT; ++m) {
       fix[m] = 0ull;
+      word[m] = 0u;
       if (src[m] >= 0) {
         const G v = map_source<P>(vd, vd, srcval, src[m], &ev[m]);
         fix[m] = (unsigned long long)__float2ll_rn(__fmul_rn(v, kFixScale));
+        if (chg_bits) word[m] = chg_bits[src[m] >> 5];
       }
     }
 #pragma unroll
-    for (int m = 0; m < kBlkVT; ++m)
+    for (int m = 0; m < kBlkVT; ++m) {
       if (fix[m]) atomicAdd(&acc[row[m]], fix[m]);
+      if (src[m] >= 0 && ((word[m] >> (src[m] & 31)) & 1u)) act[row[m]] = 1;
+    }
   }
   __syncthreads();
   for (int r = tid; r < nr; r += kBlkNT)
     gdense[r0 + r] = (G)((double)(long long)acc[r] * kFixUnscale);
+  if (chg_bits) {
+    // the bitmap words of rows r0 .. r0+nr-1; the first and last may hold
+    // rows of the neighbouring blocks, hence atomicOr
+    const int w0 = r0 >> 5, w1 = (r0 + nr - 1) >> 5;
+    for (int w = w0 + tid; w <= w1; w += kBlkNT) {
+      uint32_t bits = 0u;
+      for (int k = 0; k < 32; ++k) {
+        const int r = w * 32 + k - r0;
+        if (r >= 0 && r < nr && act[r]) bits |= 1u << k;
+      }
+      if (bits) atomicOr(&frontier[w], bits);
+    }
+  }
 }
 
 }  // namespace gr
