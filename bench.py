@@ -186,7 +186,12 @@ def main() -> int:
     # over every CSC slot of the same degree-ordered layout (a full gather of
     # this run covers > 99 % of the slots).
     floor_ms = g.source_read_ms(ordered=True)
-    avg_launch = st["gather_ms"] / max(1, launches["gather"])
+    # launches that gathered: scatterActivate launches the fused blocked
+    # gather wherever the pull may run, and the kernel stands down on the
+    # device where it does not (a few microseconds each, left in gather_ms)
+    idle = st["idle_gather_launches"]
+    working = max(1, launches["gather"] - idle)
+    avg_launch = st["gather_ms"] / working
 
     # HBM traffic per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
     # summary (calibrated against a plain copy kernel)
@@ -224,7 +229,9 @@ def main() -> int:
             "edges_reached": e_reached,
         },
         "roofline": {
-            "kernel": "k_gather_blocked<SSSPProgram> (the full gathers of the run)",
+            "kernel": "k_gather_blocked<SSSPProgram> (the full gathers of the run; those "
+                      "launched by scatterActivate in place of the pull also mark the next "
+                      "active set)",
             "blocked_launches": st["blocked_gathers"],
             "bound": "hbm",
             "achieved": round(achieved, 1),
@@ -232,9 +239,11 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": prof_traffic,
-            "bytes_per_launch": round(tile_bytes / max(1, launches["gather"])),
-            "avg_launch_ms": round(st["gather_ms"] / max(1, launches["gather"]), 5),
-            "launches": launches["gather"],
+            "bytes_per_launch": round(tile_bytes / working),
+            "avg_launch_ms": round(avg_launch, 5),
+            "launches": working,
+            "idle_launches": idle,
+            "fused_activations": st["fused_activations"],
             "algorithmic_bytes": "12 B per gathered in-edge (source id, weight, source "
                                  "distance) + 16 B per active vertex",
             "push_gathers": {
@@ -294,7 +303,7 @@ def _frac(nbytes, sec):
 def run_extras(gr, g, args, src, outdeg, w_dev):
     """The other workloads of the RMAT-24 config and the other configs.  Each
     plain program reports its step time, GTEPS, the reference's algorithmic
-    bytes per step (SURVEY-style formulas: BFS 4 E_reached + 16 V; CC and
+    bytes per step (BFS 4 E_reached + 16 V; CC and
     PageRank 8 E_in(A) + 16 #A summed over the run) over the step time as a
     fraction of HBM peak, and the same for its gather phase."""
     from graphreduce_amd import tune

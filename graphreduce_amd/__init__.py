@@ -90,6 +90,8 @@ class _Stats(C.Structure):
         ("fused_gathers", C.c_int64),
         ("fused_edges", C.c_int64),
         ("blocked_gathers", C.c_int64),
+        ("fused_activations", C.c_int64),
+        ("idle_gather_launches", C.c_int64),
     ]
 
 

@@ -294,9 +294,7 @@ __global__ void k_check_inert_edges(int64_t E, const typename P::EdgeData* __res
 // out-edge count so that the host needs no read: mode 0 = pull when the
 // out-edges of the changed vertices exceed `threshold`, 1 = push, 2 = pull.
 __device__ __forceinline__ bool use_pull(unsigned long long chg, int mode, long long threshold) {
-  if (mode == 2) return true;
-  if (mode == 1) return false;
-  return (long long)(uint32_t)chg > threshold;
+  return use_pull_rule(chg, mode, threshold);
 }
 
 // Fused activation + push gather (delta-gather programs): when the changed
@@ -304,7 +302,7 @@ __device__ __forceinline__ bool use_pull(unsigned long long chg, int mode, long 
 // the frontier bits and lowers the next gather's candidates, replacing the
 // pull or push activation and the next iteration's push relax.
 __device__ __forceinline__ bool fused_scatter(unsigned long long chg, long long fuse_max) {
-  return fuse_max >= 0 && (chg >> 32) != 0 && (long long)(uint32_t)chg <= fuse_max;
+  return fused_scatter_rule(chg, fuse_max);
 }
 
 // A fused pass over more than kScanMinEdges out-edges leaves the marking of
@@ -1187,12 +1185,14 @@ struct Engine final : EngineBase {
   ED* d_ed_blk = nullptr;
   G* d_gdense = nullptr;   // rows from blk->rows_end on stay gatherZero
   bool dense_pending = false;  // apply reads d_gdense[v] (the last gather was blocked)
-  // Fused activation (fixed-point sum programs): after a blocked gather,
-  // scatterActivate runs the next blocked gather over every row, which also
-  // marks the rows with a changed in-neighbour (the pull); the next gather
-  // only collects its values.
+  // Fused activation: scatterActivate runs the next blocked gather over every
+  // row, which also marks the rows with a changed in-neighbour (the pull);
+  // the next gather only collects its values.  Fixed-point sum programs do
+  // so after a blocked gather; min programs whenever the pull would run (the
+  // kernel decides on the device and records in ctr->fab whether it ran).
   bool last_gather_blocked = false;
   bool fab_pending = false;
+  bool fab_launched = false;  // scatterActivate enqueued the min programs' fused pass
   source_value_t<P>* d_srcval = nullptr;  // P::sourceValue of every vertex (programs with one)
   // ---- out-of-core mode: edge arrays in pinned host memory, streamed per shard
   bool streaming = false;
@@ -1533,6 +1533,7 @@ struct Engine final : EngineBase {
     iteration = 0;
     delta_chain = delta_ready = false;
     fuse_launched = fused_pending = collect_pending = dense_pending = false;
+    fab_launched = fab_pending = last_gather_blocked = false;
     it_gather = it_apply = it_scatter = false;
     bound = true;
     return GR_OK;
@@ -1911,6 +1912,8 @@ struct Engine final : EngineBase {
         stats.tile_gather_rows += n_active;
         stats.tile_gather_edges += w_edges;
         stats.blocked_gathers += 1;
+        stats.fused_activations += 1;
+        if (kDelta) stats.delta_gathers += 1;
       }
       return GR_OK;
     }
@@ -1975,11 +1978,9 @@ struct Engine final : EngineBase {
            (double)w >= blocked_share * (double)g->nE;
   }
 
-  // Gather of every row through the graph's source-listed row blocks
-  // (k_gather_blocked); apply reads the active rows' values from d_gdense.
-  // The blocks, the edge data in their order and d_gdense are set up on the
-  // first such gather of a binding.
-  int gather_blocked(int n, int w, const uint32_t* dbits) {
+  // The graph's source-listed row blocks, the edge data in their order and
+  // d_gdense, set up on the first blocked gather of a binding.
+  int blocked_setup() {
     if constexpr (kBlocked) {
       if (!blk) {
         GR_TRY(graph_blocked(g, kBlockedSum ? kBlockRowsWide : kBlockRows, &blk));
@@ -1995,6 +1996,15 @@ struct Engine final : EngineBase {
           GR_HIP(hipGetLastError());
         }
       }
+    }
+    return GR_OK;
+  }
+
+  // Gather of every row through the graph's source-listed row blocks
+  // (k_gather_blocked); apply reads the active rows' values from d_gdense.
+  int gather_blocked(int n, int w, const uint32_t* dbits) {
+    if constexpr (kBlocked) {
+      GR_TRY(blocked_setup());
       tick();
       if (blk->n_blocks > 0) {
         if constexpr (kBlockedSum)
@@ -2057,7 +2067,7 @@ struct Engine final : EngineBase {
   // (the active set was replaced or the vertex data reset): back to gatherZero.
   int drop_candidates() {
     GR_TRY(collect_flush());
-    fab_pending = false;
+    fab_pending = fab_launched = false;
     last_gather_blocked = false;
     if constexpr (kDelta) {
       if (fused_pending) {
@@ -2201,8 +2211,29 @@ struct Engine final : EngineBase {
     // stride over the tiles, so any grid of at least one workgroup is whole)
     const int push_tiles = (int)std::min<long long>(max_push_tiles, bound / kTile + 1);
     const int push_grid = push_tiles < 8192 ? (push_tiles < 1 ? 1 : push_tiles) : 8192;
+    // Min programs whose next gather may use the delta rule: where the pull
+    // would run, the blocked gather of the changed sources' out-edges takes
+    // its place and also computes the next gather's values (timed with the
+    // gathers).  Its first workgroup records in ctr->fab whether it ran.
+    bool fab = false;
+    if constexpr (kBlockedMin) {
+      fab = may_pull && delta_chain && it_gather && it_apply && delta_mode == 1 &&
+            blocked_share > 0 && g->nE > 0;
+      if (fab) {
+        GR_TRY(blocked_setup());
+        tick();
+        if (blk->n_blocks > 0)
+          k_gather_blocked<P><<<blk->n_blocks, kBlkNT, 0, stream>>>(
+              blk->brow, blk->bnr, blk->boff, blk->bsrc, blk->brl, d_ed_blk, d_vd, d_srcval,
+              d_chg_bits, d_gdense, d_bitmap, d_ctr, dir_mode, thr, fuse_max);
+        tock(kPhGather);
+        GR_HIP(hipGetLastError());
+        fab = blk->n_blocks > 0;
+      }
+    }
+    fab_launched = fab;
     tick();
-    if (may_pull) {
+    if (may_pull && !fab) {
       k_pull_stream<<<grid_1d(g->nV, 256), 256, 0, stream>>>(d_ctr, dir_mode, thr, fuse_max,
                                                               g->nV, g->off_in, g->srcs,
                                                               d_chg_bits, d_bitmap);
@@ -2283,6 +2314,11 @@ struct Engine final : EngineBase {
       fuse_launched = false;
       fused_pending = fused;
       if (fused && !delta_ready) GR_TRY(drop_candidates());  // not reached: same conditions
+    }
+    if constexpr (kBlockedMin) {
+      if (fab_launched && !h_ctr->fab) stats.idle_gather_launches += 1;
+      fab_pending = fab_launched && h_ctr->fab && delta_ready;
+      fab_launched = false;
     }
     const unsigned long long c = h_ctr->chg_last;
     stats.changed_vertices += (int64_t)h_ctr->applied_last;

@@ -34,6 +34,7 @@ struct DevCounters {
   unsigned long long chg_last;    // chg as seen by the last nextIter
   int32_t pull;                   // last scatterActivate ran in pull mode
   int32_t fused;                  // last scatterActivate also pushed the next gather
+  int32_t fab;                    // last scatterActivate ran the next blocked gather
   unsigned long long inert;       // (#non-inert << 32) | out-edges, from k_inert_scan
   unsigned long long applied_last;  // vertices whose apply returned true since
                                     // the nextIter before the last one
@@ -483,6 +484,20 @@ __global__ void k_gather_fixup(int n_tiles, const Part<typename P::GatherResult>
   gtmp[r] = finalize<P>(has, acc);
 }
 
+// Direction rules of scatterActivate, shared by its kernels: pull when the
+// changed out-edges (low half of chg) exceed the threshold (mode 0; 1 push
+// only, 2 pull only); the fused activation + push gather of delta-gather
+// programs when they number at most fuse_max (>= 0).
+__device__ __forceinline__ bool use_pull_rule(unsigned long long chg, int mode,
+                                              long long threshold) {
+  if (mode == 2) return true;
+  if (mode == 1) return false;
+  return (long long)(uint32_t)chg > threshold;
+}
+__device__ __forceinline__ bool fused_scatter_rule(unsigned long long chg, long long fuse_max) {
+  return fuse_max >= 0 && (chg >> 32) != 0 && (long long)(uint32_t)chg <= fuse_max;
+}
+
 // ------------------------------------------------------- blocked gather
 //
 // Gather of every row by row blocks (gr_blocked): one workgroup per block
@@ -499,22 +514,40 @@ __global__ void k_gather_fixup(int n_tiles, const Part<typename P::GatherResult>
 constexpr int kBlkNT = 1024;  // one workgroup per CU (128 KB of LDS): 16 waves
 constexpr int kBlkVT = 8;
 
+//
+// Fused activation (frontier != nullptr): the kernel is launched by
+// scatterActivate in place of the pull and runs only when the pull would
+// (`run` decides from the counters, the same for every workgroup; workgroup 0
+// records the outcome in ctr->fab).  delta_bits is then the bitmap of the
+// vertices whose apply just returned true: the values are the next delta
+// gather's, and a row with an in-edge from such a vertex is marked in
+// `frontier` (which holds no bit of these rows before), as the pull marks it.
 template <class P>
 __global__ __launch_bounds__(kBlkNT) void k_gather_blocked(
     const int32_t* __restrict__ brow, const int32_t* __restrict__ bnr,
     const int32_t* __restrict__ boff, const int32_t* __restrict__ bsrc,
     const uint16_t* __restrict__ brl, const typename P::EdgeData* __restrict__ ed,
     const typename P::VertexData* __restrict__ vd, const source_value_t<P>* __restrict__ srcval,
-    const uint32_t* __restrict__ delta_bits, typename P::GatherResult* __restrict__ gdense) {
+    const uint32_t* __restrict__ delta_bits, typename P::GatherResult* __restrict__ gdense,
+    uint32_t* __restrict__ frontier = nullptr, DevCounters* __restrict__ ctr = nullptr,
+    int mode = 0, long long threshold = 0, long long fuse_max = -1) {
   typedef typename P::GatherResult G;
   static_assert(std::is_same<G, int>::value && delta_gather_v<P> && !P::kGatherUsesDst,
                 "the blocked gather folds int minima of source-only gatherMaps");
   __shared__ int acc[kBlockRows];
+  __shared__ uint32_t act[kBlockRows / 32];
   const int b = blockIdx.x;
+  const int tid = threadIdx.x;
+  if (frontier) {
+    const unsigned long long c = ctr->chg;
+    const bool run = use_pull_rule(c, mode, threshold) && !fused_scatter_rule(c, fuse_max);
+    if (b == 0 && tid == 0) ctr->fab = run ? 1 : 0;
+    if (!run) return;
+  }
   const int r0 = brow[b], nr = bnr[b];
   const int e0 = boff[b], e1 = boff[b + 1];
-  const int tid = threadIdx.x;
   for (int r = tid; r < nr; r += kBlkNT) acc[r] = P::gatherZero;
+  for (int k = tid; k < kBlockRows / 32; k += kBlkNT) act[k] = 0u;
   __syncthreads();
   for (int base = e0; base < e1; base += kBlkNT * kBlkVT) {
     int src[kBlkVT], row[kBlkVT];
@@ -547,11 +580,26 @@ __global__ __launch_bounds__(kBlkNT) void k_gather_blocked(
       val[m] = src[m] >= 0 ? map_source<P>(vd, vd, srcval, src[m], &ev[m]) : P::gatherZero;
     // a value not below gatherZero cannot lower a row that starts there
 #pragma unroll
-    for (int m = 0; m < kBlkVT; ++m)
+    for (int m = 0; m < kBlkVT; ++m) {
       if (src[m] >= 0 && val[m] < P::gatherZero) atomicMin(&acc[row[m]], val[m]);
+      if (frontier && src[m] >= 0) atomicOr(&act[row[m] >> 5], 1u << (row[m] & 31));
+    }
   }
   __syncthreads();
   for (int r = tid; r < nr; r += kBlkNT) gdense[r0 + r] = acc[r];
+  if (frontier) {
+    // the bitmap words of rows r0 .. r0+nr-1; the first and last may hold
+    // rows of the neighbouring blocks, hence atomicOr
+    const int w0 = r0 >> 5, w1 = (r0 + nr - 1) >> 5;
+    for (int w = w0 + tid; w <= w1; w += kBlkNT) {
+      uint32_t bits = 0u;
+      for (int k = 0; k < 32; ++k) {
+        const int r = w * 32 + k - r0;
+        if (r >= 0 && r < nr && ((act[r >> 5] >> (r & 31)) & 1u)) bits |= 1u << k;
+      }
+      if (bits) atomicOr(&frontier[w], bits);
+    }
+  }
 }
 
 // Programs whose gather is a sum of non-negative floats may declare

@@ -345,6 +345,11 @@ typedef struct gr_engine_stats {
   int64_t blocked_gathers;   /* gathers done by the blocked kernel (counted in
                                 tile_gather_rows / tile_gather_edges, timed in
                                 gather_ms) */
+  int64_t fused_activations; /* of those, the ones that scatterActivate ran in place
+                                of the pull, marking the next active set */
+  int64_t idle_gather_launches; /* blocked-kernel launches by scatterActivate that
+                                stood down on the device (the pull would not run);
+                                in gather_launches, their few microseconds in gather_ms */
 } gr_engine_stats;
 
 /* Enables hipEvent timing of the engine phases on the engine's stream. */
