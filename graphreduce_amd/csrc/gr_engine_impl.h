@@ -537,6 +537,17 @@ __global__ __launch_bounds__(kNT) void k_scatter_tiles(
     part[t] = max(0, upper_bound_dev(chg_e, n_c, min(t * kTile, w_c)) - 1);
 }
 
+// Targets below kRelaxHub (the highest-degree vertices of the degree order)
+// are folded per workgroup in LDS first: in the late iterations most changed
+// vertices have few out-edges and those lead mostly to the same few hubs, whose
+// atomics would be serialised where they are performed.  Each workgroup then
+// lowers a hub's candidate (and marks it) once, after its last tile.
+#ifndef GR_RELAX_HUB
+#define GR_RELAX_HUB 2048
+#endif
+constexpr int kRelaxHub = GR_RELAX_HUB;
+static_assert(kRelaxHub % kWave == 0, "hub folding flushes whole waves of ids");
+
 template <class P>
 // With ctr (the fused pass of scatterActivate): runs only when fused_scatter
 // holds, takes the edge count from ctr->chg and also marks the targets in the
@@ -550,10 +561,14 @@ template <class P>
     const typename P::EdgeData* __restrict__ ed_csr, typename P::GatherResult* __restrict__ cand,
     uint32_t* __restrict__ bitmap, int scan_marks) {
   static_assert(!P::kGatherUsesDst, "push relax needs a gatherMap without the destination");
+  typedef typename P::GatherResult G;
+  constexpr int kHub = kRelaxHub > 0 ? kRelaxHub : kWave;
   __shared__ int32_t s_start[kTile + 2];
   __shared__ int32_t s_csr[kTile + 2];
   __shared__ typename P::VertexData s_vd[kTile + 2];
   __shared__ uint16_t s_row[kTile];
+  __shared__ G s_hub[kHub];               // hub candidates of this workgroup
+  __shared__ uint32_t s_hub_mark[kHub / 32];  // hubs to mark in the bitmap
   bool scan = false;
   if (ctr) {
     const unsigned long long c = ctr->chg;
@@ -563,6 +578,12 @@ template <class P>
   }
   const int n_tiles = (w_c + kTile - 1) / kTile;
   const int tid = threadIdx.x;
+  if ((int)blockIdx.x >= n_tiles) return;  // uniform per workgroup
+  if (kRelaxHub > 0) {
+    for (int i = tid; i < kHub; i += kNT) s_hub[i] = P::gatherZero;
+    for (int i = tid; i < kHub / 32; i += kNT) s_hub_mark[i] = 0u;
+    // the first tile's __syncthreads orders these before any use
+  }
   for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
     const int k0 = part[t], k1 = part[t + 1];
     const int nr = k1 - k0 + 1;
@@ -607,6 +628,17 @@ template <class P>
 #pragma unroll
     for (int m = 0; m < kVT; ++m)
       val[m] = pos[m] >= 0 ? P::gatherMap(nullptr, &s_vd[row[m]], &ev[m]) : P::gatherZero;
+    if (kRelaxHub > 0) {
+#pragma unroll
+      for (int m = 0; m < kVT; ++m) {
+        if (vv[m] >= 0 && vv[m] < kRelaxHub) {
+          if (val[m] < P::gatherZero) atomicMin(&s_hub[vv[m]], val[m]);
+          if (bitmap && !(scan && val[m] < P::gatherZero))
+            atomicOr(&s_hub_mark[vv[m] >> 5], 1u << (vv[m] & 31));
+          vv[m] = -1;  // done with this edge
+        }
+      }
+    }
     G cur[kVT];
     uint32_t word[kVT];
 #pragma unroll
@@ -628,6 +660,20 @@ template <class P>
       if (need_or & (1u << m)) atomicOr(&bitmap[vv[m] >> 5], 1u << (vv[m] & 31));
     }
     __syncthreads();
+  }
+  if (kRelaxHub > 0) {
+    // the workgroup's hub candidates and marks (the loop's last
+    // __syncthreads made the LDS folds visible)
+    for (int i = tid; i < kHub; i += kNT) {
+      const G v = s_hub[i];
+      if (v < P::gatherZero && v < cand[i]) atomicMin(&cand[i], v);
+    }
+    if (bitmap) {
+      for (int k = tid; k < kHub / 32; k += kNT) {
+        const uint32_t m = s_hub_mark[k];
+        if (m && (bitmap[k] & m) != m) atomicOr(&bitmap[k], m);
+      }
+    }
   }
 }
 
