@@ -126,6 +126,10 @@ struct gr_graph {
 namespace gr {
 constexpr int kBlockRows = 32768;       // rows per gather block (LDS: 128 KB of int)
 constexpr int kBlockRowsWide = 16384;   // ... for 8-byte accumulators (LDS: 128 KB)
+#ifndef GR_MIN_BLOCK_ROWS
+#define GR_MIN_BLOCK_ROWS kBlockRows
+#endif
+constexpr int kMinBlockRows = GR_MIN_BLOCK_ROWS;  // rows per block of the int-minimum gather
 constexpr int kBlockEdges = 1 << 19;    // in-edges per gather block (largest target)
 int graph_free(gr_graph* g);
 // The degree-ordered copy of g (cached in g; freed with g).

@@ -2029,7 +2029,7 @@ struct Engine final : EngineBase {
   int blocked_setup() {
     if constexpr (kBlocked) {
       if (!blk) {
-        GR_TRY(graph_blocked(g, kBlockedSum ? kBlockRowsWide : kBlockRows, &blk));
+        GR_TRY(graph_blocked(g, kBlockedSum ? kBlockRowsWide : kMinBlockRows, &blk));
         GR_TRY(dalloc(&d_gdense, g->nV));
         k_fill<G><<<1024, 256, 0, stream>>>(g->nV, P::gatherZero, d_gdense);
         GR_HIP(hipGetLastError());

@@ -534,8 +534,8 @@ __global__ __launch_bounds__(kBlkNT) void k_gather_blocked(
   typedef typename P::GatherResult G;
   static_assert(std::is_same<G, int>::value && delta_gather_v<P> && !P::kGatherUsesDst,
                 "the blocked gather folds int minima of source-only gatherMaps");
-  __shared__ int acc[kBlockRows];
-  __shared__ uint32_t act[kBlockRows / 32];
+  __shared__ int acc[kMinBlockRows];
+  __shared__ uint32_t act[kMinBlockRows / 32];
   const int b = blockIdx.x;
   const int tid = threadIdx.x;
   if (frontier) {
@@ -547,7 +547,7 @@ __global__ __launch_bounds__(kBlkNT) void k_gather_blocked(
   const int r0 = brow[b], nr = bnr[b];
   const int e0 = boff[b], e1 = boff[b + 1];
   for (int r = tid; r < nr; r += kBlkNT) acc[r] = P::gatherZero;
-  for (int k = tid; k < kBlockRows / 32; k += kBlkNT) act[k] = 0u;
+  for (int k = tid; k < kMinBlockRows / 32; k += kBlkNT) act[k] = 0u;
   __syncthreads();
   for (int base = e0; base < e1; base += kBlkNT * kBlkVT) {
     int src[kBlkVT], row[kBlkVT];
@@ -579,10 +579,14 @@ __global__ __launch_bounds__(kBlkNT) void k_gather_blocked(
     for (int m = 0; m < kBlkVT; ++m)
       val[m] = src[m] >= 0 ? map_source<P>(vd, vd, srcval, src[m], &ev[m]) : P::gatherZero;
     // a value not below gatherZero cannot lower a row that starts there
+    // (activation: a row whose minimum ends below gatherZero had an in-edge
+    // from a changed source; only the other such edges need a mark)
 #pragma unroll
     for (int m = 0; m < kBlkVT; ++m) {
-      if (src[m] >= 0 && val[m] < P::gatherZero) atomicMin(&acc[row[m]], val[m]);
-      if (frontier && src[m] >= 0) atomicOr(&act[row[m] >> 5], 1u << (row[m] & 31));
+      if (src[m] >= 0 && val[m] < P::gatherZero)
+        atomicMin(&acc[row[m]], val[m]);
+      else if (frontier && src[m] >= 0)
+        atomicOr(&act[row[m] >> 5], 1u << (row[m] & 31));
     }
   }
   __syncthreads();
@@ -595,7 +599,9 @@ __global__ __launch_bounds__(kBlkNT) void k_gather_blocked(
       uint32_t bits = 0u;
       for (int k = 0; k < 32; ++k) {
         const int r = w * 32 + k - r0;
-        if (r >= 0 && r < nr && ((act[r >> 5] >> (r & 31)) & 1u)) bits |= 1u << k;
+        if (r >= 0 && r < nr &&
+            (acc[r] < P::gatherZero || ((act[r >> 5] >> (r & 31)) & 1u)))
+          bits |= 1u << k;
       }
       if (bits) atomicOr(&frontier[w], bits);
     }
