@@ -5,6 +5,11 @@
 drivers under ``graphreduce_amd/apps`` to ``graphreduce_amd/bin``.  Objects go
 to ``_build/`` at the repository root.  It always recompiles (the round-end
 build check calls it on a machine without a GPU; hipcc cross-compiles).
+
+``python -m graphreduce_amd._build --variant NAME -D MACRO=VALUE ...`` builds
+the library with compile-time parameters changed (for A/B measurements, e.g.
+``-D GR_RELAX_HUB=0``) to ``graphreduce_amd/lib/variants/NAME/``; a process
+loads it when ``GRAPHREDUCE_AMD_LIB`` names it.
 """
 from __future__ import annotations
 
@@ -73,23 +78,27 @@ def _run(cmd: list[str]) -> None:
         sys.stderr.write(out + "\n")
 
 
-def build_library(verbose: bool = False) -> Path:
+def build_library(verbose: bool = False, defines: list[str] | None = None,
+                  objdir: Path | None = None, libdir: Path | None = None) -> Path:
     cc = hipcc()
-    OBJDIR.mkdir(parents=True, exist_ok=True)
-    LIBDIR.mkdir(parents=True, exist_ok=True)
+    objdir = objdir or OBJDIR
+    libdir = libdir or LIBDIR
+    objdir.mkdir(parents=True, exist_ok=True)
+    libdir.mkdir(parents=True, exist_ok=True)
+    extra = [f"-D{d}" for d in (defines or [])]
     objs = []
     cmds = []
     for src in SOURCES:
-        obj = OBJDIR / (Path(src).stem + ".o")
+        obj = objdir / (Path(src).stem + ".o")
         objs.append(obj)
         lang = ["-x", "hip"] if src.endswith(".hip") else []
-        cmds.append([cc, *HIP_FLAGS, *lang, "-c", str(CSRC / src), "-o", str(obj)])
+        cmds.append([cc, *HIP_FLAGS, *extra, *lang, "-c", str(CSRC / src), "-o", str(obj)])
     workers = min(len(cmds), max(1, min(8, os.cpu_count() or 1)))
     with ThreadPoolExecutor(max_workers=workers) as ex:
         for _ in ex.map(_run, cmds):
             pass
-    lib = LIBDIR / LIBNAME
-    tmp = LIBDIR / (LIBNAME + ".tmp")
+    lib = libdir / LIBNAME
+    tmp = libdir / (LIBNAME + ".tmp")
     _run([cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs), "-lz"])
     os.replace(tmp, lib)
     if verbose:
@@ -164,5 +173,24 @@ def build(verbose: bool = True) -> None:
     build_oracle(verbose)
 
 
+def build_variant(name: str, defines: list[str], verbose: bool = True) -> Path:
+    """The library with compile-time parameters changed, for A/B measurements."""
+    if not name.replace("_", "").replace("-", "").isalnum():
+        raise ValueError(f"variant name {name!r}: letters, digits, '_' and '-' only")
+    return build_library(verbose, defines, objdir=OBJDIR / "variants" / name,
+                         libdir=LIBDIR / "variants" / name)
+
+
 if __name__ == "__main__":
-    build(verbose=True)
+    import argparse
+
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--variant", help="build only the library, with -D flags, as a named variant")
+    ap.add_argument("-D", dest="defines", action="append", default=[], metavar="MACRO=VALUE")
+    a = ap.parse_args()
+    if a.variant:
+        build_variant(a.variant, a.defines)
+    elif a.defines:
+        ap.error("-D needs --variant (the default build uses the sources' values)")
+    else:
+        build(verbose=True)

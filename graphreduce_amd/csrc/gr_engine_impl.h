@@ -2056,7 +2056,7 @@ struct Engine final : EngineBase {
         if constexpr (kBlockedSum)
           k_gather_blocked_sum<P><<<blk->n_blocks, kBlkNT, 0, stream>>>(
               blk->brow, blk->bnr, blk->boff, blk->bsrc, blk->brl, d_ed_blk, d_vd, d_srcval,
-              d_gdense, nullptr, nullptr);
+              d_gdense, nullptr, nullptr, nullptr);
         else
           k_gather_blocked<P><<<blk->n_blocks, kBlkNT, 0, stream>>>(
               blk->brow, blk->bnr, blk->boff, blk->bsrc, blk->brl, d_ed_blk, d_vd, d_srcval,
@@ -2224,7 +2224,7 @@ struct Engine final : EngineBase {
         if (blk->n_blocks > 0)
           k_gather_blocked_sum<P><<<blk->n_blocks, kBlkNT, 0, stream>>>(
               blk->brow, blk->bnr, blk->boff, blk->bsrc, blk->brl, d_ed_blk, d_vd, d_srcval,
-              d_gdense, d_chg_bits, d_bitmap);
+              d_gdense, d_chg_bits, d_bitmap, d_ctr);
         tock(kPhGather);
         GR_HIP(hipGetLastError());
         fab_pending = true;

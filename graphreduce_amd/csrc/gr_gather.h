@@ -630,6 +630,8 @@ constexpr double kFixUnscale = 1.0 / 4294967296.0;     // 2^-32
 // With chg_bits (the vertices whose apply just returned true), the same pass
 // is also scatterActivate's pull: a row with an in-edge from such a vertex is
 // marked in the `frontier` bitmap (which holds no bit of these rows before).
+// It then stands down when ctr->chg shows no changed vertex with out-edges:
+// the next active set is empty and no gather follows.
 template <class P>
 __global__ __launch_bounds__(kBlkNT) void k_gather_blocked_sum(
     const int32_t* __restrict__ brow, const int32_t* __restrict__ bnr,
@@ -637,12 +639,13 @@ __global__ __launch_bounds__(kBlkNT) void k_gather_blocked_sum(
     const uint16_t* __restrict__ brl, const typename P::EdgeData* __restrict__ ed,
     const typename P::VertexData* __restrict__ vd, const source_value_t<P>* __restrict__ srcval,
     typename P::GatherResult* __restrict__ gdense, const uint32_t* __restrict__ chg_bits,
-    uint32_t* __restrict__ frontier) {
+    uint32_t* __restrict__ frontier, const DevCounters* __restrict__ ctr) {
   typedef typename P::GatherResult G;
   static_assert(std::is_same<G, float>::value && fixed_point_sum_v<P> && !P::kGatherUsesDst,
                 "the fixed-point blocked gather sums non-negative floats");
   __shared__ unsigned long long acc[kBlockRowsWide];
   __shared__ uint8_t act[kBlockRowsWide];
+  if (frontier && ctr->chg == 0) return;  // uniform: nothing to activate or gather
   const int b = blockIdx.x;
   const int r0 = brow[b], nr = bnr[b];
   const int e0 = boff[b], e1 = boff[b + 1];
