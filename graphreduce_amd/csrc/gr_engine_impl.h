@@ -737,11 +737,18 @@ __global__ void k_fill(int64_t n, T value, T* __restrict__ out) {
 // lanes' row offsets; rows already hit, and heavy rows (in-degree above
 // kPullHeavy, done by k_pull_heavy with a whole workgroup and early exit),
 // are jumped over.  Before that, each lane tests kPullFirst in-edges of its
-// own row, the first and the last half of them, with all loads in flight at
-// once: rows list their sources by id, which the degree order makes hubs
-// first, so a change among the hubs shows at the front of a row and one among
-// the low-degree vertices at its back.  Rows with at most kPullFirst in-edges
-// are settled either way, and the segment walk starts at the first open row.
+// own row, heavy or not, the first and the last half of them, with all loads
+// in flight at once: rows list their sources by id, which the degree order
+// makes hubs first, so a change among the hubs shows at the front of a row and
+// one among the low-degree vertices at its back.  Rows with at most kPullFirst
+// in-edges are settled either way, and the segment walk starts at the first
+// open row; k_pull_heavy skips the heavy rows this test already marked.
+#ifndef GR_PULL_HEAVY_FIRST
+#define GR_PULL_HEAVY_FIRST 1
+#endif
+#ifndef GR_PULL_GRID
+#define GR_PULL_GRID 4096
+#endif
 constexpr int kPullHeavy = 1024;
 constexpr int kPullFirst = 4;
 constexpr int kHubVertices = 1 << 20;  // push marks by byte store below this id
@@ -769,7 +776,7 @@ __device__ __forceinline__ uint64_t lane_range_mask(int lo, int hi) {  // bits [
     const bool heavy = deg > kPullHeavy;
     // kPullFirst in-edges of the lane's own row: the first and the last half
     bool first_hit = false;
-    if (!heavy && deg > 0) {
+    if (deg > 0 && (GR_PULL_HEAVY_FIRST || !heavy)) {
       int u[kPullFirst];
 #pragma unroll
       for (int k = 0; k < kPullFirst; ++k) {
@@ -830,6 +837,8 @@ __device__ __forceinline__ uint64_t lane_range_mask(int lo, int hi) {  // bits [
   if (!use_pull(ctr->chg, mode, threshold) || fused_scatter(ctr->chg, fuse_max)) return;
   for (int h = blockIdx.x; h < n_heavy; h += gridDim.x) {
     const int v = heavy[h];
+    // marked by k_pull_stream's test of the row's first and last in-edges
+    if ((bitmap[v >> 5] >> (v & 31)) & 1u) continue;  // uniform per workgroup
     const int b = off_in[v], e = off_in[v + 1];
     for (int pb = b; pb < e; pb += blockDim.x) {
       const int p = pb + threadIdx.x;
@@ -2280,7 +2289,9 @@ struct Engine final : EngineBase {
     fab_launched = fab;
     tick();
     if (may_pull && !fab) {
-      k_pull_stream<<<grid_1d(g->nV, 256), 256, 0, stream>>>(d_ctr, dir_mode, thr, fuse_max,
+      // (grid-stride over the vertices: a capped grid also stands down fast)
+      const unsigned pull_grid = GR_PULL_GRID > 0 && grid_1d(g->nV, 256) > GR_PULL_GRID ? GR_PULL_GRID : grid_1d(g->nV, 256);
+      k_pull_stream<<<pull_grid, 256, 0, stream>>>(d_ctr, dir_mode, thr, fuse_max,
                                                               g->nV, g->off_in, g->srcs,
                                                               d_chg_bits, d_bitmap);
       if (n_heavy > 0)
