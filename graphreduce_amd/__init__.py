@@ -700,10 +700,11 @@ class GASEngine:
         streamed per iteration (0: the graph lives in device memory)."""
         _check(lib().gr_engine_set_streaming(self._h, int(max_edges_per_shard)))
 
-    def setVertexOrder(self, order: int = 1) -> None:
-        """1 (default): run on the degree-ordered copy of the graph; 0: as
-        given.  Results are in the caller's ids either way; takes effect at
-        the next setGraph/bindGraph."""
+    def setVertexOrder(self, order: int = 2) -> None:
+        """1: run on the degree-ordered copy of the graph; 0: as given;
+        2 (default): the copy when some vertex's in+out degree is at least 32
+        times the mean.  Results are in the caller's ids either way; takes
+        effect at the next setGraph/bindGraph."""
         _check(lib().gr_engine_set_vertex_order(self._h, order))
 
     def setDeltaGather(self, mode=1) -> None:

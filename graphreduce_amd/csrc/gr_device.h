@@ -115,6 +115,9 @@ struct gr_graph {
   // ties by id); its rows are sorted by the new neighbour ids and its eid
   // maps index this graph's edge list.
   gr_graph* ordered = nullptr;
+  // 1 when the degrees are skewed enough for the degree-ordered copy to pay
+  // (graph_skewed; -1 until first asked)
+  int skewed = -1;
   int32_t* perm = nullptr;  // ordered id -> id   [V], device
   int32_t* inv = nullptr;   // id -> ordered id   [V], device
   // row blocks of at most kBlockRows / kBlockRowsWide rows, built on first
@@ -134,6 +137,26 @@ constexpr int kBlockEdges = 1 << 19;    // in-edges per gather block (largest ta
 int graph_free(gr_graph* g);
 // The degree-ordered copy of g (cached in g; freed with g).
 int graph_ordered(gr_graph* g, gr_graph** out);
+// Whether some vertex's in+out degree is at least kSkewFactor times the mean
+// (2E / V): then the hubs' values are worth packing into few cache lines by
+// the degree order.  A graph of uniform degree (a grid, a road map) keeps its
+// own numbering, whose neighbourhoods are already local.
+constexpr int kSkewFactor = 32;
+inline bool graph_skewed(gr_graph* g) {
+  if (g->skewed < 0 && (g->h_off_out.size() != (size_t)g->nV + 1 ||
+                         g->h_off_in.size() != (size_t)g->nV + 1))
+    return true;  // no host offsets to look at: keep the degree order
+  if (g->skewed < 0) {
+    int64_t mx = 0;
+    for (int32_t v = 0; v < g->nV; ++v) {
+      const int64_t d = (int64_t)(g->h_off_out[v + 1] - g->h_off_out[v]) +
+                        (int64_t)(g->h_off_in[v + 1] - g->h_off_in[v]);
+      if (d > mx) mx = d;
+    }
+    g->skewed = g->nV > 0 && mx * (int64_t)g->nV >= (int64_t)kSkewFactor * 2 * g->nE ? 1 : 0;
+  }
+  return g->skewed == 1;
+}
 // The row blocks of g's CSC with in-edges listed by source, at most max_rows
 // (kBlockRows or kBlockRowsWide) rows each (cached in g).
 int graph_blocked(gr_graph* g, int max_rows, const gr_blocked** out);

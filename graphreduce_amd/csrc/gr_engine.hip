@@ -221,8 +221,10 @@ extern "C" int gr_engine_set_streaming(gr_engine* e, int64_t max_edges_per_shard
 
 extern "C" int gr_engine_set_vertex_order(gr_engine* e, int order) {
   GR_TRY(check(e, false));
-  if (order != 0 && order != 1)
-    return set_error(GR_ERR_INVALID, "vertex order must be 0 (as given) or 1 (by degree)");
+  if (order < 0 || order > 2)
+    return set_error(GR_ERR_INVALID,
+                     "vertex order must be 0 (as given), 1 (by degree) or 2 (by degree when "
+                     "the degrees are skewed)");
   e->impl->vertex_order = order;
   return GR_OK;
 }

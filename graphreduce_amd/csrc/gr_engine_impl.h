@@ -15,10 +15,13 @@
 //                    out-neighbour in the next-frontier bitmap; pull:
 //                    k_pull_stream / k_pull_heavy test each vertex's in-edges
 //                    against the changed bitmap.
-//   nextIter         k_compact_count/scan/write: bitmap -> ascending active
-//                    list plus the exclusive scan of its in-degrees (the
-//                    gather's row offsets); one 32-byte device->host read
-//                    returns the new counts.
+//   nextIter         k_compact_count/scan: the frontier bitmap becomes the
+//                    active bitmap, counted with its in-degrees; the counts
+//                    reach the host through mapped memory.  k_compact_write
+//                    turns the active bitmap into the ascending active list
+//                    plus the exclusive scan of its in-degrees (the gather's
+//                    row offsets) when a phase needs the list; apply runs
+//                    from the bitmap otherwise.
 //
 // This header holds the engine's kernels and the Engine<Program> template; it
 // is shared by the library (which instantiates the built-in programs in
@@ -76,26 +79,46 @@ constexpr int kApplyNT = 256;
 constexpr int kApplyVT = 8;
 constexpr int kApplyChunk = kApplyNT * kApplyVT;
 
-// Program::apply over the active list.  Each workgroup takes a chunk of
-// kApplyChunk entries; the vertices whose apply returned true and that have
-// out-edges are appended to the changed list with ONE atomic per chunk (a
-// block scan assigns positions), keeping out-edge offsets increasing with the
-// list index; their bits are set in the changed bitmap for pull mode.
-template <class P>
+// Program::apply over the active set: the ascending active list (kBits
+// false: entry i is vertex active[i]) or the active bitmap (kBits true: entry
+// i is vertex i, taken when its bit is set; n = nV).  Each workgroup takes a
+// chunk of kApplyChunk entries, thread t entries chunk + m * kApplyNT + t.
+// The vertices whose apply returned true and that have out-edges are appended
+// to the changed list with ONE atomic per chunk, in entry order (ascending
+// vertex ids within the chunk), so that a wave's lanes store to consecutive
+// positions, with their out-edge offsets increasing with the list index;
+// their bits are set in the changed bitmap for pull mode.
+template <class P, bool kBits>
 __global__ __launch_bounds__(kApplyNT) void k_apply(
-    int n, const int32_t* __restrict__ active, const typename P::GatherResult* __restrict__ gtmp,
-    int use_gather, typename P::VertexData* __restrict__ vd, const int32_t* __restrict__ off_out,
+    int n, const int32_t* __restrict__ active, const uint32_t* __restrict__ act_bits,
+    const typename P::GatherResult* __restrict__ gtmp, int use_gather,
+    typename P::VertexData* __restrict__ vd, const int32_t* __restrict__ off_out,
     int iteration, int32_t* __restrict__ chg_v, int32_t* __restrict__ chg_e,
     uint32_t* __restrict__ chg_bits, DevCounters* __restrict__ ctr,
     source_value_t<P>* __restrict__ srcval, typename P::GatherResult* __restrict__ cand,
     unsigned long long* __restrict__ applied) {
-  __shared__ int s_a[kApplyNT / kWave], s_b[kApplyNT / kWave];
+  constexpr int kNW = kApplyNT / kWave;
+  constexpr int kWords = kApplyChunk / 32;
+  static_assert(kWords <= kApplyNT, "one bitmap word per thread");
+  __shared__ uint32_t s_words[kWords];
+  __shared__ int s_cnt[kApplyVT * kNW], s_deg[kApplyVT * kNW];
   __shared__ unsigned long long s_base;
   __shared__ unsigned s_applied;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, wv = tid / kWave;
   if (tid == 0) s_applied = 0;
   unsigned my_applied = 0;  // applies that returned true, this thread
   for (int chunk = blockIdx.x * kApplyChunk; chunk < n; chunk += gridDim.x * kApplyChunk) {
+    if constexpr (kBits) {
+      // the chunk's bitmap words; a chunk without active vertices is skipped
+      // (uniform: every thread sees the same barrier result)
+      uint32_t word = 0u;
+      if (tid < kWords) {
+        const int wi = (chunk >> 5) + tid;
+        word = wi * 32 < n ? act_bits[wi] : 0u;
+        s_words[tid] = word;
+      }
+      if (!__syncthreads_or(word != 0u)) continue;
+    }
     // In phases over the lane's kApplyVT entries (ids and gather values,
     // vertex values, apply, out-degrees of the changed ones, then the stores
     // and bit atomics), so that the loads of all entries are in flight
@@ -106,13 +129,16 @@ __global__ __launch_bounds__(kApplyNT) void k_apply(
     typename P::GatherResult gs[kApplyVT];
     VD vs[kApplyVT];
     unsigned valid_mask = 0, r_mask = 0, chg_mask = 0, want_mask = 0;
-    int my_cnt = 0, my_edges = 0;
 #pragma unroll
     for (int m = 0; m < kApplyVT; ++m) {
       const int i = chunk + m * kApplyNT + tid;
       dvs[m] = 0;
       gs[m] = P::gatherZero;
-      if (i < n) {
+      if constexpr (kBits) {
+        dvs[m] = i;
+        if (i < n && ((s_words[(m * kApplyNT + tid) >> 5] >> (i & 31)) & 1u))
+          valid_mask |= 1u << m;
+      } else if (i < n) {
         valid_mask |= 1u << m;
         dvs[m] = active[i];
         if (use_gather == 1) gs[m] = gtmp[i];
@@ -155,6 +181,10 @@ __global__ __launch_bounds__(kApplyNT) void k_apply(
     }
 #pragma unroll
     for (int m = 0; m < kApplyVT; ++m) degs[m] = (r_mask & (1u << m)) ? o1[m] - o0[m] : 0;
+    // positions in entry order: per (m, wave) the count of changed entries
+    // and the sum of their out-degrees, scanned below
+    unsigned long long ball[kApplyVT];
+    int incl[kApplyVT];
 #pragma unroll
     for (int m = 0; m < kApplyVT; ++m) {
       if (chg_mask & (1u << m)) {
@@ -162,36 +192,43 @@ __global__ __launch_bounds__(kApplyNT) void k_apply(
         if constexpr (source_value_v<P>) srcval[dvs[m]] = P::sourceValue(vs[m]);
       }
       const bool want = degs[m] > 0;  // apply returned true and out-edges exist
-      if (chg_bits) wave_set_bits_sorted((valid_mask >> m) & 1u, want, dvs[m], chg_bits);
-      if (want) {
-        want_mask |= 1u << m;
-        my_cnt += 1;
-        my_edges += degs[m];
+      if (chg_bits) wave_set_bits_sorted(chunk + m * kApplyNT + tid < n, want, dvs[m], chg_bits);
+      if (want) want_mask |= 1u << m;
+      ball[m] = __ballot(want);
+      incl[m] = wave_incl_scan_add(want ? degs[m] : 0);
+      if (lane_id() == kWave - 1) {
+        s_cnt[m * kNW + wv] = __popcll(ball[m]);
+        s_deg[m * kNW + wv] = incl[m];
       }
     }
     my_applied += __popc(r_mask);
-    int tc, te;
-    const int ec = block_excl_scan_add<kApplyNT>(my_cnt, s_a, &tc);
-    const int ee = block_excl_scan_add<kApplyNT>(my_edges, s_b, &te);
-    if (tc > 0) {
-      if (tid == 0)
-        s_base = atomicAdd(&ctr->chg, ((unsigned long long)tc << 32) |
-                                          (unsigned long long)(uint32_t)te);
-      __syncthreads();
+    __syncthreads();
+    if (tid == 0) {
+      int c = 0, e = 0;
+      for (int k = 0; k < kApplyVT * kNW; ++k) {
+        const int a = s_cnt[k], b = s_deg[k];
+        s_cnt[k] = c;
+        s_deg[k] = e;
+        c += a;
+        e += b;
+      }
+      s_base = c ? atomicAdd(&ctr->chg, ((unsigned long long)c << 32) |
+                                            (unsigned long long)(uint32_t)e)
+                 : 0ull;
+    }
+    __syncthreads();
+    if (want_mask) {
       const unsigned long long base = s_base;
-      int idx = (int)(base >> 32) + ec;
-      int eo = (int)(uint32_t)base + ee;
 #pragma unroll
       for (int m = 0; m < kApplyVT; ++m) {
         if (want_mask & (1u << m)) {
+          const int idx = (int)(base >> 32) + s_cnt[m * kNW + wv] + lanes_below(ball[m]);
           chg_v[idx] = dvs[m];
-          chg_e[idx] = eo;
-          ++idx;
-          eo += degs[m];
+          chg_e[idx] = (int)(uint32_t)base + s_deg[m * kNW + wv] + incl[m] - degs[m];
         }
       }
-      __syncthreads();
     }
+    __syncthreads();  // the next chunk reuses s_words, s_cnt, s_deg and s_base
   }
   // every apply that returned true counts toward sum #C, also those of
   // vertices without out-edges, which join no changed list
@@ -890,10 +927,15 @@ __device__ __forceinline__ ByteWork byte_work(const uint32_t* __restrict__ bitma
   return b;
 }
 
-[[maybe_unused]] static __global__ __launch_bounds__(kCompactNT) void k_compact_count(const uint32_t* __restrict__ bitmap,
-                                                              int n_words, int nV,
-                                                              const int32_t* __restrict__ off_in,
-                                                              int need_deg, int2* __restrict__ blk) {
+// Counts the frontier's vertices and in-degrees per block of kCompactNT words,
+// then moves those words into the active bitmap `act_bits` (the next
+// iteration's active set) and clears them in the frontier and in `chg_clear`
+// (the changed bitmap the next apply fills), so that scatterActivate starts
+// from an empty frontier.
+[[maybe_unused]] static __global__ __launch_bounds__(kCompactNT) void k_compact_count(
+    uint32_t* __restrict__ bitmap, int n_words, int nV, const int32_t* __restrict__ off_in,
+    int need_deg, int2* __restrict__ blk, uint32_t* __restrict__ act_bits,
+    uint32_t* __restrict__ chg_clear) {
   __shared__ int s_a[kCompactNT / kWave], s_b[kCompactNT / kWave];
   int cnt = 0, deg = 0;
   for (int r = 0; r < kCompactRounds; ++r) {
@@ -908,7 +950,7 @@ __device__ __forceinline__ ByteWork byte_work(const uint32_t* __restrict__ bitma
     s_a[threadIdx.x / kWave] = cnt;
     s_b[threadIdx.x / kWave] = deg;
   }
-  __syncthreads();
+  __syncthreads();  // also: every thread has read its bytes before the words move
   if (threadIdx.x == 0) {
     int a = 0, b = 0;
     for (int i = 0; i < kCompactNT / kWave; ++i) {
@@ -916,6 +958,14 @@ __device__ __forceinline__ ByteWork byte_work(const uint32_t* __restrict__ bitma
       b += s_b[i];
     }
     blk[blockIdx.x] = make_int2(a, b);
+  }
+  // the block's bytes are words blockIdx.x * kCompactNT ... + kCompactNT - 1
+  static_assert(kCompactRounds * kCompactNT / 4 == kCompactNT, "one word per thread");
+  const int wi = blockIdx.x * kCompactNT + threadIdx.x;
+  if (wi < n_words) {
+    act_bits[wi] = bitmap[wi];
+    bitmap[wi] = 0u;
+    if (chg_clear) chg_clear[wi] = 0u;
   }
 }
 
@@ -966,29 +1016,24 @@ __device__ __forceinline__ ByteWork byte_work(const uint32_t* __restrict__ bitma
   }
 }
 
-// Writes the ascending active list, its exclusive in-degree scan and the CSC
-// base per row; clears the frontier bitmap and the previous iteration's
-// changed bitmap (the host swaps it in as the buffer the next apply fills).
+// Writes the ascending active list of the active bitmap (as k_compact_count
+// left it), its exclusive in-degree scan and the CSC base per row.  Run when a
+// phase needs the list (the merge-path gather, the streamed gather, a flush
+// of pending candidates, apply after such a gather).
 [[maybe_unused]] static __global__ __launch_bounds__(kCompactNT) void k_compact_write(
-    uint32_t* __restrict__ bitmap, int n_words, int nV, const int32_t* __restrict__ off_in,
+    const uint32_t* __restrict__ bitmap, int n_words, int nV, const int32_t* __restrict__ off_in,
     int need_deg, const int2* __restrict__ blk, const DevCounters* __restrict__ ctr,
-    int32_t* __restrict__ active, int32_t* __restrict__ escan, int32_t* __restrict__ cbase,
-    uint32_t* __restrict__ chg_bits) {
+    int32_t* __restrict__ active, int32_t* __restrict__ escan, int32_t* __restrict__ cbase) {
   // a round's entries (at most 8 per thread) are staged here and written out
   // coalesced: each thread's own entries are consecutive, a wave's are not
   constexpr int kRoundMax = 8 * kCompactNT;
   __shared__ unsigned long long s_scan[kCompactNT / kWave];
   __shared__ int32_t s_act[kRoundMax], s_esc[kRoundMax], s_cb[kRoundMax];
   const int2 base = blk[blockIdx.x];
-  // a block without frontier bits (its count is the gap to the next block's
-  // offset) only clears its words of the changed bitmap
+  // a block without active bits (its count is the gap to the next block's
+  // offset) has nothing to write
   const int next = blockIdx.x + 1 < (int)gridDim.x ? blk[blockIdx.x + 1].x : ctr->n_active;
-  if (next == base.x) {
-    const int wi = blockIdx.x * (kCompactRounds * kCompactNT / 4) + threadIdx.x;
-    if (chg_bits && threadIdx.x < kCompactRounds * kCompactNT / 4 && wi < n_words)
-      chg_bits[wi] = 0u;
-    return;
-  }
+  if (next == base.x) return;
   int pos_run = base.x, deg_run = base.y;
   for (int r = 0; r < kCompactRounds; ++r) {
     const int bi = (blockIdx.x * kCompactRounds + r) * kCompactNT + threadIdx.x;
@@ -1002,12 +1047,6 @@ __device__ __forceinline__ ByteWork byte_work(const uint32_t* __restrict__ bitma
     int dpos = deg_run + (int)(ex >> 16);
     pos_run += round_cnt;
     deg_run += (int)(tot >> 16);
-    // every thread has read its byte before the scan's barriers
-    const int wi = bi >> 2;
-    if ((bi & 3) == 0 && wi < n_words) {
-      if (chg_bits) chg_bits[wi] = 0u;
-      bitmap[wi] = 0u;
-    }
     if (round_cnt == 0) continue;  // block-uniform
     uint32_t bits = b.bits;
     while (bits) {
@@ -1193,7 +1232,9 @@ struct EngineBase {
   int dir_mode = 0;
   double dir_alpha = 15.0;
   int gather_vt = 8;  // merge-path items per thread in the gather tiles (4, 8 or 16)
-  int vertex_order = 1;  // 1: engine works on the degree-ordered copy of the graph
+  // 1: the engine works on the degree-ordered copy of the graph; 0: on the
+  // graph as given; 2: the copy when the degrees are skewed (graph_skewed)
+  int vertex_order = 2;
   int64_t stream_max_edges = 0;  // > 0: out-of-core shards of at most this many edges
   double fuse_share = 0.4;        // fused activation + push gather up to this share of E
   // blocked gather (programs that qualify) for gathers over at least this
@@ -1276,7 +1317,12 @@ struct Engine final : EngineBase {
   G* d_gtmp = nullptr;
   int32_t* d_chg_v = nullptr;
   int32_t* d_chg_e = nullptr;
-  uint32_t* d_bitmap = nullptr;
+  uint32_t* d_bitmap = nullptr;   // next frontier (scatterActivate marks it; empty otherwise)
+  // The active set after nextIter is the bitmap d_act_bits; the ascending list
+  // (d_active, d_escan, d_cbase) is written from it only when a phase needs it
+  // (ensure_list); setActive writes the list itself.
+  uint32_t* d_act_bits = nullptr;
+  bool list_valid = true;
   uint32_t* d_chg_bits = nullptr;  // vertices changed by this iteration's apply
   uint32_t* d_chg_prev = nullptr;  // ... by the previous iteration's apply
   int2* d_blk = nullptr;
@@ -1343,8 +1389,8 @@ struct Engine final : EngineBase {
   size_t edge_size() const override { return kEdgeEmpty ? 0 : sizeof(ED); }
 
   void release() {
-    // nextIter returns while k_compact_write may still run: drain the stream
-    // before its buffers go
+    // nextIter returns while the compaction scan may still run: drain the
+    // stream before its buffers go
     if (stream) (void)hipStreamSynchronize(stream);
     for (void* p : {(void*)d_ed_blk, (void*)d_gdense})
       if (p) (void)hipFree(p);
@@ -1356,7 +1402,7 @@ struct Engine final : EngineBase {
                     d_vd_user,   d_ed,     d_ed_csr,   d_cand,   d_active, d_escan,
                     d_gtmp,      d_chg_v,  d_chg_e,    d_bitmap, d_chg_bits, d_chg_prev,
                     d_blk,       d_part,   d_push_part, d_carry, d_head,   d_ctr,
-                    d_applied};
+                    d_applied,   d_act_bits};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     if (h_ctr) (void)hipHostFree(h_ctr);
@@ -1410,7 +1456,8 @@ struct Engine final : EngineBase {
     d_cbase = nullptr;
     d_active = d_escan = d_chg_v = d_chg_e = d_part = d_push_part = nullptr;
     d_gtmp = nullptr;
-    d_bitmap = d_chg_bits = d_chg_prev = nullptr;
+    d_bitmap = d_chg_bits = d_chg_prev = d_act_bits = nullptr;
+    list_valid = true;
     d_blk = nullptr;
     d_carry = d_head = nullptr;
     d_ctr = h_ctr = d_hctr = nullptr;
@@ -1450,7 +1497,8 @@ struct Engine final : EngineBase {
     owns_graph = owns;
     host_vdata = vdata;
     streaming = stream_mode;  // edge arrays stay in host memory (bind_streaming)
-    if (vertex_order == 1 && graph->nV > 0 && !streaming) {
+    if (graph->nV > 0 && !streaming &&
+        (vertex_order == 1 || (vertex_order == 2 && graph_skewed(graph)))) {
       GR_TRY(graph_ordered(graph, &g));
       d_perm = graph->perm;
       d_inv = graph->inv;
@@ -1473,6 +1521,7 @@ struct Engine final : EngineBase {
     GR_TRY(dalloc(&d_chg_v, V));
     GR_TRY(dalloc(&d_chg_e, V));
     GR_TRY(dalloc(&d_bitmap, n_words + 1));
+    GR_TRY(dalloc(&d_act_bits, n_words + 1));
     GR_TRY(dalloc(&d_chg_bits, n_words + 1));
     GR_TRY(dalloc(&d_chg_prev, n_words + 1));
     GR_TRY(dalloc(&d_blk, n_cblocks));
@@ -1490,9 +1539,11 @@ struct Engine final : EngineBase {
     GR_HIP(hipHostGetDevicePointer((void**)&d_hseq, h_seq, 0));
     __atomic_store_n(h_seq, ctr_seq, __ATOMIC_RELEASE);
     GR_HIP(hipMemsetAsync(d_bitmap, 0, sizeof(uint32_t) * (n_words + 1), stream));
+    GR_HIP(hipMemsetAsync(d_act_bits, 0, sizeof(uint32_t) * (n_words + 1), stream));
     GR_HIP(hipMemsetAsync(d_chg_bits, 0, sizeof(uint32_t) * (n_words + 1), stream));
     GR_HIP(hipMemsetAsync(d_chg_prev, 0, sizeof(uint32_t) * (n_words + 1), stream));
     GR_HIP(hipMemsetAsync(d_ctr, 0, sizeof(DevCounters), stream));
+    list_valid = true;
     GR_HIP(hipMemsetAsync(d_applied, 0,
                           sizeof(unsigned long long) * kAppliedSlots * kAppliedStride, stream));
     {
@@ -1747,6 +1798,7 @@ struct Engine final : EngineBase {
   // Gather over the CSC shards that hold active rows; shard k + 1 is copied
   // while shard k is gathered (two staging buffers).
   int gather_streamed() {
+    GR_TRY(ensure_list());
     const int n = n_active;
     const int S = (int)csc_vb.size() - 1;
     k_shard_positions<<<grid_1d((int64_t)S + 1, 256), 256, 0, stream>>>(
@@ -1885,6 +1937,7 @@ struct Engine final : EngineBase {
                                                                          d_active, d_escan, d_cbase,
                                                                          d_ctr);
     GR_HIP(hipGetLastError());
+    list_valid = true;
     n_active = n;
     w_edges = g->h_off_in[end] - g->h_off_in[start];
     n_applies = 0;
@@ -1924,6 +1977,7 @@ struct Engine final : EngineBase {
                           sizeof(unsigned long long) * kAppliedSlots * kAppliedStride, stream));
     k_set_counters<<<1, 1, 0, stream>>>(d_ctr, n, (int32_t)acc);
     GR_HIP(hipGetLastError());
+    list_valid = true;
     GR_HIP(hipStreamSynchronize(stream));  // the host vectors go out of scope
     n_active = n;
     w_edges = (int32_t)acc;
@@ -2002,6 +2056,7 @@ struct Engine final : EngineBase {
       }
     }
     if (blocked_gather_applies(w)) return gather_blocked(n, w, dbits);
+    GR_TRY(ensure_list());
     const int tile = kNT * gather_vt;
     const int n_tiles = (int)(((int64_t)n + w + tile - 1) / tile);
     k_merge_partition<<<grid_1d((int64_t)n_tiles + 1, 256), 256, 0, stream>>>(d_escan, n, w,
@@ -2109,6 +2164,7 @@ struct Engine final : EngineBase {
   int collect_flush() {
     if constexpr (kDelta) {
       if (collect_pending) {
+        GR_TRY(ensure_list());
         k_relax_collect<P><<<grid_1d(n_active, 256), 256, 0, stream>>>(n_active, d_active, d_cand,
                                                                        d_gtmp);
         GR_HIP(hipGetLastError());
@@ -2190,15 +2246,31 @@ struct Engine final : EngineBase {
     const int use_gather = !P::kHasGather ? 0 : collect_pending ? 2 : dense_pending ? 3 : 1;
     collect_pending = false;
     dense_pending = false;
+    // from the active bitmap unless the list is there anyway or the gather
+    // values are in list order (use_gather 1)
+    const bool bits = !list_valid && use_gather != 1 &&
+                      (int64_t)n_active * kBitsApplyDiv >= (int64_t)g->nV;
+    if (!bits) GR_TRY(ensure_list());
     tick();
-    k_apply<P><<<grid_1d(n_active, kApplyChunk), kApplyNT, 0, stream>>>(
-        n_active, d_active, d_gtmp, use_gather, d_vd, g->off_out, iteration, d_chg_v, d_chg_e,
-        changed_bits_on() ? d_chg_bits : nullptr, d_ctr, d_srcval,
-        use_gather == 3 ? d_gdense : d_cand, d_applied);
+    uint32_t* cbits = changed_bits_on() ? d_chg_bits : nullptr;
+    G* cv = use_gather == 3 ? d_gdense : d_cand;
+    if (bits)
+      k_apply<P, true><<<grid_1d(g->nV, kApplyChunk), kApplyNT, 0, stream>>>(
+          g->nV, nullptr, d_act_bits, d_gtmp, use_gather, d_vd, g->off_out, iteration, d_chg_v,
+          d_chg_e, cbits, d_ctr, d_srcval, cv, d_applied);
+    else
+      k_apply<P, false><<<grid_1d(n_active, kApplyChunk), kApplyNT, 0, stream>>>(
+          n_active, d_active, nullptr, d_gtmp, use_gather, d_vd, g->off_out, iteration, d_chg_v,
+          d_chg_e, cbits, d_ctr, d_srcval, cv, d_applied);
     tock(kPhApply);
     GR_HIP(hipGetLastError());
     return GR_OK;
   }
+
+  // Active sets of at least nV / kBitsApplyDiv vertices are applied from the
+  // bitmap: a pass over all chunks costs about what writing and reading the
+  // list saves (BFS at RMAT-24: 1.4 % active levels ran faster from the list).
+  static constexpr int kBitsApplyDiv = 32;
 
   // At least the out-edges of any k vertices: the largest buckets first, the
   // last one taken at its bucket's largest degree.
@@ -2347,16 +2419,14 @@ struct Engine final : EngineBase {
     GR_TRY(collect_flush());
     const int need_deg = P::kHasGather ? 1 : 0;
     tick();
-    k_compact_count<<<n_cblocks, kCompactNT, 0, stream>>>(d_bitmap, n_words, g->nV, g->off_in,
-                                                          need_deg, d_blk);
+    k_compact_count<<<n_cblocks, kCompactNT, 0, stream>>>(
+        d_bitmap, n_words, g->nV, g->off_in, need_deg, d_blk, d_act_bits,
+        changed_bits_on() ? d_chg_prev : nullptr);
     ++ctr_seq;
     k_compact_scan<<<1, 1024, 0, stream>>>(d_blk, n_cblocks, d_ctr, need_deg ? d_escan : nullptr,
                                            dir_mode, pull_threshold(), d_hctr, d_hseq, ctr_seq,
                                            d_applied);
-    k_compact_write<<<n_cblocks, kCompactNT, 0, stream>>>(d_bitmap, n_words, g->nV, g->off_in,
-                                                          need_deg, d_blk, d_ctr, d_active, d_escan,
-                                                          d_cbase,
-                                                          changed_bits_on() ? d_chg_prev : nullptr);
+    list_valid = false;
     tock(kPhNext);
     GR_HIP(hipGetLastError());
     // this iteration's changes become the previous ones; the cleared buffer
@@ -2394,7 +2464,7 @@ struct Engine final : EngineBase {
 
   // Waits until k_compact_scan has published counters number `want` to h_ctr.
   // The host polls the sequence number instead of synchronising the stream:
-  // it learns the next frontier's size before k_compact_write has finished
+  // it learns the next frontier's size as soon as the scan has published it
   // and queues the next iteration's kernels behind it.  The polling is
   // bounded; after kSpinLimit the stream is synchronised, after which the
   // number must be there.
@@ -2409,6 +2479,20 @@ struct Engine final : EngineBase {
     GR_HIP(hipStreamSynchronize(stream));
     if (__atomic_load_n(h_seq, __ATOMIC_ACQUIRE) != want)
       return set_error(GR_ERR_HIP, "frontier counters were not published");
+    return GR_OK;
+  }
+
+  // The active list of the active bitmap, for the phases that need it.  The
+  // per-block offsets in d_blk and ctr->n_active are those nextIter's scan
+  // left: nothing else writes them before the list is next replaced.
+  int ensure_list() {
+    if (list_valid) return GR_OK;
+    list_valid = true;
+    if (n_active == 0) return GR_OK;
+    k_compact_write<<<n_cblocks, kCompactNT, 0, stream>>>(d_act_bits, n_words, g->nV, g->off_in,
+                                                          P::kHasGather ? 1 : 0, d_blk, d_ctr,
+                                                          d_active, d_escan, d_cbase);
+    GR_HIP(hipGetLastError());
     return GR_OK;
   }
 

@@ -242,15 +242,18 @@ int gr_engine_set_direction(gr_engine* e, int mode, double alpha);
  * threads).  Chosen by the autotuner; default 8. */
 int gr_engine_set_gather_tile(gr_engine* e, int items_per_thread);
 
-/* Internal vertex order (no reference counterpart; default 1).  With 1 the
+/* Internal vertex order (no reference counterpart; default 2).  With 1 the
  * engine runs on a copy of the graph whose vertices are renumbered by
  * descending in+out degree (ties by id), built once per graph on first use
  * and shared by the engines bound to it; every CSC/CSR row keeps its edge
  * order.  The hot vertices' values then share cache lines, which cuts the
- * random traffic of gather and activation.  Every argument and result stays
- * in the caller's ids (vertex data, setActive ranges, getResults), so results,
- * iteration counts and work counters are those of order 0.  Takes effect at
- * the next setGraph / bindGraph. */
+ * random traffic of gather and activation.  With 0 the engine runs on the
+ * graph as given; with 2 it takes the copy only when some vertex's in+out
+ * degree is at least 32 times the mean (an RMAT graph, not a grid or road
+ * map, whose own numbering already keeps neighbourhoods local).  Every
+ * argument and result stays in the caller's ids (vertex data, setActive
+ * ranges, getResults), so results, iteration counts and work counters are
+ * those of order 0.  Takes effect at the next setGraph / bindGraph. */
 int gr_engine_set_vertex_order(gr_engine* e, int order);
 
 /* Out-of-core mode, the reference's shard streaming (gpugas.h:1398-1503,

@@ -417,12 +417,17 @@ def _ordered_run(gr, prog, n, vd, s, d, w=None, start=None, order=1):
     return out, trips, st
 
 
-@pytest.mark.parametrize("case", ["bfs", "sssp", "sssp_range", "cc"])
+@pytest.mark.parametrize("case", ["bfs", "sssp", "sssp_range", "cc", "cc_grid"])
 def test_vertex_order_gives_the_same_results(gpu, case):
     """The degree-ordered layout is internal: values in the caller's ids,
-    trips and work counters equal those of the layout as given."""
+    trips and work counters equal those of the layout as given, and of the
+    automatic choice (order 2: the RMAT graphs are skewed, the grid is not)."""
     gr = gpu
-    n, s, d = _rmat(gr, 14, seed=8, sym=(case == "cc"))
+    if case == "cc_grid":
+        s, d = gr.grid_edges(96, 0.01, seed=5)
+        n = 96 * 96
+    else:
+        n, s, d = _rmat(gr, 14, seed=8, sym=(case == "cc"))
     w = gr.hash_weights(len(s), seed=8)
     src = _max_out_src(n, s)
     start = None
@@ -443,11 +448,13 @@ def test_vertex_order_gives_the_same_results(gpu, case):
         ref, ref_trips, _ = oracle.cc(n, s, d)
     a, ta, sa = _ordered_run(gr, prog, n, vd, s, d, w, start, order=0)
     b, tb, sb = _ordered_run(gr, prog, n, vd, s, d, w, start, order=1)
+    c, tc, sc = _ordered_run(gr, prog, n, vd, s, d, w, start, order=2)
     assert np.array_equal(a.view(np.int32), b.view(np.int32))
-    assert ta == tb
+    assert np.array_equal(a.view(np.int32), c.view(np.int32))
+    assert ta == tb == tc
     for k in ("iterations", "active_vertices", "gathered_edges", "changed_vertices",
               "pushed_edges"):
-        assert sa[k] == sb[k], k
+        assert sa[k] == sb[k] == sc[k], k
     if ref is not None:
         got = b["depth"] if case == "bfs" else b
         assert np.array_equal(got, ref)
