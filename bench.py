@@ -76,6 +76,16 @@ def run_loop(eng, all_active_n=None, source=None):
     return it
 
 
+def run_engine(eng, n):
+    """The reference's CC and PageRank apps: every vertex active, then the
+    engine's own loop (connected_component.cu:75-77, pagerank.cu:95-97:
+    setActive + run(), gpugas.h:1792-1807)."""
+    eng.resetVertexData()
+    eng.setActive(0, n)
+    eng.setIterationCount(0)
+    return eng.run()
+
+
 def timed(fn, steps, sync):
     sync()
     t0 = time.perf_counter()
@@ -336,14 +346,14 @@ def run_extras(gr, g, args, src, outdeg, w_dev):
                                     for k in ("apply", "scatter", "next")},
                           pull_iterations=st["pull_iterations"])
         log(f"BFS: {out['bfs']}")
-    if on("pagerank"):  # plain PageRank (pagerank.cu loop: activation-driven)
+    if on("pagerank"):  # plain PageRank (pagerank.cu: setActive + run, activation-driven)
         vd = np.zeros(n, gr.PROGRAMS[gr.PAGERANK]["vertex"])
         vd["rank"] = np.float32(0.15)
         vd["numOutEdges"] = outdeg.astype(np.int32)
         eng = gr.GASEngine(gr.PAGERANK)
         eng.bindGraph(g, vd)
         tune.apply_tuning(eng, gr.PAGERANK)
-        step = lambda: run_loop(eng, all_active_n=n)  # noqa: E731
+        step = lambda: run_engine(eng, n)  # noqa: E731
         trips = step()
         sec = timed(step, 2, gr.synchronize)
         # one more run with the phase events for the gather figures
@@ -361,13 +371,13 @@ def run_extras(gr, g, args, src, outdeg, w_dev):
                                gather_alg_gbs=round(pb / gs_ / 1e9, 1),
                                gather_frac=_frac(pb, gs_))
         log(f"PR: {out['pagerank']}")
-    if on("cc"):  # plain CC on the symmetrised graph
+    if on("cc"):  # plain CC on the symmetrised graph (connected_component.cu: setActive + run)
         gs = gr.Graph.rmat(args.scale, 16, seed=args.seed, symmetrize=True)
         vd = np.arange(n, dtype=np.int32)
         eng = gr.GASEngine(gr.CC)
         eng.bindGraph(gs, vd)
         tune.apply_tuning(eng, gr.CC)
-        step = lambda: run_loop(eng, all_active_n=n)  # noqa: E731
+        step = lambda: run_engine(eng, n)  # noqa: E731
         trips = step()
         sec = timed(step, max(3, args.steps), gr.synchronize)
         eng.resetStats()
@@ -666,7 +676,7 @@ def run_cc_grid(gr):
     eng.bindGraph(gg, vd)
     gr.synchronize()
     t0 = time.perf_counter()
-    trips = run_loop(eng, all_active_n=n)
+    trips = run_engine(eng, n)
     gr.synchronize()
     sec = time.perf_counter() - t0
     lab = eng.readVertexData()

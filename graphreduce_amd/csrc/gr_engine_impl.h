@@ -772,10 +772,10 @@ __global__ void k_fill(int64_t n, T value, T* __restrict__ out) {
 // the chunk contains a hit (a second ballot), so no cross-lane shuffles are
 // needed.  The row holding the chunk start is found from a ballot over the
 // lanes' row offsets; rows already hit, and heavy rows (in-degree above
-// kPullHeavy, done by k_pull_heavy with a whole workgroup and early exit),
+// kPullHeavy, done by k_pull_heavy with a wave per row and early exit),
 // are jumped over.  Before that, each lane tests kPullFirst in-edges of its
-// own row, heavy or not, the first and the last half of them, with all loads
-// in flight at once: rows list their sources by id, which the degree order
+// own row, heavy or not, the first and the last half of them, in two rounds
+// from both ends inwards: rows list their sources by id, which the degree order
 // makes hubs first, so a change among the hubs shows at the front of a row and
 // one among the low-degree vertices at its back.  Rows with at most kPullFirst
 // in-edges are settled either way, and the segment walk starts at the first
@@ -811,22 +811,22 @@ __device__ __forceinline__ uint64_t lane_range_mask(int lo, int hi) {  // bits [
     const int my_end = mine ? off_in[base + lane + 1] : INT_MAX;
     const int deg = mine ? my_end - my_off : 0;
     const bool heavy = deg > kPullHeavy;
-    // kPullFirst in-edges of the lane's own row: the first and the last half
+    // kPullFirst in-edges of the lane's own row, the first and the last half,
+    // in two rounds from the ends inwards: the first and the last in-edge
+    // settle most rows (at RMAT-24 the first alone 80 % of the rows with
+    // in-edges in one pull, the last 62 % in the other), and the test's cost
+    // is its random bitmap reads
     bool first_hit = false;
     if (deg > 0 && (GR_PULL_HEAVY_FIRST || !heavy)) {
-      int u[kPullFirst];
 #pragma unroll
-      for (int k = 0; k < kPullFirst; ++k) {
-        const int p = deg <= kPullFirst ? my_off + k
-                      : k < kPullFirst / 2 ? my_off + k : my_end - (kPullFirst - k);
-        u[k] = k < deg ? srcs[p] : -1;
+      for (int r = 0; r < kPullFirst / 2; ++r) {
+        const int pa = my_off + r, pz = my_end - 1 - r;  // from the front, from the back
+        const int ua = !first_hit && pa < my_end ? srcs[pa] : -1;
+        const int uz = !first_hit && pz > pa ? srcs[pz] : -1;
+        const uint32_t wa = ua >= 0 ? chg_bits[ua >> 5] : 0u;
+        const uint32_t wz = uz >= 0 ? chg_bits[uz >> 5] : 0u;
+        first_hit |= (ua >= 0 && ((wa >> (ua & 31)) & 1u)) || (uz >= 0 && ((wz >> (uz & 31)) & 1u));
       }
-      uint32_t wd[kPullFirst];
-#pragma unroll
-      for (int k = 0; k < kPullFirst; ++k) wd[k] = u[k] >= 0 ? chg_bits[u[k] >> 5] : 0u;
-#pragma unroll
-      for (int k = 0; k < kPullFirst; ++k)
-        first_hit |= u[k] >= 0 && ((wd[k] >> (u[k] & 31)) & 1u);
     }
     uint64_t hits = __ballot(first_hit);
     // rows still open: more than kPullFirst in-edges, not heavy, no hit yet
@@ -866,26 +866,35 @@ __device__ __forceinline__ uint64_t lane_range_mask(int lo, int hi) {  // bits [
   }
 }
 
+// The heavy rows' walk: one wave per row reads 64 in-edges per round until one
+// comes from a changed vertex; rows that k_pull_stream's test of the first and
+// last in-edges marked already are skipped.  The walk's cost is its random
+// changed-bitmap reads: at RMAT-24 the heavy rows' first hits sit at a median
+// of about 250 in-edges, so steps of a wave read fewer past the hit than
+// steps of a workgroup.
 [[maybe_unused]] static __global__ __launch_bounds__(256) void k_pull_heavy(
     const DevCounters* __restrict__ ctr, int mode, long long threshold, long long fuse_max,
     const int32_t* __restrict__ heavy, int n_heavy, const int32_t* __restrict__ off_in,
     const int32_t* __restrict__ srcs, const uint32_t* __restrict__ chg_bits,
     uint32_t* __restrict__ bitmap) {
   if (!use_pull(ctr->chg, mode, threshold) || fused_scatter(ctr->chg, fuse_max)) return;
-  for (int h = blockIdx.x; h < n_heavy; h += gridDim.x) {
+  const int lane = lane_id();
+  const int wpb = blockDim.x / kWave;
+  const int nw = gridDim.x * wpb;
+  for (int h = blockIdx.x * wpb + (int)threadIdx.x / kWave; h < n_heavy; h += nw) {
     const int v = heavy[h];
-    // marked by k_pull_stream's test of the row's first and last in-edges
-    if ((bitmap[v >> 5] >> (v & 31)) & 1u) continue;  // uniform per workgroup
+    // one load instruction: every lane of the wave sees the same word
+    if ((bitmap[v >> 5] >> (v & 31)) & 1u) continue;
     const int b = off_in[v], e = off_in[v + 1];
-    for (int pb = b; pb < e; pb += blockDim.x) {
-      const int p = pb + threadIdx.x;
+    for (int pb = b; pb < e; pb += kWave) {
+      const int p = pb + lane;
       bool hit = false;
       if (p < e) {
         const int u = srcs[p];
         hit = (chg_bits[u >> 5] >> (u & 31)) & 1u;
       }
-      if (__syncthreads_or(hit)) {
-        if (threadIdx.x == 0) atomicOr(&bitmap[v >> 5], 1u << (v & 31));
+      if (__ballot(hit)) {
+        if (lane == 0) atomicOr(&bitmap[v >> 5], 1u << (v & 31));
         break;
       }
     }
@@ -2367,7 +2376,7 @@ struct Engine final : EngineBase {
                                                               g->nV, g->off_in, g->srcs,
                                                               d_chg_bits, d_bitmap);
       if (n_heavy > 0)
-        k_pull_heavy<<<n_heavy < 4096 ? n_heavy : 4096, 256, 0, stream>>>(
+        k_pull_heavy<<<std::min(grid_1d(n_heavy, 4), 4096u), 256, 0, stream>>>(
             d_ctr, dir_mode, thr, fuse_max, d_heavy, n_heavy, g->off_in, g->srcs, d_chg_bits,
             d_bitmap);
     }
