@@ -21,6 +21,7 @@ namespace gr {
 
 constexpr int kLaneNT = 256;
 constexpr int kLaneHeavy = 4096;
+constexpr int kLaneBatch = 16;  // rows read at once by a wave summing lanes
 constexpr int kLaneChunk = 4096;
 
 // (vertex, first in-edge, end in-edge, index of the vertex's first chunk)

@@ -43,6 +43,7 @@ namespace gr {
 constexpr int kSrvNT = 256;
 constexpr int kLanes = 64;
 constexpr int kSrvHeavy = 64;
+constexpr int kPprSegEdges = 4096;  // in-edges of a 64-vertex segment the PPR walk takes
 
 // ------------------------------------------------------------------ kernels
 
@@ -324,38 +325,176 @@ __global__ void k_srv_ppr_reset(int nV, uint64_t mask, LaneSeeds seeds, float* _
   }
 }
 
-// contrib[u][s] = rank[u][s] / outdeg[u] (0 without out-edges)
-__global__ void k_srv_ppr_contrib(int nV, uint64_t active, const int32_t* __restrict__ off_out,
-                                  const float* __restrict__ rank, float* __restrict__ contrib) {
+// Value of `x` in lane `l` as a wave-uniform (scalar) value; v_readlane
+// ignores the exec mask, so the lane need not be active.
+__device__ __forceinline__ int lane_value(int x, int l) { return __builtin_amdgcn_readlane(x, l); }
+__device__ __forceinline__ uint64_t lane_value64(uint64_t x, int l) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), l);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// contrib[u][s] = rank[u][s] / outdeg[u] (0 without out-edges).  A wave takes
+// 64 consecutive vertices: their out-degrees in one read (lane per vertex),
+// then their rows kLaneBatch at a time with all reads in flight.  Every loop
+// bound and cross-lane value is a scalar, so all lanes run every iteration.
+__global__ __launch_bounds__(kSrvNT) void k_srv_ppr_contrib(int nV, uint64_t active,
+                                                          const int32_t* __restrict__ off_out,
+                                                          const float* __restrict__ rank,
+                                                          float* __restrict__ contrib) {
   const int lane = lane_id();
-  const int wpb = blockDim.x / kWave;
-  if (!((active >> lane) & 1ull)) return;
-  for (int v = blockIdx.x * wpb + threadIdx.x / kWave; v < nV; v += gridDim.x * wpb) {
-    const int d = off_out[v + 1] - off_out[v];
-    const float r = rank[(size_t)v * kLanes + lane];
-    contrib[(size_t)v * kLanes + lane] = d ? __fdiv_rn(r, (float)d) : 0.0f;
+  const int wpb = kSrvNT / kWave;
+  const int n_waves = gridDim.x * wpb;
+  const bool mine = (active >> lane) & 1ull;
+  const int w0 = __builtin_amdgcn_readfirstlane(blockIdx.x * wpb + (int)threadIdx.x / kWave);
+  for (int v0 = w0 * kWave; v0 < nV; v0 += n_waves * kWave) {
+    const int cnt = min(kWave, nV - v0);
+    const int deg = lane < cnt ? off_out[v0 + lane + 1] - off_out[v0 + lane] : 0;
+    for (int r0 = 0; r0 < cnt; r0 += kLaneBatch) {
+      float x[kLaneBatch];
+#pragma unroll
+      for (int j = 0; j < kLaneBatch; ++j)
+        x[j] = (mine && r0 + j < cnt) ? rank[(size_t)(v0 + r0 + j) * kLanes + lane] : 0.0f;
+#pragma unroll
+      for (int j = 0; j < kLaneBatch; ++j) {
+        if (r0 + j < cnt) {
+          const int d = lane_value(deg, r0 + j);
+          if (mine) contrib[(size_t)(v0 + r0 + j) * kLanes + lane] = d ? __fdiv_rn(x[j], (float)d) : 0.0f;
+        }
+      }
+    }
   }
 }
 
 // r'[v][s] = 0.15 * [v == seed_s] + 0.85 * sum_{u->v} contrib[u][s], in-edge
-// order fixed; sources without mass for any active lane are skipped.
-__global__ __launch_bounds__(kSrvNT) void k_srv_ppr_step(
-    int nV, uint64_t active, LaneSeeds seeds, const int32_t* __restrict__ off_in,
+// order fixed; sources without mass for any active lane are skipped (adding
+// +0.0 changes no sum that starts at +0.0).
+//
+// k_srv_ppr_step_rows: one wave per vertex of [v_begin, v_end), the vertices
+// of the degree order's head, whose 64-vertex segments hold many in-edges.
+__global__ __launch_bounds__(kSrvNT) void k_srv_ppr_step_rows(
+    int v_begin, int v_end, uint64_t active, LaneSeeds seeds, const int32_t* __restrict__ off_in,
     const int32_t* __restrict__ srcs, const float* __restrict__ contrib,
     const uint64_t* __restrict__ nz_in, uint64_t* __restrict__ nz_out, float* __restrict__ rank) {
   const int lane = lane_id();
-  const int wpb = blockDim.x / kWave;
+  const int wpb = kSrvNT / kWave;
   const int n_waves = gridDim.x * wpb;
   const bool mine = (active >> lane) & 1ull;
-  for (int v = blockIdx.x * wpb + threadIdx.x / kWave; v < nV; v += n_waves) {
-    const int b = off_in[v], e = off_in[v + 1];
+  const int my_seed = seeds.vertex[lane];
+  const int w0 = __builtin_amdgcn_readfirstlane(blockIdx.x * wpb + (int)threadIdx.x / kWave);
+  for (int v = v_begin + w0; v < v_end; v += n_waves) {
+    const int b = __builtin_amdgcn_readfirstlane(off_in[v]);
+    const int e = __builtin_amdgcn_readfirstlane(off_in[v + 1]);
     if (e - b > kLaneHeavy) continue;  // k_srv_ppr_heavy_*
     const float acc = lane_sum_in_edges(b, e, active, srcs, contrib, nz_in);
     float nr = __fmul_rn(0.85f, acc);
-    if (mine && seeds.vertex[lane] == v) nr = __fadd_rn(nr, 0.15f);
+    if (mine && my_seed == v) nr = __fadd_rn(nr, 0.15f);
     if (mine) rank[(size_t)v * kLanes + lane] = nr;
     const uint64_t nzm = __ballot(mine && nr != 0.0f);
     if (lane == 0) nz_out[v] = (nz_in[v] & ~active) | nzm;
+  }
+}
+
+// k_srv_ppr_step: a wave takes 64 consecutive vertices from v_begin (a
+// multiple of 64) on.  Without a heavy row among them it walks their CSC segment once,
+// 64 in-edges per round, and adds the mass rows kLaneBatch at a time in edge
+// order, closing a vertex's sum when the walk passes its last in-edge: the
+// same additions in the same order as a walk per vertex, without a chain of
+// reads per vertex (most vertices have a few in-edges or none).  A segment
+// with a heavy row (summed by k_srv_ppr_heavy_*) takes its vertices one by
+// one.  The segment's offsets live one per lane; every loop bound, row index
+// and source id the loops use is read out of its lane as a scalar
+// (lane_value), so the wave runs each iteration with all lanes: a lane-wise
+// loop test would let the compiler run lanes through different iterations,
+// in which a shuffle from a lane that is not running reads 0.
+__global__ __launch_bounds__(kSrvNT) void k_srv_ppr_step(
+    int v_begin, int nV, uint64_t active, LaneSeeds seeds, const int32_t* __restrict__ off_in,
+    const int32_t* __restrict__ srcs, const float* __restrict__ contrib,
+    const uint64_t* __restrict__ nz_in, uint64_t* __restrict__ nz_out, float* __restrict__ rank) {
+  const int lane = lane_id();
+  const int wpb = kSrvNT / kWave;
+  const int n_waves = gridDim.x * wpb;
+  const bool mine = (active >> lane) & 1ull;
+  const int my_seed = seeds.vertex[lane];
+  const int w0 = __builtin_amdgcn_readfirstlane(blockIdx.x * wpb + (int)threadIdx.x / kWave);
+  for (int v0 = v_begin + w0 * kWave; v0 < nV; v0 += n_waves * kWave) {
+    const int cnt = min(kWave, nV - v0);
+    const int beg = lane < cnt ? off_in[v0 + lane] : 0;
+    const int end = lane < cnt ? off_in[v0 + lane + 1] : 0;
+    const uint64_t nzr = lane < cnt ? nz_in[v0 + lane] : 0ull;
+    if (__ballot(lane < cnt && end - beg > kLaneHeavy)) {
+      for (int r = 0; r < cnt; ++r) {
+        const int v = v0 + r;
+        const int b = lane_value(beg, r), e = lane_value(end, r);
+        if (e - b > kLaneHeavy) continue;  // k_srv_ppr_heavy_*
+        const float acc = lane_sum_in_edges(b, e, active, srcs, contrib, nz_in);
+        float nr = __fmul_rn(0.85f, acc);
+        if (mine && my_seed == v) nr = __fadd_rn(nr, 0.15f);
+        if (mine) rank[(size_t)v * kLanes + lane] = nr;
+        const uint64_t nzm = __ballot(mine && nr != 0.0f);
+        if (lane == 0) nz_out[v] = (lane_value64(nzr, r) & ~active) | nzm;
+      }
+      continue;
+    }
+    const int seg_b = lane_value(beg, 0), seg_e = lane_value(end, cnt - 1);
+    int r = 0;  // vertex (within the segment) whose in-edges the walk is in
+    float acc = 0.0f;
+    for (int pb = seg_b; pb < seg_e; pb += kWave) {
+      const int p = pb + lane;
+      int u = 0;
+      uint64_t m = 0;
+      if (p < seg_e) {
+        u = srcs[p];
+        m = nz_in[u] & active;
+      }
+      uint64_t todo = __ballot(m != 0);
+      while (todo) {
+        int k[kLaneBatch];
+        int nb = 0;
+#pragma unroll
+        for (int j = 0; j < kLaneBatch; ++j) {
+          k[j] = todo ? __builtin_ctzll(todo) : 0;
+          if (todo) {
+            todo &= todo - 1;
+            nb = j + 1;
+          }
+        }
+        float x[kLaneBatch];
+#pragma unroll
+        for (int j = 0; j < kLaneBatch; ++j) {
+          const int uu = lane_value(u, k[j]);
+          x[j] = (mine && j < nb) ? contrib[(size_t)uu * kLanes + lane] : 0.0f;
+        }
+#pragma unroll
+        for (int j = 0; j < kLaneBatch; ++j) {
+          if (j < nb) {
+            const int pos = pb + k[j];
+            // close the vertices whose in-edges all lie before this edge
+            // (the segment's last vertex ends past every edge: r < cnt)
+            while (lane_value(end, r) <= pos) {
+              const int v = v0 + r;
+              float nr = __fmul_rn(0.85f, acc);
+              if (mine && my_seed == v) nr = __fadd_rn(nr, 0.15f);
+              if (mine) rank[(size_t)v * kLanes + lane] = nr;
+              const uint64_t nzm = __ballot(mine && nr != 0.0f);
+              if (lane == 0) nz_out[v] = (lane_value64(nzr, r) & ~active) | nzm;
+              acc = 0.0f;
+              ++r;
+            }
+            if (mine) acc = __fadd_rn(acc, x[j]);
+          }
+        }
+      }
+    }
+    for (; r < cnt; ++r) {
+      const int v = v0 + r;
+      float nr = __fmul_rn(0.85f, acc);
+      if (mine && my_seed == v) nr = __fadd_rn(nr, 0.15f);
+      if (mine) rank[(size_t)v * kLanes + lane] = nr;
+      const uint64_t nzm = __ballot(mine && nr != 0.0f);
+      if (lane == 0) nz_out[v] = (lane_value64(nzr, r) & ~active) | nzm;
+      acc = 0.0f;
+    }
   }
 }
 
@@ -507,6 +646,9 @@ struct gr_server {
   gr::LaneTables lanes;       // heavy-vertex chunks of the CSC
   gr::LaneTables out_lanes;   // ... of the CSR (BFS push)
   float* ppr_partial = nullptr;  // [chunks][64]
+  // PPR steps: vertices below ppr_split (a multiple of 64; the degree order's
+  // head) one wave each, the segments from there on one wave per 64 vertices
+  int32_t ppr_split = 0;
   std::mutex mu;
   std::condition_variable cv;
   std::deque<int64_t> queue[3];
@@ -699,8 +841,14 @@ int launch_step(gr_server* sv, Pool& p) {
     const uint64_t* nzi = p.parity ? p.m1 : p.m0;
     uint64_t* nzo = p.parity ? p.m0 : p.m1;
     k_srv_ppr_contrib<<<grid_v(nV), kSrvNT, 0, p.stream>>>(nV, p.active, g->off_out, p.f0, p.f1);
-    k_srv_ppr_step<<<grid_v(nV), kSrvNT, 0, p.stream>>>(nV, p.active, p.seeds, g->off_in, g->srcs,
-                                                        p.f1, nzi, nzo, p.f0);
+    const int split = sv->ppr_split;
+    if (split > 0)
+      k_srv_ppr_step_rows<<<grid_v(split), kSrvNT, 0, p.stream>>>(0, split, p.active, p.seeds,
+                                                                  g->off_in, g->srcs, p.f1, nzi,
+                                                                  nzo, p.f0);
+    if (split < nV)  // one wave per 64-vertex segment
+      k_srv_ppr_step<<<grid_v((nV - split + kLanes - 1) / kLanes), kSrvNT, 0, p.stream>>>(
+          split, nV, p.active, p.seeds, g->off_in, g->srcs, p.f1, nzi, nzo, p.f0);
     if (sv->lanes.n_chunks) {
       k_srv_ppr_heavy_partial<<<grid_1d(sv->lanes.n_chunks, kSrvNT / kWave), kSrvNT, 0, p.stream>>>(
           sv->lanes.chunks, sv->lanes.n_chunks, p.active, g->srcs, p.f1, nzi, sv->ppr_partial);
@@ -867,6 +1015,20 @@ extern "C" int gr_server_create(gr_graph* g, const int32_t* weights_device,
       rc = set_error(GR_ERR_HIP, "server: clearing the SSSP flags");
   }
   if (rc == GR_OK) rc = sv->lanes.build(g->h_off_in);
+  if (rc == GR_OK) {
+    // the last 64-vertex segment from which on every segment has at most
+    // kPprSegEdges in-edges and no heavy row: the segment walk's share
+    const std::vector<int32_t>& off = g->h_off_in;
+    int32_t split = (g->nV + kLanes - 1) / kLanes * kLanes;
+    while (split > 0) {
+      const int32_t a = split - kLanes, b = std::min<int32_t>(split, g->nV);
+      bool light = off[b] - off[a] <= kPprSegEdges;
+      for (int32_t v = a; light && v < b; ++v) light = off[v + 1] - off[v] <= kLaneHeavy;
+      if (!light) break;
+      split = a;
+    }
+    sv->ppr_split = std::min<int32_t>(split, g->nV);
+  }
   if (rc == GR_OK) rc = sv->out_lanes.build(g->h_off_out);
   if (rc == GR_OK)
     rc = alloc_dev((void**)&sv->ppr_partial, sizeof(float) * kLanes * ((size_t)sv->lanes.n_chunks + 1));
