@@ -1057,6 +1057,7 @@ extern "C" int gr_mssp(gr_graph* g, const int32_t* weights_device, const int32_t
   LaneTables lt, lt_out;
   GR_TRY(lt.build(g->h_off_in));
   GR_TRY(lt_out.build(g->h_off_out));
+  const int32_t seg_split = graph_lane_split(g);
   hipStream_t st;
   GR_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   GR_HIP(hipDeviceSynchronize());
@@ -1099,7 +1100,7 @@ extern "C" int gr_mssp(gr_graph* g, const int32_t* weights_device, const int32_t
                                                 lt_out.n_chunks, cin, rr);
     k_lane_sssp_filter<<<vgrid, kLaneNT, 0, st>>>(nV, g->off_in, cout, rr);
     k_lane_sssp_light<<<rgrid < 65536 ? rgrid : 65536, kLaneNT, 0, st>>>(
-        nV, g->off_in, g->off_out, g->srcs, w_csc.p, dist.p, cin, cout, any.p, rr);
+        nV, g->off_in, g->off_out, g->srcs, w_csc.p, dist.p, cin, cout, any.p, rr, seg_split);
     if (lt.n_chunks) {
       k_lane_sssp_heavy<<<hgrid, kLaneNT, 0, st>>>(lt.chunks, lt.n_chunks, g->off_out, g->srcs,
                                                    w_csc.p, dist.p, cin, cout, any.p, rr);

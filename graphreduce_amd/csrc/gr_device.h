@@ -118,6 +118,9 @@ struct gr_graph {
   // 1 when the degrees are skewed enough for the degree-ordered copy to pay
   // (graph_skewed; -1 until first asked)
   int skewed = -1;
+  // first vertex of the lane kernels' segment walk (lane_seg_split; -1 until
+  // first asked)
+  int32_t lane_split = -1;
   int32_t* perm = nullptr;  // ordered id -> id   [V], device
   int32_t* inv = nullptr;   // id -> ordered id   [V], device
   // row blocks of at most kBlockRows / kBlockRowsWide rows, built on first

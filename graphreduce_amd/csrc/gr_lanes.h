@@ -13,6 +13,7 @@
 #ifndef GRAPHREDUCE_AMD_GR_LANES_H
 #define GRAPHREDUCE_AMD_GR_LANES_H
 
+#include <algorithm>
 #include <vector>
 
 #include "gr_device.h"
@@ -50,6 +51,35 @@ inline void build_lane_chunks(const std::vector<int32_t>& off_in, std::vector<La
   }
 }
 
+// Rows per segment of the segment walks, and the most in-edges a segment
+// walked by one wave may hold.
+constexpr int kLaneSegRows = 16;
+constexpr int kLaneSegEdges = 1024;
+
+// First vertex (a multiple of kLaneSegRows) from which on every segment of
+// kLaneSegRows vertices holds at most kLaneSegEdges in-edges and no heavy
+// row: the tail of the degree order, which the relax kernels walk a segment
+// per wave; the vertices before it are taken one per wave.
+inline int32_t lane_seg_split(const std::vector<int32_t>& off_in) {
+  const int32_t nV = (int32_t)off_in.size() - 1;
+  int32_t split = (nV + kLaneSegRows - 1) / kLaneSegRows * kLaneSegRows;
+  while (split > 0) {
+    const int32_t a = split - kLaneSegRows, b = std::min<int32_t>(split, nV);
+    bool light = off_in[b] - off_in[a] <= kLaneSegEdges;
+    for (int32_t v = a; light && v < b; ++v) light = off_in[v + 1] - off_in[v] <= kLaneHeavy;
+    if (!light) break;
+    split = a;
+  }
+  return std::min(split, nV);
+}
+
+// lane_seg_split of a graph's CSC, computed once per graph (a pass over the
+// vertices of the tail).
+inline int32_t graph_lane_split(gr_graph* g) {
+  if (g->lane_split < 0) g->lane_split = lane_seg_split(g->h_off_in);
+  return g->lane_split;
+}
+
 // Device copies of the chunk tables of a graph.
 struct LaneTables {
   LaneChunk* chunks = nullptr;
@@ -80,8 +110,18 @@ __device__ __forceinline__ uint64_t shfl_u64(uint64_t x, int src) {
          (uint64_t)__shfl((uint32_t)x, src, kWave);
 }
 
-// Per-lane minimum of dist[u][lane] + w over the in-edges [b, e) whose source
-// changed for this lane; up to four changed edges are loaded at once.
+// Value of `x` in lane `l` as a wave-uniform (scalar) value; v_readlane
+// ignores the exec mask, so the lane need not be active.
+__device__ __forceinline__ int lane_value(int x, int l) { return __builtin_amdgcn_readlane(x, l); }
+__device__ __forceinline__ uint64_t lane_value64(uint64_t x, int l) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), l);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Per-lane minimum of dist[u][lane] + w over the in-edges [b, e) (b and e
+// wave-uniform) whose source changed for this lane; up to kLaneBatch changed
+// sources' rows are read at once.
 __device__ __forceinline__ int32_t lane_min_in_edges(int b, int e, const int32_t* __restrict__ srcs,
                                                      const int32_t* __restrict__ w_csc,
                                                      const int32_t* __restrict__ dist,
@@ -100,30 +140,23 @@ __device__ __forceinline__ int32_t lane_min_in_edges(int b, int e, const int32_t
     }
     uint64_t todo = __ballot(cu != 0);
     while (todo) {
-      int k[4], uu[4], ww[4];
-      uint64_t cm[4];
-      int cnt = 0;
+      int uu[kLaneBatch], ww[kLaneBatch];
+      uint64_t cm[kLaneBatch];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        k[j] = todo ? __builtin_ctzll(todo) : 0;
-        if (todo) {
-          todo &= todo - 1;
-          cnt = j + 1;
-        }
+      for (int j = 0; j < kLaneBatch; ++j) {
+        const int k = todo ? __builtin_ctzll(todo) : 0;
+        cm[j] = todo ? lane_value64(cu, k) : 0ull;
+        uu[j] = lane_value(u, k);
+        ww[j] = lane_value(wt, k);
+        todo &= todo - 1;
       }
+      int32_t du[kLaneBatch];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        uu[j] = __shfl(u, k[j], kWave);
-        ww[j] = __shfl(wt, k[j], kWave);
-        cm[j] = shfl_u64(cu, k[j]);
-      }
-      int32_t du[4];
+      for (int j = 0; j < kLaneBatch; ++j)
+        du[j] = (cm[j] & my_bit) ? dist[(size_t)uu[j] * kWave + lane] : 0;
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        du[j] = (j < cnt && (cm[j] & my_bit)) ? dist[(size_t)uu[j] * kWave + lane] : 0;
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (j < cnt && (cm[j] & my_bit)) {
+      for (int j = 0; j < kLaneBatch; ++j)
+        if (cm[j] & my_bit) {
           const int32_t cand = du[j] + ww[j];
           if (cand < best) best = cand;
         }
@@ -260,25 +293,34 @@ static __global__ __launch_bounds__(kLaneNT) void k_lane_sssp_filter(
 // of them or the filtered list.  chg_out[v] = lanes whose distance at v
 // dropped (heavy vertices get 0 here and their bits from k_lane_sssp_heavy);
 // *any |= lanes changed anywhere; rr.count / rr.edges += changed vertices and
-// their out-edges.
+// their out-edges.  In unfiltered rounds the vertices from seg_split on (the
+// tail of the degree order: few in-edges each) are taken kLaneSegRows at a
+// time: one walk over their CSC segment, closing a vertex when the walk
+// passes its last in-edge, instead of a chain of reads per vertex.  Every loop
+// bound and cross-lane value is read out of its lane as a scalar
+// (lane_value), so each wave runs every iteration with all of its lanes.
 static __global__ __launch_bounds__(kLaneNT) void k_lane_sssp_light(
     int nV, const int32_t* __restrict__ off_in, const int32_t* __restrict__ off_out,
     const int32_t* __restrict__ srcs, const int32_t* __restrict__ w_csc,
     int32_t* __restrict__ dist, const uint64_t* __restrict__ chg_in,
-    uint64_t* __restrict__ chg_out, unsigned long long* __restrict__ any, LaneRound rr) {
+    uint64_t* __restrict__ chg_out, unsigned long long* __restrict__ any, LaneRound rr,
+    int seg_split) {
   if (lane_round_idle(rr)) return;
   const bool filtered = lane_round_filtered(rr);
-  const int n_items = filtered ? (int)*rr.list_len : nV;
+  const int n_items = filtered ? (int)*rr.list_len : seg_split;
   __shared__ unsigned long long s_acc[kLaneNT / kWave], s_cnt[kLaneNT / kWave],
       s_edg[kLaneNT / kWave];
   const int lane = lane_id();
-  const int wpb = blockDim.x / kWave;
+  const int wpb = kLaneNT / kWave;
   const int n_waves = gridDim.x * wpb;
+  const int w0 = __builtin_amdgcn_readfirstlane(blockIdx.x * wpb + (int)threadIdx.x / kWave);
+  const uint64_t my_bit = 1ull << lane;
   uint64_t acc = 0;
   unsigned long long cnt = 0, edg = 0;
-  for (int i = blockIdx.x * wpb + threadIdx.x / kWave; i < n_items; i += n_waves) {
-    const int v = filtered ? rr.list[i] : i;
-    const int b = off_in[v], e = off_in[v + 1];
+  for (int i = w0; i < n_items; i += n_waves) {
+    const int v = filtered ? __builtin_amdgcn_readfirstlane(rr.list[i]) : i;
+    const int b = __builtin_amdgcn_readfirstlane(off_in[v]);
+    const int e = __builtin_amdgcn_readfirstlane(off_in[v + 1]);
     if (e - b > kLaneHeavy) {
       if (lane == 0) chg_out[v] = 0;
       continue;
@@ -294,6 +336,83 @@ static __global__ __launch_bounds__(kLaneNT) void k_lane_sssp_light(
       cnt += 1;
       if (lane == 0 && rr.edges) edg += (unsigned long long)(off_out[v + 1] - off_out[v]);
     }
+  }
+  const int n_seg = filtered ? 0 : (nV - seg_split + kLaneSegRows - 1) / kLaneSegRows;
+  for (int sg = w0; sg < n_seg; sg += n_waves) {
+    const int v0 = seg_split + sg * kLaneSegRows;
+    const int rows = min(kLaneSegRows, nV - v0);
+    // lanes 0 .. rows hold the rows' CSC and CSR offsets
+    const int off = lane <= rows ? off_in[v0 + lane] : 0;
+    const int oo = (lane <= rows && rr.edges) ? off_out[v0 + lane] : 0;
+    int32_t bef[kLaneSegRows];
+#pragma unroll
+    for (int r = 0; r < kLaneSegRows; ++r)
+      bef[r] = r < rows ? dist[(size_t)(v0 + r) * kWave + lane] : 0;
+    const int seg_b = lane_value(off, 0), seg_e = lane_value(off, rows);
+    int r = 0;                 // vertex (within the segment) whose in-edges the walk is in
+    int32_t best = INT_MAX;    // its candidates' minimum, this lane
+    // closes vertex r: a lane improves where a candidate beat the distance
+    auto close_row = [&]() {
+      int32_t b0 = bef[0];
+#pragma unroll
+      for (int k = 1; k < kLaneSegRows; ++k)
+        if (k == r) b0 = bef[k];
+      const int v = v0 + r;
+      const bool improved = best < b0;
+      if (improved) dist[(size_t)v * kWave + lane] = best;
+      const uint64_t m = __ballot(improved);
+      if (lane == 0) chg_out[v] = m;
+      acc |= m;
+      if (m) {
+        cnt += 1;
+        if (lane == 0 && rr.edges)
+          edg += (unsigned long long)(lane_value(oo, r + 1) - lane_value(oo, r));
+      }
+      best = INT_MAX;
+      ++r;
+    };
+    for (int pb = seg_b; pb < seg_e; pb += kWave) {
+      const int p = pb + lane;
+      int u = 0, wt = 0;
+      uint64_t cu = 0;
+      if (p < seg_e) {
+        u = srcs[p];
+        cu = chg_in[u];
+        if (cu) wt = w_csc[p];
+      }
+      uint64_t todo = __ballot(cu != 0);
+      while (todo) {
+        int kk[kLaneBatch], uu[kLaneBatch], ww[kLaneBatch];
+        uint64_t cm[kLaneBatch];
+        int nb = 0;
+#pragma unroll
+        for (int j = 0; j < kLaneBatch; ++j) {
+          kk[j] = todo ? __builtin_ctzll(todo) : 0;
+          cm[j] = todo ? lane_value64(cu, kk[j]) : 0ull;
+          uu[j] = lane_value(u, kk[j]);
+          ww[j] = lane_value(wt, kk[j]);
+          if (todo) nb = j + 1;
+          todo &= todo - 1;
+        }
+        int32_t du[kLaneBatch];
+#pragma unroll
+        for (int j = 0; j < kLaneBatch; ++j)
+          du[j] = (cm[j] & my_bit) ? dist[(size_t)uu[j] * kWave + lane] : 0;
+#pragma unroll
+        for (int j = 0; j < kLaneBatch; ++j) {
+          if (j < nb) {
+            // close the vertices whose in-edges all lie before this edge (the
+            // segment's last vertex ends past every edge: r < rows)
+            while (lane_value(off, r + 1) <= pb + kk[j]) close_row();
+            if (cm[j] & my_bit) {
+              const int32_t cand = du[j] + ww[j];
+              if (cand < best) best = cand;
+            }
+          }
+        }
+      }
+    }
+    while (r < rows) close_row();
   }
   // one atomic per counter and workgroup: per-wave atomics on these words
   // serialise over hundreds of thousands of waves

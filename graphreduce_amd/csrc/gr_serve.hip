@@ -325,15 +325,6 @@ __global__ void k_srv_ppr_reset(int nV, uint64_t mask, LaneSeeds seeds, float* _
   }
 }
 
-// Value of `x` in lane `l` as a wave-uniform (scalar) value; v_readlane
-// ignores the exec mask, so the lane need not be active.
-__device__ __forceinline__ int lane_value(int x, int l) { return __builtin_amdgcn_readlane(x, l); }
-__device__ __forceinline__ uint64_t lane_value64(uint64_t x, int l) {
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, l);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), l);
-  return ((uint64_t)hi << 32) | lo;
-}
-
 // contrib[u][s] = rank[u][s] / outdeg[u] (0 without out-edges).  A wave takes
 // 64 consecutive vertices: their out-degrees in one read (lane per vertex),
 // then their rows kLaneBatch at a time with all reads in flight.  Every loop
@@ -649,6 +640,7 @@ struct gr_server {
   // PPR steps: vertices below ppr_split (a multiple of 64; the degree order's
   // head) one wave each, the segments from there on one wave per 64 vertices
   int32_t ppr_split = 0;
+  int32_t lane_split = 0;  // the SSSP relax walk's (graph_lane_split)
   std::mutex mu;
   std::condition_variable cv;
   std::deque<int64_t> queue[3];
@@ -828,7 +820,8 @@ int launch_step(gr_server* sv, Pool& p) {
                                                       sv->out_lanes.n_chunks, cin, rr);
     k_lane_sssp_filter<<<vgrid, kLaneNT, 0, p.stream>>>(nV, g->off_in, cout, rr);
     k_lane_sssp_light<<<grid_v(nV), kLaneNT, 0, p.stream>>>(nV, g->off_in, g->off_out, g->srcs,
-                                                            sv->w_csc, p.i32, cin, cout, any, rr);
+                                                            sv->w_csc, p.i32, cin, cout, any, rr,
+                                                            sv->lane_split);
     if (sv->lanes.n_chunks) {
       k_lane_sssp_heavy<<<grid_1d(sv->lanes.n_chunks, kLaneNT / kWave), kLaneNT, 0, p.stream>>>(
           sv->lanes.chunks, sv->lanes.n_chunks, g->off_out, g->srcs, sv->w_csc, p.i32, cin, cout,
@@ -1015,6 +1008,7 @@ extern "C" int gr_server_create(gr_graph* g, const int32_t* weights_device,
       rc = set_error(GR_ERR_HIP, "server: clearing the SSSP flags");
   }
   if (rc == GR_OK) rc = sv->lanes.build(g->h_off_in);
+  if (rc == GR_OK) sv->lane_split = graph_lane_split(g);
   if (rc == GR_OK) {
     // the last 64-vertex segment from which on every segment has at most
     // kPprSegEdges in-edges and no heavy row: the segment walk's share
