@@ -39,6 +39,14 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
+# The query server runs its three pools (BFS, SSSP, PPR) on three streams.
+# With HIP's default of 4 hardware queues per process, streams created
+# earlier in the process (the engines' own) can leave two pools on one
+# queue, whose kernels then run one after another: a BFS level waits behind
+# a PPR pass.  8 queues give every pool its own.  Set before HIP starts.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
+
 METRIC = "GTEPS + HBM GB/s fraction at RMAT-24; batched vs plain aggregate TEPS, p99 lat"
 HBM_PEAK_GBS = 8000.0
 

@@ -458,7 +458,11 @@ typedef struct gr_server_stats {
 } gr_server_stats;
 
 /* weights_device (int32 per edge, edge-list order) may be NULL when no SSSP
- * query will be submitted.  The graph must outlive the server. */
+ * query will be submitted.  The graph must outlive the server.  The three
+ * pools run on three streams of their own; give the process at least 8 HIP
+ * hardware queues (GPU_MAX_HW_QUEUES=8 in the environment before HIP starts;
+ * HIP's default is 4), or two pools may share a queue and run their kernels
+ * one after another (a BFS level waiting behind a PPR pass). */
 int gr_server_create(gr_graph* g, const int32_t* weights_device, const gr_server_config* config,
                      gr_server** out);
 int gr_server_destroy(gr_server* s);
