@@ -484,6 +484,8 @@ __device__ __forceinline__ float lane_sum_in_edges(int b, int e, uint64_t active
                                                    const int32_t* __restrict__ srcs,
                                                    const float* __restrict__ contrib,
                                                    const uint64_t* __restrict__ nz_in) {
+  // b and e are wave-uniform; up to kLaneBatch mass rows are read at once
+  // and added in edge order
   const int lane = lane_id();
   const bool mine = (active >> lane) & 1ull;
   float acc = 0.0f;
@@ -497,25 +499,21 @@ __device__ __forceinline__ float lane_sum_in_edges(int b, int e, uint64_t active
     }
     uint64_t todo = __ballot(m != 0);
     while (todo) {
-      int k[4];
-      int cnt = 0;
+      int uu[kLaneBatch];
+      int nb = 0;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        k[j] = todo ? __builtin_ctzll(todo) : 0;
-        if (todo) {
-          todo &= todo - 1;
-          cnt = j + 1;
-        }
+      for (int j = 0; j < kLaneBatch; ++j) {
+        uu[j] = lane_value(u, todo ? __builtin_ctzll(todo) : 0);
+        if (todo) nb = j + 1;
+        todo &= todo - 1;
       }
-      int uu[4];
+      float x[kLaneBatch];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) uu[j] = __shfl(u, k[j], kWave);
-      float x[4];
+      for (int j = 0; j < kLaneBatch; ++j)
+        x[j] = (mine && j < nb) ? contrib[(size_t)uu[j] * kWave + lane] : 0.0f;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) x[j] = (mine && j < cnt) ? contrib[(size_t)uu[j] * kWave + lane] : 0.0f;
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        if (mine && j < cnt) acc = __fadd_rn(acc, x[j]);
+      for (int j = 0; j < kLaneBatch; ++j)
+        if (mine && j < nb) acc = __fadd_rn(acc, x[j]);
     }
   }
   return acc;

@@ -497,9 +497,11 @@ __global__ __launch_bounds__(kSrvNT) void k_srv_ppr_heavy_partial(
   const int lane = lane_id();
   const int wpb = blockDim.x / kWave;
   const int n_waves = gridDim.x * wpb;
-  for (int c = blockIdx.x * wpb + threadIdx.x / kWave; c < n_chunks; c += n_waves) {
-    const LaneChunk ch = chunks[c];
-    partial[(size_t)c * kLanes + lane] = lane_sum_in_edges(ch.b, ch.e, active, srcs, contrib, nz_in);
+  const int w0 = __builtin_amdgcn_readfirstlane(blockIdx.x * wpb + (int)threadIdx.x / kWave);
+  for (int c = w0; c < n_chunks; c += n_waves) {
+    const int b = __builtin_amdgcn_readfirstlane(chunks[c].b);
+    const int e = __builtin_amdgcn_readfirstlane(chunks[c].e);
+    partial[(size_t)c * kLanes + lane] = lane_sum_in_edges(b, e, active, srcs, contrib, nz_in);
   }
 }
 
