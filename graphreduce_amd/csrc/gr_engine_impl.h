@@ -203,18 +203,21 @@ __global__ __launch_bounds__(kApplyNT) void k_apply(
     }
     my_applied += __popc(r_mask);
     __syncthreads();
-    if (tid == 0) {
-      int c = 0, e = 0;
-      for (int k = 0; k < kApplyVT * kNW; ++k) {
-        const int a = s_cnt[k], b = s_deg[k];
-        s_cnt[k] = c;
-        s_deg[k] = e;
-        c += a;
-        e += b;
+    if (wv == 0) {  // the (m, wave) table's exclusive scan, one entry per lane
+      static_assert(kApplyVT * kNW <= kWave, "the table fits one wave");
+      const int lane = lane_id();
+      const int a = lane < kApplyVT * kNW ? s_cnt[lane] : 0;
+      const int b = lane < kApplyVT * kNW ? s_deg[lane] : 0;
+      const int ia = wave_incl_scan_add(a), ib = wave_incl_scan_add(b);
+      if (lane < kApplyVT * kNW) {
+        s_cnt[lane] = ia - a;
+        s_deg[lane] = ib - b;
       }
-      s_base = c ? atomicAdd(&ctr->chg, ((unsigned long long)c << 32) |
-                                            (unsigned long long)(uint32_t)e)
-                 : 0ull;
+      const int c = __shfl(ia, kWave - 1, kWave), e = __shfl(ib, kWave - 1, kWave);
+      if (lane == 0)
+        s_base = c ? atomicAdd(&ctr->chg, ((unsigned long long)c << 32) |
+                                              (unsigned long long)(uint32_t)e)
+                   : 0ull;
     }
     __syncthreads();
     if (want_mask) {
