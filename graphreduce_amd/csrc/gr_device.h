@@ -38,6 +38,18 @@ __device__ __forceinline__ int lanes_below(uint64_t mask) {
                                         __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
+// Value of `x` in lane `l` as a wave-uniform (scalar) value; v_readlane
+// ignores the exec mask, so the lane need not be active.  Loop bounds read
+// this way keep a loop scalar: a bound the compiler must treat as per-lane
+// lets it run lanes through different iterations, in which a shuffle from a
+// lane that is not running reads 0.
+__device__ __forceinline__ int lane_value(int x, int l) { return __builtin_amdgcn_readlane(x, l); }
+__device__ __forceinline__ uint64_t lane_value64(uint64_t x, int l) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, l);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), l);
+  return ((uint64_t)hi << 32) | lo;
+}
+
 template <class T>
 __device__ __forceinline__ T wave_incl_scan_add(T v) {
   const int l = lane_id();

@@ -775,7 +775,7 @@ __global__ void k_fill(int64_t n, T value, T* __restrict__ out) {
 // the chunk contains a hit (a second ballot), so no cross-lane shuffles are
 // needed.  The row holding the chunk start is found from a ballot over the
 // lanes' row offsets; rows already hit, and heavy rows (in-degree above
-// kPullHeavy, done by k_pull_heavy with a wave per row and early exit),
+// kPullHeavy, done by k_pull_heavy with a wave per row in growing rounds and early exit),
 // are jumped over.  Before that, each lane tests kPullFirst in-edges of its
 // own row, heavy or not, the first and the last half of them, in two rounds
 // from both ends inwards: rows list their sources by id, which the degree order
@@ -807,7 +807,8 @@ __device__ __forceinline__ uint64_t lane_range_mask(int lo, int hi) {  // bits [
   const int lane = lane_id();
   const int wpb = blockDim.x / kWave;
   const int nw = gridDim.x * wpb;
-  for (int base = (blockIdx.x * wpb + threadIdx.x / kWave) * kWave; base < nV; base += nw * kWave) {
+  const int w0 = __builtin_amdgcn_readfirstlane(blockIdx.x * wpb + (int)threadIdx.x / kWave);
+  for (int base = w0 * kWave; base < nV; base += nw * kWave) {
     const int cnt = min(kWave, nV - base);
     const bool mine = lane < cnt;
     const int my_off = mine ? off_in[base + lane] : INT_MAX;
@@ -835,15 +836,16 @@ __device__ __forceinline__ uint64_t lane_range_mask(int lo, int hi) {  // bits [
     // rows still open: more than kPullFirst in-edges, not heavy, no hit yet
     const uint64_t need = __ballot(deg > kPullFirst && !heavy && !first_hit);
     const uint64_t settled = ~need;  // for the walk: nothing left to find there
-    const int seg1 = __shfl(my_end, cnt - 1, kWave);
+    // the walk's bounds are scalars (lane_value), so the loop is a scalar loop
+    const int seg1 = lane_value(my_end, cnt - 1);
     int pb = seg1;
-    if (need) pb = __shfl(my_off, __builtin_ctzll(need), kWave) + kPullFirst / 2;
+    if (need) pb = lane_value(my_off, __builtin_ctzll(need)) + kPullFirst / 2;
     while (pb < seg1 && (hits & need) != need) {
       // row holding edge pb: the last lane whose row starts at or before pb
       const uint64_t started = __ballot(mine && my_off <= pb);
       const int r0 = 63 - __builtin_clzll(started);
       if (((settled | hits) >> r0) & 1ull) {
-        pb = __shfl(my_end, r0, kWave);
+        pb = lane_value(my_end, r0);
         continue;
       }
       const int p = pb + lane;
@@ -869,12 +871,13 @@ __device__ __forceinline__ uint64_t lane_range_mask(int lo, int hi) {  // bits [
   }
 }
 
-// The heavy rows' walk: one wave per row reads 64 in-edges per round until one
-// comes from a changed vertex; rows that k_pull_stream's test of the first and
-// last in-edges marked already are skipped.  The walk's cost is its random
+// The heavy rows' walk: one wave per row reads its in-edges in rounds of 64,
+// 128, then 256 (up to four per lane, all in flight) until one comes from a
+// changed vertex; rows that k_pull_stream's test of the first and last
+// in-edges marked already are skipped.  The walk's cost is its random
 // changed-bitmap reads: at RMAT-24 the heavy rows' first hits sit at a median
-// of about 250 in-edges, so steps of a wave read fewer past the hit than
-// steps of a workgroup.
+// of about 250 in-edges (at most about 2,500), so a wave reads little past
+// the hit and the longest walk takes few rounds.  Row bounds are scalars.
 [[maybe_unused]] static __global__ __launch_bounds__(256) void k_pull_heavy(
     const DevCounters* __restrict__ ctr, int mode, long long threshold, long long fuse_max,
     const int32_t* __restrict__ heavy, int n_heavy, const int32_t* __restrict__ off_in,
@@ -884,18 +887,25 @@ __device__ __forceinline__ uint64_t lane_range_mask(int lo, int hi) {  // bits [
   const int lane = lane_id();
   const int wpb = blockDim.x / kWave;
   const int nw = gridDim.x * wpb;
-  for (int h = blockIdx.x * wpb + (int)threadIdx.x / kWave; h < n_heavy; h += nw) {
-    const int v = heavy[h];
+  const int w0 = __builtin_amdgcn_readfirstlane(blockIdx.x * wpb + (int)threadIdx.x / kWave);
+  for (int h = w0; h < n_heavy; h += nw) {
+    const int v = __builtin_amdgcn_readfirstlane(heavy[h]);
     // one load instruction: every lane of the wave sees the same word
-    if ((bitmap[v >> 5] >> (v & 31)) & 1u) continue;
-    const int b = off_in[v], e = off_in[v + 1];
-    for (int pb = b; pb < e; pb += kWave) {
-      const int p = pb + lane;
+    if ((__builtin_amdgcn_readfirstlane(bitmap[v >> 5]) >> (v & 31)) & 1u) continue;
+    const int b = __builtin_amdgcn_readfirstlane(off_in[v]);
+    const int e = __builtin_amdgcn_readfirstlane(off_in[v + 1]);
+    int k = 1;  // 64-edge steps this round
+    for (int pb = b; pb < e; pb += k * kWave, k = k < 4 ? 2 * k : 4) {
       bool hit = false;
-      if (p < e) {
-        const int u = srcs[p];
-        hit = (chg_bits[u >> 5] >> (u & 31)) & 1u;
+      int u[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int p = pb + j * kWave + lane;
+        u[j] = (j < k && p < e) ? srcs[p] : -1;
       }
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (u[j] >= 0) hit |= (chg_bits[u[j] >> 5] >> (u[j] & 31)) & 1u;
       if (__ballot(hit)) {
         if (lane == 0) atomicOr(&bitmap[v >> 5], 1u << (v & 31));
         break;

@@ -110,15 +110,6 @@ __device__ __forceinline__ uint64_t shfl_u64(uint64_t x, int src) {
          (uint64_t)__shfl((uint32_t)x, src, kWave);
 }
 
-// Value of `x` in lane `l` as a wave-uniform (scalar) value; v_readlane
-// ignores the exec mask, so the lane need not be active.
-__device__ __forceinline__ int lane_value(int x, int l) { return __builtin_amdgcn_readlane(x, l); }
-__device__ __forceinline__ uint64_t lane_value64(uint64_t x, int l) {
-  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, l);
-  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), l);
-  return ((uint64_t)hi << 32) | lo;
-}
-
 // Per-lane minimum of dist[u][lane] + w over the in-edges [b, e) (b and e
 // wave-uniform) whose source changed for this lane; up to kLaneBatch changed
 // sources' rows are read at once.
