@@ -61,6 +61,21 @@ __device__ __forceinline__ T wave_incl_scan_add(T v) {
   return v;
 }
 
+// Inclusive prefix sum of an int over the wave by DPP moves, without the LDS
+// crossbar that shuffles use: shifts by 1, 2, 4 and 8 lanes within each row of
+// 16 lanes (lanes whose source lies before their row add 0), then the last
+// lane of row 0 into row 1 and of row 2 into row 3 (row_bcast:15, rows 1 and
+// 3 written), then lane 31 into rows 2 and 3 (row_bcast:31).
+__device__ __forceinline__ int wave_incl_scan_add_dpp(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return v;
+}
+
 template <class T>
 __device__ __forceinline__ T wave_reduce_add(T v) {
 #pragma unroll
@@ -97,19 +112,31 @@ __device__ __forceinline__ T block_excl_scan_add(T v, T* scratch, T* total) {
 // Row blocks of the CSC whose in-edges are listed by source (for the blocked
 // gather): block b holds rows [brow[b], brow[b] + bnr[b]) -- at most
 // gr::kBlockRows (or gr::kBlockRowsWide) rows and, unless one row alone has more, the edge target
-// below -- and their in-edges are positions [boff[b], boff[b + 1]) of
-// bsrc / brl / bpos, sorted by source (stable).  Rows from rows_end on have
-// no in-edges and belong to no block.  The edge target per block is
+// below -- and its in-edges, sorted by source (stable), fill slots
+// [boff[b], boff[b + 1]) of the packed layout.  Rows from rows_end on have no
+// in-edges and belong to no block.  The edge target per block is
 // E / (2 * compute units), at most kBlockEdges and at least 4096.
+//
+// Packed layout: slot s holds bpk[s] = step << 16 | row, where row is the
+// in-edge's row - brow[b] and step its source minus the source of slot s - 1.
+// Slots come in groups of gr::kPackGroup; the first slot of a group stores
+// step 0 and gbase[group] holds its source, so that a source is gbase plus a
+// prefix sum of steps within the group.  Where two consecutive in-edges of a
+// block have sources more than gr::kPackStep apart, pad slots (row
+// gr::kPackPad) of step kPackStep come between them; every block starts on a
+// group and its last group is filled up with pad slots of step 0.  bpos[s] is
+// the CSC position of slot s's in-edge (-1 for a pad slot), which lays the
+// edge data out in slot order.
 struct gr_blocked {
   int32_t n_blocks = 0;
   int32_t rows_end = 0;
+  int64_t n_slots = 0;      // a multiple of kPackGroup
   int32_t* brow = nullptr;  // [n_blocks]
   int32_t* bnr = nullptr;   // [n_blocks]
-  int32_t* boff = nullptr;  // [n_blocks + 1]
-  int32_t* bsrc = nullptr;  // [E] source of the edge
-  uint16_t* brl = nullptr;  // [E] its row - brow of the block
-  int32_t* bpos = nullptr;  // [E] its CSC position
+  int32_t* boff = nullptr;  // [n_blocks + 1] first slot of each block
+  uint32_t* bpk = nullptr;  // [n_slots] step << 16 | row
+  int32_t* gbase = nullptr; // [n_slots / kPackGroup] source of each group's first slot
+  int32_t* bpos = nullptr;  // [n_slots] CSC position, -1 for a pad slot
 };
 
 struct gr_graph {
@@ -149,6 +176,11 @@ constexpr int kBlockRowsWide = 16384;   // ... for 8-byte accumulators (LDS: 128
 #endif
 constexpr int kMinBlockRows = GR_MIN_BLOCK_ROWS;  // rows per block of the int-minimum gather
 constexpr int kBlockEdges = 1 << 19;    // in-edges per gather block (largest target)
+constexpr int kPackGroup = 512;          // slots per group of the packed layout
+constexpr uint32_t kPackPad = 0xFFFFu;   // row of a pad slot
+constexpr int32_t kPackStep = 0xFFFF;    // largest source step one slot holds
+static_assert(kBlockRows <= (int)kPackPad && kBlockRowsWide <= (int)kPackPad,
+              "a block's rows fit below the pad row");
 int graph_free(gr_graph* g);
 // The degree-ordered copy of g (cached in g; freed with g).
 int graph_ordered(gr_graph* g, gr_graph** out);

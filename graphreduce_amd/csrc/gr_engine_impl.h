@@ -1120,14 +1120,15 @@ __device__ __forceinline__ ByteWork byte_work(const uint32_t* __restrict__ bitma
 }
 
 // Gathers n elements of `words` 4-byte words: out[i] = in[eid[i]] (eid may be
-// nullptr: identity).  Moves edge data from edge-list order into CSC order,
+// nullptr: identity; an eid of -1 writes zeros).  Moves edge data from
+// edge-list order into CSC order and into the blocked gather's slot order,
 // and vertex data between user order and the degree-ordered layout.
 [[maybe_unused]] static __global__ void k_permute_words(const uint32_t* __restrict__ in, const int32_t* __restrict__ eid,
                                 int64_t n, int words, uint32_t* __restrict__ out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int64_t s = eid ? (int64_t)eid[i] : i;
-  for (int k = 0; k < words; ++k) out[i * words + k] = in[s * words + k];
+  for (int k = 0; k < words; ++k) out[i * words + k] = s >= 0 ? in[s * words + k] : 0u;
 }
 
 // ------------------------------------------------ out-of-core shards
@@ -2121,10 +2122,10 @@ struct Engine final : EngineBase {
         GR_HIP(hipGetLastError());
         if constexpr (!kEdgeEmpty) {
           static_assert(sizeof(ED) % 4 == 0, "edge data is moved as 4-byte words");
-          GR_TRY(dalloc(&d_ed_blk, g->nE));
-          k_permute_words<<<grid_1d(g->nE, 256), 256, 0, stream>>>(
-              reinterpret_cast<const uint32_t*>(d_ed), blk->bpos, g->nE, (int)(sizeof(ED) / 4),
-              reinterpret_cast<uint32_t*>(d_ed_blk));
+          GR_TRY(dalloc(&d_ed_blk, blk->n_slots));
+          k_permute_words<<<grid_1d(blk->n_slots, 256), 256, 0, stream>>>(
+              reinterpret_cast<const uint32_t*>(d_ed), blk->bpos, blk->n_slots,
+              (int)(sizeof(ED) / 4), reinterpret_cast<uint32_t*>(d_ed_blk));
           GR_HIP(hipGetLastError());
         }
       }
@@ -2141,11 +2142,11 @@ struct Engine final : EngineBase {
       if (blk->n_blocks > 0) {
         if constexpr (kBlockedSum)
           k_gather_blocked_sum<P><<<blk->n_blocks, kBlkNT, 0, stream>>>(
-              blk->brow, blk->bnr, blk->boff, blk->bsrc, blk->brl, d_ed_blk, d_vd, d_srcval,
+              blk->brow, blk->bnr, blk->boff, blk->bpk, blk->gbase, d_ed_blk, d_vd, d_srcval,
               d_gdense, nullptr, nullptr, nullptr);
         else
           k_gather_blocked<P><<<blk->n_blocks, kBlkNT, 0, stream>>>(
-              blk->brow, blk->bnr, blk->boff, blk->bsrc, blk->brl, d_ed_blk, d_vd, d_srcval,
+              blk->brow, blk->bnr, blk->boff, blk->bpk, blk->gbase, d_ed_blk, d_vd, d_srcval,
               dbits, d_gdense);
       }
       tock(kPhGather);
@@ -2326,7 +2327,7 @@ struct Engine final : EngineBase {
         tick();
         if (blk->n_blocks > 0)
           k_gather_blocked_sum<P><<<blk->n_blocks, kBlkNT, 0, stream>>>(
-              blk->brow, blk->bnr, blk->boff, blk->bsrc, blk->brl, d_ed_blk, d_vd, d_srcval,
+              blk->brow, blk->bnr, blk->boff, blk->bpk, blk->gbase, d_ed_blk, d_vd, d_srcval,
               d_gdense, d_chg_bits, d_bitmap, d_ctr);
         tock(kPhGather);
         GR_HIP(hipGetLastError());
@@ -2373,7 +2374,7 @@ struct Engine final : EngineBase {
         tick();
         if (blk->n_blocks > 0)
           k_gather_blocked<P><<<blk->n_blocks, kBlkNT, 0, stream>>>(
-              blk->brow, blk->bnr, blk->boff, blk->bsrc, blk->brl, d_ed_blk, d_vd, d_srcval,
+              blk->brow, blk->bnr, blk->boff, blk->bpk, blk->gbase, d_ed_blk, d_vd, d_srcval,
               d_chg_bits, d_gdense, d_bitmap, d_ctr, dir_mode, thr, fuse_max);
         tock(kPhGather);
         GR_HIP(hipGetLastError());

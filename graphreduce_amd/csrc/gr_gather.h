@@ -511,8 +511,45 @@ __device__ __forceinline__ bool fused_scatter_rule(unsigned long long chg, long 
 // (refgas.h:141-160) in any order.  delta_bits (optional) skips the sources
 // outside the bitmap as k_gather_tiles does.  Writes gdense[row] for every row
 // of every block (vertex-indexed).
+//
+// The in-edges come packed (gr_blocked): each wave takes one group of
+// kPackGroup slots per pass, lane l its slots l, l + 64, ... (one coalesced
+// 4-byte load per 64 slots, so that the 64 sources of one load are
+// consecutive in the block's source order and their random value reads share
+// cache lines); the sources are the group's base plus the prefix sums of the
+// steps, a wave scan per 64 slots.
 constexpr int kBlkNT = 1024;  // one workgroup per CU (128 KB of LDS): 16 waves
 constexpr int kBlkVT = 8;
+static_assert(kBlkVT * kWave == kPackGroup, "a wave's pass is one group");
+
+// The sources and rows of this lane's slots of group `gs` (its first slot);
+// a pad slot yields source -1.  Every lane of the wave calls it together.
+__device__ __forceinline__ void unpack_slots(const uint32_t* __restrict__ bpk,
+                                             const int32_t* __restrict__ gbase, int gs,
+                                             int (&src)[kBlkVT], int (&row)[kBlkVT]) {
+  const int lane = lane_id();
+  uint32_t w[kBlkVT];
+#pragma unroll
+  for (int m = 0; m < kBlkVT; ++m) w[m] = __builtin_nontemporal_load(bpk + gs + m * kWave + lane);
+  int carry = gbase[gs / kPackGroup];
+#pragma unroll
+  for (int m = 0; m < kBlkVT; ++m) {
+    const int incl = wave_incl_scan_add_dpp((int)(w[m] >> 16));
+    row[m] = (int)(w[m] & 0xFFFFu);
+    src[m] = row[m] == (int)kPackPad ? -1 : carry + incl;
+    carry += lane_value(incl, kWave - 1);
+  }
+}
+
+// Edge data of this lane's slots of group `gs` where src >= 0.
+template <class ED>
+__device__ __forceinline__ void load_slot_edges(const ED* __restrict__ ed, int gs,
+                                                const int (&src)[kBlkVT], ED (&ev)[kBlkVT]) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int m = 0; m < kBlkVT; ++m)
+    if (src[m] >= 0) ev[m] = stream_load(ed + gs + m * kWave + lane);
+}
 
 //
 // Fused activation (frontier != nullptr): the kernel is launched by
@@ -525,8 +562,8 @@ constexpr int kBlkVT = 8;
 template <class P>
 __global__ __launch_bounds__(kBlkNT) void k_gather_blocked(
     const int32_t* __restrict__ brow, const int32_t* __restrict__ bnr,
-    const int32_t* __restrict__ boff, const int32_t* __restrict__ bsrc,
-    const uint16_t* __restrict__ brl, const typename P::EdgeData* __restrict__ ed,
+    const int32_t* __restrict__ boff, const uint32_t* __restrict__ bpk,
+    const int32_t* __restrict__ gbase, const typename P::EdgeData* __restrict__ ed,
     const typename P::VertexData* __restrict__ vd, const source_value_t<P>* __restrict__ srcval,
     const uint32_t* __restrict__ delta_bits, typename P::GatherResult* __restrict__ gdense,
     uint32_t* __restrict__ frontier = nullptr, DevCounters* __restrict__ ctr = nullptr,
@@ -549,19 +586,11 @@ __global__ __launch_bounds__(kBlkNT) void k_gather_blocked(
   for (int r = tid; r < nr; r += kBlkNT) acc[r] = P::gatherZero;
   for (int k = tid; k < kMinBlockRows / 32; k += kBlkNT) act[k] = 0u;
   __syncthreads();
-  for (int base = e0; base < e1; base += kBlkNT * kBlkVT) {
+  const int wv = __builtin_amdgcn_readfirstlane(tid / kWave);
+  for (int gs = e0 + wv * kPackGroup; gs < e1; gs += kBlkNT * kBlkVT) {
     int src[kBlkVT], row[kBlkVT];
     typename P::EdgeData ev[kBlkVT];
-#pragma unroll
-    for (int m = 0; m < kBlkVT; ++m) {
-      const int e = base + m * kBlkNT + tid;
-      src[m] = -1;
-      row[m] = 0;
-      if (e < e1) {
-        src[m] = __builtin_nontemporal_load(bsrc + e);
-        row[m] = brl[e];
-      }
-    }
+    unpack_slots(bpk, gbase, gs, src, row);
     if (delta_bits) {
       uint32_t word[kBlkVT];
 #pragma unroll
@@ -571,9 +600,7 @@ __global__ __launch_bounds__(kBlkNT) void k_gather_blocked(
         if (!((word[m] >> (src[m] & 31)) & 1u)) src[m] = -1;
     }
     // edge data only for the edges that count
-#pragma unroll
-    for (int m = 0; m < kBlkVT; ++m)
-      if (src[m] >= 0) ev[m] = stream_load(ed + base + m * kBlkNT + tid);
+    load_slot_edges(ed, gs, src, ev);
     G val[kBlkVT];
 #pragma unroll
     for (int m = 0; m < kBlkVT; ++m)
@@ -635,8 +662,8 @@ constexpr double kFixUnscale = 1.0 / 4294967296.0;     // 2^-32
 template <class P>
 __global__ __launch_bounds__(kBlkNT) void k_gather_blocked_sum(
     const int32_t* __restrict__ brow, const int32_t* __restrict__ bnr,
-    const int32_t* __restrict__ boff, const int32_t* __restrict__ bsrc,
-    const uint16_t* __restrict__ brl, const typename P::EdgeData* __restrict__ ed,
+    const int32_t* __restrict__ boff, const uint32_t* __restrict__ bpk,
+    const int32_t* __restrict__ gbase, const typename P::EdgeData* __restrict__ ed,
     const typename P::VertexData* __restrict__ vd, const source_value_t<P>* __restrict__ srcval,
     typename P::GatherResult* __restrict__ gdense, const uint32_t* __restrict__ chg_bits,
     uint32_t* __restrict__ frontier, const DevCounters* __restrict__ ctr) {
@@ -655,20 +682,12 @@ __global__ __launch_bounds__(kBlkNT) void k_gather_blocked_sum(
     act[r] = 0;
   }
   __syncthreads();
-  for (int base = e0; base < e1; base += kBlkNT * kBlkVT) {
+  const int wv = __builtin_amdgcn_readfirstlane(tid / kWave);
+  for (int gs = e0 + wv * kPackGroup; gs < e1; gs += kBlkNT * kBlkVT) {
     int src[kBlkVT], row[kBlkVT];
     typename P::EdgeData ev[kBlkVT];
-#pragma unroll
-    for (int m = 0; m < kBlkVT; ++m) {
-      const int e = base + m * kBlkNT + tid;
-      src[m] = -1;
-      row[m] = 0;
-      if (e < e1) {
-        src[m] = __builtin_nontemporal_load(bsrc + e);
-        row[m] = brl[e];
-        ev[m] = stream_load(ed + e);
-      }
-    }
+    unpack_slots(bpk, gbase, gs, src, row);
+    load_slot_edges(ed, gs, src, ev);
     unsigned long long fix[kBlkVT];
     uint32_t word[kBlkVT];
 #pragma unroll

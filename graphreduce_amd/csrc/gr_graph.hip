@@ -128,6 +128,10 @@ struct DevBuf {
     p = nullptr;
     return r;
   }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+  }
 };
 
 // Sorts (keys, vals) by the low `bits` bits of the keys; results in *_out.
@@ -184,8 +188,8 @@ int graph_free(gr_graph* g) {
   if (g->ordered) graph_free(g->ordered);
   for (gr_blocked* b : {g->blocked, g->blocked_half}) {
     if (!b) continue;
-    for (void* p : {(void*)b->brow, (void*)b->bnr, (void*)b->boff, (void*)b->bsrc,
-                    (void*)b->brl, (void*)b->bpos})
+    for (void* p : {(void*)b->brow, (void*)b->bnr, (void*)b->boff, (void*)b->bpk,
+                    (void*)b->gbase, (void*)b->bpos})
       if (p) (void)hipFree(p);
     delete b;
   }
@@ -334,6 +338,111 @@ __global__ void k_block_split(int64_t nE, const uint64_t* __restrict__ keys,
   bpos[k] = j;
 }
 
+// Slots of sorted in-edge k in the packed layout: its own, and one pad slot
+// per kPackStep of source distance beyond the first from the block's previous
+// in-edge.  size[nE] = 0 closes the array for the exclusive scan.
+__global__ void k_pack_sizes(int64_t nE, const uint64_t* __restrict__ keys, int shift,
+                             const int32_t* __restrict__ bsrc, int32_t* __restrict__ size) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > nE) return;
+  if (k == nE) {
+    size[k] = 0;
+    return;
+  }
+  int32_t n = 1;
+  if (k > 0 && (keys[k] >> shift) == (keys[k - 1] >> shift)) {
+    const int32_t gap = bsrc[k] - bsrc[k - 1];  // >= 0: sorted by source within the block
+    if (gap > kPackStep) n += (gap - 1) / kPackStep;
+  }
+  size[k] = n;
+}
+
+// out[i] = in[idx[i]].
+__global__ void k_take(int n, const int32_t* __restrict__ idx, const int32_t* __restrict__ in,
+                       int32_t* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[idx[i]];
+}
+
+// Every slot a pad of step 0, no in-edge.
+__global__ void k_pack_fill(int64_t n, uint32_t* __restrict__ bpk, int32_t* __restrict__ bpos) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  bpk[i] = kPackPad;
+  bpos[i] = -1;
+}
+
+__device__ __forceinline__ void pack_slot(int64_t s, int32_t step, uint32_t row, int32_t src,
+                                          int32_t pos, uint32_t* __restrict__ bpk,
+                                          int32_t* __restrict__ gbase,
+                                          int32_t* __restrict__ bpos) {
+  if (s % kPackGroup == 0) {  // a group's first slot: its source is the base
+    gbase[s / kPackGroup] = src;
+    step = 0;
+  }
+  bpk[s] = ((uint32_t)step << 16) | row;
+  bpos[s] = pos;
+}
+
+// Sorted in-edge k and the pad slots before it, into the packed layout:
+// in-edge k of block b starts its run at slot pstart[b] + scan[k] - scan of
+// the block's first in-edge (estart[b]).
+__global__ void k_pack_scatter(int64_t nE, const uint64_t* __restrict__ keys, int shift,
+                               const int32_t* __restrict__ bsrc, const uint16_t* __restrict__ brl,
+                               const int32_t* __restrict__ pos, const int32_t* __restrict__ scan,
+                               const int32_t* __restrict__ estart,
+                               const int32_t* __restrict__ pstart, uint32_t* __restrict__ bpk,
+                               int32_t* __restrict__ gbase, int32_t* __restrict__ bpos) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nE) return;
+  const int32_t b = (int32_t)(keys[k] >> shift);
+  const bool first = k == 0 || (keys[k - 1] >> shift) != (uint64_t)b;
+  const int64_t s0 = (int64_t)pstart[b] + (scan[k] - estart[b]);
+  const int32_t pads = scan[k + 1] - scan[k] - 1;
+  const int32_t prev = first ? bsrc[k] : bsrc[k - 1];
+  for (int32_t i = 0; i < pads; ++i)
+    pack_slot(s0 + i, kPackStep, kPackPad, prev + (i + 1) * kPackStep, -1, bpk, gbase, bpos);
+  const int32_t step = first ? 0 : bsrc[k] - prev - pads * kPackStep;
+  pack_slot(s0 + pads, step, brl[k], bsrc[k], pos[k], bpk, gbase, bpos);
+}
+
+// Decodes every group of the packed layout as the blocked gather does and
+// compares each slot with the CSC: err[0] counts mismatches (a step in a
+// group's first slot, a pad slot naming an in-edge, a source or row other than
+// the CSC's), err[1] the slots that name an in-edge.
+__global__ void k_pack_check(int64_t n_groups, const uint32_t* __restrict__ bpk,
+                             const int32_t* __restrict__ gbase, const int32_t* __restrict__ bpos,
+                             const int32_t* __restrict__ pstart, int nb,
+                             const int32_t* __restrict__ brow, const int32_t* __restrict__ srcs,
+                             const int32_t* __restrict__ rowid, int64_t nE,
+                             unsigned long long* __restrict__ err) {
+  const int64_t gi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gi >= n_groups) return;
+  const int64_t s0 = gi * kPackGroup;
+  int lo = 0, hi = nb;  // the block: last pstart[b] <= s0
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) / 2;
+    if ((int64_t)pstart[mid] <= s0) lo = mid; else hi = mid;
+  }
+  unsigned long long bad = 0, real = 0;
+  int32_t src = gbase[gi];
+  for (int i = 0; i < kPackGroup; ++i) {
+    const uint32_t w = bpk[s0 + i];
+    if (i == 0) bad += (w >> 16) != 0u;
+    else src += (int32_t)(w >> 16);
+    const uint32_t row = w & 0xFFFFu;
+    const int32_t j = bpos[s0 + i];
+    if (row == kPackPad) {
+      bad += j != -1;
+      continue;
+    }
+    ++real;
+    if (j < 0 || (int64_t)j >= nE || srcs[j] != src || rowid[j] - brow[lo] != (int32_t)row) ++bad;
+  }
+  if (bad) atomicAdd(&err[0], bad);
+  if (real) atomicAdd(&err[1], real);
+}
+
 std::mutex g_block_mu;
 
 }  // namespace
@@ -380,18 +489,24 @@ int graph_blocked(gr_graph* g, int max_rows, const gr_blocked** out) {
   }
   boff.push_back(off[last]);
   const int32_t nb = (int32_t)brow.size();
+  // Slot bound: every in-edge, the pads of a block (its source range over
+  // kPackStep, at most nV / kPackStep + 1) and a group of fill per block.
+  const int64_t slot_bound =
+      nE + (int64_t)nb * ((int64_t)nV / kPackStep + 1 + kPackGroup) + kPackGroup;
+  if (slot_bound >= (int64_t)INT32_MAX)
+    return set_errorf(GR_ERR_UNSUPPORTED, "graph_blocked: %lld slots exceed int32",
+                      (long long)slot_bound);
   gr_blocked* b = new gr_blocked();
   b->n_blocks = nb;
   b->rows_end = last;
-  DevBuf<int32_t> d_brow, d_bnr, d_boff, d_bsrc, d_bpos, d_rowblk;
-  DevBuf<uint16_t> d_brl;
+  DevBuf<int32_t> d_brow, d_bnr, d_boff, d_rowblk, d_gbase, d_bpos;
+  DevBuf<uint32_t> d_bpk;
   int rc = GR_OK;
   auto fail = [&](int code) {
     delete b;
     return code;
   };
   if ((rc = d_brow.alloc(nb)) || (rc = d_bnr.alloc(nb)) || (rc = d_boff.alloc(nb + 1)) ||
-      (rc = d_bsrc.alloc(nE)) || (rc = d_brl.alloc(nE)) || (rc = d_bpos.alloc(nE)) ||
       (rc = d_rowblk.alloc(nV)))
     return fail(rc);
   if (nb > 0) {
@@ -399,14 +514,15 @@ int graph_blocked(gr_graph* g, int max_rows, const gr_blocked** out) {
         hipMemcpy(d_bnr.p, bnr.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice))
       return fail(set_error(GR_ERR_HIP, "graph_blocked: block table upload"));
   }
-  if (hipMemcpy(d_boff.p, boff.data(), sizeof(int32_t) * (nb + 1), hipMemcpyHostToDevice) ||
-      (nV > 0 && hipMemcpy(d_rowblk.p, row_block.data(), sizeof(int32_t) * nV,
-                           hipMemcpyHostToDevice)))
+  if (nV > 0 && hipMemcpy(d_rowblk.p, row_block.data(), sizeof(int32_t) * nV,
+                          hipMemcpyHostToDevice))
     return fail(set_error(GR_ERR_HIP, "graph_blocked: block table upload"));
+  std::vector<int32_t> pstart((size_t)nb + 1, 0);
   if (nE > 0) {
     const int shift = bits_for(nV);
     DevBuf<uint64_t> keys, ksorted;
-    DevBuf<int32_t> vals, vsorted, rowid;
+    DevBuf<int32_t> vals, vsorted, rowid, bsrc, size, scan, eidx, estart, d_pstart;
+    DevBuf<uint16_t> brl;
     if ((rc = keys.alloc(nE)) || (rc = ksorted.alloc(nE)) || (rc = vals.alloc(nE)) ||
         (rc = vsorted.alloc(nE)) || (rc = rowid.alloc(nE)))
       return fail(rc);
@@ -416,17 +532,79 @@ int graph_blocked(gr_graph* g, int max_rows, const gr_blocked** out) {
     if (hipGetLastError() != hipSuccess) return fail(set_error(GR_ERR_HIP, "k_block_keys"));
     if ((rc = radix_sort_pairs(keys.p, ksorted.p, vals.p, vsorted.p, nE, shift + bits_for(nb))))
       return fail(rc);
+    keys.reset();
+    vals.reset();
+    // sorted in-edges: source, row in block, CSC position
+    DevBuf<int32_t> pos;
+    if ((rc = bsrc.alloc(nE)) || (rc = brl.alloc(nE)) || (rc = pos.alloc(nE))) return fail(rc);
     k_block_split<<<grid_for(nE), kBlock>>>(nE, ksorted.p, vsorted.p, shift, rowid.p, d_brow.p,
-                                            d_bsrc.p, d_brl.p, d_bpos.p);
+                                            bsrc.p, brl.p, pos.p);
     if (hipGetLastError() != hipSuccess) return fail(set_error(GR_ERR_HIP, "k_block_split"));
-    if (hipDeviceSynchronize() != hipSuccess)
+    vsorted.reset();
+    // slots per in-edge, their exclusive scan (nE + 1 entries), the scan at
+    // each block's first in-edge
+    if ((rc = size.alloc(nE + 1)) || (rc = scan.alloc(nE + 1)) || (rc = eidx.alloc(nb + 1)) ||
+        (rc = estart.alloc(nb + 1)))
+      return fail(rc);
+    k_pack_sizes<<<grid_for(nE + 1), kBlock>>>(nE, ksorted.p, shift, bsrc.p, size.p);
+    if (hipGetLastError() != hipSuccess) return fail(set_error(GR_ERR_HIP, "k_pack_sizes"));
+    {
+      size_t temp = 0;
+      if (hipcub::DeviceScan::ExclusiveSum(nullptr, temp, size.p, scan.p, (int)(nE + 1)))
+        return fail(set_error(GR_ERR_HIP, "graph_blocked: slot scan"));
+      DevBuf<uint8_t> t;
+      if ((rc = t.alloc((int64_t)temp))) return fail(rc);
+      if (hipcub::DeviceScan::ExclusiveSum(t.p, temp, size.p, scan.p, (int)(nE + 1)))
+        return fail(set_error(GR_ERR_HIP, "graph_blocked: slot scan"));
+    }
+    if (hipMemcpy(eidx.p, boff.data(), sizeof(int32_t) * (nb + 1), hipMemcpyHostToDevice))
+      return fail(set_error(GR_ERR_HIP, "graph_blocked: block table upload"));
+    k_take<<<grid_for(nb + 1), kBlock>>>(nb + 1, eidx.p, scan.p, estart.p);
+    if (hipGetLastError() != hipSuccess) return fail(set_error(GR_ERR_HIP, "k_take"));
+    std::vector<int32_t> h_estart((size_t)nb + 1);
+    if (hipMemcpy(h_estart.data(), estart.p, sizeof(int32_t) * (nb + 1), hipMemcpyDeviceToHost))
+      return fail(set_error(GR_ERR_HIP, "graph_blocked: slot counts"));
+    // every block starts on a group; its last group is filled up
+    for (int32_t k = 0; k < nb; ++k) {
+      const int64_t len = (int64_t)h_estart[k + 1] - h_estart[k];
+      pstart[k + 1] = (int32_t)((pstart[k] + len + kPackGroup - 1) / kPackGroup * kPackGroup);
+    }
+    b->n_slots = pstart[nb];
+    if ((rc = d_pstart.alloc(nb + 1)) || (rc = d_bpk.alloc(b->n_slots)) ||
+        (rc = d_bpos.alloc(b->n_slots)) || (rc = d_gbase.alloc(b->n_slots / kPackGroup)))
+      return fail(rc);
+    if (hipMemcpy(d_pstart.p, pstart.data(), sizeof(int32_t) * (nb + 1), hipMemcpyHostToDevice))
+      return fail(set_error(GR_ERR_HIP, "graph_blocked: block table upload"));
+    k_pack_fill<<<grid_for(b->n_slots), kBlock>>>(b->n_slots, d_bpk.p, d_bpos.p);
+    if (hipGetLastError() != hipSuccess) return fail(set_error(GR_ERR_HIP, "k_pack_fill"));
+    k_pack_scatter<<<grid_for(nE), kBlock>>>(nE, ksorted.p, shift, bsrc.p, brl.p, pos.p, scan.p,
+                                             estart.p, d_pstart.p, d_bpk.p, d_gbase.p, d_bpos.p);
+    if (hipGetLastError() != hipSuccess) return fail(set_error(GR_ERR_HIP, "k_pack_scatter"));
+    DevBuf<unsigned long long> err;
+    if ((rc = err.alloc(2))) return fail(rc);
+    if (hipMemset(err.p, 0, 2 * sizeof(unsigned long long)))
+      return fail(set_error(GR_ERR_HIP, "graph_blocked: check counters"));
+    const int64_t n_groups = b->n_slots / kPackGroup;
+    k_pack_check<<<grid_for(n_groups), kBlock>>>(n_groups, d_bpk.p, d_gbase.p, d_bpos.p,
+                                                 d_pstart.p, nb, d_brow.p, g->srcs, rowid.p, nE,
+                                                 err.p);
+    if (hipGetLastError() != hipSuccess) return fail(set_error(GR_ERR_HIP, "k_pack_check"));
+    unsigned long long h_err[2] = {0, 0};
+    if (hipMemcpy(h_err, err.p, sizeof(h_err), hipMemcpyDeviceToHost) != hipSuccess)
       return fail(set_error(GR_ERR_HIP, "blocked CSC build failed"));
+    if (h_err[0] != 0 || h_err[1] != (unsigned long long)nE)
+      return fail(set_errorf(GR_ERR_HIP,
+                             "graph_blocked: packed layout check: %llu bad slots, %llu of %lld "
+                             "in-edges placed",
+                             h_err[0], h_err[1], (long long)nE));
   }
+  if (hipMemcpy(d_boff.p, pstart.data(), sizeof(int32_t) * (nb + 1), hipMemcpyHostToDevice))
+    return fail(set_error(GR_ERR_HIP, "graph_blocked: block table upload"));
   b->brow = d_brow.release();
   b->bnr = d_bnr.release();
   b->boff = d_boff.release();
-  b->bsrc = d_bsrc.release();
-  b->brl = d_brl.release();
+  b->bpk = d_bpk.release();
+  b->gbase = d_gbase.release();
   b->bpos = d_bpos.release();
   slot = b;
   *out = b;
