@@ -522,16 +522,20 @@ constexpr int kBlkNT = 1024;  // one workgroup per CU (128 KB of LDS): 16 waves
 constexpr int kBlkVT = 8;
 static_assert(kBlkVT * kWave == kPackGroup, "a wave's pass is one group");
 
-// The sources and rows of this lane's slots of group `gs` (its first slot);
-// a pad slot yields source -1.  Every lane of the wave calls it together.
-__device__ __forceinline__ void unpack_slots(const uint32_t* __restrict__ bpk,
-                                             const int32_t* __restrict__ gbase, int gs,
-                                             int (&src)[kBlkVT], int (&row)[kBlkVT]) {
+// This lane's packed slots of group `gs` (its first slot).
+__device__ __forceinline__ void load_slot_words(const uint32_t* __restrict__ bpk, int gs,
+                                                uint32_t (&w)[kBlkVT]) {
   const int lane = lane_id();
-  uint32_t w[kBlkVT];
 #pragma unroll
   for (int m = 0; m < kBlkVT; ++m) w[m] = __builtin_nontemporal_load(bpk + gs + m * kWave + lane);
-  int carry = gbase[gs / kPackGroup];
+}
+
+// The sources and rows of this lane's slots of a group whose first slot's
+// source is `base`; a pad slot yields source -1.  Every lane of the wave
+// calls it together.
+__device__ __forceinline__ void decode_slots(const uint32_t (&w)[kBlkVT], int base,
+                                             int (&src)[kBlkVT], int (&row)[kBlkVT]) {
+  int carry = base;
 #pragma unroll
   for (int m = 0; m < kBlkVT; ++m) {
     const int incl = wave_incl_scan_add_dpp((int)(w[m] >> 16));
@@ -586,11 +590,19 @@ __global__ __launch_bounds__(kBlkNT) void k_gather_blocked(
   for (int r = tid; r < nr; r += kBlkNT) acc[r] = P::gatherZero;
   for (int k = tid; k < kMinBlockRows / 32; k += kBlkNT) act[k] = 0u;
   __syncthreads();
+  // each wave's next group is loaded while it works on the current one
+  constexpr int kPass = kBlkNT * kBlkVT;
   const int wv = __builtin_amdgcn_readfirstlane(tid / kWave);
-  for (int gs = e0 + wv * kPackGroup; gs < e1; gs += kBlkNT * kBlkVT) {
+  uint32_t wcur[kBlkVT] = {};
+  if (e0 + wv * kPackGroup < e1) load_slot_words(bpk, e0 + wv * kPackGroup, wcur);
+  for (int gs = e0 + wv * kPackGroup; gs < e1; gs += kPass) {
+    uint32_t wnext[kBlkVT] = {};
+    if (gs + kPass < e1) load_slot_words(bpk, gs + kPass, wnext);
     int src[kBlkVT], row[kBlkVT];
     typename P::EdgeData ev[kBlkVT];
-    unpack_slots(bpk, gbase, gs, src, row);
+    decode_slots(wcur, gbase[gs / kPackGroup], src, row);
+#pragma unroll
+    for (int m = 0; m < kBlkVT; ++m) wcur[m] = wnext[m];
     if (delta_bits) {
       uint32_t word[kBlkVT];
 #pragma unroll
@@ -682,11 +694,18 @@ __global__ __launch_bounds__(kBlkNT) void k_gather_blocked_sum(
     act[r] = 0;
   }
   __syncthreads();
+  constexpr int kPass = kBlkNT * kBlkVT;
   const int wv = __builtin_amdgcn_readfirstlane(tid / kWave);
-  for (int gs = e0 + wv * kPackGroup; gs < e1; gs += kBlkNT * kBlkVT) {
+  uint32_t wcur[kBlkVT] = {};
+  if (e0 + wv * kPackGroup < e1) load_slot_words(bpk, e0 + wv * kPackGroup, wcur);
+  for (int gs = e0 + wv * kPackGroup; gs < e1; gs += kPass) {
+    uint32_t wnext[kBlkVT] = {};
+    if (gs + kPass < e1) load_slot_words(bpk, gs + kPass, wnext);
     int src[kBlkVT], row[kBlkVT];
     typename P::EdgeData ev[kBlkVT];
-    unpack_slots(bpk, gbase, gs, src, row);
+    decode_slots(wcur, gbase[gs / kPackGroup], src, row);
+#pragma unroll
+    for (int m = 0; m < kBlkVT; ++m) wcur[m] = wnext[m];
     load_slot_edges(ed, gs, src, ev);
     unsigned long long fix[kBlkVT];
     uint32_t word[kBlkVT];
