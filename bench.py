@@ -425,8 +425,10 @@ def run_batched(gr, g, args, outdeg, w_dev):
     same sources (seeded random vertices with out-degree > 0), every run on
     the same clock: host wall time around the call, synchronised on both
     sides.  The plain engines carry the tuned settings and are bound before
-    timing; the batch call includes its own state allocations and leaves its
-    results in a preallocated device array.  All 64 rows are compared."""
+    timing; the batches run on a context (gr_batch) created before timing
+    and leave their results in a preallocated device array; the one-shot
+    calls, which create and free their context inside, are timed beside
+    them.  All 64 rows are compared."""
     from graphreduce_amd import tune
 
     n = g.n_vertices
@@ -454,29 +456,41 @@ def run_batched(gr, g, args, outdeg, w_dev):
         er_tot += int(outdeg[row >= 0].sum())
         plain_rows.append(row)
     eng.close()
-    gr.msbfs(g, srcs, out=out)  # warm
+    # the batch context is created before timing, as the plain engine is bound
+    # before timing; the one-shot call (context created and freed inside) is
+    # timed beside it
+    bctx = gr.Batch(g, w_dev)
+    bctx.bfs(srcs, out=out)  # warm
     gr.synchronize()
     t0 = time.perf_counter()
-    _, info = gr.msbfs(g, srcs, out=out)
+    _, info = bctx.bfs(srcs, out=out)
     gr.synchronize()
     t_b = time.perf_counter() - t0
     depth = gr.download(out, np.int32, (64, n))
     same = all(np.array_equal(depth[i], plain_rows[i]) for i in range(64))
+    gr.msbfs(g, srcs, out=out)  # warm
+    gr.synchronize()
+    t0 = time.perf_counter()
+    gr.msbfs(g, srcs, out=out)
+    gr.synchronize()
+    t_once = time.perf_counter() - t0
     res["bfs"] = dict(plain_sequential_ms=round(t_plain * 1e3, 3), batched_ms=round(t_b * 1e3, 3),
                       batched_device_ms=round(info["ms"], 3),
                       plain_aggregate_gteps=round(er_tot / t_plain / 1e9, 3),
                       batched_aggregate_gteps=round(er_tot / t_b / 1e9, 3),
                       speedup=round(t_plain / t_b, 2), levels=info["levels"],
-                      rows_compared=64, identical_to_plain=bool(same))
+                      rows_compared=64, identical_to_plain=bool(same),
+                      one_shot_ms=round(t_once * 1e3, 3),
+                      one_shot_speedup=round(t_plain / t_once, 2))
     log(f"batched BFS: {res['bfs']}")
     plain_bfs = plain_rows
     del depth
     # the same batch as ONE persistent cooperative launch of every level
     # (GR_MSBFS_PERSISTENT): measured beside the default per-level launches
-    gr.msbfs(g, srcs, persistent=True, out=out)  # warm
+    bctx.bfs(srcs, persistent=True, out=out)  # warm
     gr.synchronize()
     t0 = time.perf_counter()
-    _, pinfo = gr.msbfs(g, srcs, persistent=True, out=out)
+    _, pinfo = bctx.bfs(srcs, persistent=True, out=out)
     gr.synchronize()
     t_p = time.perf_counter() - t0
     pdepth = gr.download(out, np.int32, (64, n))
@@ -508,20 +522,29 @@ def run_batched(gr, g, args, outdeg, w_dev):
         er_tot += int(outdeg[row < gr.SSSP_INF].sum())
         rows.append(row)
         eng.close()
-    gr.mssp(g, w_dev, srcs, out=out)  # warm
+    bctx.sssp(srcs, out=out)  # warm
     gr.synchronize()
     t0 = time.perf_counter()
-    _, info = gr.mssp(g, w_dev, srcs, out=out)
+    _, info = bctx.sssp(srcs, out=out)
     gr.synchronize()
     t_b = time.perf_counter() - t0
     dist = gr.download(out, np.int32, (64, n))
     same = all(np.array_equal(dist[i], rows[i]) for i in range(64))
+    bctx.close()
+    gr.mssp(g, w_dev, srcs, out=out)  # warm
+    gr.synchronize()
+    t0 = time.perf_counter()
+    gr.mssp(g, w_dev, srcs, out=out)
+    gr.synchronize()
+    t_once = time.perf_counter() - t0
     res["sssp"] = dict(plain_sequential_ms=round(t_plain * 1e3, 3), batched_ms=round(t_b * 1e3, 3),
                        batched_device_ms=round(info["ms"], 3),
                        plain_aggregate_gteps=round(er_tot / t_plain / 1e9, 3),
                        batched_aggregate_gteps=round(er_tot / t_b / 1e9, 3),
                        speedup=round(t_plain / t_b, 2), rounds=info["levels"],
-                       rows_compared=64, identical_to_plain=bool(same))
+                       rows_compared=64, identical_to_plain=bool(same),
+                       one_shot_ms=round(t_once * 1e3, 3),
+                       one_shot_speedup=round(t_plain / t_once, 2))
     log(f"batched SSSP: {res['sssp']}")
     out.free()
     return res

@@ -24,7 +24,7 @@ __all__ = [
     "GRError", "lib", "Graph", "GASEngine", "PROGRAMS", "BFS", "SSSP", "PAGERANK", "CC",
     "rmat_edges", "grid_edges", "hash_weights", "device_count", "device_info",
     "run_bfs", "run_sssp", "run_cc", "run_pagerank", "SSSP_INF", "pagerank_precise",
-    "msbfs", "mssp", "upload_i32", "Server", "QUERY_BFS", "QUERY_SSSP", "QUERY_PPR",
+    "msbfs", "mssp", "Batch", "upload_i32", "Server", "QUERY_BFS", "QUERY_SSSP", "QUERY_PPR",
     "SSSP_CONGESTION", "read_graph", "write_gr", "write_mtx", "edge_list_to_csr",
 ]
 
@@ -191,6 +191,11 @@ _SIGS = {
     "gr_msbfs": (C.c_int, [_P, _I32P, C.c_int, _P, C.c_int, C.c_double,
                             C.POINTER(_BatchResult)]),
     "gr_mssp": (C.c_int, [_P, _P, _I32P, C.c_int, _P, C.POINTER(_BatchResult)]),
+    "gr_batch_create": (C.c_int, [_P, _P, C.POINTER(_P)]),
+    "gr_batch_bfs": (C.c_int, [_P, _I32P, C.c_int, _P, C.c_int, C.c_double,
+                               C.POINTER(_BatchResult)]),
+    "gr_batch_sssp": (C.c_int, [_P, _I32P, C.c_int, _P, C.POINTER(_BatchResult)]),
+    "gr_batch_destroy": (C.c_int, [_P]),
     "gr_server_create": (C.c_int, [_P, _P, C.POINTER(_ServerConfig), C.POINTER(_P)]),
     "gr_server_destroy": (C.c_int, [_P]),
     "gr_server_submit": (C.c_int, [_P, C.c_int, C.c_int32, C.c_int, _I64P]),
@@ -923,6 +928,67 @@ def mssp(graph: Graph, weights: "DeviceArray", sources, out: "DeviceArray | None
     finally:
         lib().gr_device_free(d)
     return out, {name: getattr(r, name) for name, _ in _BatchResult._fields_}
+
+
+class Batch:
+    """A batch context (gr_batch_create): the state of batched runs on one
+    graph, allocated by the first run that needs it and reused by later runs,
+    as an engine binds its graph once.  weights (a DeviceArray of int32 per
+    edge in edge-list order) is needed for sssp().  The graph must stay open
+    while the context is."""
+
+    def __init__(self, graph: Graph, weights: "DeviceArray | None" = None):
+        h = _P()
+        _check(lib().gr_batch_create(graph.handle, weights.ptr if weights is not None else None,
+                                     C.byref(h)))
+        self._h = h
+        self._graph = graph  # kept alive with the context
+        self.n_vertices = graph.n_vertices
+
+    def _run(self, fn, src, out, *args):
+        n = self.n_vertices
+        nbytes = 4 * n * len(src)
+        r = _BatchResult()
+        if out is not None:
+            if out.nbytes < nbytes:
+                raise GRError(1, "batch: output array too small")
+            _check(fn(self._h, _p32(src), len(src), out.ptr, *args, C.byref(r)))
+            return None, {name: getattr(r, name) for name, _ in _BatchResult._fields_}
+        d = C.c_void_p()
+        _check(lib().gr_device_alloc(nbytes, C.byref(d)))
+        try:
+            _check(fn(self._h, _p32(src), len(src), d, *args, C.byref(r)))
+            res = np.empty((len(src), n), np.int32)
+            _check(lib().gr_memcpy_d2h(res.ctypes.data, d, nbytes))
+        finally:
+            lib().gr_device_free(d)
+        return res, {name: getattr(r, name) for name, _ in _BatchResult._fields_}
+
+    def bfs(self, sources, direction: int = 0, alpha: float = 20.0, persistent: bool = False,
+            pull_filter: bool = False, out: "DeviceArray | None" = None):
+        """msbfs() on this context; same arguments and results."""
+        if persistent:
+            direction |= MSBFS_PERSISTENT
+        if pull_filter:
+            direction |= MSBFS_FILTER
+        return self._run(lib().gr_batch_bfs, _batch_sources(sources), out, direction, alpha)
+
+    def sssp(self, sources, out: "DeviceArray | None" = None):
+        """mssp() on this context's weights; same results."""
+        return self._run(lib().gr_batch_sssp, _batch_sources(sources), out)
+
+    def close(self) -> None:
+        if self._h:
+            lib().gr_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        if _SHUTDOWN:
+            return
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def device_empty(nbytes: int) -> DeviceArray:

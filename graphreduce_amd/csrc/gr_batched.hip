@@ -27,7 +27,10 @@
 // once into a hipGraph and replayed.
 #include <hip/hip_cooperative_groups.h>
 
+#include <string.h>
+
 #include <algorithm>
+#include <new>
 #include <vector>
 
 #include "gr_device.h"
@@ -850,120 +853,193 @@ using namespace gr;
 constexpr bool kStreamPullEnabled = false;
 constexpr int kCoopBlocksPerCu = 4;  // cooperative MS-BFS grid: workgroups per CU
 
-extern "C" int gr_msbfs(gr_graph* g, const int32_t* sources, int n_sources, int32_t* depth_device,
-                        int direction, double alpha, gr_batch_result* result) {
+// ---------------------------------------------------------- batch context
+//
+// The state of batched runs on one graph, allocated on the first run that
+// needs it and reused by every later run on the context, as an engine binds
+// its graph once: the three mask arrays (BFS visit/frontier/next; the first
+// two double as SSSP's changed masks), the 1-bit frontier map, the [V][64]
+// rows (BFS depths or SSSP distances), the heavy-row chunk tables of the CSC
+// and CSR, and for SSSP the weights in CSC order and the recorded rounds.
+// Runs on one context are serialised on its stream.
+struct gr_batch {
+  gr_graph* user_g = nullptr;  // the caller's graph
+  gr_graph* g = nullptr;       // its degree-ordered copy (cached in user_g)
+  int nV = 0;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  bool tables = false;
+  gr::LaneTables in_chunks, out_chunks;  // heavy CSC rows / CSR rows
+  gr::DBuf<uint64_t> m0, m1, m2;
+  gr::DBuf<uint32_t> fbits;
+  gr::DBuf<int32_t> rows;  // nV * 64
+  gr::DBuf<int32_t> d_src, d_src_user;
+  gr::DBuf<gr::MsCounters> ctr;
+  gr::DBuf<gr::MsLevelCounters> lc;
+  void* h_pinned = nullptr;  // host-side counters (pinned)
+  // SSSP
+  bool has_weights = false;
+  gr::DBuf<int32_t> w_csc;
+  gr::DBuf<unsigned long long> cnt, any;
+  gr::DBuf<uint8_t> dirty;
+  gr::DBuf<int32_t> list;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int32_t seg_split = 0;
+
+  ~gr_batch() {
+    if (st) (void)hipStreamSynchronize(st);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (h_pinned) (void)hipHostFree(h_pinned);
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+namespace gr {
+namespace {
+
+constexpr int kSsspCnt = 16;    // counter slots per group (see batch_sssp)
+constexpr int kSsspRounds = 8;  // rounds recorded into one hipGraph
+static_assert(kSsspRounds <= kSsspCnt, "one counter slot per round");
+constexpr size_t kPinnedBytes = 4 * kSsspCnt * sizeof(unsigned long long);
+static_assert(kPinnedBytes >= sizeof(MsLevelCounters) && kPinnedBytes >= sizeof(MsCounters),
+              "the pinned block holds every counter set");
+
+int batch_common(gr_batch* b) {
+  if (b->m0.p) return GR_OK;
+  const int nV = b->nV;
+  GR_TRY(b->m0.alloc(nV));
+  GR_TRY(b->m1.alloc(nV));
+  GR_TRY(b->m2.alloc(nV));
+  GR_TRY(b->fbits.alloc((size_t)nV / 32 + 2));
+  GR_TRY(b->rows.alloc((size_t)nV * kWave));
+  GR_TRY(b->d_src.alloc(kWave));
+  GR_TRY(b->d_src_user.alloc(kWave));
+  GR_TRY(b->ctr.alloc(1));
+  GR_TRY(b->lc.alloc(1));
+  GR_HIP(hipHostMalloc(&b->h_pinned, kPinnedBytes, hipHostMallocDefault));
+  return GR_OK;
+}
+
+int batch_tables(gr_batch* b) {
+  if (b->tables) return GR_OK;
+  GR_TRY(b->in_chunks.build(b->g->h_off_in));
+  GR_TRY(b->out_chunks.build(b->g->h_off_out));
+  b->tables = true;
+  return GR_OK;
+}
+
+// The caller's sources, checked, into d_src (layout ids) on the stream.
+int batch_sources(gr_batch* b, const char* who, const int32_t* sources, int n_sources) {
+  if (!sources || n_sources < 1 || n_sources > 64)
+    return set_errorf(GR_ERR_INVALID, "%s: bad arguments", who);
+  for (int s = 0; s < n_sources; ++s)
+    if (sources[s] < 0 || sources[s] >= b->nV)
+      return set_errorf(GR_ERR_INVALID, "%s: source %d out of range", who, sources[s]);
+  int32_t* h = static_cast<int32_t*>(b->h_pinned);
+  GR_HIP(hipStreamSynchronize(b->st));  // the pinned block is free
+  memcpy(h, sources, sizeof(int32_t) * n_sources);
+  GR_HIP(hipMemcpyAsync(b->d_src_user.p, h, sizeof(int32_t) * n_sources, hipMemcpyHostToDevice,
+                        b->st));
+  k_map_ids<<<1, 64, 0, b->st>>>(n_sources, b->d_src_user.p, b->user_g->inv, b->d_src.p);
+  GR_HIP(hipGetLastError());
+  return GR_OK;
+}
+
+int batch_finish(gr_batch* b, int levels, int grid, gr_batch_result* result) {
+  GR_HIP(hipEventRecord(b->e1, b->st));
+  GR_HIP(hipEventSynchronize(b->e1));
+  float ms = 0;
+  GR_HIP(hipEventElapsedTime(&ms, b->e0, b->e1));
+  if (result) {
+    result->levels = levels;
+    result->ms = ms;
+    result->grid = grid;
+  }
+  return GR_OK;
+}
+
+int batch_bfs(gr_batch* b, const int32_t* sources, int n_sources, int32_t* depth_device,
+              int direction, double alpha, gr_batch_result* result) {
   const bool stream = (direction & GR_MSBFS_STREAM_PULL) != 0;
   const bool persistent = (direction & GR_MSBFS_PERSISTENT) != 0;
   const bool filter = (direction & GR_MSBFS_FILTER) != 0;
   direction &= ~(GR_MSBFS_STREAM_PULL | GR_MSBFS_PERSISTENT | GR_MSBFS_FILTER);
-  const bool multi = !stream && !persistent;
-  if (!g || !sources || !depth_device || n_sources < 1 || n_sources > 64 || direction < 0 ||
-      direction > 2)
-    return set_error(GR_ERR_INVALID, "gr_msbfs: bad arguments");
+  if (!depth_device || direction < 0 || direction > 2)
+    return set_error(GR_ERR_INVALID, "gr_batch_bfs: bad arguments");
   if (stream && !kStreamPullEnabled)
     return set_error(GR_ERR_INVALID,
-                     "gr_msbfs: the streaming pull kernel is disabled (its launch did not finish "
-                     "on the GPU; cause not established)");
-  const int nV = g->nV;
-  for (int s = 0; s < n_sources; ++s)
-    if (sources[s] < 0 || sources[s] >= nV)
-      return set_errorf(GR_ERR_INVALID, "gr_msbfs: source %d out of range", sources[s]);
-  // runs on the degree-ordered copy of the graph; depth rows are produced in
-  // its ids and moved to the caller's ids at the end
-  gr_graph* user_g = g;
-  GR_TRY(graph_ordered(user_g, &g));
-  DBuf<uint64_t> visit, frontier, next;
-  DBuf<uint32_t> fbits;
-  DBuf<int32_t> d_src, d_src_user, depth_layout;
-  DBuf<MsCounters> ctr;
-  LaneTables lt;
-  GR_TRY(visit.alloc(nV));
-  GR_TRY(frontier.alloc(nV));
-  GR_TRY(next.alloc(nV));
-  if (stream || multi) GR_TRY(fbits.alloc((size_t)nV / 32 + 2));
-  if (stream) GR_TRY(lt.build(g->h_off_in));
-  GR_TRY(d_src.alloc(n_sources));
-  GR_TRY(d_src_user.alloc(n_sources));
-  // multi-kernel: [V][64] depths (no initialisation: the visit bits say which
-  // entries are valid); the single-launch kernels: [ns][V] set to -1 first
-  GR_TRY(depth_layout.alloc(multi ? (size_t)nV * kWave : (size_t)nV * n_sources));
-  GR_TRY(ctr.alloc(1));
-  GR_HIP(hipMemcpy(d_src_user.p, sources, sizeof(int32_t) * n_sources, hipMemcpyHostToDevice));
-  k_map_ids<<<1, 64>>>(n_sources, d_src_user.p, user_g->inv, d_src.p);
-  GR_HIP(hipGetLastError());
-  // depth = -1 everywhere: the counterpart of the host-side vertex init before setGraph
-  if (!multi)
-    GR_HIP(hipMemset(depth_layout.p, 0xFF, sizeof(int32_t) * (size_t)nV * n_sources));
-  if (multi) {
-    DBuf<MsLevelCounters> lc;
-    GR_TRY(lc.alloc(1));
-    LaneTables in_chunks, out_chunks;  // heavy CSC rows / CSR rows
-    GR_TRY(in_chunks.build(g->h_off_in));
-    GR_TRY(out_chunks.build(g->h_off_out));
-    hipEvent_t e0, e1;
-    GR_HIP(hipEventCreate(&e0));
-    GR_HIP(hipEventCreate(&e1));
-    GR_HIP(hipDeviceSynchronize());
-    GR_HIP(hipEventRecord(e0, 0));
-    k_ms_init<<<1024, 256>>>(nV, visit.p, frontier.p, next.p, fbits.p);
-    k_ms_sources<<<1, 64>>>(n_sources, d_src.p, nV, visit.p, frontier.p, fbits.p,
-                            depth_layout.p, 1, g->off_out, ctr.p);
+                     "gr_batch_bfs: the streaming pull kernel is disabled (its launch did not "
+                     "finish on the GPU; cause not established)");
+  GR_TRY(batch_common(b));
+  GR_TRY(batch_sources(b, "gr_batch_bfs", sources, n_sources));
+  GR_TRY(batch_tables(b));
+  gr_graph* g = b->g;
+  const int nV = b->nV;
+  hipStream_t st = b->st;
+  uint64_t *visit = b->m0.p, *frontier = b->m1.p, *next = b->m2.p;
+  uint32_t* fbits = b->fbits.p;
+  int32_t* rows = b->rows.p;
+  if (!stream && !persistent) {
+    // one expand and one update launch group per level; the host reads the
+    // level's two counters through the pinned block
+    MsLevelCounters* hlc = static_cast<MsLevelCounters*>(b->h_pinned);
+    MsCounters* hc0 = static_cast<MsCounters*>(b->h_pinned);
+    GR_HIP(hipEventRecord(b->e0, st));
+    // [V][64] depths need no initialisation: the visit bits say which
+    // entries are valid
+    k_ms_init<<<1024, 256, 0, st>>>(nV, visit, frontier, next, fbits);
+    k_ms_sources<<<1, 64, 0, st>>>(n_sources, b->d_src.p, nV, visit, frontier, fbits, rows, 1,
+                                   g->off_out, b->ctr.p);
     GR_HIP(hipGetLastError());
-    MsCounters hc0;
-    GR_HIP(hipMemcpy(&hc0, ctr.p, sizeof(hc0), hipMemcpyDeviceToHost));
-    unsigned long long nv_count = hc0.new_vertices[0], fe_count = hc0.frontier_edges[0];
+    GR_HIP(hipMemcpyAsync(hc0, b->ctr.p, sizeof(MsCounters), hipMemcpyDeviceToHost, st));
+    GR_HIP(hipStreamSynchronize(st));
+    unsigned long long nv_count = hc0->new_vertices[0], fe_count = hc0->frontier_edges[0];
     const uint64_t all = n_sources >= 64 ? ~0ull : ((1ull << n_sources) - 1);
     const long long threshold = (long long)((double)g->nE / (alpha > 0 ? alpha : 1.0));
     const int grid = (int)std::min<int64_t>(((int64_t)nV + kBatchNT - 1) / kBatchNT, 32768);
+    const LaneTables& inc = b->in_chunks;
+    const LaneTables& outc = b->out_chunks;
     int levels = 0;
     for (int level = 0; level < nV + 1 && nv_count > 0; ++level) {
       const bool pull = direction == 2 || (direction == 0 && (long long)fe_count > threshold);
       if (pull) {
-        k_ms_pull<<<grid, kBatchNT>>>(nV, all, g->off_in, g->srcs, visit.p, frontier.p,
-                                      filter ? fbits.p : nullptr, next.p);
-        if (in_chunks.n_chunks)
-          k_ms_pull_chunks<<<grid_1d(in_chunks.n_chunks, kBatchNT / kWave), kBatchNT>>>(
-              in_chunks.chunks, in_chunks.n_chunks, all, g->srcs, visit.p, frontier.p, next.p);
+        k_ms_pull<<<grid, kBatchNT, 0, st>>>(nV, all, g->off_in, g->srcs, visit, frontier,
+                                             filter ? fbits : nullptr, next);
+        if (inc.n_chunks)
+          k_ms_pull_chunks<<<grid_1d(inc.n_chunks, kBatchNT / kWave), kBatchNT, 0, st>>>(
+              inc.chunks, inc.n_chunks, all, g->srcs, visit, frontier, next);
       } else {
-        k_ms_push<<<grid, kBatchNT>>>(nV, g->off_out, g->dsts, visit.p, frontier.p, fbits.p,
-                                      next.p);
-        if (out_chunks.n_chunks)
-          k_ms_push_chunks<<<grid_1d(out_chunks.n_chunks, kBatchNT / kWave), kBatchNT>>>(
-              out_chunks.chunks, out_chunks.n_chunks, g->dsts, visit.p, frontier.p, fbits.p,
-              next.p);
+        k_ms_push<<<grid, kBatchNT, 0, st>>>(nV, g->off_out, g->dsts, visit, frontier, fbits,
+                                             next);
+        if (outc.n_chunks)
+          k_ms_push_chunks<<<grid_1d(outc.n_chunks, kBatchNT / kWave), kBatchNT, 0, st>>>(
+              outc.chunks, outc.n_chunks, g->dsts, visit, frontier, fbits, next);
       }
-      GR_HIP(hipMemsetAsync(lc.p, 0, sizeof(MsLevelCounters), 0));
-      k_ms_update<<<grid, kBatchNT>>>(nV, all, level, g->off_out, visit.p, frontier.p, next.p,
-                                      fbits.p, depth_layout.p, lc.p);
+      GR_HIP(hipMemsetAsync(b->lc.p, 0, sizeof(MsLevelCounters), st));
+      k_ms_update<<<grid, kBatchNT, 0, st>>>(nV, all, level, g->off_out, visit, frontier, next,
+                                             fbits, rows, b->lc.p);
       GR_HIP(hipGetLastError());
-      MsLevelCounters h;
-      GR_HIP(hipMemcpy(&h, lc.p, sizeof(h), hipMemcpyDeviceToHost));
-      nv_count = h.new_vertices;
-      fe_count = h.frontier_edges;
+      GR_HIP(hipMemcpyAsync(hlc, b->lc.p, sizeof(MsLevelCounters), hipMemcpyDeviceToHost, st));
+      GR_HIP(hipStreamSynchronize(st));
+      nv_count = hlc->new_vertices;
+      fe_count = hlc->frontier_edges;
       levels = level + 1;
     }
-    k_ms_depth_out<<<(unsigned)std::min<int64_t>(((int64_t)nV + kWave - 1) / kWave, 65536), 256>>>(
-        nV, n_sources, depth_layout.p, visit.p, user_g->inv, depth_device);
+    k_ms_depth_out<<<(unsigned)std::min<int64_t>(((int64_t)nV + kWave - 1) / kWave, 65536), 256,
+                     0, st>>>(nV, n_sources, rows, visit, b->user_g->inv, depth_device);
     GR_HIP(hipGetLastError());
-    GR_HIP(hipEventRecord(e1, 0));
-    GR_HIP(hipEventSynchronize(e1));
-    float ms = 0;
-    GR_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (result) {
-      result->levels = levels;
-      result->ms = ms;
-      result->grid = grid;
-    }
-    return GR_OK;
+    return batch_finish(b, levels, grid, result);
   }
   const void* kernel = stream ? (const void*)k_msbfs_stream : (const void*)k_msbfs;
   int dev = 0, cus = 0, per_cu = 0;
   GR_HIP(hipGetDevice(&dev));
   GR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   GR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBatchNT, 0));
-  if (per_cu < 1) return set_error(GR_ERR_HIP, "gr_msbfs: kernel cannot be resident");
+  if (per_cu < 1) return set_error(GR_ERR_HIP, "gr_batch_bfs: kernel cannot be resident");
   // Every workgroup must be resident at once (grid.sync waits for all of
   // them).  The compiler's resource report for gfx950 (k_msbfs: 66 SGPRs,
   // 36 VGPRs, no LDS; k_msbfs_stream: 92 SGPRs, 48 VGPRs, no LDS) gives
@@ -972,186 +1048,225 @@ extern "C" int gr_msbfs(gr_graph* g, const int32_t* sources, int n_sources, int3
   // so the launch stays resident even when other work holds part of every
   // CU; the level loops are grid-strided and do not depend on the grid size.
   const int grid = cus * std::max(1, std::min(per_cu, kCoopBlocksPerCu));
-  hipEvent_t e0, e1;
-  GR_HIP(hipEventCreate(&e0));
-  GR_HIP(hipEventCreate(&e1));
-  GR_HIP(hipDeviceSynchronize());
-  GR_HIP(hipEventRecord(e0, 0));
-  k_ms_init<<<1024, 256>>>(nV, visit.p, frontier.p, next.p, fbits.p);
-  k_ms_sources<<<1, 64>>>(n_sources, d_src.p, nV, visit.p, frontier.p, fbits.p,
-                          depth_layout.p, 0, g->off_out, ctr.p);
+  GR_HIP(hipEventRecord(b->e0, st));
+  // the single-launch kernels write [ns][V] rows: depth = -1 everywhere
+  // first, the counterpart of the host-side vertex init before setGraph
+  GR_HIP(hipMemsetAsync(rows, 0xFF, sizeof(int32_t) * (size_t)nV * n_sources, st));
+  k_ms_init<<<1024, 256, 0, st>>>(nV, visit, frontier, next, fbits);
+  k_ms_sources<<<1, 64, 0, st>>>(n_sources, b->d_src.p, nV, visit, frontier, fbits, rows, 0,
+                                 g->off_out, b->ctr.p);
   GR_HIP(hipGetLastError());
   int nVa = nV, ns = n_sources, mode = direction, max_levels = nV + 1;
   long long nE = g->nE;
   double al = alpha;
   const int32_t *off_out = g->off_out, *dsts = g->dsts, *off_in = g->off_in, *srcs = g->srcs;
-  uint64_t *pv = visit.p, *pf = frontier.p, *pn = next.p;
-  uint32_t* pfb = fbits.p;
-  int32_t* pd = depth_layout.p;
-  const LaneChunk* pch = lt.chunks;
-  int nch = lt.n_chunks;
-  MsCounters* pc = ctr.p;
+  uint64_t *pv = visit, *pf = frontier, *pn = next;
+  uint32_t* pfb = fbits;
+  int32_t* pd = rows;
+  const LaneChunk* pch = b->in_chunks.chunks;
+  int nch = b->in_chunks.n_chunks;
+  MsCounters* pc = b->ctr.p;
   if (stream) {
     void* args[] = {&nVa, &nE, &ns, &mode, &al, &off_out, &dsts, &off_in, &srcs, &pv,
                     &pf,  &pn, &pfb, &pd,  &pch, &nch,   &pc,   &max_levels};
-    GR_HIP(hipLaunchCooperativeKernel(kernel, dim3(grid), dim3(kBatchNT), args, 0, 0));
+    GR_HIP(hipLaunchCooperativeKernel(kernel, dim3(grid), dim3(kBatchNT), args, 0, st));
   } else {
     void* args[] = {&nVa, &nE, &ns, &mode, &al, &off_out, &dsts, &off_in, &srcs,
                     &pv,  &pf, &pn, &pd,   &pc, &max_levels};
-    GR_HIP(hipLaunchCooperativeKernel(kernel, dim3(grid), dim3(kBatchNT), args, 0, 0));
+    GR_HIP(hipLaunchCooperativeKernel(kernel, dim3(grid), dim3(kBatchNT), args, 0, st));
   }
-  k_rows_to_user<<<4096, 256>>>(nV, n_sources, depth_layout.p, user_g->inv, depth_device);
+  k_rows_to_user<<<4096, 256, 0, st>>>(nV, n_sources, rows, b->user_g->inv, depth_device);
   GR_HIP(hipGetLastError());
-  GR_HIP(hipEventRecord(e1, 0));
-  GR_HIP(hipEventSynchronize(e1));
-  float ms = 0;
-  GR_HIP(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  MsCounters hc;
-  GR_HIP(hipMemcpy(&hc, ctr.p, sizeof(hc), hipMemcpyDeviceToHost));
-  if (result) {
-    result->levels = hc.levels;
-    result->ms = ms;
-    result->grid = grid;
-  }
+  MsCounters* hc = static_cast<MsCounters*>(b->h_pinned);
+  GR_HIP(hipMemcpyAsync(hc, b->ctr.p, sizeof(MsCounters), hipMemcpyDeviceToHost, st));
+  GR_TRY(batch_finish(b, 0, grid, result));
+  if (result) result->levels = hc->levels;
   return GR_OK;
 }
 
-extern "C" int gr_mssp(gr_graph* g, const int32_t* weights_device, const int32_t* sources,
-                       int n_sources, int32_t* dist_device, gr_batch_result* result) {
-  if (!g || !weights_device || !sources || !dist_device || n_sources < 1 || n_sources > 64)
-    return set_error(GR_ERR_INVALID, "gr_mssp: bad arguments");
-  const int nV = g->nV;
-  for (int s = 0; s < n_sources; ++s)
-    if (sources[s] < 0 || sources[s] >= nV)
-      return set_errorf(GR_ERR_INVALID, "gr_mssp: source %d out of range", sources[s]);
-  const int32_t inf = 2147483647 - 100000;  // SSSP::gatherZero, sssp.cu:32-33
-  // runs on the degree-ordered copy of the graph (hot vertices share lines)
-  gr_graph* user_g = g;
-  GR_TRY(graph_ordered(user_g, &g));
-  DBuf<int32_t> dist, w_csc, d_src, d_src_user;
-  DBuf<uint64_t> chg_a, chg_b;
-  DBuf<unsigned long long> cnt, any;
-  DBuf<uint8_t> dirty;
-  DBuf<int32_t> list;
-  GR_TRY(dist.alloc((size_t)nV * kWave));
-  GR_TRY(w_csc.alloc((size_t)g->nE));
-  GR_TRY(d_src.alloc(n_sources));
-  GR_TRY(d_src_user.alloc(n_sources));
-  GR_TRY(chg_a.alloc(nV));
-  GR_TRY(chg_b.alloc(nV));
-  // per round r of a replay: cnt[r] changed vertices, cnt[kCnt + r] their
-  // out-edges, cnt[2 kCnt + r] filtered-list length; cnt[3 kCnt + 0/1]: the
-  // previous replay's last round (changed vertices, out-edges)
-  constexpr int kCnt = 16;
-  GR_TRY(cnt.alloc(4 * kCnt));
-  GR_TRY(any.alloc(1));
-  GR_TRY(dirty.alloc(nV));
-  GR_TRY(list.alloc(nV));
-  GR_HIP(hipMemset(dirty.p, 0, (size_t)nV));
-  GR_HIP(hipMemcpy(d_src_user.p, sources, sizeof(int32_t) * n_sources, hipMemcpyHostToDevice));
-  k_map_ids<<<1, 64>>>(n_sources, d_src_user.p, user_g->inv, d_src.p);
-  if (g->nE)
-    k_permute_i32<<<grid_1d(g->nE, 256), 256>>>(weights_device, g->csc_eid, g->nE, w_csc.p);
-  LaneTables lt, lt_out;
-  GR_TRY(lt.build(g->h_off_in));
-  GR_TRY(lt_out.build(g->h_off_out));
-  const int32_t seg_split = graph_lane_split(g);
-  hipStream_t st;
-  GR_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-  GR_HIP(hipDeviceSynchronize());
-  hipEvent_t e0, e1;
-  GR_HIP(hipEventCreate(&e0));
-  GR_HIP(hipEventCreate(&e1));
-  GR_HIP(hipEventRecord(e0, st));
-  k_mssp_init<<<4096, 256, 0, st>>>(nV, n_sources, d_src.p, dist.p, chg_a.p, inf);
-  k_mssp_sources<<<1, 64, 0, st>>>(n_sources, d_src.p, dist.p, chg_a.p);
-  GR_HIP(hipGetLastError());
-  // One graph holds kRounds rounds (ping-pong of the changed masks); replays
-  // continue until a round changes nothing.
-  constexpr int kRounds = 8;
-  static_assert(kRounds <= kCnt, "one counter slot per round");
+// The SSSP state and the recorded rounds, on the first SSSP run of a context.
+int batch_sssp_setup(gr_batch* b) {
+  if (b->exec) return GR_OK;
+  gr_graph* g = b->g;
+  const int nV = b->nV;
+  hipStream_t st = b->st;
+  // per round r of a replay: cnt[r] changed vertices, cnt[kSsspCnt + r] their
+  // out-edges, cnt[2 kSsspCnt + r] filtered-list length; cnt[3 kSsspCnt +
+  // 0/1]: the previous replay's last round (changed vertices, out-edges)
+  GR_TRY(b->cnt.alloc(4 * kSsspCnt));
+  GR_TRY(b->any.alloc(1));
+  GR_TRY(b->dirty.alloc(nV));
+  GR_TRY(b->list.alloc(nV));
+  b->seg_split = graph_lane_split(g);
+  const LaneTables& lt = b->in_chunks;
+  const LaneTables& lt_out = b->out_chunks;
+  int32_t* dist = b->rows.p;
   const unsigned rgrid = grid_1d((int64_t)nV, kLaneNT / kWave);
   const unsigned hgrid = grid_1d((int64_t)lt.n_chunks, kLaneNT / kWave);
   const unsigned vgrid = std::min<unsigned>(grid_1d((int64_t)nV, kLaneNT), 16384);
   // a filtered round pays one flag write per out-edge of the changed vertices;
   // it replaces the scan of every in-edge when those are a small share
   const long long filter_max = g->nE / 8;
-  hipGraph_t graph;
-  hipGraphExec_t exec;
+  unsigned long long* cnt = b->cnt.p;
   GR_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  for (int r = 0; r < kRounds; ++r) {
-    uint64_t* cin = (r & 1) ? chg_b.p : chg_a.p;
-    uint64_t* cout = (r & 1) ? chg_a.p : chg_b.p;
+  for (int r = 0; r < kSsspRounds; ++r) {
+    uint64_t* cin = (r & 1) ? b->m1.p : b->m0.p;
+    uint64_t* cout = (r & 1) ? b->m0.p : b->m1.p;
     // a round after a round without change returns at once (exact: nothing
     // changed feeds it), so the rest of a replay after convergence is free
     LaneRound rr;
-    rr.prev_count = r ? cnt.p + (r - 1) : cnt.p + 3 * kCnt;
-    rr.prev_edges = r ? cnt.p + kCnt + (r - 1) : cnt.p + 3 * kCnt + 1;
-    rr.count = cnt.p + r;
-    rr.edges = cnt.p + kCnt + r;
-    rr.dirty = dirty.p;
-    rr.list = list.p;
-    rr.list_len = reinterpret_cast<unsigned int*>(cnt.p + 2 * kCnt + r);
+    rr.prev_count = r ? cnt + (r - 1) : cnt + 3 * kSsspCnt;
+    rr.prev_edges = r ? cnt + kSsspCnt + (r - 1) : cnt + 3 * kSsspCnt + 1;
+    rr.count = cnt + r;
+    rr.edges = cnt + kSsspCnt + r;
+    rr.dirty = b->dirty.p;
+    rr.list = b->list.p;
+    rr.list_len = reinterpret_cast<unsigned int*>(cnt + 2 * kSsspCnt + r);
     rr.filter_max = filter_max;
     rr.host_filter = -1;
     k_lane_sssp_mark<<<vgrid, kLaneNT, 0, st>>>(nV, g->off_out, g->dsts, lt_out.chunks,
                                                 lt_out.n_chunks, cin, rr);
     k_lane_sssp_filter<<<vgrid, kLaneNT, 0, st>>>(nV, g->off_in, cout, rr);
     k_lane_sssp_light<<<rgrid < 65536 ? rgrid : 65536, kLaneNT, 0, st>>>(
-        nV, g->off_in, g->off_out, g->srcs, w_csc.p, dist.p, cin, cout, any.p, rr, seg_split);
+        nV, g->off_in, g->off_out, g->srcs, b->w_csc.p, dist, cin, cout, b->any.p, rr,
+        b->seg_split);
     if (lt.n_chunks) {
       k_lane_sssp_heavy<<<hgrid, kLaneNT, 0, st>>>(lt.chunks, lt.n_chunks, g->off_out, g->srcs,
-                                                   w_csc.p, dist.p, cin, cout, any.p, rr);
-      k_lane_sssp_clear_heavy<<<grid_1d(lt.n_heavy, 256), 256, 0, st>>>(lt.heavy, lt.n_heavy, rr);
+                                                   b->w_csc.p, dist, cin, cout, b->any.p, rr);
+      k_lane_sssp_clear_heavy<<<grid_1d(lt.n_heavy, 256), 256, 0, st>>>(lt.heavy, lt.n_heavy,
+                                                                        rr);
     }
   }
-  GR_HIP(hipStreamEndCapture(st, &graph));
-  GR_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  GR_HIP(hipStreamEndCapture(st, &b->graph));
+  GR_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  return GR_OK;
+}
+
+int batch_sssp(gr_batch* b, const int32_t* sources, int n_sources, int32_t* dist_device,
+               gr_batch_result* result) {
+  if (!dist_device) return set_error(GR_ERR_INVALID, "gr_batch_sssp: bad arguments");
+  if (!b->has_weights)
+    return set_error(GR_ERR_INVALID, "gr_batch_sssp: the context was created without weights");
+  const int32_t inf = 2147483647 - 100000;  // SSSP::gatherZero, sssp.cu:32-33
+  GR_TRY(batch_common(b));
+  GR_TRY(batch_sources(b, "gr_batch_sssp", sources, n_sources));
+  GR_TRY(batch_tables(b));
+  GR_TRY(batch_sssp_setup(b));
+  const int nV = b->nV;
+  hipStream_t st = b->st;
+  int32_t* dist = b->rows.p;
+  GR_HIP(hipEventRecord(b->e0, st));
+  // the flags are all zero between rounds of a run; cleared here so that a
+  // run never depends on how the previous one ended
+  GR_HIP(hipMemsetAsync(b->dirty.p, 0, (size_t)nV, st));
+  GR_HIP(hipMemsetAsync(b->any.p, 0, sizeof(unsigned long long), st));
+  k_mssp_init<<<4096, 256, 0, st>>>(nV, n_sources, b->d_src.p, dist, b->m0.p, inf);
+  k_mssp_sources<<<1, 64, 0, st>>>(n_sources, b->d_src.p, dist, b->m0.p);
+  GR_HIP(hipGetLastError());
+  unsigned long long* hall = static_cast<unsigned long long*>(b->h_pinned);
   int rounds = 0;
-  unsigned long long hc[kRounds];
   // the first replay's first round has no previous round: not idle, unfiltered
   unsigned long long carry[2] = {1ull, ~0ull};
   for (;;) {
-    // counters are cleared by kernel-free copies (host arrays), and the last
-    // round's counts carried into the next replay's first round
-    unsigned long long init[4 * kCnt] = {0};
-    init[3 * kCnt] = carry[0];
-    init[3 * kCnt + 1] = carry[1];
-    GR_HIP(hipMemcpyAsync(cnt.p, init, sizeof(init), hipMemcpyHostToDevice, st));
-    GR_HIP(hipGraphLaunch(exec, st));
-    unsigned long long hall[4 * kCnt];
-    GR_HIP(hipMemcpyAsync(hall, cnt.p, sizeof(hall), hipMemcpyDeviceToHost, st));
+    // counters are cleared by a kernel-free copy from the pinned block, and
+    // the last round's counts carried into the next replay's first round
+    GR_HIP(hipStreamSynchronize(st));  // the pinned block is free
+    memset(hall, 0, kPinnedBytes);
+    hall[3 * kSsspCnt] = carry[0];
+    hall[3 * kSsspCnt + 1] = carry[1];
+    GR_HIP(hipMemcpyAsync(b->cnt.p, hall, kPinnedBytes, hipMemcpyHostToDevice, st));
+    GR_HIP(hipGraphLaunch(b->exec, st));
+    GR_HIP(hipMemcpyAsync(hall, b->cnt.p, kPinnedBytes, hipMemcpyDeviceToHost, st));
     GR_HIP(hipStreamSynchronize(st));
-    for (int r = 0; r < kRounds; ++r) hc[r] = hall[r];
-    carry[0] = hall[kRounds - 1];
-    carry[1] = hall[kCnt + kRounds - 1];
+    carry[0] = hall[kSsspRounds - 1];
+    carry[1] = hall[kSsspCnt + kSsspRounds - 1];
     int last = -1;
-    for (int r = 0; r < kRounds; ++r)
-      if (hc[r]) last = r;
-    if (last < kRounds - 1) {
+    for (int r = 0; r < kSsspRounds; ++r)
+      if (hall[r]) last = r;
+    if (last < kSsspRounds - 1) {
       rounds += last + 1;
       break;
     }
-    rounds += kRounds;
+    rounds += kSsspRounds;
   }
   k_mssp_transpose<<<(unsigned)std::min<int64_t>(((int64_t)nV + kWave - 1) / kWave, 65536), 256, 0,
-                     st>>>(nV, n_sources, dist.p, user_g->inv, dist_device);
+                     st>>>(nV, n_sources, dist, b->user_g->inv, dist_device);
   GR_HIP(hipGetLastError());
-  GR_HIP(hipEventRecord(e1, st));
-  GR_HIP(hipEventSynchronize(e1));
-  float ms = 0;
-  GR_HIP(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipGraphExecDestroy(exec);
-  (void)hipGraphDestroy(graph);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipStreamDestroy(st);
-  if (result) {
-    result->levels = rounds;
-    result->ms = ms;
-    result->grid = (int)rgrid;
+  return batch_finish(b, rounds, (int)grid_1d((int64_t)nV, kLaneNT / kWave), result);
+}
+
+}  // namespace
+}  // namespace gr
+
+extern "C" int gr_batch_create(gr_graph* g, const int32_t* weights_device, gr_batch** out) {
+  if (!g || !out) return set_error(GR_ERR_INVALID, "gr_batch_create: bad arguments");
+  *out = nullptr;
+  gr_batch* b = new (std::nothrow) gr_batch;
+  if (!b) return set_error(GR_ERR_NOMEM, "gr_batch_create: out of host memory");
+  b->user_g = g;
+  b->nV = g->nV;
+  int rc = graph_ordered(g, &b->g);
+  if (rc == GR_OK && hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking) != hipSuccess)
+    rc = set_error(GR_ERR_HIP, "gr_batch_create: hipStreamCreateWithFlags failed");
+  if (rc == GR_OK &&
+      (hipEventCreate(&b->e0) != hipSuccess || hipEventCreate(&b->e1) != hipSuccess))
+    rc = set_error(GR_ERR_HIP, "gr_batch_create: hipEventCreate failed");
+  if (rc == GR_OK && weights_device) {
+    // the weights in the CSC order of the degree-ordered copy
+    b->has_weights = true;
+    rc = b->w_csc.alloc((size_t)b->g->nE);
+    if (rc == GR_OK && b->g->nE) {
+      k_permute_i32<<<grid_1d(b->g->nE, 256), 256, 0, b->st>>>(weights_device, b->g->csc_eid,
+                                                              b->g->nE, b->w_csc.p);
+      if (hipGetLastError() != hipSuccess || hipStreamSynchronize(b->st) != hipSuccess)
+        rc = set_error(GR_ERR_HIP, "gr_batch_create: weight permutation failed");
+    }
   }
+  if (rc != GR_OK) {
+    delete b;
+    return rc;
+  }
+  *out = b;
   return GR_OK;
+}
+
+extern "C" int gr_batch_bfs(gr_batch* b, const int32_t* sources, int n_sources,
+                            int32_t* depth_device, int direction, double alpha,
+                            gr_batch_result* result) {
+  if (!b) return set_error(GR_ERR_INVALID, "gr_batch_bfs: bad arguments");
+  return batch_bfs(b, sources, n_sources, depth_device, direction, alpha, result);
+}
+
+extern "C" int gr_batch_sssp(gr_batch* b, const int32_t* sources, int n_sources,
+                             int32_t* dist_device, gr_batch_result* result) {
+  if (!b) return set_error(GR_ERR_INVALID, "gr_batch_sssp: bad arguments");
+  return batch_sssp(b, sources, n_sources, dist_device, result);
+}
+
+extern "C" int gr_batch_destroy(gr_batch* b) {
+  delete b;
+  return GR_OK;
+}
+
+// One-shot forms: a context for one run.
+extern "C" int gr_msbfs(gr_graph* g, const int32_t* sources, int n_sources, int32_t* depth_device,
+                        int direction, double alpha, gr_batch_result* result) {
+  if (!g || !sources || !depth_device || n_sources < 1 || n_sources > 64)
+    return set_error(GR_ERR_INVALID, "gr_msbfs: bad arguments");
+  gr_batch* b = nullptr;
+  GR_TRY(gr_batch_create(g, nullptr, &b));
+  const int rc = batch_bfs(b, sources, n_sources, depth_device, direction, alpha, result);
+  delete b;
+  return rc;
+}
+
+extern "C" int gr_mssp(gr_graph* g, const int32_t* weights_device, const int32_t* sources,
+                       int n_sources, int32_t* dist_device, gr_batch_result* result) {
+  if (!g || !weights_device || !sources || !dist_device || n_sources < 1 || n_sources > 64)
+    return set_error(GR_ERR_INVALID, "gr_mssp: bad arguments");
+  gr_batch* b = nullptr;
+  GR_TRY(gr_batch_create(g, weights_device, &b));
+  const int rc = batch_sssp(b, sources, n_sources, dist_device, result);
+  delete b;
+  return rc;
 }

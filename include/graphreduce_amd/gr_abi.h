@@ -428,6 +428,26 @@ int gr_msbfs(gr_graph* g, const int32_t* sources, int n_sources, int32_t* depth_
 int gr_mssp(gr_graph* g, const int32_t* weights_device, const int32_t* sources, int n_sources,
             int32_t* dist_device, gr_batch_result* result);
 
+/* A batch context: the state of batched runs on one graph (the 64-bit masks,
+ * the [V][64] rows, the heavy-row chunk tables, the weights in CSC order and
+ * the recorded SSSP rounds), allocated by the first run that needs it and
+ * reused by every later run on the context, as an engine binds its graph
+ * once.  gr_msbfs / gr_mssp are a context for one run.  weights_device
+ * (int32 per edge, edge-list order; read once, at creation) may be NULL for
+ * BFS-only use.  The graph must outlive the context; runs on one context are
+ * serialised (one caller at a time).  Device memory held at V vertices and
+ * E edges: about 282 B per vertex (+ 4 B per edge and 5 B per vertex with
+ * weights once SSSP has run). */
+typedef struct gr_batch gr_batch;
+int gr_batch_create(gr_graph* g, const int32_t* weights_device, gr_batch** out);
+/* gr_msbfs on the context (same arguments and results). */
+int gr_batch_bfs(gr_batch* b, const int32_t* sources, int n_sources, int32_t* depth_device,
+                 int direction, double alpha, gr_batch_result* result);
+/* gr_mssp on the context's weights; GR_ERR_INVALID if it has none. */
+int gr_batch_sssp(gr_batch* b, const int32_t* sources, int n_sources, int32_t* dist_device,
+                  gr_batch_result* result);
+int gr_batch_destroy(gr_batch* b);
+
 /* ------------------------------------------------------------------ serving */
 /* Continuous-batching query server (no reference counterpart).  Each query
  * kind has a pool of up to 64 resident lanes; a host thread admits queued

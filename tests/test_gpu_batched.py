@@ -128,3 +128,43 @@ def test_msbfs_variants_agree(gpu, direction):
         assert np.array_equal(outs[0][i], ref)
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
     g.close()
+
+
+def test_batch_context_reuse(gpu):
+    """One context, several runs of both kinds and sizes in a row: every run
+    equals the oracle, so no run depends on state another one left."""
+    gr = gpu
+    g = gr.Graph.rmat(13, 16, seed=4)
+    n = g.n_vertices
+    s, d = g.edges()
+    w = gr.hash_weights(len(s), seed=9)
+    w_dev = gr.upload_i32(w)
+    hub, _ = g.max_out_degree_vertex()
+    b = gr.Batch(g, w_dev)
+    big = _sources(n, 64, 21, extra=(hub,))
+    small = _sources(n, 5, 22)
+    for srcs, kw in ((big, {}), (small, {}), (big, {"direction": 2}), (small, {"persistent": True})):
+        depth, info = b.bfs(srcs, **kw)
+        assert info["levels"] >= 1
+        for i, src in enumerate(srcs):
+            ref, _, _ = oracle.bfs(n, s, d, src)
+            assert np.array_equal(depth[i], ref), f"BFS {kw} source {src} (row {i})"
+    for srcs in (big, small, big[:17]):
+        dist, _ = b.sssp(srcs)
+        for i, src in enumerate(srcs):
+            ref, _, _ = oracle.sssp(n, s, d, w, src)
+            assert np.array_equal(dist[i], ref), f"SSSP source {src} (row {i})"
+    # the one-shot forms give the same rows
+    depth1, _ = gr.msbfs(g, small)
+    depth2, _ = b.bfs(small)
+    assert np.array_equal(depth1, depth2)
+    b.close()
+    # a context without weights runs BFS and refuses SSSP
+    b2 = gr.Batch(g)
+    depth3, _ = b2.bfs(small)
+    assert np.array_equal(depth1, depth3)
+    with pytest.raises(gr.GRError, match="without weights"):
+        b2.sssp(small)
+    b2.close()
+    w_dev.free()
+    g.close()

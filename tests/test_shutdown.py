@@ -39,7 +39,8 @@ g = make(gr.Graph, "_h"); del g; gc.collect()
 assert calls == ["gr_graph_destroy"], calls
 calls.clear()
 
-keep = [make(gr.Graph, "_h"), make(gr.GASEngine, "_h"), make(gr.DeviceArray, "ptr")]
+keep = [make(gr.Graph, "_h"), make(gr.GASEngine, "_h"), make(gr.DeviceArray, "ptr"),
+        make(gr.Batch, "_h")]
 srv = make(gr.Server, "_h")
 gr._LIVE_SERVERS.add(srv)
 gr._at_exit()
