@@ -1,23 +1,26 @@
 // Batched multi-source traversals: up to 64 BFS or SSSP sources share one
 // pass over the graph, one bit (BFS) or one lane (SSSP) per source.
 //
-// Multi-source BFS (k_msbfs): ONE persistent cooperative launch runs every
-// level.  Per vertex three 64-bit masks live in HBM: visit (sources that
-// reached it), frontier (sources for which it is on the current level) and
-// next.  Each level is an expand phase and an update phase separated by grid
-// barriers:
+// Multi-source BFS: per vertex three 64-bit masks live in HBM: visit
+// (sources that reached it), frontier (sources for which it is on the
+// current level) and next.  Each level is an expand phase and an update
+// phase:
 //   expand, push  every frontier vertex ORs its frontier mask into next[u] of
-//                 its CSR out-neighbours (skipping those that every one of its
-//                 sources already visited);
+//                 its CSR out-neighbours (skipping bits next[u] or visit[u]
+//                 already hold);
 //   expand, pull  every vertex with unvisited sources ORs the frontier masks of
 //                 its CSC in-neighbours, stopping once all of its unvisited
 //                 sources are covered (thread per vertex; a wave per vertex
 //                 above kMsHeavy in-edges);
 //   update        new = next & ~visit; visit |= new; frontier = new; the
 //                 level is written for every new (source, vertex) pair.
-// The direction is chosen per level from the frontier's out-edge count.
-// Depths equal those of independent plain runs (the BFS level of a vertex
-// does not depend on the schedule), which the tests check.
+// Rows above kLaneHeavy edges are cut into chunks spread over all waves.
+// The default runs one expand and one update launch per level; k_msbfs runs
+// every level in ONE persistent cooperative launch with grid barriers
+// between the phases (the same level bodies).  The direction is chosen per
+// level from the frontier's out-edge count.  Depths equal those of
+// independent plain runs (the BFS level of a vertex does not depend on the
+// schedule), which the tests check.
 //
 // Multi-source SSSP (k_lane_sssp_light/heavy in gr_lanes.h): Bellman-Ford
 // rounds over 64 lanes, one wave per destination vertex, lane s relaxing
@@ -110,166 +113,33 @@ __global__ void k_ms_sources(int ns, const int32_t* __restrict__ sources, int nV
   ctr->error = 0;
 }
 
-__global__ __launch_bounds__(kBatchNT) void k_msbfs(
-    int nV, long long nE, int ns, int mode, double alpha, const int32_t* __restrict__ off_out,
-    const int32_t* __restrict__ dsts, const int32_t* __restrict__ off_in,
-    const int32_t* __restrict__ srcs, uint64_t* __restrict__ visit,
-    uint64_t* __restrict__ frontier, uint64_t* __restrict__ next, int32_t* __restrict__ depth,
-    MsCounters* __restrict__ ctr, int max_levels) {
-  cg::grid_group grid = cg::this_grid();
-  const uint64_t all = ns >= 64 ? ~0ull : ((1ull << ns) - 1);
-  const int lane = lane_id();
-  const int wpb = blockDim.x / kWave;
-  const int wave_id = blockIdx.x * wpb + threadIdx.x / kWave;
-  const int n_waves = gridDim.x * wpb;
-  const long long threshold = (long long)((double)nE / (alpha > 0 ? alpha : 1.0));
-  for (int level = 0; level < max_levels; ++level) {
-    const int slot = level % 3;
-    // Coherent (atomic) loads after the grid barrier: every workgroup must see
-    // the same counts so that all of them leave the loop at the same level
-    // and pick the same direction.
-    const unsigned long long nv_count =
-        __hip_atomic_load(&ctr->new_vertices[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned long long fe_count =
-        __hip_atomic_load(&ctr->frontier_edges[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (nv_count == 0) break;
-    const bool pull = mode == 2 || (mode == 0 && (long long)fe_count > threshold);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      const int nx = (level + 1) % 3;
-      __hip_atomic_store(&ctr->new_vertices[nx], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&ctr->frontier_edges[nx], 0ull, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    }
-    // ---------------------------------------------------------- expand
-    for (int base = wave_id * kWave; base < nV; base += n_waves * kWave) {
-      const int v = base + lane;
-      const bool valid = v < nV;
-      if (!pull) {
-        const uint64_t f = valid ? frontier[v] : 0ull;
-        int b = 0, e = 0;
-        if (f) {
-          b = off_out[v];
-          e = off_out[v + 1];
-        }
-        const bool light = (e - b) <= kMsHeavy;
-        if (f && light) {
-          for (int p = b; p < e; ++p) {
-            const int u = dsts[p];
-            if (f & ~visit[u]) atomicOr((unsigned long long*)&next[u], (unsigned long long)f);
-          }
-        }
-        uint64_t heavy = __ballot(f && !light);
-        while (heavy) {
-          const int hl = __builtin_ctzll(heavy);
-          heavy &= heavy - 1;
-          const int hb = __shfl(b, hl, kWave), he = __shfl(e, hl, kWave);
-          const uint32_t flo = __shfl((uint32_t)f, hl, kWave);
-          const uint32_t fhi = __shfl((uint32_t)(f >> 32), hl, kWave);
-          const uint64_t hf = ((uint64_t)fhi << 32) | flo;
-          for (int p = hb + lane; p < he; p += kWave) {
-            const int u = dsts[p];
-            if (hf & ~visit[u]) atomicOr((unsigned long long*)&next[u], (unsigned long long)hf);
-          }
-        }
-      } else {
-        const uint64_t need = valid ? (all & ~visit[v]) : 0ull;
-        int b = 0, e = 0;
-        if (need) {
-          b = off_in[v];
-          e = off_in[v + 1];
-        }
-        const bool light = (e - b) <= kMsHeavy;
-        uint64_t acc = 0;
-        if (need && light) {
-          for (int p = b; p < e; ++p) {
-            acc |= frontier[srcs[p]];
-            if ((acc & need) == need) break;
-          }
-          next[v] = acc & need;
-        }
-        uint64_t heavy = __ballot(need && !light);
-        while (heavy) {
-          const int hl = __builtin_ctzll(heavy);
-          heavy &= heavy - 1;
-          const int hb = __shfl(b, hl, kWave), he = __shfl(e, hl, kWave);
-          const uint32_t nlo = __shfl((uint32_t)need, hl, kWave);
-          const uint32_t nhi = __shfl((uint32_t)(need >> 32), hl, kWave);
-          const uint64_t hn = ((uint64_t)nhi << 32) | nlo;
-          uint64_t hacc = 0;
-          for (int pb = hb; pb < he; pb += kWave) {  // wave-uniform trip count
-            const int p = pb + lane;
-            const uint64_t m = p < he ? frontier[srcs[p]] : 0ull;
-            hacc |= wave_or64(m);
-            if ((hacc & hn) == hn) break;
-          }
-          if (lane == hl) next[v] = hacc & hn;
-        }
-      }
-    }
-    grid_barrier(grid);
-    // ---------------------------------------------------------- update
-    const int slot_next = (level + 1) % 3;
-    for (int base = wave_id * kWave; base < nV; base += n_waves * kWave) {
-      const int v = base + lane;
-      uint64_t nw = 0;
-      int deg = 0;
-      if (v < nV) {
-        const uint64_t vis = visit[v];
-        nw = next[v] & ~vis & all;
-        frontier[v] = nw;
-        if (nw) {
-          next[v] = 0;
-          visit[v] = vis | nw;
-          deg = off_out[v + 1] - off_out[v];
-          uint64_t bits = nw;
-          while (bits) {
-            const int s = __builtin_ctzll(bits);
-            bits &= bits - 1;
-            depth[(size_t)s * nV + v] = level + 1;
-          }
-        } else if (!pull) {
-          next[v] = 0;
-        }
-      }
-      const uint64_t nb = __ballot(nw != 0);
-      unsigned long long fe = (unsigned long long)deg;
-#pragma unroll
-      for (int off = kWave / 2; off > 0; off >>= 1) fe += __shfl_xor(fe, off, kWave);
-      if (lane == 0 && nb) {
-        atomicAdd(&ctr->new_vertices[slot_next], (unsigned long long)__popcll(nb));
-        atomicAdd(&ctr->frontier_edges[slot_next], fe);
-      }
-    }
-    grid_barrier(grid);
-    if (blockIdx.x == 0 && threadIdx.x == 0) ctr->levels = level + 1;
-  }
-}
-
-// ---------------------------------------------- multi-kernel MS-BFS
+// ------------------------------------------------------- MS-BFS level bodies
 //
-// The same level structure as k_msbfs as ordinary launches (no grid
-// barrier): one expand kernel (push or pull) and one update kernel per level;
-// the host reads the level's two counters after the update.  fbits is the
-// frontier bitmap (bit v = frontier[v] != 0), written by the update: push reads
-// the 8-byte frontier mask only of frontier vertices, and pull may skip the
-// mask of in-neighbours outside the frontier.
+// One level is an expand phase (push or pull) and an update phase.  The
+// bodies below are grid-strided from wave `w0` over `nw` waves, so that the
+// per-level launches (k_ms_push ... k_ms_update, the host reading the level's
+// two counters after the update) and the persistent launch (k_msbfs, grid
+// barriers between the phases) run the same code.  fbits is the frontier
+// bitmap (bit v = frontier[v] != 0), written by the update: push reads the
+// 8-byte frontier mask only of frontier vertices, and pull may skip the mask
+// of in-neighbours outside the frontier.  Rows above kLaneHeavy edges are
+// left to the chunk bodies, which spread them over all waves in kLaneChunk
+// pieces (a single wave per hub would be the critical path).
 
 struct MsLevelCounters {
   unsigned long long new_vertices;
   unsigned long long frontier_edges;
 };
 
-__global__ __launch_bounds__(kBatchNT) void k_ms_push(int nV, const int32_t* __restrict__ off_out,
-                                                      const int32_t* __restrict__ dsts,
-                                                      const uint64_t* __restrict__ visit,
-                                                      const uint64_t* __restrict__ frontier,
-                                                      const uint32_t* __restrict__ fbits,
-                                                      uint64_t* __restrict__ next) {
+__device__ __forceinline__ void ms_push_body(int w0, int nw, int nV,
+                                             const int32_t* __restrict__ off_out,
+                                             const int32_t* __restrict__ dsts,
+                                             const uint64_t* __restrict__ visit,
+                                             const uint64_t* __restrict__ frontier,
+                                             const uint32_t* __restrict__ fbits,
+                                             uint64_t* __restrict__ next) {
   const int lane = lane_id();
-  const int wpb = blockDim.x / kWave;
-  const int n_waves = gridDim.x * wpb;
-  for (int base = (blockIdx.x * wpb + threadIdx.x / kWave) * kWave; base < nV;
-       base += n_waves * kWave) {
+  for (int base = w0 * kWave; base < nV; base += nw * kWave) {
     const int v = base + lane;
     const uint32_t word = (v < nV) ? fbits[v >> 5] : 0u;
     const bool inf = (word >> (v & 31)) & 1u;
@@ -279,7 +149,7 @@ __global__ __launch_bounds__(kBatchNT) void k_ms_push(int nV, const int32_t* __r
     if (f) {
       b = off_out[v];
       e = off_out[v + 1];
-      if (e - b > kLaneHeavy) f = 0;  // chunked: k_ms_push_chunks
+      if (e - b > kLaneHeavy) f = 0;  // chunked: ms_push_chunk_body
     }
     const bool light = (e - b) <= kMsHeavy;
     if (f && light) {
@@ -297,14 +167,14 @@ __global__ __launch_bounds__(kBatchNT) void k_ms_push(int nV, const int32_t* __r
           if (want[j]) atomicOr((unsigned long long*)&next[u[j]], (unsigned long long)want[j]);
       }
     }
+    // medium rows one wave each; the row's bounds and mask are read out of
+    // its lane as scalars, so the walk is a scalar loop
     uint64_t heavy = __ballot(f && !light);
     while (heavy) {
       const int hl = __builtin_ctzll(heavy);
       heavy &= heavy - 1;
-      const int hb = __shfl(b, hl, kWave), he = __shfl(e, hl, kWave);
-      const uint32_t flo = __shfl((uint32_t)f, hl, kWave);
-      const uint32_t fhi = __shfl((uint32_t)(f >> 32), hl, kWave);
-      const uint64_t hf = ((uint64_t)fhi << 32) | flo;
+      const int hb = lane_value(b, hl), he = lane_value(e, hl);
+      const uint64_t hf = lane_value64(f, hl);
       for (int p = hb + lane; p < he; p += kWave) {
         const int u = dsts[p];
         const uint64_t want = hf & ~visit[u] & ~next[u];
@@ -321,25 +191,22 @@ __device__ __forceinline__ uint64_t ms_mask_of(int u, const uint64_t* __restrict
 }
 
 // filter: the frontier bitmap, or nullptr to read every in-neighbour's mask
-__global__ __launch_bounds__(kBatchNT) void k_ms_pull(int nV, uint64_t all,
-                                                      const int32_t* __restrict__ off_in,
-                                                      const int32_t* __restrict__ srcs,
-                                                      const uint64_t* __restrict__ visit,
-                                                      const uint64_t* __restrict__ frontier,
-                                                      const uint32_t* __restrict__ filter,
-                                                      uint64_t* __restrict__ next) {
+__device__ __forceinline__ void ms_pull_body(int w0, int nw, int nV, uint64_t all,
+                                             const int32_t* __restrict__ off_in,
+                                             const int32_t* __restrict__ srcs,
+                                             const uint64_t* __restrict__ visit,
+                                             const uint64_t* __restrict__ frontier,
+                                             const uint32_t* __restrict__ filter,
+                                             uint64_t* __restrict__ next) {
   const int lane = lane_id();
-  const int wpb = blockDim.x / kWave;
-  const int n_waves = gridDim.x * wpb;
-  for (int base = (blockIdx.x * wpb + threadIdx.x / kWave) * kWave; base < nV;
-       base += n_waves * kWave) {
+  for (int base = w0 * kWave; base < nV; base += nw * kWave) {
     const int v = base + lane;
     uint64_t need = v < nV ? (all & ~visit[v]) : 0ull;
     int b = 0, e = 0;
     if (need) {
       b = off_in[v];
       e = off_in[v + 1];
-      if (e - b > kLaneHeavy) need = 0;  // chunked: k_ms_pull_chunks (atomicOr)
+      if (e - b > kLaneHeavy) need = 0;  // chunked: ms_pull_chunk_body (atomicOr)
     }
     const bool light = (e - b) <= kMsHeavy;
     if (need && light) {
@@ -359,12 +226,10 @@ __global__ __launch_bounds__(kBatchNT) void k_ms_pull(int nV, uint64_t all,
     while (heavy) {
       const int hl = __builtin_ctzll(heavy);
       heavy &= heavy - 1;
-      const int hb = __shfl(b, hl, kWave), he = __shfl(e, hl, kWave);
-      const uint32_t nlo = __shfl((uint32_t)need, hl, kWave);
-      const uint32_t nhi = __shfl((uint32_t)(need >> 32), hl, kWave);
-      const uint64_t hn = ((uint64_t)nhi << 32) | nlo;
+      const int hb = lane_value(b, hl), he = lane_value(e, hl);
+      const uint64_t hn = lane_value64(need, hl);
       uint64_t hacc = 0;
-      for (int pb = hb; pb < he; pb += kWave) {  // wave-uniform trip count
+      for (int pb = hb; pb < he; pb += kWave) {  // scalar trip count
         const int p = pb + lane;
         const uint64_t m = p < he ? ms_mask_of(srcs[p], frontier, filter) : 0ull;
         hacc |= wave_or64(m);
@@ -375,16 +240,15 @@ __global__ __launch_bounds__(kBatchNT) void k_ms_pull(int nV, uint64_t all,
   }
 }
 
-// Vertices with more than kLaneHeavy edges, in chunks of kLaneChunk edges
-// spread over all waves (a single wave per hub would be the critical path).
-__global__ __launch_bounds__(kBatchNT) void k_ms_push_chunks(
-    const LaneChunk* __restrict__ chunks, int n_chunks, const int32_t* __restrict__ dsts,
-    const uint64_t* __restrict__ visit, const uint64_t* __restrict__ frontier,
-    const uint32_t* __restrict__ fbits, uint64_t* __restrict__ next) {
+__device__ __forceinline__ void ms_push_chunk_body(int w0, int nw,
+                                                   const LaneChunk* __restrict__ chunks,
+                                                   int n_chunks, const int32_t* __restrict__ dsts,
+                                                   const uint64_t* __restrict__ visit,
+                                                   const uint64_t* __restrict__ frontier,
+                                                   const uint32_t* __restrict__ fbits,
+                                                   uint64_t* __restrict__ next) {
   const int lane = lane_id();
-  const int wpb = blockDim.x / kWave;
-  const int n_waves = gridDim.x * wpb;
-  for (int c = blockIdx.x * wpb + threadIdx.x / kWave; c < n_chunks; c += n_waves) {
+  for (int c = w0; c < n_chunks; c += nw) {
     const LaneChunk ch = chunks[c];
     if (!((fbits[ch.v >> 5] >> (ch.v & 31)) & 1u)) continue;
     const uint64_t f = frontier[ch.v];
@@ -396,14 +260,15 @@ __global__ __launch_bounds__(kBatchNT) void k_ms_push_chunks(
   }
 }
 
-__global__ __launch_bounds__(kBatchNT) void k_ms_pull_chunks(
-    const LaneChunk* __restrict__ chunks, int n_chunks, uint64_t all,
-    const int32_t* __restrict__ srcs, const uint64_t* __restrict__ visit,
-    const uint64_t* __restrict__ frontier, uint64_t* __restrict__ next) {
+__device__ __forceinline__ void ms_pull_chunk_body(int w0, int nw,
+                                                   const LaneChunk* __restrict__ chunks,
+                                                   int n_chunks, uint64_t all,
+                                                   const int32_t* __restrict__ srcs,
+                                                   const uint64_t* __restrict__ visit,
+                                                   const uint64_t* __restrict__ frontier,
+                                                   uint64_t* __restrict__ next) {
   const int lane = lane_id();
-  const int wpb = blockDim.x / kWave;
-  const int n_waves = gridDim.x * wpb;
-  for (int c = blockIdx.x * wpb + threadIdx.x / kWave; c < n_chunks; c += n_waves) {
+  for (int c = w0; c < n_chunks; c += nw) {
     const LaneChunk ch = chunks[c];
     const uint64_t need = all & ~visit[ch.v];
     if (!need) continue;
@@ -418,35 +283,41 @@ __global__ __launch_bounds__(kBatchNT) void k_ms_pull_chunks(
   }
 }
 
-__global__ __launch_bounds__(kBatchNT) void k_ms_update(
-    int nV, uint64_t all, int level, const int32_t* __restrict__ off_out,
-    uint64_t* __restrict__ visit, uint64_t* __restrict__ frontier, uint64_t* __restrict__ next,
-    uint32_t* __restrict__ fbits, int32_t* __restrict__ depth_vs,
-    MsLevelCounters* __restrict__ ctr) {
+// The update of every vertex: new = next & ~visit; visit |= new; frontier =
+// new; the level is written for every new (source, vertex) pair into the
+// vertex-major depth rows; the new frontier's vertex and out-edge counts are
+// added to *nv_out / *fe_out once per workgroup.  Every thread of the
+// workgroup calls it (it synchronises the workgroup once).
+__device__ __forceinline__ void ms_update_body(int w0, int nw, int nV, uint64_t all, int level,
+                                               const int32_t* __restrict__ off_out,
+                                               uint64_t* __restrict__ visit,
+                                               uint64_t* __restrict__ frontier,
+                                               uint64_t* __restrict__ next,
+                                               uint32_t* __restrict__ fbits,
+                                               int32_t* __restrict__ depth_vs,
+                                               unsigned long long* __restrict__ nv_out,
+                                               unsigned long long* __restrict__ fe_out) {
   __shared__ unsigned long long s_nv[kBatchNT / kWave], s_fe[kBatchNT / kWave];
   const int lane = lane_id();
-  const int wpb = blockDim.x / kWave;
-  const int n_waves = gridDim.x * wpb;
   // the level counts are summed per wave over its vertices, then per
   // workgroup: one atomic pair per workgroup (not per wave) on the counters
   unsigned long long my_nv = 0, my_fe = 0;
-  for (int base = (blockIdx.x * wpb + threadIdx.x / kWave) * kWave; base < nV;
-       base += n_waves * kWave) {
+  for (int base = w0 * kWave; base < nV; base += nw * kWave) {
     const int v = base + lane;
-    uint64_t nw = 0;
+    uint64_t nwm = 0;
     int deg = 0;
     if (v < nV) {
       const uint64_t vis = visit[v];
       const uint64_t nxt = next[v];
-      nw = nxt & ~vis & all;
+      nwm = nxt & ~vis & all;
       if (nxt) next[v] = 0;
-      if (nw || frontier[v]) frontier[v] = nw;
-      if (nw) {
-        visit[v] = vis | nw;
+      if (nwm || frontier[v]) frontier[v] = nwm;
+      if (nwm) {
+        visit[v] = vis | nwm;
         deg = off_out[v + 1] - off_out[v];
       }
     }
-    const uint64_t nb = __ballot(nw != 0);
+    const uint64_t nb = __ballot(nwm != 0);
     // depths of the new (source, vertex) pairs: for each vertex of the wave
     // with new bits, lane s writes the vertex's row entry s (one masked,
     // coalesced 256-B store per vertex)
@@ -454,9 +325,7 @@ __global__ __launch_bounds__(kBatchNT) void k_ms_update(
     while (rows) {
       const int r = __builtin_ctzll(rows);
       rows &= rows - 1;
-      const uint32_t lo = __shfl((uint32_t)nw, r, kWave);
-      const uint32_t hi = __shfl((uint32_t)(nw >> 32), r, kWave);
-      const uint64_t m = ((uint64_t)hi << 32) | lo;
+      const uint64_t m = lane_value64(nwm, r);
       if ((m >> lane) & 1ull) depth_vs[(size_t)(base + r) * kWave + lane] = level + 1;
     }
     if ((lane & 31) == 0 && v < nV) fbits[v >> 5] = (uint32_t)(nb >> lane);
@@ -473,14 +342,111 @@ __global__ __launch_bounds__(kBatchNT) void k_ms_update(
   __syncthreads();
   if (threadIdx.x == 0) {
     unsigned long long nv = 0, fe = 0;
-    for (int i = 0; i < wpb; ++i) {
+    for (int i = 0; i < (int)(blockDim.x / kWave); ++i) {
       nv += s_nv[i];
       fe += s_fe[i];
     }
     if (nv) {
-      atomicAdd(&ctr->new_vertices, nv);
-      atomicAdd(&ctr->frontier_edges, fe);
+      atomicAdd(nv_out, nv);
+      atomicAdd(fe_out, fe);
     }
+  }
+}
+
+__device__ __forceinline__ int ms_wave0() {
+  return blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+}
+__device__ __forceinline__ int ms_waves() { return gridDim.x * (blockDim.x / kWave); }
+
+__global__ __launch_bounds__(kBatchNT) void k_ms_push(int nV, const int32_t* __restrict__ off_out,
+                                                      const int32_t* __restrict__ dsts,
+                                                      const uint64_t* __restrict__ visit,
+                                                      const uint64_t* __restrict__ frontier,
+                                                      const uint32_t* __restrict__ fbits,
+                                                      uint64_t* __restrict__ next) {
+  ms_push_body(ms_wave0(), ms_waves(), nV, off_out, dsts, visit, frontier, fbits, next);
+}
+
+__global__ __launch_bounds__(kBatchNT) void k_ms_pull(int nV, uint64_t all,
+                                                      const int32_t* __restrict__ off_in,
+                                                      const int32_t* __restrict__ srcs,
+                                                      const uint64_t* __restrict__ visit,
+                                                      const uint64_t* __restrict__ frontier,
+                                                      const uint32_t* __restrict__ filter,
+                                                      uint64_t* __restrict__ next) {
+  ms_pull_body(ms_wave0(), ms_waves(), nV, all, off_in, srcs, visit, frontier, filter, next);
+}
+
+__global__ __launch_bounds__(kBatchNT) void k_ms_push_chunks(
+    const LaneChunk* __restrict__ chunks, int n_chunks, const int32_t* __restrict__ dsts,
+    const uint64_t* __restrict__ visit, const uint64_t* __restrict__ frontier,
+    const uint32_t* __restrict__ fbits, uint64_t* __restrict__ next) {
+  ms_push_chunk_body(ms_wave0(), ms_waves(), chunks, n_chunks, dsts, visit, frontier, fbits, next);
+}
+
+__global__ __launch_bounds__(kBatchNT) void k_ms_pull_chunks(
+    const LaneChunk* __restrict__ chunks, int n_chunks, uint64_t all,
+    const int32_t* __restrict__ srcs, const uint64_t* __restrict__ visit,
+    const uint64_t* __restrict__ frontier, uint64_t* __restrict__ next) {
+  ms_pull_chunk_body(ms_wave0(), ms_waves(), chunks, n_chunks, all, srcs, visit, frontier, next);
+}
+
+__global__ __launch_bounds__(kBatchNT) void k_ms_update(
+    int nV, uint64_t all, int level, const int32_t* __restrict__ off_out,
+    uint64_t* __restrict__ visit, uint64_t* __restrict__ frontier, uint64_t* __restrict__ next,
+    uint32_t* __restrict__ fbits, int32_t* __restrict__ depth_vs,
+    MsLevelCounters* __restrict__ ctr) {
+  ms_update_body(ms_wave0(), ms_waves(), nV, all, level, off_out, visit, frontier, next, fbits,
+                 depth_vs, &ctr->new_vertices, &ctr->frontier_edges);
+}
+
+// Every level in ONE cooperative launch: the same expand and update bodies,
+// separated by grid barriers.  The level's counts live in a ring of three
+// slots: level L reads slot L % 3 (written by the previous update, or by
+// k_ms_sources), workgroup 0 clears slot (L + 1) % 3 before the barrier that
+// precedes the update which adds into it.  Every workgroup reads the counts
+// with agent-scope atomic loads after a fenced grid barrier, so all of them
+// take the same direction and leave the level loop at the same level.
+__global__ __launch_bounds__(kBatchNT) void k_msbfs(
+    int nV, long long nE, int ns, int mode, double alpha, int filter,
+    const int32_t* __restrict__ off_out, const int32_t* __restrict__ dsts,
+    const int32_t* __restrict__ off_in, const int32_t* __restrict__ srcs,
+    uint64_t* __restrict__ visit, uint64_t* __restrict__ frontier, uint64_t* __restrict__ next,
+    uint32_t* __restrict__ fbits, int32_t* __restrict__ depth_vs,
+    const LaneChunk* __restrict__ in_chunks, int n_in_chunks,
+    const LaneChunk* __restrict__ out_chunks, int n_out_chunks, MsCounters* __restrict__ ctr,
+    int max_levels) {
+  cg::grid_group grid = cg::this_grid();
+  const uint64_t all = ns >= 64 ? ~0ull : ((1ull << ns) - 1);
+  const int w0 = ms_wave0(), nw = ms_waves();
+  const long long threshold = (long long)((double)nE / (alpha > 0 ? alpha : 1.0));
+  for (int level = 0; level < max_levels; ++level) {
+    const int slot = level % 3;
+    const unsigned long long nv_count =
+        __hip_atomic_load(&ctr->new_vertices[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long fe_count =
+        __hip_atomic_load(&ctr->frontier_edges[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (nv_count == 0) break;
+    const bool pull = mode == 2 || (mode == 0 && (long long)fe_count > threshold);
+    const int nx = (level + 1) % 3;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      __hip_atomic_store(&ctr->new_vertices[nx], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ctr->frontier_edges[nx], 0ull, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (pull) {
+      ms_pull_body(w0, nw, nV, all, off_in, srcs, visit, frontier, filter ? fbits : nullptr,
+                   next);
+      ms_pull_chunk_body(w0, nw, in_chunks, n_in_chunks, all, srcs, visit, frontier, next);
+    } else {
+      ms_push_body(w0, nw, nV, off_out, dsts, visit, frontier, fbits, next);
+      ms_push_chunk_body(w0, nw, out_chunks, n_out_chunks, dsts, visit, frontier, fbits, next);
+    }
+    grid_barrier(grid);
+    ms_update_body(w0, nw, nV, all, level, off_out, visit, frontier, next, fbits, depth_vs,
+                   &ctr->new_vertices[nx], &ctr->frontier_edges[nx]);
+    grid_barrier(grid);
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctr->levels = level + 1;
   }
 }
 
@@ -1041,21 +1007,14 @@ int batch_bfs(gr_batch* b, const int32_t* sources, int n_sources, int32_t* depth
   GR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kBatchNT, 0));
   if (per_cu < 1) return set_error(GR_ERR_HIP, "gr_batch_bfs: kernel cannot be resident");
   // Every workgroup must be resident at once (grid.sync waits for all of
-  // them).  The compiler's resource report for gfx950 (k_msbfs: 66 SGPRs,
-  // 36 VGPRs, no LDS; k_msbfs_stream: 92 SGPRs, 48 VGPRs, no LDS) gives
+  // them).  The compiler's resource report for gfx950 (k_msbfs: 100 SGPRs,
+  // 44 VGPRs, 64 B of LDS; k_msbfs_stream: 92 SGPRs, 48 VGPRs, no LDS) gives
   // 8 waves per SIMD for both, i.e. 8 workgroups of 4 waves per CU, which is
   // what the occupancy query returns.  The grid takes at most half of that,
   // so the launch stays resident even when other work holds part of every
   // CU; the level loops are grid-strided and do not depend on the grid size.
   const int grid = cus * std::max(1, std::min(per_cu, kCoopBlocksPerCu));
   GR_HIP(hipEventRecord(b->e0, st));
-  // the single-launch kernels write [ns][V] rows: depth = -1 everywhere
-  // first, the counterpart of the host-side vertex init before setGraph
-  GR_HIP(hipMemsetAsync(rows, 0xFF, sizeof(int32_t) * (size_t)nV * n_sources, st));
-  k_ms_init<<<1024, 256, 0, st>>>(nV, visit, frontier, next, fbits);
-  k_ms_sources<<<1, 64, 0, st>>>(n_sources, b->d_src.p, nV, visit, frontier, fbits, rows, 0,
-                                 g->off_out, b->ctr.p);
-  GR_HIP(hipGetLastError());
   int nVa = nV, ns = n_sources, mode = direction, max_levels = nV + 1;
   long long nE = g->nE;
   double al = alpha;
@@ -1063,19 +1022,35 @@ int batch_bfs(gr_batch* b, const int32_t* sources, int n_sources, int32_t* depth
   uint64_t *pv = visit, *pf = frontier, *pn = next;
   uint32_t* pfb = fbits;
   int32_t* pd = rows;
-  const LaneChunk* pch = b->in_chunks.chunks;
-  int nch = b->in_chunks.n_chunks;
   MsCounters* pc = b->ctr.p;
+  k_ms_init<<<1024, 256, 0, st>>>(nV, visit, frontier, next, fbits);
   if (stream) {
+    // the streaming kernel writes [ns][V] rows: depth = -1 everywhere first,
+    // the counterpart of the host-side vertex init before setGraph
+    GR_HIP(hipMemsetAsync(rows, 0xFF, sizeof(int32_t) * (size_t)nV * n_sources, st));
+    k_ms_sources<<<1, 64, 0, st>>>(n_sources, b->d_src.p, nV, visit, frontier, fbits, rows, 0,
+                                   g->off_out, b->ctr.p);
+    GR_HIP(hipGetLastError());
+    const LaneChunk* pch = b->in_chunks.chunks;
+    int nch = b->in_chunks.n_chunks;
     void* args[] = {&nVa, &nE, &ns, &mode, &al, &off_out, &dsts, &off_in, &srcs, &pv,
                     &pf,  &pn, &pfb, &pd,  &pch, &nch,   &pc,   &max_levels};
     GR_HIP(hipLaunchCooperativeKernel(kernel, dim3(grid), dim3(kBatchNT), args, 0, st));
+    k_rows_to_user<<<4096, 256, 0, st>>>(nV, n_sources, rows, b->user_g->inv, depth_device);
   } else {
-    void* args[] = {&nVa, &nE, &ns, &mode, &al, &off_out, &dsts, &off_in, &srcs,
-                    &pv,  &pf, &pn, &pd,   &pc, &max_levels};
+    // vertex-major [V][64] depths, valid where the visit bit is set
+    k_ms_sources<<<1, 64, 0, st>>>(n_sources, b->d_src.p, nV, visit, frontier, fbits, rows, 1,
+                                   g->off_out, b->ctr.p);
+    GR_HIP(hipGetLastError());
+    int flt = filter ? 1 : 0;
+    const LaneChunk *pin = b->in_chunks.chunks, *pout = b->out_chunks.chunks;
+    int nin = b->in_chunks.n_chunks, nout = b->out_chunks.n_chunks;
+    void* args[] = {&nVa, &nE,  &ns,  &mode, &al,  &flt, &off_out, &dsts, &off_in, &srcs, &pv,
+                    &pf,  &pn,  &pfb, &pd,   &pin, &nin, &pout,    &nout, &pc,     &max_levels};
     GR_HIP(hipLaunchCooperativeKernel(kernel, dim3(grid), dim3(kBatchNT), args, 0, st));
+    k_ms_depth_out<<<(unsigned)std::min<int64_t>(((int64_t)nV + kWave - 1) / kWave, 65536), 256,
+                     0, st>>>(nV, n_sources, rows, visit, b->user_g->inv, depth_device);
   }
-  k_rows_to_user<<<4096, 256, 0, st>>>(nV, n_sources, rows, b->user_g->inv, depth_device);
   GR_HIP(hipGetLastError());
   MsCounters* hc = static_cast<MsCounters*>(b->h_pinned);
   GR_HIP(hipMemcpyAsync(hc, b->ctr.p, sizeof(MsCounters), hipMemcpyDeviceToHost, st));

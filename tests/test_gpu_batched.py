@@ -67,12 +67,15 @@ def test_msbfs_partial_batch_and_plain_engine(gpu):
     g.close()
 
 
-def test_msbfs_grid_deep_levels(gpu):
+@pytest.mark.parametrize("persistent", [False, True])
+def test_msbfs_grid_deep_levels(gpu, persistent):
+    """Over 150 levels: the persistent launch's counter ring and barriers
+    hold level after level."""
     gr = gpu
     g = gr.Graph.grid(96, 0.01, seed=5)
     s, d = g.edges()
     srcs = [0, 95, 96 * 96 - 1, 4000]
-    depth, info = gr.msbfs(g, srcs)
+    depth, info = gr.msbfs(g, srcs, persistent=persistent)
     assert info["levels"] > 150
     for i, src in enumerate(srcs):
         ref, _, _ = oracle.bfs(g.n_vertices, s, d, src)

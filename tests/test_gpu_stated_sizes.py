@@ -117,6 +117,9 @@ def test_c5_batched_bfs_64_rows(rmat24):
     for i in (0, 21, 42, 63):
         ref, _, _ = oracle.bfs_csr(n, h["off_out"], h["dsts"], h["off_in"], h["srcs"], srcs[i])
         assert np.array_equal(depth[i], ref), f"row {i} vs oracle"
+    _say("C5 batched BFS: 64 sources, one persistent launch")
+    pdepth, _ = gr.msbfs(g, srcs, persistent=True)
+    assert np.array_equal(pdepth, depth)
 
 
 def test_c5_batched_sssp_64_rows(rmat24):
