@@ -817,7 +817,10 @@ using namespace gr;
 // workgroups per CU, as the occupancy query says).  The cause is not
 // established.
 constexpr bool kStreamPullEnabled = false;
-constexpr int kCoopBlocksPerCu = 4;  // cooperative MS-BFS grid: workgroups per CU
+#ifndef GR_COOP_BLOCKS_PER_CU
+#define GR_COOP_BLOCKS_PER_CU 4
+#endif
+constexpr int kCoopBlocksPerCu = GR_COOP_BLOCKS_PER_CU;  // cooperative MS-BFS grid: workgroups per CU
 
 // ---------------------------------------------------------- batch context
 //

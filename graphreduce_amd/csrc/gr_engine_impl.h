@@ -79,6 +79,45 @@ constexpr int kApplyNT = 256;
 constexpr int kApplyVT = 8;
 constexpr int kApplyChunk = kApplyNT * kApplyVT;
 
+// Deferred changed list of a bitmap apply.  cc[c]: chunk c's (count << 32 |
+// out-edges) of the vertices whose apply returned true and that have
+// out-edges.  Chunks form groups of kApplyGroup; each chunk adds ONE word to
+// its group's sc[group * kGroupStride] (groups 128 bytes apart): the chunk
+// count in bits 50 and up, its vertex count in bits 32-49 (at most
+// kApplyGroup * kApplyChunk = 2^17 per group), its out-edges in bits 0-31
+// (the packed totals keep out-edges below 2^32, as ctr->chg does).  The
+// chunk whose add completes the group knows the group total from the value
+// the add returned, keeps it in sc_final[group], resets sc and adds the
+// total to ctr->chg: one same-address atomic per group instead of one per
+// chunk, and no fence.  k_chg_write finds a chunk's list position from
+// sc_final and cc.
+constexpr int kApplyGroup = 64;
+constexpr int kGroupStride = 16;  // unsigned long longs between groups
+constexpr int kGroupDoneShift = 50;
+static_assert(kApplyGroup * kApplyChunk < (1 << (kGroupDoneShift - 32)),
+              "a group's vertex count fits below the done count");
+struct ApplyDefer {
+  unsigned long long* cc = nullptr;  // nullptr: the list is appended at once
+  unsigned long long* sc = nullptr;
+  unsigned long long* sc_final = nullptr;
+};
+
+__device__ __forceinline__ void defer_chunk_done(const ApplyDefer& d, int c, int n_chunks,
+                                                 unsigned long long tot,
+                                                 DevCounters* __restrict__ ctr) {
+  d.cc[c] = tot;
+  const int grp = c / kApplyGroup;
+  const unsigned members = (unsigned)min(kApplyGroup, n_chunks - grp * kApplyGroup);
+  const unsigned long long mine = (1ull << kGroupDoneShift) + tot;
+  const unsigned long long old = atomicAdd(&d.sc[grp * kGroupStride], mine);
+  if ((unsigned)(old >> kGroupDoneShift) == members - 1) {
+    const unsigned long long g = (old + mine) & ((1ull << kGroupDoneShift) - 1);
+    d.sc[grp * kGroupStride] = 0ull;  // every chunk of the group has added
+    d.sc_final[grp] = g;
+    if (g) atomicAdd(&ctr->chg, g);
+  }
+}
+
 // Program::apply over the active set: the ascending active list (kBits
 // false: entry i is vertex active[i]) or the active bitmap (kBits true: entry
 // i is vertex i, taken when its bit is set; n = nV).  Each workgroup takes a
@@ -88,6 +127,12 @@ constexpr int kApplyChunk = kApplyNT * kApplyVT;
 // vertex ids within the chunk), so that a wave's lanes store to consecutive
 // positions, with their out-edge offsets increasing with the list index;
 // their bits are set in the changed bitmap for pull mode.
+// Deferred form (defer.cc != nullptr, bitmap form with a changed bitmap):
+// no list is written; each chunk's counts go to ApplyDefer (above), from
+// which k_chg_write makes the list when a phase needs it.  One same-address
+// atomic per workgroup serialises at the memory side (about 12 ns each; 8192
+// of them took 100 us in one launch on an MI355X), which is what the
+// deferred form avoids for large active sets.
 template <class P, bool kBits>
 __global__ __launch_bounds__(kApplyNT) void k_apply(
     int n, const int32_t* __restrict__ active, const uint32_t* __restrict__ act_bits,
@@ -96,7 +141,7 @@ __global__ __launch_bounds__(kApplyNT) void k_apply(
     int iteration, int32_t* __restrict__ chg_v, int32_t* __restrict__ chg_e,
     uint32_t* __restrict__ chg_bits, DevCounters* __restrict__ ctr,
     source_value_t<P>* __restrict__ srcval, typename P::GatherResult* __restrict__ cand,
-    unsigned long long* __restrict__ applied) {
+    unsigned long long* __restrict__ applied, ApplyDefer defer) {
   constexpr int kNW = kApplyNT / kWave;
   constexpr int kWords = kApplyChunk / 32;
   static_assert(kWords <= kApplyNT, "one bitmap word per thread");
@@ -117,7 +162,12 @@ __global__ __launch_bounds__(kApplyNT) void k_apply(
         word = wi * 32 < n ? act_bits[wi] : 0u;
         s_words[tid] = word;
       }
-      if (!__syncthreads_or(word != 0u)) continue;
+      if (!__syncthreads_or(word != 0u)) {
+        if (defer.cc && tid == 0)
+          defer_chunk_done(defer, chunk / kApplyChunk, (n + kApplyChunk - 1) / kApplyChunk, 0ull,
+                           ctr);
+        continue;
+      }
     }
     // In phases over the lane's kApplyVT entries (ids and gather values,
     // vertex values, apply, out-degrees of the changed ones, then the stores
@@ -185,6 +235,30 @@ __global__ __launch_bounds__(kApplyNT) void k_apply(
     // and the sum of their out-degrees, scanned below
     unsigned long long ball[kApplyVT];
     int incl[kApplyVT];
+    if (defer.cc) {  // deferred: stores, bits and the chunk's counts only
+      unsigned long long mine = 0ull;
+#pragma unroll
+      for (int m = 0; m < kApplyVT; ++m) {
+        if (chg_mask & (1u << m)) {
+          vd[dvs[m]] = vs[m];
+          if constexpr (source_value_v<P>) srcval[dvs[m]] = P::sourceValue(vs[m]);
+        }
+        const bool want = degs[m] > 0;
+        wave_set_bits_sorted(chunk + m * kApplyNT + tid < n, want, dvs[m], chg_bits);
+        if (want) mine += (1ull << 32) | (unsigned long long)(uint32_t)degs[m];
+      }
+      my_applied += __popc(r_mask);
+      mine = wave_reduce_add(mine);
+      if (tid == 0) s_base = 0ull;
+      __syncthreads();
+      if (lane_id() == 0 && mine) atomicAdd(&s_base, mine);
+      __syncthreads();
+      if (tid == 0)
+        defer_chunk_done(defer, chunk / kApplyChunk, (n + kApplyChunk - 1) / kApplyChunk, s_base,
+                         ctr);
+      __syncthreads();  // the next chunk reuses s_words and s_base
+      continue;
+    }
 #pragma unroll
     for (int m = 0; m < kApplyVT; ++m) {
       if (chg_mask & (1u << m)) {
@@ -242,6 +316,78 @@ __global__ __launch_bounds__(kApplyNT) void k_apply(
   if (tid == 0 && s_applied)
     atomicAdd(&applied[(blockIdx.x % kAppliedSlots) * kAppliedStride],
               (unsigned long long)s_applied);
+}
+
+// The changed list of a deferred apply: chunk c (kApplyChunk vertices, the
+// 64 words of the changed bitmap from word 64 c) starts at the list index and
+// out-edge offset that the groups before its group (sc_final) and the chunks
+// of its group before it (cc) add up to; thread t takes byte t of the chunk
+// (8 vertices), the list is ascending in vertex id, each entry's out-edge
+// offset the running sum of the out-degrees before it.  A chunk's entries are
+// staged in LDS and written out coalesced.
+constexpr int kChgNT = kApplyChunk / 8;
+//
+// standdown != 0: scatterActivate's blocked gather will mark the next active
+// set in place of the pull (the same rule on the same counters) and no phase
+// of this iteration or the next one reads the list: nothing is written.
+[[maybe_unused]] static __global__ __launch_bounds__(kChgNT) void k_chg_write(
+    const uint32_t* __restrict__ chg_bits, int nV, const int32_t* __restrict__ off_out,
+    const unsigned long long* __restrict__ cc, const unsigned long long* __restrict__ sc_final,
+    int n_chunks, int32_t* __restrict__ chg_v, int32_t* __restrict__ chg_e,
+    const DevCounters* __restrict__ ctr, int standdown, int mode, long long threshold,
+    long long fuse_max) {
+  __shared__ unsigned long long s_scan[kChgNT / kWave];
+  __shared__ unsigned long long s_pref;
+  __shared__ int32_t s_v[kApplyChunk], s_e[kApplyChunk];
+  if (standdown) {  // uniform: every workgroup reads the same counters
+    const unsigned long long c = ctr->chg;
+    if (use_pull_rule(c, mode, threshold) && !fused_scatter_rule(c, fuse_max)) return;
+  }
+  for (int c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+    const unsigned long long own = cc[c];
+    if (own == 0ull) continue;  // uniform: no entries in this chunk
+    if (threadIdx.x < kWave) {
+      const int grp = c / kApplyGroup;
+      unsigned long long p = 0ull;
+      for (int i = threadIdx.x; i < grp; i += kWave) p += sc_final[i];
+      for (int i = grp * kApplyGroup + (int)threadIdx.x; i < c; i += kWave) p += cc[i];
+      p = wave_reduce_add(p);
+      if (threadIdx.x == 0) s_pref = p;
+    }
+    __syncthreads();
+    const unsigned long long b0 = s_pref;
+    const int v0 = c * kApplyChunk + (int)threadIdx.x * 8;
+    const uint32_t word = v0 < nV ? chg_bits[v0 >> 5] : 0u;
+    uint32_t bits = (word >> ((threadIdx.x & 3) * 8)) & 0xFFu;
+    int o[9];
+    unsigned long long mine = 0ull;
+    if (bits) {
+#pragma unroll
+      for (int j = 0; j < 9; ++j) o[j] = off_out[min(v0 + j, nV)];
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if ((bits >> j) & 1u) mine += (1ull << 32) | (unsigned long long)(uint32_t)(o[j + 1] - o[j]);
+    }
+    unsigned long long tot;
+    const unsigned long long ex = block_excl_scan_add<kChgNT>(mine, s_scan, &tot);
+    int pos = (int)(ex >> 32);
+    int dpos = (int)(uint32_t)b0 + (int)(uint32_t)ex;
+    while (bits) {
+      const int j = __builtin_ctz(bits);
+      bits &= bits - 1;
+      s_v[pos] = v0 + j;
+      s_e[pos] = dpos;
+      dpos += o[j + 1] - o[j];
+      ++pos;
+    }
+    __syncthreads();
+    const int cnt = (int)(tot >> 32), first = (int)(b0 >> 32);
+    for (int i = threadIdx.x; i < cnt; i += kChgNT) {
+      chg_v[first + i] = s_v[i];
+      chg_e[first + i] = s_e[i];
+    }
+    __syncthreads();  // the next chunk stages into the same arrays
+  }
 }
 
 // srcval[v] = P::sourceValue(vd[v]) for every vertex.
@@ -959,12 +1105,42 @@ __device__ __forceinline__ ByteWork byte_work(const uint32_t* __restrict__ bitma
     int need_deg, int2* __restrict__ blk, uint32_t* __restrict__ act_bits,
     uint32_t* __restrict__ chg_clear) {
   __shared__ int s_a[kCompactNT / kWave], s_b[kCompactNT / kWave];
+  // a round's 8 * kCompactNT + 1 CSC offsets, loaded coalesced when many of
+  // its bytes have bits (a dense frontier): each thread's 9 overlapping
+  // offsets would otherwise cost a strided load each
+  constexpr int kRoundV = 8 * kCompactNT;
+  __shared__ int32_t s_off[kRoundV + 1];
   int cnt = 0, deg = 0;
   for (int r = 0; r < kCompactRounds; ++r) {
     const int bi = (blockIdx.x * kCompactRounds + r) * kCompactNT + threadIdx.x;
-    const ByteWork b = byte_work(bitmap, n_words, nV, off_in, need_deg, bi);
-    cnt += b.cnt;
-    deg += b.deg;
+    const int wi = bi >> 2;
+    const uint32_t word = (wi < n_words) ? bitmap[wi] : 0u;
+    const uint32_t bits = (word >> ((bi & 3) * 8)) & 0xFFu;
+    cnt += __popc(bits);
+    if (!need_deg) continue;
+    const int n_set = __syncthreads_count(bits != 0u);  // uniform
+    if (n_set == 0) continue;
+    if (n_set >= kCompactNT / 8) {
+      const int v0r = (blockIdx.x * kCompactRounds + r) * kRoundV;
+      for (int i = threadIdx.x; i <= kRoundV; i += kCompactNT) s_off[i] = off_in[min(v0r + i, nV)];
+      __syncthreads();
+      if (bits) {
+        const int* o = s_off + threadIdx.x * 8;
+        if (bits == 0xFFu) {
+          deg += o[8] - o[0];
+        } else {
+#pragma unroll
+          for (int j = 0; j < 8; ++j)
+            if ((bits >> j) & 1u) deg += o[j + 1] - o[j];
+        }
+      }
+      __syncthreads();  // the next round stages into the same array
+    } else if (bits) {
+      const int v0 = bi * 8;
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if ((bits >> j) & 1u) deg += off_in[v0 + j + 1] - off_in[v0 + j];  // bit set => v0 + j < nV
+    }
   }
   cnt = wave_reduce_add(cnt);
   deg = wave_reduce_add(deg);
@@ -1346,6 +1522,14 @@ struct Engine final : EngineBase {
   // (ensure_list); setActive writes the list itself.
   uint32_t* d_act_bits = nullptr;
   bool list_valid = true;
+  // Deferred changed list (large bitmap applies): per-chunk counts, scanned
+  // in place; the list itself is written when a phase needs it.
+  unsigned long long* d_cc = nullptr;
+  unsigned long long* d_sc = nullptr;
+  unsigned long long* d_sc_final = nullptr;
+  int n_apply_chunks = 0;
+  bool chg_pending = false;
+  bool chg_cond = false;  // written only if the blocked gather did not run
   uint32_t* d_chg_bits = nullptr;  // vertices changed by this iteration's apply
   uint32_t* d_chg_prev = nullptr;  // ... by the previous iteration's apply
   int2* d_blk = nullptr;
@@ -1425,7 +1609,7 @@ struct Engine final : EngineBase {
                     d_vd_user,   d_ed,     d_ed_csr,   d_cand,   d_active, d_escan,
                     d_gtmp,      d_chg_v,  d_chg_e,    d_bitmap, d_chg_bits, d_chg_prev,
                     d_blk,       d_part,   d_push_part, d_carry, d_head,   d_ctr,
-                    d_applied,   d_act_bits};
+                    d_applied,   d_act_bits, d_cc,     d_sc,     d_sc_final};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     if (h_ctr) (void)hipHostFree(h_ctr);
@@ -1480,6 +1664,8 @@ struct Engine final : EngineBase {
     d_active = d_escan = d_chg_v = d_chg_e = d_part = d_push_part = nullptr;
     d_gtmp = nullptr;
     d_bitmap = d_chg_bits = d_chg_prev = d_act_bits = nullptr;
+    d_cc = d_sc = d_sc_final = nullptr;
+    chg_pending = chg_cond = false;
     list_valid = true;
     d_blk = nullptr;
     d_carry = d_head = nullptr;
@@ -1547,6 +1733,14 @@ struct Engine final : EngineBase {
     GR_TRY(dalloc(&d_act_bits, n_words + 1));
     GR_TRY(dalloc(&d_chg_bits, n_words + 1));
     GR_TRY(dalloc(&d_chg_prev, n_words + 1));
+    n_apply_chunks = (int)(((int64_t)V + kApplyChunk - 1) / kApplyChunk);
+    GR_TRY(dalloc(&d_cc, (int64_t)n_apply_chunks));
+    {
+      const int64_t groups = (n_apply_chunks + kApplyGroup - 1) / kApplyGroup;
+      GR_TRY(dalloc(&d_sc, groups * kGroupStride));
+      GR_TRY(dalloc(&d_sc_final, groups));
+      GR_HIP(hipMemsetAsync(d_sc, 0, sizeof(unsigned long long) * groups * kGroupStride, stream));
+    }
     GR_TRY(dalloc(&d_blk, n_cblocks));
     GR_TRY(dalloc(&d_part, max_tiles + 2));
     GR_TRY(dalloc(&d_push_part, max_push_tiles + 2));
@@ -2200,6 +2394,7 @@ struct Engine final : EngineBase {
   // Candidates relaxed by a fused scatterActivate whose gather will not run
   // (the active set was replaced or the vertex data reset): back to gatherZero.
   int drop_candidates() {
+    GR_TRY(ensure_chg_list());  // before the changed bitmaps are cleared
     GR_TRY(collect_flush());
     fab_pending = fab_launched = false;
     last_gather_blocked = false;
@@ -2217,6 +2412,7 @@ struct Engine final : EngineBase {
   // of the non-inert vertices.
   int inert_scan() {
     if constexpr (kInert) {
+      chg_pending = chg_cond = false;  // the list is replaced by the non-inert vertices
       tick();
       GR_HIP(hipMemsetAsync(&d_ctr->inert, 0, sizeof(unsigned long long), stream));
       k_inert_scan<P><<<grid_1d(g->nV, kApplyChunk), kApplyNT, 0, stream>>>(
@@ -2236,6 +2432,7 @@ struct Engine final : EngineBase {
   // Delta gather by push: relax the out-edges of the previous iteration's
   // changed list into d_cand, then read the candidates of the active rows.
   int gather_push() {
+    GR_TRY(ensure_chg_list());
     const int n = n_active;
     const int n_c = (int)chg_count, w_c = (int)chg_edges;
     tick();
@@ -2274,18 +2471,50 @@ struct Engine final : EngineBase {
     const bool bits = !list_valid && use_gather != 1 &&
                       (int64_t)n_active * kBitsApplyDiv >= (int64_t)g->nV;
     if (!bits) GR_TRY(ensure_list());
+    // a second apply of the same iteration appends to the first one's list
+    GR_TRY(ensure_chg_list());
     tick();
     uint32_t* cbits = changed_bits_on() ? d_chg_bits : nullptr;
     G* cv = use_gather == 3 ? d_gdense : d_cand;
+    // The first apply of an iteration over the bitmap defers its changed
+    // list (no per-workgroup atomic on the list counter); the changed bitmap
+    // holds exactly its vertices then, from which k_chg_write makes the list.
+    const bool defer = bits && cbits && n_applies == 1;
+    ApplyDefer ad;
+    if (defer) {
+      ad.cc = d_cc;
+      ad.sc = d_sc;
+      ad.sc_final = d_sc_final;
+    }
     if (bits)
       k_apply<P, true><<<grid_1d(g->nV, kApplyChunk), kApplyNT, 0, stream>>>(
           g->nV, nullptr, d_act_bits, d_gtmp, use_gather, d_vd, g->off_out, iteration, d_chg_v,
-          d_chg_e, cbits, d_ctr, d_srcval, cv, d_applied);
+          d_chg_e, cbits, d_ctr, d_srcval, cv, d_applied, ad);
     else
       k_apply<P, false><<<grid_1d(n_active, kApplyChunk), kApplyNT, 0, stream>>>(
           n_active, d_active, nullptr, d_gtmp, use_gather, d_vd, g->off_out, iteration, d_chg_v,
-          d_chg_e, cbits, d_ctr, d_srcval, cv, d_applied);
+          d_chg_e, cbits, d_ctr, d_srcval, cv, d_applied, ApplyDefer());
+    if (defer) chg_pending = true;
     tock(kPhApply);
+    GR_HIP(hipGetLastError());
+    return GR_OK;
+  }
+
+  // The changed list of a deferred apply, from the changed bitmap it left.
+  // With `standdown` (scatterActivate launching the blocked gather in place
+  // of the pull, mode / thr / fuse_max its rule) the kernel writes nothing
+  // when that gather runs; a later phase of the same iteration that needs the
+  // list then writes it unconditionally (chg_cond).
+  int ensure_chg_list(bool standdown = false, int mode = 0, long long thr = 0,
+                      long long fuse_max = -1) {
+    if (!chg_pending && !chg_cond) return GR_OK;
+    standdown = standdown && chg_pending;
+    chg_pending = false;
+    chg_cond = standdown;
+    const int grid = n_apply_chunks < 8192 ? n_apply_chunks : 8192;
+    k_chg_write<<<grid < 1 ? 1 : grid, kChgNT, 0, stream>>>(
+        d_chg_bits, g->nV, g->off_out, d_cc, d_sc_final, n_apply_chunks, d_chg_v, d_chg_e, d_ctr,
+        standdown ? 1 : 0, mode, thr, fuse_max);
     GR_HIP(hipGetLastError());
     return GR_OK;
   }
@@ -2371,14 +2600,26 @@ struct Engine final : EngineBase {
             blocked_share > 0 && g->nE > 0;
       if (fab) {
         GR_TRY(blocked_setup());
+        fab = blk->n_blocks > 0;
+      }
+    }
+    // The push, relax and scatter kernels below read the changed list; where
+    // the blocked gather runs in place of the pull they stand down, and the
+    // next gather is that gather's values: the list is then not needed
+    // (unless Program::scatter runs).
+    {
+      bool need_always = false;
+      if constexpr (kScatter) need_always = have_scatter;
+      GR_TRY(ensure_chg_list(fab && !need_always, dir_mode, thr, fuse_max));
+    }
+    if constexpr (kBlockedMin) {
+      if (fab) {
         tick();
-        if (blk->n_blocks > 0)
-          k_gather_blocked<P><<<blk->n_blocks, kBlkNT, 0, stream>>>(
-              blk->brow, blk->bnr, blk->boff, blk->bpk, blk->gbase, d_ed_blk, d_vd, d_srcval,
-              d_chg_bits, d_gdense, d_bitmap, d_ctr, dir_mode, thr, fuse_max);
+        k_gather_blocked<P><<<blk->n_blocks, kBlkNT, 0, stream>>>(
+            blk->brow, blk->bnr, blk->boff, blk->bpk, blk->gbase, d_ed_blk, d_vd, d_srcval,
+            d_chg_bits, d_gdense, d_bitmap, d_ctr, dir_mode, thr, fuse_max);
         tock(kPhGather);
         GR_HIP(hipGetLastError());
-        fab = blk->n_blocks > 0;
       }
     }
     fab_launched = fab;
@@ -2452,6 +2693,11 @@ struct Engine final : EngineBase {
     list_valid = false;
     tock(kPhNext);
     GR_HIP(hipGetLastError());
+    // a deferred list no phase asked for: the next delta gather may push the
+    // out-edges of this iteration's changed vertices, other programs never
+    // read it again
+    if (kDelta && chg_pending) GR_TRY(ensure_chg_list());
+    chg_pending = chg_cond = false;
     // this iteration's changes become the previous ones; the cleared buffer
     // receives the next apply's
     std::swap(d_chg_bits, d_chg_prev);
