@@ -169,6 +169,12 @@ struct gr_graph {
 };
 
 namespace gr {
+// Grid-strided kernels whose workgroups each add to one counter take at most
+// this many workgroups (8 per CU of 4 waves: every CU full): device-scope
+// atomics on one address are performed one after another at the memory side,
+// about 12 ns each, so a launch of 65,536 such workgroups would spend
+// 0.8 ms on them.
+constexpr int kAtomicGrid = 2048;
 constexpr int kBlockRows = 32768;       // rows per gather block (LDS: 128 KB of int)
 constexpr int kBlockRowsWide = 16384;   // ... for 8-byte accumulators (LDS: 128 KB)
 #ifndef GR_MIN_BLOCK_ROWS

@@ -532,6 +532,9 @@ __global__ __launch_bounds__(kSrvNT) void k_srv_checksum_lanes(int nV, uint64_t 
                                                                const T* __restrict__ data,
                                                                const int32_t* __restrict__ perm,
                                                                SrvCounters* __restrict__ ctr) {
+  // the workgroup's waves are summed per lane in LDS first: one atomic per
+  // lane and workgroup (per-wave atomics on the same 64 words serialise)
+  __shared__ unsigned long long s_h[kSrvNT / kWave][kLanes];
   const int lane = lane_id();
   const int wpb = blockDim.x / kWave;
   const int n_waves = gridDim.x * wpb;
@@ -545,7 +548,13 @@ __global__ __launch_bounds__(kSrvNT) void k_srv_checksum_lanes(int nV, uint64_t 
     const uint32_t id = perm ? (uint32_t)perm[v] : (uint32_t)v;  // the caller's vertex id
     h += mix64(((uint64_t)id << 32) | bits);
   }
-  if (mine && h) atomicAdd(&ctr->checksum[lane], h);
+  s_h[threadIdx.x / kWave][lane] = h;
+  __syncthreads();
+  if (threadIdx.x < kWave) {
+    unsigned long long t = 0;
+    for (int w = 0; w < wpb; ++w) t += s_h[w][lane];
+    if (mine && t) atomicAdd(&ctr->checksum[lane], t);
+  }
 }
 
 // Order-independent checksum of one lane's result row: sum of mix64(v, value)
@@ -798,7 +807,7 @@ int launch_step(gr_server* sv, Pool& p) {
       k_srv_bfs_push_chunks<<<grid_1d(sv->out_lanes.n_chunks, kSrvNT / kWave), kSrvNT, 0,
                               p.stream>>>(sv->out_lanes.chunks, sv->out_lanes.n_chunks, g->dsts,
                                           p.m0, p.m1, p.m2);
-    k_srv_bfs_update<<<grid_1d(nV, kSrvNT), kSrvNT, 0, p.stream>>>(
+    k_srv_bfs_update<<<std::min<unsigned>(grid_1d(nV, kSrvNT), kAtomicGrid), kSrvNT, 0, p.stream>>>(
         nV, p.step, p.active, p.seeds, g->off_out, p.m0, p.m1, p.m2, p.i32, p.ctr);
   } else if (p.kind == GR_QUERY_SSSP) {
     const uint64_t* cin = p.parity ? p.m1 : p.m0;
@@ -816,7 +825,7 @@ int launch_step(gr_server* sv, Pool& p) {
     rr.list_len = &p.ctr->list_len;
     rr.filter_max = g->nE / 8;
     rr.host_filter = (long long)p.last_fe < rr.filter_max ? 1 : 0;
-    const unsigned vgrid = std::min<unsigned>(grid_1d((int64_t)nV, kLaneNT), 16384);
+    const unsigned vgrid = std::min<unsigned>(grid_1d((int64_t)nV, kLaneNT), kAtomicGrid);
     k_lane_sssp_mark<<<vgrid, kLaneNT, 0, p.stream>>>(nV, g->off_out, g->dsts,
                                                       sv->out_lanes.chunks,
                                                       sv->out_lanes.n_chunks, cin, rr);
@@ -883,10 +892,10 @@ int retire(gr_server* sv, Pool& p, uint64_t done) {
   const int nV = sv->g->nV;
   GR_HIP(hipMemsetAsync(p.ctr, 0, sizeof(SrvCounters), p.stream));
   if (p.kind == GR_QUERY_BFS || p.kind == GR_QUERY_SSSP)
-    k_srv_checksum_lanes<int32_t><<<grid_v(nV), kSrvNT, 0, p.stream>>>(nV, done, p.i32, sv->perm,
+    k_srv_checksum_lanes<int32_t><<<std::min(grid_v(nV), (unsigned)kAtomicGrid), kSrvNT, 0, p.stream>>>(nV, done, p.i32, sv->perm,
                                                                         p.ctr);
   else
-    k_srv_checksum_lanes<float><<<grid_v(nV), kSrvNT, 0, p.stream>>>(nV, done, p.f0, sv->perm,
+    k_srv_checksum_lanes<float><<<std::min(grid_v(nV), (unsigned)kAtomicGrid), kSrvNT, 0, p.stream>>>(nV, done, p.f0, sv->perm,
                                                                       p.ctr);
   GR_HIP(hipGetLastError());
   GR_HIP(hipMemcpyAsync(p.h_ctr, p.ctr, sizeof(SrvCounters), hipMemcpyDeviceToHost, p.stream));
