@@ -810,17 +810,14 @@ struct DBuf {
 using namespace gr;
 
 // The streaming-pull kernel (k_msbfs_stream) stays off: its cooperative
-// launch did not finish on an MI355X (RMAT-14, 64 sources, automatic
-// direction) although a lane-level CPU emulation of the same code terminates
-// with exact depths on that graph.  Residency is not the difference from
-// k_msbfs: the compiler reports 8 waves per SIMD for both kernels (8
-// workgroups per CU, as the occupancy query says).  The cause is not
-// established.
+// launch, at the occupancy query's 8 workgroups per CU, did not finish on an
+// MI355X (RMAT-14, 64 sources, automatic direction) although a lane-level
+// CPU emulation of the same code terminates with exact depths on that graph.
+// k_msbfs did not finish at 8 workgroups per CU either and passes at 4, so
+// cooperative grids take at most 4 per CU; whether that was this kernel's
+// cause is not established.
 constexpr bool kStreamPullEnabled = false;
-#ifndef GR_COOP_BLOCKS_PER_CU
-#define GR_COOP_BLOCKS_PER_CU 4
-#endif
-constexpr int kCoopBlocksPerCu = GR_COOP_BLOCKS_PER_CU;  // cooperative MS-BFS grid: workgroups per CU
+constexpr int kCoopBlocksPerCu = 4;  // cooperative MS-BFS grid: workgroups per CU (not 8: see above)
 
 // ---------------------------------------------------------- batch context
 //
