@@ -2589,7 +2589,9 @@ struct Engine final : EngineBase {
     // tiles the changed out-edges can fill: the push kernels' grids (they
     // stride over the tiles, so any grid of at least one workgroup is whole)
     const int push_tiles = (int)std::min<long long>(max_push_tiles, bound / kTile + 1);
-    const int push_grid = push_tiles < 8192 ? (push_tiles < 1 ? 1 : push_tiles) : 8192;
+    // (2,048 workgroups fill every CU; a launch the device stands down costs
+    // less with fewer of them)
+    const int push_grid = push_tiles < 2048 ? (push_tiles < 1 ? 1 : push_tiles) : 2048;
     // Min programs whose next gather may use the delta rule: where the pull
     // would run, the blocked gather of the changed sources' out-edges takes
     // its place and also computes the next gather's values (timed with the
@@ -2670,7 +2672,7 @@ struct Engine final : EngineBase {
                                                    edge_csr_ptr(), d_cand, d_bitmap, 1);
         const int64_t n_scan = (int64_t)n_words * 32;
         if (bound > kScanMinEdges)
-          k_cand_to_bits<P><<<grid_1d(n_scan, 256) < 4096 ? grid_1d(n_scan, 256) : 4096, 256, 0,
+          k_cand_to_bits<P><<<grid_1d(n_scan, 256) < 2048 ? grid_1d(n_scan, 256) : 2048, 256, 0,
                               stream>>>(d_ctr, fuse_max, n_words, d_cand, d_bitmap);
         tock(kPhPush);
       }
