@@ -562,7 +562,7 @@ __device__ __forceinline__ void label_push_rows(const int32_t* s_start, int nr, 
 
 // Push activation.  Targets below n_hub (the highest-degree vertices in the
 // degree-ordered layout) are marked with a plain byte store in hub_flags,
-// which k_hub_fold folds into the bitmap: many changed vertices share a few
+// which nextIter's compaction folds into the bitmap: many changed vertices share a few
 // hub targets, and atomics on the same word are serialised where they are
 // performed, while stores to one line merge in each L2.  The other targets
 // are marked with an atomicOr unless their bit is already visible.
@@ -627,32 +627,6 @@ __device__ __forceinline__ void label_push_rows(const int32_t* s_start, int nr, 
       if (need & (1u << m)) atomicOr(&bitmap[dst[m] >> 5], 1u << (dst[m] & 31));
     }
     __syncthreads();
-  }
-}
-
-// After a push activation: bitmap word w |= the 32 hub flag bytes of its
-// vertices, which are cleared (n_hub is a multiple of 32; the push kernel
-// marked no other vertex of these words, and no other kernel writes them now).
-[[maybe_unused]] static __global__ void k_hub_fold(const DevCounters* __restrict__ ctr, int mode, long long threshold,
-                           long long fuse_max, int n_hub, uint8_t* __restrict__ hub_flags,
-                           uint32_t* __restrict__ bitmap) {
-  const unsigned long long c = ctr->chg;
-  if (use_pull(c, mode, threshold) || fused_scatter(c, fuse_max)) return;
-  for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < n_hub / 32;
-       w += gridDim.x * blockDim.x) {
-    uint4* t = reinterpret_cast<uint4*>(hub_flags + (size_t)w * 32);
-    const uint4 a = t[0], b = t[1];
-    const uint32_t q[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-    uint32_t m = 0;
-#pragma unroll
-    for (int i = 0; i < 8; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) m |= ((q[i] >> (8 * j)) & 1u) << (4 * i + j);
-    if (m) {
-      bitmap[w] |= m;
-      t[0] = make_uint4(0, 0, 0, 0);
-      t[1] = make_uint4(0, 0, 0, 0);
-    }
   }
 }
 
@@ -1095,26 +1069,92 @@ __device__ __forceinline__ ByteWork byte_work(const uint32_t* __restrict__ bitma
   return b;
 }
 
-// Counts the frontier's vertices and in-degrees per block of kCompactNT words,
-// then moves those words into the active bitmap `act_bits` (the next
+// nextIter's compaction in one launch.  Each block of kCompactNT words first
+// folds the push pass's hub flags into its words (when the push ran: the same
+// rule as the push kernels, on the same counters; the flags are cleared),
+// counts the frontier's vertices and in-degrees, keeps its counts in
+// blk[block], moves its words into the active bitmap `act_bits` (the next
 // iteration's active set) and clears them in the frontier and in `chg_clear`
 // (the changed bitmap the next apply fills), so that scatterActivate starts
-// from an empty frontier.
+// from an empty frontier.  Blocks form groups of kApplyGroup: each adds one
+// packed word to its group (done count in bits 52 and up, vertices in bits
+// 32-51, in-degrees in bits 0-31); the block that completes a group keeps
+// the group's counts in fin.grp (for the list writer), adds them to fin.tot
+// and then counts the group done, the second atomic issued only once the
+// first has returned (its operand depends on the returned value), so that no
+// fence is needed (a device-scope fence writes L2 back on this GPU); the
+// block that completes the last group takes the totals out of fin.tot and
+// publishes them: the
+// active set's size and in-degrees, the changed-list counter's last value
+// (reset for the next apply), the applies that returned true, written to the
+// host's mapped copy and then `seq` to `host_seq`, so that the host can go on
+// before the rest of the stream has finished.
+struct HubFold {
+  uint8_t* flags = nullptr;  // nullptr: no push pass in this iteration
+  int n_hub = 0;
+  int mode = 0;
+  long long threshold = 0;
+  long long fuse_max = -1;
+};
+struct CompactFinal {
+  unsigned long long* gsum;  // [groups * kGroupStride], zero between launches
+  int2* grp;                 // [groups]
+  unsigned long long* tot;   // (vertices << 32 | in-degrees), zero between launches
+  unsigned int* gdone;       // zero between launches
+  DevCounters* ctr;
+  int32_t* escan;            // nullptr: no in-degree scan end to write
+  int mode;
+  long long threshold;
+  DevCounters* host;
+  uint32_t* host_seq;
+  uint32_t seq;
+  unsigned long long* applied;
+};
+constexpr int kCompactDoneShift = 52;
+static_assert((long long)kApplyGroup * 8 * kCompactRounds * kCompactNT <
+                  (1ll << (kCompactDoneShift - 32)),
+              "a group's vertex count fits below the done count");
+
 [[maybe_unused]] static __global__ __launch_bounds__(kCompactNT) void k_compact_count(
     uint32_t* __restrict__ bitmap, int n_words, int nV, const int32_t* __restrict__ off_in,
     int need_deg, int2* __restrict__ blk, uint32_t* __restrict__ act_bits,
-    uint32_t* __restrict__ chg_clear) {
+    uint32_t* __restrict__ chg_clear, HubFold hub, CompactFinal fin) {
   __shared__ int s_a[kCompactNT / kWave], s_b[kCompactNT / kWave];
+  __shared__ uint32_t s_word[kCompactNT];
+  __shared__ int s_last;
   // a round's 8 * kCompactNT + 1 CSC offsets, loaded coalesced when many of
   // its bytes have bits (a dense frontier): each thread's 9 overlapping
   // offsets would otherwise cost a strided load each
   constexpr int kRoundV = 8 * kCompactNT;
   __shared__ int32_t s_off[kRoundV + 1];
+  static_assert(kCompactRounds * kCompactNT / 4 == kCompactNT, "one word per thread");
+  const int wi = blockIdx.x * kCompactNT + threadIdx.x;  // this thread's word
+  uint32_t myword = wi < n_words ? bitmap[wi] : 0u;
+  if (hub.flags) {  // uniform: every block reads the same counters
+    const unsigned long long c = fin.ctr->chg;
+    if (!(use_pull(c, hub.mode, hub.threshold) || fused_scatter(c, hub.fuse_max)) &&
+        wi < hub.n_hub / 32) {
+      uint4* t = reinterpret_cast<uint4*>(hub.flags + (size_t)wi * 32);
+      const uint4 a = t[0], b = t[1];
+      const uint32_t q[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+      uint32_t m = 0;
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) m |= ((q[i] >> (8 * j)) & 1u) << (4 * i + j);
+      if (m) {
+        myword |= m;
+        t[0] = make_uint4(0, 0, 0, 0);
+        t[1] = make_uint4(0, 0, 0, 0);
+      }
+    }
+  }
+  s_word[threadIdx.x] = myword;
+  __syncthreads();
   int cnt = 0, deg = 0;
   for (int r = 0; r < kCompactRounds; ++r) {
     const int bi = (blockIdx.x * kCompactRounds + r) * kCompactNT + threadIdx.x;
-    const int wi = bi >> 2;
-    const uint32_t word = (wi < n_words) ? bitmap[wi] : 0u;
+    const uint32_t word = s_word[r * (kCompactNT / 4) + threadIdx.x / 4];
     const uint32_t bits = (word >> ((bi & 3) * 8)) & 0xFFu;
     cnt += __popc(bits);
     if (!need_deg) continue;
@@ -1148,7 +1188,14 @@ __device__ __forceinline__ ByteWork byte_work(const uint32_t* __restrict__ bitma
     s_a[threadIdx.x / kWave] = cnt;
     s_b[threadIdx.x / kWave] = deg;
   }
-  __syncthreads();  // also: every thread has read its bytes before the words move
+  if (wi < n_words) {
+    act_bits[wi] = myword;
+    bitmap[wi] = 0u;
+    if (chg_clear) chg_clear[wi] = 0u;
+  }
+  __syncthreads();
+  const int nb = (int)gridDim.x;
+  const int n_groups = (nb + kApplyGroup - 1) / kApplyGroup;
   if (threadIdx.x == 0) {
     int a = 0, b = 0;
     for (int i = 0; i < kCompactNT / kWave; ++i) {
@@ -1156,62 +1203,72 @@ __device__ __forceinline__ ByteWork byte_work(const uint32_t* __restrict__ bitma
       b += s_b[i];
     }
     blk[blockIdx.x] = make_int2(a, b);
+    const int grp = (int)blockIdx.x / kApplyGroup;
+    const unsigned members = (unsigned)min(kApplyGroup, nb - grp * kApplyGroup);
+    const unsigned long long mine = (1ull << kCompactDoneShift) |
+                                    ((unsigned long long)(uint32_t)a << 32) | (uint32_t)b;
+    const unsigned long long old = atomicAdd(&fin.gsum[grp * kGroupStride], mine);
+    int last = 0;
+    if ((unsigned)(old >> kCompactDoneShift) == members - 1) {
+      const unsigned long long g = (old + mine) & ((1ull << kCompactDoneShift) - 1);
+      fin.gsum[grp * kGroupStride] = 0ull;  // every block of the group has added
+      fin.grp[grp] = make_int2((int)(g >> 32), (int)(uint32_t)g);
+      // (vertices < 2^31: bit 63 of the returned total is 0, but the done
+      // count's add depends on it and waits for the first add to return)
+      const unsigned long long t = atomicAdd(fin.tot, g);
+      last = atomicAdd(fin.gdone, 1u + (unsigned)(t >> 63)) == (unsigned)(n_groups - 1) ? 1 : 0;
+    }
+    s_last = last;
   }
-  // the block's bytes are words blockIdx.x * kCompactNT ... + kCompactNT - 1
-  static_assert(kCompactRounds * kCompactNT / 4 == kCompactNT, "one word per thread");
-  const int wi = blockIdx.x * kCompactNT + threadIdx.x;
-  if (wi < n_words) {
-    act_bits[wi] = bitmap[wi];
-    bitmap[wi] = 0u;
-    if (chg_clear) chg_clear[wi] = 0u;
+  __syncthreads();
+  if (!s_last) return;  // uniform
+  unsigned long long my_applied = 0;
+  if (threadIdx.x < kAppliedSlots) {
+    my_applied = fin.applied[threadIdx.x * kAppliedStride];
+    fin.applied[threadIdx.x * kAppliedStride] = 0;
+  }
+  my_applied = wave_reduce_add(my_applied);  // slots 0..63: wave 0
+  if (threadIdx.x == 0) {
+    // every group's add was performed before its done count
+    const unsigned long long t = atomicExch(fin.tot, 0ull);
+    const long long ta = (long long)(t >> 32), tb = (long long)(uint32_t)t;
+    DevCounters* ctr = fin.ctr;
+    *fin.gdone = 0u;
+    ctr->n_active = (int32_t)ta;
+    ctr->w_edges = (int32_t)tb;
+    if (fin.escan) fin.escan[ta] = (int32_t)tb;
+    ctr->chg_last = ctr->chg;
+    ctr->pull = use_pull(ctr->chg, fin.mode, fin.threshold) && !ctr->fused ? 1 : 0;
+    ctr->chg = 0;
+    ctr->applied_last = my_applied;
+    *fin.host = *ctr;
+    __threadfence_system();
+    __hip_atomic_store(fin.host_seq, fin.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
-// Single workgroup: exclusive scan of the per-block counts; publishes the
-// totals, resets the changed-list counter for the next apply and writes the
-// counters to the host's mapped copy (`host`; no separate device-to-host
-// copy), then `seq` to `host_seq` once they are visible to the host, so that
-// the host can go on before the rest of the stream has finished.
-[[maybe_unused]] static __global__ __launch_bounds__(1024) void k_compact_scan(int2* __restrict__ blk, int nb,
-                                                       DevCounters* __restrict__ ctr,
-                                                       int32_t* __restrict__ escan, int mode,
-                                                       long long threshold,
-                                                       DevCounters* __restrict__ host,
-                                                       uint32_t* __restrict__ host_seq,
-                                                       uint32_t seq,
-                                                       unsigned long long* __restrict__ applied) {
-  __shared__ int s_a[1024 / kWave], s_b[1024 / kWave];
-  __shared__ unsigned long long s_applied;
-  unsigned long long my_applied = 0;
-  if (threadIdx.x < kAppliedSlots) {
-    my_applied = applied[threadIdx.x * kAppliedStride];
-    applied[threadIdx.x * kAppliedStride] = 0;
+// The exclusive (vertices, in-degrees) before compaction block b: the groups
+// before its group and the blocks of its group before it.  Wave 0 computes
+// it into *out; every thread of the block calls this.
+__device__ __forceinline__ void compact_block_base(const int2* __restrict__ blk,
+                                                   const int2* __restrict__ grp, int b,
+                                                   int2* out) {
+  if (threadIdx.x < kWave) {
+    const int g = b / kApplyGroup;
+    long long a = 0, d = 0;
+    for (int i = threadIdx.x; i < g; i += kWave) {
+      a += grp[i].x;
+      d += grp[i].y;
+    }
+    for (int i = g * kApplyGroup + (int)threadIdx.x; i < b; i += kWave) {
+      a += blk[i].x;
+      d += blk[i].y;
+    }
+    a = wave_reduce_add(a);
+    d = wave_reduce_add(d);
+    if (threadIdx.x == 0) *out = make_int2((int)a, (int)d);
   }
-  my_applied = wave_reduce_add(my_applied);  // slots 0..63: wave 0
-  if (threadIdx.x == 0) s_applied = my_applied;
-  int run_a = 0, run_b = 0;
-  for (int base = 0; base < nb; base += 1024) {
-    const int i = base + threadIdx.x;
-    const int2 v = (i < nb) ? blk[i] : make_int2(0, 0);
-    int ta, tb;
-    const int ea = block_excl_scan_add<1024>(v.x, s_a, &ta);
-    const int eb = block_excl_scan_add<1024>(v.y, s_b, &tb);
-    if (i < nb) blk[i] = make_int2(run_a + ea, run_b + eb);
-    run_a += ta;
-    run_b += tb;
-  }
-  if (threadIdx.x == 0) {
-    ctr->n_active = run_a;
-    ctr->w_edges = run_b;
-    if (escan) escan[run_a] = run_b;
-    ctr->chg_last = ctr->chg;
-    ctr->pull = use_pull(ctr->chg, mode, threshold) && !ctr->fused ? 1 : 0;
-    ctr->chg = 0;
-    ctr->applied_last = s_applied;
-    *host = *ctr;
-    __threadfence_system();
-    __hip_atomic_store(host_seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-  }
+  __syncthreads();
 }
 
 // Writes the ascending active list of the active bitmap (as k_compact_count
@@ -1220,18 +1277,18 @@ __device__ __forceinline__ ByteWork byte_work(const uint32_t* __restrict__ bitma
 // of pending candidates, apply after such a gather).
 [[maybe_unused]] static __global__ __launch_bounds__(kCompactNT) void k_compact_write(
     const uint32_t* __restrict__ bitmap, int n_words, int nV, const int32_t* __restrict__ off_in,
-    int need_deg, const int2* __restrict__ blk, const DevCounters* __restrict__ ctr,
+    int need_deg, const int2* __restrict__ blk, const int2* __restrict__ grp,
     int32_t* __restrict__ active, int32_t* __restrict__ escan, int32_t* __restrict__ cbase) {
   // a round's entries (at most 8 per thread) are staged here and written out
   // coalesced: each thread's own entries are consecutive, a wave's are not
   constexpr int kRoundMax = 8 * kCompactNT;
   __shared__ unsigned long long s_scan[kCompactNT / kWave];
   __shared__ int32_t s_act[kRoundMax], s_esc[kRoundMax], s_cb[kRoundMax];
-  const int2 base = blk[blockIdx.x];
-  // a block without active bits (its count is the gap to the next block's
-  // offset) has nothing to write
-  const int next = blockIdx.x + 1 < (int)gridDim.x ? blk[blockIdx.x + 1].x : ctr->n_active;
-  if (next == base.x) return;
+  // a block without active bits has nothing to write
+  if (blk[blockIdx.x].x == 0) return;  // uniform
+  __shared__ int2 s_base;
+  compact_block_base(blk, grp, (int)blockIdx.x, &s_base);
+  const int2 base = s_base;
   int pos_run = base.x, deg_run = base.y;
   for (int r = 0; r < kCompactRounds; ++r) {
     const int bi = (blockIdx.x * kCompactRounds + r) * kCompactNT + threadIdx.x;
@@ -1533,6 +1590,13 @@ struct Engine final : EngineBase {
   uint32_t* d_chg_bits = nullptr;  // vertices changed by this iteration's apply
   uint32_t* d_chg_prev = nullptr;  // ... by the previous iteration's apply
   int2* d_blk = nullptr;
+  // nextIter's compaction: per-group sums (zero between launches), group
+  // counts, groups done (zero between launches)
+  unsigned long long* d_cgsum = nullptr;
+  int2* d_cgrp = nullptr;
+  unsigned long long* d_cgtot = nullptr;
+  unsigned int* d_cgdone = nullptr;
+  HubFold hub;  // hub marks a push left for the next compaction to fold
   int32_t* d_part = nullptr;
   int32_t* d_push_part = nullptr;
   Part<G>* d_carry = nullptr;
@@ -1609,7 +1673,8 @@ struct Engine final : EngineBase {
                     d_vd_user,   d_ed,     d_ed_csr,   d_cand,   d_active, d_escan,
                     d_gtmp,      d_chg_v,  d_chg_e,    d_bitmap, d_chg_bits, d_chg_prev,
                     d_blk,       d_part,   d_push_part, d_carry, d_head,   d_ctr,
-                    d_applied,   d_act_bits, d_cc,     d_sc,     d_sc_final};
+                    d_applied,   d_act_bits, d_cc,     d_sc,     d_sc_final, d_cgsum,
+                    d_cgrp,      d_cgtot,  d_cgdone};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     if (h_ctr) (void)hipHostFree(h_ctr);
@@ -1664,7 +1729,10 @@ struct Engine final : EngineBase {
     d_active = d_escan = d_chg_v = d_chg_e = d_part = d_push_part = nullptr;
     d_gtmp = nullptr;
     d_bitmap = d_chg_bits = d_chg_prev = d_act_bits = nullptr;
-    d_cc = d_sc = d_sc_final = nullptr;
+    d_cc = d_sc = d_sc_final = d_cgsum = d_cgtot = nullptr;
+    d_cgrp = nullptr;
+    d_cgdone = nullptr;
+    hub = HubFold();
     chg_pending = chg_cond = false;
     list_valid = true;
     d_blk = nullptr;
@@ -1742,6 +1810,17 @@ struct Engine final : EngineBase {
       GR_HIP(hipMemsetAsync(d_sc, 0, sizeof(unsigned long long) * groups * kGroupStride, stream));
     }
     GR_TRY(dalloc(&d_blk, n_cblocks));
+    {
+      const int64_t groups = (n_cblocks + kApplyGroup - 1) / kApplyGroup;
+      GR_TRY(dalloc(&d_cgsum, groups * kGroupStride));
+      GR_TRY(dalloc(&d_cgrp, groups));
+      GR_TRY(dalloc(&d_cgtot, 1));
+      GR_TRY(dalloc(&d_cgdone, 1));
+      GR_HIP(hipMemsetAsync(d_cgtot, 0, sizeof(unsigned long long), stream));
+      GR_HIP(hipMemsetAsync(d_cgsum, 0, sizeof(unsigned long long) * groups * kGroupStride,
+                            stream));
+      GR_HIP(hipMemsetAsync(d_cgdone, 0, sizeof(unsigned int), stream));
+    }
     GR_TRY(dalloc(&d_part, max_tiles + 2));
     GR_TRY(dalloc(&d_push_part, max_push_tiles + 2));
     GR_TRY(dalloc(&d_carry, max_tiles + 1));
@@ -2395,6 +2474,10 @@ struct Engine final : EngineBase {
   // (the active set was replaced or the vertex data reset): back to gatherZero.
   int drop_candidates() {
     GR_TRY(ensure_chg_list());  // before the changed bitmaps are cleared
+    if (hub.flags) {  // hub marks of a push whose nextIter will not run
+      GR_HIP(hipMemsetAsync(d_hub_flags, 0, (size_t)n_hub, stream));
+      hub = HubFold();
+    }
     GR_TRY(collect_flush());
     fab_pending = fab_launched = false;
     last_gather_blocked = false;
@@ -2646,9 +2729,15 @@ struct Engine final : EngineBase {
         k_push_tiles<<<push_grid, kNT, 0, stream>>>(d_ctr, dir_mode, thr, fuse_max, d_push_part,
                                                d_chg_v, d_chg_e, g->off_out, g->dsts, d_bitmap,
                                                d_hub_flags, n_hub);
-        if (n_hub > 0)
-          k_hub_fold<<<grid_1d(n_hub / 32, 256), 256, 0, stream>>>(d_ctr, dir_mode, thr, fuse_max,
-                                                                   n_hub, d_hub_flags, d_bitmap);
+        // nextIter's compaction folds the hub marks into the frontier (under
+        // the rule the push ran by)
+        if (n_hub > 0) {
+          hub.flags = d_hub_flags;
+          hub.n_hub = n_hub;
+          hub.mode = dir_mode;
+          hub.threshold = thr;
+          hub.fuse_max = fuse_max;
+        }
       }
     }
     if constexpr (kScatter) {
@@ -2685,13 +2774,24 @@ struct Engine final : EngineBase {
     GR_TRY(collect_flush());
     const int need_deg = P::kHasGather ? 1 : 0;
     tick();
+    ++ctr_seq;
+    CompactFinal fin;
+    fin.gsum = d_cgsum;
+    fin.grp = d_cgrp;
+    fin.tot = d_cgtot;
+    fin.gdone = d_cgdone;
+    fin.ctr = d_ctr;
+    fin.escan = need_deg ? d_escan : nullptr;
+    fin.mode = dir_mode;
+    fin.threshold = pull_threshold();
+    fin.host = d_hctr;
+    fin.host_seq = d_hseq;
+    fin.seq = ctr_seq;
+    fin.applied = d_applied;
     k_compact_count<<<n_cblocks, kCompactNT, 0, stream>>>(
         d_bitmap, n_words, g->nV, g->off_in, need_deg, d_blk, d_act_bits,
-        changed_bits_on() ? d_chg_prev : nullptr);
-    ++ctr_seq;
-    k_compact_scan<<<1, 1024, 0, stream>>>(d_blk, n_cblocks, d_ctr, need_deg ? d_escan : nullptr,
-                                           dir_mode, pull_threshold(), d_hctr, d_hseq, ctr_seq,
-                                           d_applied);
+        changed_bits_on() ? d_chg_prev : nullptr, hub, fin);
+    hub = HubFold();
     list_valid = false;
     tock(kPhNext);
     GR_HIP(hipGetLastError());
@@ -2733,7 +2833,7 @@ struct Engine final : EngineBase {
     return GR_OK;
   }
 
-  // Waits until k_compact_scan has published counters number `want` to h_ctr.
+  // Waits until k_compact_count has published counters number `want` to h_ctr.
   // The host polls the sequence number instead of synchronising the stream:
   // it learns the next frontier's size as soon as the scan has published it
   // and queues the next iteration's kernels behind it.  The polling is
@@ -2761,7 +2861,7 @@ struct Engine final : EngineBase {
     list_valid = true;
     if (n_active == 0) return GR_OK;
     k_compact_write<<<n_cblocks, kCompactNT, 0, stream>>>(d_act_bits, n_words, g->nV, g->off_in,
-                                                          P::kHasGather ? 1 : 0, d_blk, d_ctr,
+                                                          P::kHasGather ? 1 : 0, d_blk, d_cgrp,
                                                           d_active, d_escan, d_cbase);
     GR_HIP(hipGetLastError());
     return GR_OK;
