@@ -2218,6 +2218,7 @@ struct Engine final : EngineBase {
     GR_TRY(drop_candidates());
     const int n = end - start;
     if (d_inv && n < g->nV) return set_active_list(start, end);
+    GR_TRY(drop_hub_marks());
     GR_HIP(hipMemsetAsync(d_bitmap, 0, sizeof(uint32_t) * (n_words + 1), stream));
     GR_HIP(hipMemsetAsync(d_chg_bits, 0, sizeof(uint32_t) * (n_words + 1), stream));
     GR_HIP(hipMemsetAsync(d_chg_prev, 0, sizeof(uint32_t) * (n_words + 1), stream));
@@ -2258,6 +2259,7 @@ struct Engine final : EngineBase {
     }
     escan[n] = (int32_t)acc;
     cbase[n] = 0;
+    GR_TRY(drop_hub_marks());
     GR_HIP(hipMemsetAsync(d_bitmap, 0, sizeof(uint32_t) * (n_words + 1), stream));
     GR_HIP(hipMemsetAsync(d_chg_bits, 0, sizeof(uint32_t) * (n_words + 1), stream));
     GR_HIP(hipMemsetAsync(d_chg_prev, 0, sizeof(uint32_t) * (n_words + 1), stream));
@@ -2472,12 +2474,18 @@ struct Engine final : EngineBase {
 
   // Candidates relaxed by a fused scatterActivate whose gather will not run
   // (the active set was replaced or the vertex data reset): back to gatherZero.
-  int drop_candidates() {
-    GR_TRY(ensure_chg_list());  // before the changed bitmaps are cleared
-    if (hub.flags) {  // hub marks of a push whose nextIter will not run
+  // Hub marks of a push whose frontier is being cleared (setActive): the
+  // next compaction must not fold them in.
+  int drop_hub_marks() {
+    if (hub.flags) {
       GR_HIP(hipMemsetAsync(d_hub_flags, 0, (size_t)n_hub, stream));
       hub = HubFold();
     }
+    return GR_OK;
+  }
+
+  int drop_candidates() {
+    GR_TRY(ensure_chg_list());  // before the changed bitmaps are cleared
     GR_TRY(collect_flush());
     fab_pending = fab_launched = false;
     last_gather_blocked = false;
