@@ -327,9 +327,9 @@ __global__ __launch_bounds__(kApplyNT) void k_apply(
 // staged in LDS and written out coalesced.
 constexpr int kChgNT = kApplyChunk / 8;
 //
-// standdown != 0: scatterActivate's blocked gather will mark the next active
-// set in place of the pull (the same rule on the same counters) and no phase
-// of this iteration or the next one reads the list: nothing is written.
+// standdown != 0: when scatterActivate's pull (or the blocked gather in its
+// place) runs, by the same rule on the same counters, no phase of this
+// iteration or the next one reads the list: nothing is written then.
 [[maybe_unused]] static __global__ __launch_bounds__(kChgNT) void k_chg_write(
     const uint32_t* __restrict__ chg_bits, int nV, const int32_t* __restrict__ off_out,
     const unsigned long long* __restrict__ cc, const unsigned long long* __restrict__ sc_final,
@@ -997,7 +997,9 @@ __device__ __forceinline__ uint64_t lane_range_mask(int lo, int hi) {  // bits [
 // in-edges marked already are skipped.  The walk's cost is its random
 // changed-bitmap reads: at RMAT-24 the heavy rows' first hits sit at a median
 // of about 250 in-edges (at most about 2,500), so a wave reads little past
-// the hit and the longest walk takes few rounds.  Row bounds are scalars.
+// the hit and the longest walk takes few rounds.  (A walk from both ends
+// inwards was slower: 104 -> 122 us in BFS's dense pull.)  Row bounds are
+// scalars.
 [[maybe_unused]] static __global__ __launch_bounds__(256) void k_pull_heavy(
     const DevCounters* __restrict__ ctr, int mode, long long threshold, long long fuse_max,
     const int32_t* __restrict__ heavy, int n_heavy, const int32_t* __restrict__ off_in,
@@ -2697,13 +2699,15 @@ struct Engine final : EngineBase {
       }
     }
     // The push, relax and scatter kernels below read the changed list; where
-    // the blocked gather runs in place of the pull they stand down, and the
-    // next gather is that gather's values: the list is then not needed
-    // (unless Program::scatter runs).
+    // the pull (or the blocked gather in its place) runs they stand down.
+    // After the blocked gather the next gather is that gather's values, and
+    // programs without the delta rule never push a gather: the list is then
+    // not needed (unless Program::scatter runs).
     {
       bool need_always = false;
       if constexpr (kScatter) need_always = have_scatter;
-      GR_TRY(ensure_chg_list(fab && !need_always, dir_mode, thr, fuse_max));
+      const bool may_skip = fab || (!kDelta && may_pull);
+      GR_TRY(ensure_chg_list(may_skip && !need_always, dir_mode, thr, fuse_max));
     }
     if constexpr (kBlockedMin) {
       if (fab) {
