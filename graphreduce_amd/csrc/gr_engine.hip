@@ -131,13 +131,20 @@ extern "C" int gr_engine_get_results(gr_engine* e) {
 extern "C" int gr_engine_run(gr_engine* e, int32_t* iterations) {
   GR_TRY(check(e));
   int32_t it = 0;
+  int rc = GR_OK;
+  e->impl->run_mode = true;
   while (e->impl->n_active) {
-    GR_TRY(e->impl->gather());
-    GR_TRY(e->impl->apply());
-    GR_TRY(e->impl->scatter_activate(true));
-    GR_TRY(e->impl->next_iter());
+    if ((rc = e->impl->gather()) != GR_OK) break;
+    if ((rc = e->impl->apply()) != GR_OK) break;
+    if ((rc = e->impl->scatter_activate(true)) != GR_OK) break;
+    if ((rc = e->impl->next_iter()) != GR_OK) break;
     ++it;
   }
+  e->impl->run_mode = false;
+  // an apply queued by the last nextIter ran over an empty active set
+  const int rc_end = e->impl->end_run();
+  GR_TRY(rc);
+  GR_TRY(rc_end);
   if (iterations) *iterations = it;
   return GR_OK;
 }

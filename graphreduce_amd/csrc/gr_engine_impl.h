@@ -1482,11 +1482,18 @@ struct EngineBase {
   virtual int sync_stats(gr_engine_stats* out) = 0;
   virtual size_t vertex_size() const = 0;
   virtual size_t edge_size() const = 0;  // 0 for an empty EdgeData
+  // after gr_engine_run's loop: drops an apply its last nextIter queued (it
+  // ran over the empty active set the loop ended on)
+  virtual int end_run() { return GR_OK; }
 
   int32_t n_active = 0;
   int32_t w_edges = 0;
   int32_t iteration = 0;
   bool bound = false;
+  // set by gr_engine_run for the length of its loop: no caller sees the
+  // state between nextIter and the next apply, so nextIter may queue that
+  // apply before the host has the next frontier's size
+  bool run_mode = false;
   int dir_mode = 0;
   double dir_alpha = 15.0;
   int gather_vt = 8;  // merge-path items per thread in the gather tiles (4, 8 or 16)
@@ -1589,6 +1596,7 @@ struct Engine final : EngineBase {
   int n_apply_chunks = 0;
   bool chg_pending = false;
   bool chg_cond = false;  // written only if the blocked gather did not run
+  bool pre_applied = false;  // nextIter queued the next iteration's apply
   uint32_t* d_chg_bits = nullptr;  // vertices changed by this iteration's apply
   uint32_t* d_chg_prev = nullptr;  // ... by the previous iteration's apply
   int2* d_blk = nullptr;
@@ -2554,6 +2562,15 @@ struct Engine final : EngineBase {
 
   int apply() override {
     it_apply = true;
+    if (pre_applied) {  // queued by nextIter (run loop): the same launch
+      pre_applied = false;
+      if (n_active == 0) return GR_OK;
+      ++n_applies;
+      collect_pending = false;
+      dense_pending = false;
+      chg_pending = true;
+      return GR_OK;
+    }
     if (n_active == 0) return GR_OK;
     ++n_applies;
     const int use_gather = !P::kHasGather ? 0 : collect_pending ? 2 : dense_pending ? 3 : 1;
@@ -2590,6 +2607,11 @@ struct Engine final : EngineBase {
     if (defer) chg_pending = true;
     tock(kPhApply);
     GR_HIP(hipGetLastError());
+    return GR_OK;
+  }
+
+  int end_run() override {
+    pre_applied = false;
     return GR_OK;
   }
 
@@ -2818,6 +2840,26 @@ struct Engine final : EngineBase {
     delta_chain = delta_chain && it_gather && it_apply && it_scatter;
     delta_ready = kDelta && delta_chain && changed_bits_on();
     it_gather = it_apply = it_scatter = false;
+    if constexpr (kBlockedSum) {
+      // In the run loop after the fused blocked sum, the next gather only
+      // hands its values on and the next apply is the bitmap form over them
+      // whatever the frontier's size (an empty one leaves every chunk
+      // unread): queue it now, so that the GPU does not wait for the host to
+      // learn the size first.
+      if (run_mode && fab_pending && P::kHasGather && changed_bits_on() && !streaming) {
+        tick();
+        ApplyDefer ad;
+        ad.cc = d_cc;
+        ad.sc = d_sc;
+        ad.sc_final = d_sc_final;
+        k_apply<P, true><<<grid_1d(g->nV, kApplyChunk), kApplyNT, 0, stream>>>(
+            g->nV, nullptr, d_act_bits, d_gtmp, 3, d_vd, g->off_out, iteration + 1, d_chg_v,
+            d_chg_e, d_chg_bits, d_ctr, d_srcval, d_gdense, d_applied, ad);
+        tock(kPhApply);
+        GR_HIP(hipGetLastError());
+        pre_applied = true;
+      }
+    }
     GR_TRY(wait_counters(ctr_seq));
     if constexpr (kDelta) {
       const bool fused = fuse_launched && h_ctr->fused;
