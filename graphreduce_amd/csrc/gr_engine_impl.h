@@ -118,6 +118,36 @@ __device__ __forceinline__ void defer_chunk_done(const ApplyDefer& d, int c, int
   }
 }
 
+// The kept source value of an apply's entry.  Without flag_prev it is
+// rewritten where the vertex data changed.  With flag_prev (programs with
+// source_flag_v, bitmap form): its sign bit is set where the apply returned
+// true with out-edges (`want`) and cleared where the previous apply set it
+// (flag_prev, that apply's changed bitmap) and this one does not, so that
+// the flags are exactly this apply's changed set afterwards.
+template <class P, bool kBits>
+__device__ __forceinline__ void keep_source_value(source_value_t<P>* __restrict__ srcval,
+                                                  const uint32_t* __restrict__ flag_prev,
+                                                  const uint32_t* s_pwords, int e, int n, int chunk,
+                                                  int v, const typename P::VertexData& vs,
+                                                  bool valid, bool changed, bool want) {
+  if constexpr (source_value_v<P>) {
+    if constexpr (source_flag_v<P> && kBits) {
+      if (flag_prev) {
+        const int i = chunk + e;
+        const bool was = i < n && ((s_pwords[e >> 5] >> (i & 31)) & 1u);
+        if (valid && (changed || want || was)) {
+          const float sv = P::sourceValue(vs);
+          srcval[v] = want ? flagged(sv) : sv;
+        } else if (was) {
+          srcval[v] = unflag(srcval[v]);
+        }
+        return;
+      }
+    }
+    if (changed) srcval[v] = P::sourceValue(vs);
+  }
+}
+
 // Program::apply over the active set: the ascending active list (kBits
 // false: entry i is vertex active[i]) or the active bitmap (kBits true: entry
 // i is vertex i, taken when its bit is set; n = nV).  Each workgroup takes a
@@ -141,11 +171,13 @@ __global__ __launch_bounds__(kApplyNT) void k_apply(
     int iteration, int32_t* __restrict__ chg_v, int32_t* __restrict__ chg_e,
     uint32_t* __restrict__ chg_bits, DevCounters* __restrict__ ctr,
     source_value_t<P>* __restrict__ srcval, typename P::GatherResult* __restrict__ cand,
-    unsigned long long* __restrict__ applied, ApplyDefer defer) {
+    unsigned long long* __restrict__ applied, ApplyDefer defer,
+    const uint32_t* __restrict__ flag_prev = nullptr) {
   constexpr int kNW = kApplyNT / kWave;
   constexpr int kWords = kApplyChunk / 32;
   static_assert(kWords <= kApplyNT, "one bitmap word per thread");
   __shared__ uint32_t s_words[kWords];
+  __shared__ uint32_t s_pwords[kWords];  // flag_prev words of the chunk
   __shared__ int s_cnt[kApplyVT * kNW], s_deg[kApplyVT * kNW];
   __shared__ unsigned long long s_base;
   __shared__ unsigned s_applied;
@@ -156,13 +188,17 @@ __global__ __launch_bounds__(kApplyNT) void k_apply(
     if constexpr (kBits) {
       // the chunk's bitmap words; a chunk without active vertices is skipped
       // (uniform: every thread sees the same barrier result)
-      uint32_t word = 0u;
+      uint32_t word = 0u, pword = 0u;
       if (tid < kWords) {
         const int wi = (chunk >> 5) + tid;
         word = wi * 32 < n ? act_bits[wi] : 0u;
         s_words[tid] = word;
+        if (flag_prev) {
+          pword = wi * 32 < n ? flag_prev[wi] : 0u;
+          s_pwords[tid] = pword;
+        }
       }
-      if (!__syncthreads_or(word != 0u)) {
+      if (!__syncthreads_or((word | pword) != 0u)) {
         if (defer.cc && tid == 0)
           defer_chunk_done(defer, chunk / kApplyChunk, (n + kApplyChunk - 1) / kApplyChunk, 0ull,
                            ctr);
@@ -239,11 +275,11 @@ __global__ __launch_bounds__(kApplyNT) void k_apply(
       unsigned long long mine = 0ull;
 #pragma unroll
       for (int m = 0; m < kApplyVT; ++m) {
-        if (chg_mask & (1u << m)) {
-          vd[dvs[m]] = vs[m];
-          if constexpr (source_value_v<P>) srcval[dvs[m]] = P::sourceValue(vs[m]);
-        }
         const bool want = degs[m] > 0;
+        if (chg_mask & (1u << m)) vd[dvs[m]] = vs[m];
+        keep_source_value<P, kBits>(srcval, flag_prev, s_pwords, m * kApplyNT + tid, n, chunk,
+                                    dvs[m], vs[m], (valid_mask >> m) & 1u, (chg_mask >> m) & 1u,
+                                    want);
         wave_set_bits_sorted(chunk + m * kApplyNT + tid < n, want, dvs[m], chg_bits);
         if (want) mine += (1ull << 32) | (unsigned long long)(uint32_t)degs[m];
       }
@@ -261,11 +297,11 @@ __global__ __launch_bounds__(kApplyNT) void k_apply(
     }
 #pragma unroll
     for (int m = 0; m < kApplyVT; ++m) {
-      if (chg_mask & (1u << m)) {
-        vd[dvs[m]] = vs[m];
-        if constexpr (source_value_v<P>) srcval[dvs[m]] = P::sourceValue(vs[m]);
-      }
       const bool want = degs[m] > 0;  // apply returned true and out-edges exist
+      if (chg_mask & (1u << m)) vd[dvs[m]] = vs[m];
+      keep_source_value<P, kBits>(srcval, flag_prev, s_pwords, m * kApplyNT + tid, n, chunk,
+                                  dvs[m], vs[m], (valid_mask >> m) & 1u, (chg_mask >> m) & 1u,
+                                  want);
       if (chg_bits) wave_set_bits_sorted(chunk + m * kApplyNT + tid < n, want, dvs[m], chg_bits);
       if (want) want_mask |= 1u << m;
       ball[m] = __ballot(want);
@@ -1597,6 +1633,11 @@ struct Engine final : EngineBase {
   bool chg_pending = false;
   bool chg_cond = false;  // written only if the blocked gather did not run
   bool pre_applied = false;  // nextIter queued the next iteration's apply
+  // source_flag_v programs: the sign bits of d_srcval hold at most the last
+  // flag-keeping apply's changed set (src_flags_ok), exactly it (gather_flags:
+  // the fused blocked sum may read them instead of the changed bitmap)
+  bool src_flags_ok = true;
+  bool gather_flags = false;
   uint32_t* d_chg_bits = nullptr;  // vertices changed by this iteration's apply
   uint32_t* d_chg_prev = nullptr;  // ... by the previous iteration's apply
   int2* d_blk = nullptr;
@@ -2229,6 +2270,7 @@ struct Engine final : EngineBase {
     const int n = end - start;
     if (d_inv && n < g->nV) return set_active_list(start, end);
     GR_TRY(drop_hub_marks());
+    src_flags_ok = gather_flags = false;  // the changed bitmaps are cleared below
     GR_HIP(hipMemsetAsync(d_bitmap, 0, sizeof(uint32_t) * (n_words + 1), stream));
     GR_HIP(hipMemsetAsync(d_chg_bits, 0, sizeof(uint32_t) * (n_words + 1), stream));
     GR_HIP(hipMemsetAsync(d_chg_prev, 0, sizeof(uint32_t) * (n_words + 1), stream));
@@ -2270,6 +2312,7 @@ struct Engine final : EngineBase {
     escan[n] = (int32_t)acc;
     cbase[n] = 0;
     GR_TRY(drop_hub_marks());
+    src_flags_ok = gather_flags = false;  // the changed bitmaps are cleared below
     GR_HIP(hipMemsetAsync(d_bitmap, 0, sizeof(uint32_t) * (n_words + 1), stream));
     GR_HIP(hipMemsetAsync(d_chg_bits, 0, sizeof(uint32_t) * (n_words + 1), stream));
     GR_HIP(hipMemsetAsync(d_chg_prev, 0, sizeof(uint32_t) * (n_words + 1), stream));
@@ -2596,10 +2639,12 @@ struct Engine final : EngineBase {
       ad.sc = d_sc;
       ad.sc_final = d_sc_final;
     }
+    const uint32_t* fprev = nullptr;
+    GR_TRY(source_flags_for_apply(defer, &fprev));
     if (bits)
       k_apply<P, true><<<grid_1d(g->nV, kApplyChunk), kApplyNT, 0, stream>>>(
           g->nV, nullptr, d_act_bits, d_gtmp, use_gather, d_vd, g->off_out, iteration, d_chg_v,
-          d_chg_e, cbits, d_ctr, d_srcval, cv, d_applied, ad);
+          d_chg_e, cbits, d_ctr, d_srcval, cv, d_applied, ad, fprev);
     else
       k_apply<P, false><<<grid_1d(n_active, kApplyChunk), kApplyNT, 0, stream>>>(
           n_active, d_active, nullptr, d_gtmp, use_gather, d_vd, g->off_out, iteration, d_chg_v,
@@ -2612,6 +2657,28 @@ struct Engine final : EngineBase {
 
   int end_run() override {
     pre_applied = false;
+    return GR_OK;
+  }
+
+  // Source flags for an apply: a flag-keeping apply (the first of an
+  // iteration, bitmap form, with changed bitmaps) clears the previous
+  // apply's flags that it does not set again, which needs the flags to be
+  // at most that apply's changed set (d_chg_prev): otherwise the kept values
+  // are rewritten clean first.  Any other apply leaves flags behind that the
+  // fused blocked sum must not read.
+  int source_flags_for_apply(bool keeps, const uint32_t** fprev) {
+    *fprev = nullptr;
+    if constexpr (source_flag_v<P>) {
+      if (keeps && changed_bits_on() && !streaming) {
+        if (!src_flags_ok) GR_TRY(refresh_source_values());
+        *fprev = d_chg_prev;
+        src_flags_ok = gather_flags = true;
+      } else {
+        src_flags_ok = gather_flags = false;
+      }
+    } else {
+      (void)keeps;
+    }
     return GR_OK;
   }
 
@@ -2672,7 +2739,7 @@ struct Engine final : EngineBase {
         if (blk->n_blocks > 0)
           k_gather_blocked_sum<P><<<blk->n_blocks, kBlkNT, 0, stream>>>(
               blk->brow, blk->bnr, blk->boff, blk->bpk, blk->gbase, d_ed_blk, d_vd, d_srcval,
-              d_gdense, d_chg_bits, d_bitmap, d_ctr);
+              d_gdense, d_chg_bits, d_bitmap, d_ctr, gather_flags ? 1 : 0);
         tock(kPhGather);
         GR_HIP(hipGetLastError());
         fab_pending = true;
@@ -2847,6 +2914,8 @@ struct Engine final : EngineBase {
       // unread): queue it now, so that the GPU does not wait for the host to
       // learn the size first.
       if (run_mode && fab_pending && P::kHasGather && changed_bits_on() && !streaming) {
+        const uint32_t* fprev = nullptr;
+        GR_TRY(source_flags_for_apply(true, &fprev));
         tick();
         ApplyDefer ad;
         ad.cc = d_cc;
@@ -2854,7 +2923,7 @@ struct Engine final : EngineBase {
         ad.sc_final = d_sc_final;
         k_apply<P, true><<<grid_1d(g->nV, kApplyChunk), kApplyNT, 0, stream>>>(
             g->nV, nullptr, d_act_bits, d_gtmp, 3, d_vd, g->off_out, iteration + 1, d_chg_v,
-            d_chg_e, d_chg_bits, d_ctr, d_srcval, d_gdense, d_applied, ad);
+            d_chg_e, d_chg_bits, d_ctr, d_srcval, d_gdense, d_applied, ad, fprev);
         tock(kPhApply);
         GR_HIP(hipGetLastError());
         pre_applied = true;
@@ -2934,6 +3003,8 @@ struct Engine final : EngineBase {
       k_source_values<P><<<grid_1d(g->nV, 256) < 8192 ? grid_1d(g->nV, 256) : 8192, 256, 0,
                            stream>>>(g->nV, d_vd, d_srcval);
       GR_HIP(hipGetLastError());
+      src_flags_ok = true;  // no flags
+      gather_flags = false;
     }
     return GR_OK;
   }

@@ -207,13 +207,48 @@ __device__ __forceinline__ int edge_pos(const int32_t* __restrict__ ed_index, in
     return c;
 }
 
+// Programs whose gather is a sum of non-negative floats may declare
+// kFixedPointSum: the blocked gather then adds their terms exactly in 64-bit
+// fixed point (each term rounded once to a multiple of 2^-32) and rounds the
+// row's sum once to float, so the result does not depend on the order in
+// which the LDS additions land (determinism) and is at least as close to the
+// exact sum as a float tree sum.
+template <class P, class = void>
+struct fixed_point_sum : std::false_type {};
+template <class P>
+struct fixed_point_sum<P, std::void_t<decltype(P::kFixedPointSum)>>
+    : std::integral_constant<bool, P::kFixedPointSum> {};
+template <class P>
+constexpr bool fixed_point_sum_v = fixed_point_sum<P>::value;
+
+// Programs with kFixedPointSum whose kept source value is a float must keep
+// that value non-negative (PageRank's rank / out-degree is), so its sign bit
+// is free: the
+// engine may keep there whether the vertex's last apply returned true and it
+// has out-edges (flagged(), unflag()), which the fused blocked sum then reads
+// with the value instead of the changed bitmap.  Every other reader masks it.
+template <class P>
+constexpr bool source_flag_v =
+    fixed_point_sum_v<P> && source_value_v<P> && std::is_same<source_value_t<P>, float>::value;
+__device__ __forceinline__ float unflag(float s) {
+  return __uint_as_float(__float_as_uint(s) & 0x7FFFFFFFu);
+}
+__device__ __forceinline__ float flagged(float s) {
+  return __uint_as_float(__float_as_uint(s) | 0x80000000u);
+}
+
 // gatherMap of a source, through its kept source value when there is one.
 template <class P>
 __device__ __forceinline__ typename P::GatherResult map_source(
     const typename P::VertexData* dst, const typename P::VertexData* vd,
     const source_value_t<P>* srcval, int src, const typename P::EdgeData* edge) {
   if constexpr (source_value_v<P>) {
-    if (srcval) return P::gatherMapValue(srcval[src], edge);
+    if (srcval) {
+      if constexpr (source_flag_v<P>)
+        return P::gatherMapValue(unflag(srcval[src]), edge);
+      else
+        return P::gatherMapValue(srcval[src], edge);
+    }
   }
   return P::gatherMap(dst, vd + src, edge);
 }
@@ -647,20 +682,6 @@ __global__ __launch_bounds__(kBlkNT) void k_gather_blocked(
   }
 }
 
-// Programs whose gather is a sum of non-negative floats may declare
-// kFixedPointSum: the blocked gather then adds their terms exactly in 64-bit
-// fixed point (each term rounded once to a multiple of 2^-32) and rounds the
-// row's sum once to float, so the result does not depend on the order in
-// which the LDS additions land (determinism) and is at least as close to the
-// exact sum as a float tree sum.
-template <class P, class = void>
-struct fixed_point_sum : std::false_type {};
-template <class P>
-struct fixed_point_sum<P, std::void_t<decltype(P::kFixedPointSum)>>
-    : std::integral_constant<bool, P::kFixedPointSum> {};
-template <class P>
-constexpr bool fixed_point_sum_v = fixed_point_sum<P>::value;
-
 constexpr float kFixScale = 4294967296.0f;             // 2^32
 constexpr double kFixUnscale = 1.0 / 4294967296.0;     // 2^-32
 
@@ -678,7 +699,7 @@ __global__ __launch_bounds__(kBlkNT) void k_gather_blocked_sum(
     const int32_t* __restrict__ gbase, const typename P::EdgeData* __restrict__ ed,
     const typename P::VertexData* __restrict__ vd, const source_value_t<P>* __restrict__ srcval,
     typename P::GatherResult* __restrict__ gdense, const uint32_t* __restrict__ chg_bits,
-    uint32_t* __restrict__ frontier, const DevCounters* __restrict__ ctr) {
+    uint32_t* __restrict__ frontier, const DevCounters* __restrict__ ctr, int use_flags = 0) {
   typedef typename P::GatherResult G;
   static_assert(std::is_same<G, float>::value && fixed_point_sum_v<P> && !P::kGatherUsesDst,
                 "the fixed-point blocked gather sums non-negative floats");
@@ -708,21 +729,43 @@ __global__ __launch_bounds__(kBlkNT) void k_gather_blocked_sum(
     for (int m = 0; m < kBlkVT; ++m) wcur[m] = wnext[m];
     load_slot_edges(ed, gs, src, ev);
     unsigned long long fix[kBlkVT];
-    uint32_t word[kBlkVT];
+    bool chg[kBlkVT];
+    if constexpr (source_flag_v<P>) {
+      // every lane's loads first (all in flight together), then the terms
+      uint32_t raw[kBlkVT], word[kBlkVT];
 #pragma unroll
-    for (int m = 0; m < kBlkVT; ++m) {
-      fix[m] = 0ull;
-      word[m] = 0u;
-      if (src[m] >= 0) {
-        const G v = map_source<P>(vd, vd, srcval, src[m], &ev[m]);
-        fix[m] = (unsigned long long)__float2ll_rn(__fmul_rn(v, kFixScale));
-        if (chg_bits) word[m] = chg_bits[src[m] >> 5];
+      for (int m = 0; m < kBlkVT; ++m) raw[m] = src[m] >= 0 ? __float_as_uint(srcval[src[m]]) : 0u;
+      const bool bits = chg_bits && !use_flags;  // uniform
+#pragma unroll
+      for (int m = 0; m < kBlkVT; ++m) word[m] = bits && src[m] >= 0 ? chg_bits[src[m] >> 5] : 0u;
+#pragma unroll
+      for (int m = 0; m < kBlkVT; ++m) {
+        const G v = P::gatherMapValue(__uint_as_float(raw[m] & 0x7FFFFFFFu), &ev[m]);
+        fix[m] = src[m] >= 0 ? (unsigned long long)__float2ll_rn(__fmul_rn(v, kFixScale)) : 0ull;
+        chg[m] = src[m] >= 0 && chg_bits &&
+                 (use_flags ? (raw[m] >> 31) != 0u : ((word[m] >> (src[m] & 31)) & 1u) != 0u);
       }
+    } else {
+      (void)use_flags;
+      uint32_t word[kBlkVT];
+#pragma unroll
+      for (int m = 0; m < kBlkVT; ++m) {
+        fix[m] = 0ull;
+        word[m] = 0u;
+        if (src[m] >= 0) {
+          const G v = map_source<P>(vd, vd, srcval, src[m], &ev[m]);
+          fix[m] = (unsigned long long)__float2ll_rn(__fmul_rn(v, kFixScale));
+          if (chg_bits) word[m] = chg_bits[src[m] >> 5];
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < kBlkVT; ++m)
+        chg[m] = src[m] >= 0 && ((word[m] >> (src[m] & 31)) & 1u);
     }
 #pragma unroll
     for (int m = 0; m < kBlkVT; ++m) {
       if (fix[m]) atomicAdd(&acc[row[m]], fix[m]);
-      if (src[m] >= 0 && ((word[m] >> (src[m] & 31)) & 1u)) act[row[m]] = 1;
+      if (chg[m]) act[row[m]] = 1;
     }
   }
   __syncthreads();
