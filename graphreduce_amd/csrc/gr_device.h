@@ -181,6 +181,16 @@ constexpr int kBlockRowsWide = 16384;   // ... for 8-byte accumulators (LDS: 128
 #define GR_MIN_BLOCK_ROWS kBlockRows
 #endif
 constexpr int kMinBlockRows = GR_MIN_BLOCK_ROWS;  // rows per block of the int-minimum gather
+// ... on a graph of uniform degree (graph_skewed false): two workgroups per
+// CU, since there the source reads mostly hit the cache and a workgroup's
+// passes over its slots, one after another, are what take the time
+#ifndef GR_MIN_BLOCK_ROWS_FLAT
+#define GR_MIN_BLOCK_ROWS_FLAT kBlockRowsWide
+#endif
+constexpr int kMinBlockRowsFlat = GR_MIN_BLOCK_ROWS_FLAT;
+static_assert((kMinBlockRows == kBlockRows || kMinBlockRows == kBlockRowsWide) &&
+                  (kMinBlockRowsFlat == kBlockRows || kMinBlockRowsFlat == kBlockRowsWide),
+              "the row blocks are built for these two limits");
 constexpr int kBlockEdges = 1 << 19;    // in-edges per gather block (largest target)
 constexpr int kPackGroup = 512;          // slots per group of the packed layout
 constexpr uint32_t kPackPad = 0xFFFFu;   // row of a pad slot

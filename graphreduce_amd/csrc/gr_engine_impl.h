@@ -1579,6 +1579,7 @@ struct Engine final : EngineBase {
                                       !P::kGatherUsesDst && !P::kHasScatter;
   static constexpr bool kBlocked = kBlockedMin || kBlockedSum;
   const gr_blocked* blk = nullptr;
+  int blk_rows = kBlockRows;  // the blocks' row limit (kBlockRows or kBlockRowsWide)
   ED* d_ed_blk = nullptr;
   G* d_gdense = nullptr;   // rows from blk->rows_end on stay gatherZero
   bool dense_pending = false;  // apply reads d_gdense[v] (the last gather was blocked)
@@ -2444,7 +2445,8 @@ struct Engine final : EngineBase {
   int blocked_setup() {
     if constexpr (kBlocked) {
       if (!blk) {
-        GR_TRY(graph_blocked(g, kBlockedSum ? kBlockRowsWide : kMinBlockRows, &blk));
+        blk_rows = kBlockedSum ? kBlockRowsWide : graph_skewed(g) ? kMinBlockRows : kMinBlockRowsFlat;
+        GR_TRY(graph_blocked(g, blk_rows, &blk));
         GR_TRY(dalloc(&d_gdense, g->nV));
         k_fill<G><<<1024, 256, 0, stream>>>(g->nV, P::gatherZero, d_gdense);
         GR_HIP(hipGetLastError());
@@ -2461,6 +2463,28 @@ struct Engine final : EngineBase {
     return GR_OK;
   }
 
+  // k_gather_blocked over the row blocks, for the blocks' row limit.
+  void launch_blocked_min(const uint32_t* dbits, uint32_t* frontier, DevCounters* ctr,
+                          int mode = 0, long long thr = 0, long long fuse_max = -1) {
+    if constexpr (kBlockedMin) {
+      if (blk_rows == kBlockRows)
+        k_gather_blocked<P, kBlockRows><<<blk->n_blocks, kBlkNT, 0, stream>>>(
+            blk->brow, blk->bnr, blk->boff, blk->bpk, blk->gbase, d_ed_blk, d_vd, d_srcval,
+            dbits, d_gdense, frontier, ctr, mode, thr, fuse_max);
+      else
+        k_gather_blocked<P, kBlockRowsWide><<<blk->n_blocks, kBlkNT, 0, stream>>>(
+            blk->brow, blk->bnr, blk->boff, blk->bpk, blk->gbase, d_ed_blk, d_vd, d_srcval,
+            dbits, d_gdense, frontier, ctr, mode, thr, fuse_max);
+    } else {
+      (void)dbits;
+      (void)frontier;
+      (void)ctr;
+      (void)mode;
+      (void)thr;
+      (void)fuse_max;
+    }
+  }
+
   // Gather of every row through the graph's source-listed row blocks
   // (k_gather_blocked); apply reads the active rows' values from d_gdense.
   int gather_blocked(int n, int w, const uint32_t* dbits) {
@@ -2473,9 +2497,7 @@ struct Engine final : EngineBase {
               blk->brow, blk->bnr, blk->boff, blk->bpk, blk->gbase, d_ed_blk, d_vd, d_srcval,
               d_gdense, nullptr, nullptr, nullptr);
         else
-          k_gather_blocked<P><<<blk->n_blocks, kBlkNT, 0, stream>>>(
-              blk->brow, blk->bnr, blk->boff, blk->bpk, blk->gbase, d_ed_blk, d_vd, d_srcval,
-              dbits, d_gdense);
+          launch_blocked_min(dbits, nullptr, nullptr);
       }
       tock(kPhGather);
       GR_HIP(hipGetLastError());
@@ -2801,9 +2823,7 @@ struct Engine final : EngineBase {
     if constexpr (kBlockedMin) {
       if (fab) {
         tick();
-        k_gather_blocked<P><<<blk->n_blocks, kBlkNT, 0, stream>>>(
-            blk->brow, blk->bnr, blk->boff, blk->bpk, blk->gbase, d_ed_blk, d_vd, d_srcval,
-            d_chg_bits, d_gdense, d_bitmap, d_ctr, dir_mode, thr, fuse_max);
+        launch_blocked_min(d_chg_bits, d_bitmap, d_ctr, dir_mode, thr, fuse_max);
         tock(kPhGather);
         GR_HIP(hipGetLastError());
       }

@@ -598,7 +598,9 @@ __device__ __forceinline__ void load_slot_edges(const ED* __restrict__ ed, int g
 // vertices whose apply just returned true: the values are the next delta
 // gather's, and a row with an in-edge from such a vertex is marked in
 // `frontier` (which holds no bit of these rows before), as the pull marks it.
-template <class P>
+// kRows is the blocks' row limit (the LDS minima): kBlockRows, one workgroup
+// per CU, or kBlockRowsWide, two (40-50 VGPRs leave room for 8 waves per SIMD).
+template <class P, int kRows = kMinBlockRows>
 __global__ __launch_bounds__(kBlkNT) void k_gather_blocked(
     const int32_t* __restrict__ brow, const int32_t* __restrict__ bnr,
     const int32_t* __restrict__ boff, const uint32_t* __restrict__ bpk,
@@ -610,8 +612,8 @@ __global__ __launch_bounds__(kBlkNT) void k_gather_blocked(
   typedef typename P::GatherResult G;
   static_assert(std::is_same<G, int>::value && delta_gather_v<P> && !P::kGatherUsesDst,
                 "the blocked gather folds int minima of source-only gatherMaps");
-  __shared__ int acc[kMinBlockRows];
-  __shared__ uint32_t act[kMinBlockRows / 32];
+  __shared__ int acc[kRows];
+  __shared__ uint32_t act[kRows / 32];
   const int b = blockIdx.x;
   const int tid = threadIdx.x;
   if (frontier) {
@@ -623,7 +625,7 @@ __global__ __launch_bounds__(kBlkNT) void k_gather_blocked(
   const int r0 = brow[b], nr = bnr[b];
   const int e0 = boff[b], e1 = boff[b + 1];
   for (int r = tid; r < nr; r += kBlkNT) acc[r] = P::gatherZero;
-  for (int k = tid; k < kMinBlockRows / 32; k += kBlkNT) act[k] = 0u;
+  for (int k = tid; k < kRows / 32; k += kBlkNT) act[k] = 0u;
   __syncthreads();
   // each wave's next group is loaded while it works on the current one
   constexpr int kPass = kBlkNT * kBlkVT;
