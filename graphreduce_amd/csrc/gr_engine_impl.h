@@ -118,6 +118,10 @@ __device__ __forceinline__ void defer_chunk_done(const ApplyDefer& d, int c, int
   }
 }
 
+// A one-byte degree of kDegSat stands for kDegSat or more: the offsets hold
+// the degree.
+constexpr int kDegSat = 255;
+
 // The kept source value of an apply's entry.  Without flag_prev it is
 // rewritten where the vertex data changed.  With flag_prev (programs with
 // source_flag_v, bitmap form): its sign bit is set where the apply returned
@@ -168,7 +172,8 @@ __global__ __launch_bounds__(kApplyNT) void k_apply(
     int n, const int32_t* __restrict__ active, const uint32_t* __restrict__ act_bits,
     const typename P::GatherResult* __restrict__ gtmp, int use_gather,
     typename P::VertexData* __restrict__ vd, const int32_t* __restrict__ off_out,
-    int iteration, int32_t* __restrict__ chg_v, int32_t* __restrict__ chg_e,
+    const uint8_t* __restrict__ deg8, int iteration, int32_t* __restrict__ chg_v,
+    int32_t* __restrict__ chg_e,
     uint32_t* __restrict__ chg_bits, DevCounters* __restrict__ ctr,
     source_value_t<P>* __restrict__ srcval, typename P::GatherResult* __restrict__ cand,
     unsigned long long* __restrict__ applied, ApplyDefer defer,
@@ -257,16 +262,17 @@ __global__ __launch_bounds__(kApplyNT) void k_apply(
         if (!pod_equal(vs[m], old)) chg_mask |= 1u << m;
       }
     }
-    // unconditional offset reads (vertex 0's for the others) keep them batched
-    int o0[kApplyVT], o1[kApplyVT];
+    // out-degrees of the entries whose apply returned true, one byte each
+    // (unconditional reads, vertex 0's for the others, keep them batched);
+    // the offsets only for a degree of kDegSat or more
+    uint32_t d8[kApplyVT];
+#pragma unroll
+    for (int m = 0; m < kApplyVT; ++m) d8[m] = deg8[(r_mask & (1u << m)) ? dvs[m] : 0];
 #pragma unroll
     for (int m = 0; m < kApplyVT; ++m) {
-      const int32_t* o = off_out + ((r_mask & (1u << m)) ? dvs[m] : 0);
-      o0[m] = o[0];
-      o1[m] = o[1];
+      degs[m] = (r_mask & (1u << m)) ? (int)d8[m] : 0;
+      if (degs[m] == kDegSat) degs[m] = off_out[dvs[m] + 1] - off_out[dvs[m]];
     }
-#pragma unroll
-    for (int m = 0; m < kApplyVT; ++m) degs[m] = (r_mask & (1u << m)) ? o1[m] - o0[m] : 0;
     // positions in entry order: per (m, wave) the count of changed entries
     // and the sum of their out-degrees, scanned below
     unsigned long long ball[kApplyVT];
@@ -1107,6 +1113,24 @@ __device__ __forceinline__ ByteWork byte_work(const uint32_t* __restrict__ bitma
   return b;
 }
 
+// One-byte degrees: deg[v] = min(degree, kDegSat) for v < nV, 0 for the
+// padding up to n_pad.
+[[maybe_unused]] static __global__ void k_degree8(int nV, int n_pad,
+                                                  const int32_t* __restrict__ off_in,
+                                                  const int32_t* __restrict__ off_out,
+                                                  uint8_t* __restrict__ din,
+                                                  uint8_t* __restrict__ dout) {
+  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n_pad; v += gridDim.x * blockDim.x) {
+    int a = 0, b = 0;
+    if (v < nV) {
+      a = min(off_in[v + 1] - off_in[v], kDegSat);
+      b = min(off_out[v + 1] - off_out[v], kDegSat);
+    }
+    din[v] = (uint8_t)a;
+    dout[v] = (uint8_t)b;
+  }
+}
+
 // nextIter's compaction in one launch.  Each block of kCompactNT words first
 // folds the push pass's hub flags into its words (when the push ran: the same
 // rule as the push kernels, on the same counters; the flags are cleared),
@@ -1155,16 +1179,11 @@ static_assert((long long)kApplyGroup * 8 * kCompactRounds * kCompactNT <
 
 [[maybe_unused]] static __global__ __launch_bounds__(kCompactNT) void k_compact_count(
     uint32_t* __restrict__ bitmap, int n_words, int nV, const int32_t* __restrict__ off_in,
-    int need_deg, int2* __restrict__ blk, uint32_t* __restrict__ act_bits,
+    const uint8_t* __restrict__ deg8, int need_deg, int2* __restrict__ blk, uint32_t* __restrict__ act_bits,
     uint32_t* __restrict__ chg_clear, HubFold hub, CompactFinal fin) {
   __shared__ int s_a[kCompactNT / kWave], s_b[kCompactNT / kWave];
   __shared__ uint32_t s_word[kCompactNT];
   __shared__ int s_last;
-  // a round's 8 * kCompactNT + 1 CSC offsets, loaded coalesced when many of
-  // its bytes have bits (a dense frontier): each thread's 9 overlapping
-  // offsets would otherwise cost a strided load each
-  constexpr int kRoundV = 8 * kCompactNT;
-  __shared__ int32_t s_off[kRoundV + 1];
   static_assert(kCompactRounds * kCompactNT / 4 == kCompactNT, "one word per thread");
   const int wi = blockIdx.x * kCompactNT + threadIdx.x;  // this thread's word
   uint32_t myword = wi < n_words ? bitmap[wi] : 0u;
@@ -1195,29 +1214,18 @@ static_assert((long long)kApplyGroup * 8 * kCompactRounds * kCompactNT <
     const uint32_t word = s_word[r * (kCompactNT / 4) + threadIdx.x / 4];
     const uint32_t bits = (word >> ((bi & 3) * 8)) & 0xFFu;
     cnt += __popc(bits);
-    if (!need_deg) continue;
-    const int n_set = __syncthreads_count(bits != 0u);  // uniform
-    if (n_set == 0) continue;
-    if (n_set >= kCompactNT / 8) {
-      const int v0r = (blockIdx.x * kCompactRounds + r) * kRoundV;
-      for (int i = threadIdx.x; i <= kRoundV; i += kCompactNT) s_off[i] = off_in[min(v0r + i, nV)];
-      __syncthreads();
-      if (bits) {
-        const int* o = s_off + threadIdx.x * 8;
-        if (bits == 0xFFu) {
-          deg += o[8] - o[0];
-        } else {
+    if (!need_deg || !bits) continue;
+    // the byte's 8 vertices' one-byte in-degrees, one 8-byte load (a set bit
+    // means bi < 4 * n_words, and deg8 covers every word's vertices); the
+    // offsets only for a degree of kDegSat or more
+    const uint2 d = *reinterpret_cast<const uint2*>(deg8 + (size_t)bi * 8);
+    const int v0 = bi * 8;
 #pragma unroll
-          for (int j = 0; j < 8; ++j)
-            if ((bits >> j) & 1u) deg += o[j + 1] - o[j];
-        }
+    for (int j = 0; j < 8; ++j) {
+      if ((bits >> j) & 1u) {  // bit set => v0 + j < nV
+        const int dj = (int)(((j < 4 ? d.x : d.y) >> (8 * (j & 3))) & 0xFFu);
+        deg += dj == kDegSat ? off_in[v0 + j + 1] - off_in[v0 + j] : dj;
       }
-      __syncthreads();  // the next round stages into the same array
-    } else if (bits) {
-      const int v0 = bi * 8;
-#pragma unroll
-      for (int j = 0; j < 8; ++j)
-        if ((bits >> j) & 1u) deg += off_in[v0 + j + 1] - off_in[v0 + j];  // bit set => v0 + j < nV
     }
   }
   cnt = wave_reduce_add(cnt);
@@ -1578,6 +1586,10 @@ struct Engine final : EngineBase {
   static constexpr bool kBlockedSum = std::is_same<G, float>::value && fixed_point_sum_v<P> &&
                                       !P::kGatherUsesDst && !P::kHasScatter;
   static constexpr bool kBlocked = kBlockedMin || kBlockedSum;
+  // in- and out-degrees of the vertices in one byte each, kDegSat standing
+  // for kDegSat or more (then the offsets are read)
+  uint8_t* d_deg_in8 = nullptr;
+  uint8_t* d_deg_out8 = nullptr;
   const gr_blocked* blk = nullptr;
   int blk_rows = kBlockRows;  // the blocks' row limit (kBlockRows or kBlockRowsWide)
   ED* d_ed_blk = nullptr;
@@ -1726,7 +1738,7 @@ struct Engine final : EngineBase {
                     d_gtmp,      d_chg_v,  d_chg_e,    d_bitmap, d_chg_bits, d_chg_prev,
                     d_blk,       d_part,   d_push_part, d_carry, d_head,   d_ctr,
                     d_applied,   d_act_bits, d_cc,     d_sc,     d_sc_final, d_cgsum,
-                    d_cgrp,      d_cgtot,  d_cgdone};
+                    d_cgrp,      d_cgtot,  d_cgdone, d_deg_in8, d_deg_out8};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     if (h_ctr) (void)hipHostFree(h_ctr);
@@ -1784,6 +1796,7 @@ struct Engine final : EngineBase {
     d_cc = d_sc = d_sc_final = d_cgsum = d_cgtot = nullptr;
     d_cgrp = nullptr;
     d_cgdone = nullptr;
+    d_deg_in8 = d_deg_out8 = nullptr;
     hub = HubFold();
     chg_pending = chg_cond = false;
     list_valid = true;
@@ -1852,6 +1865,15 @@ struct Engine final : EngineBase {
     GR_TRY(dalloc(&d_bitmap, n_words + 1));
     GR_TRY(dalloc(&d_act_bits, n_words + 1));
     GR_TRY(dalloc(&d_chg_bits, n_words + 1));
+    // one-byte degrees for the counting passes: every vertex of the bitmaps'
+    // words has a byte (0 past nV)
+    GR_TRY(dalloc(&d_deg_in8, (int64_t)n_words * 32));
+    GR_TRY(dalloc(&d_deg_out8, (int64_t)n_words * 32));
+    if (n_words > 0) {
+      k_degree8<<<grid_1d((int64_t)n_words * 32, 256), 256, 0, stream>>>(
+          g->nV, n_words * 32, g->off_in, g->off_out, d_deg_in8, d_deg_out8);
+      GR_HIP(hipGetLastError());
+    }
     GR_TRY(dalloc(&d_chg_prev, n_words + 1));
     n_apply_chunks = (int)(((int64_t)V + kApplyChunk - 1) / kApplyChunk);
     GR_TRY(dalloc(&d_cc, (int64_t)n_apply_chunks));
@@ -2665,11 +2687,11 @@ struct Engine final : EngineBase {
     GR_TRY(source_flags_for_apply(defer, &fprev));
     if (bits)
       k_apply<P, true><<<grid_1d(g->nV, kApplyChunk), kApplyNT, 0, stream>>>(
-          g->nV, nullptr, d_act_bits, d_gtmp, use_gather, d_vd, g->off_out, iteration, d_chg_v,
+          g->nV, nullptr, d_act_bits, d_gtmp, use_gather, d_vd, g->off_out, d_deg_out8, iteration, d_chg_v,
           d_chg_e, cbits, d_ctr, d_srcval, cv, d_applied, ad, fprev);
     else
       k_apply<P, false><<<grid_1d(n_active, kApplyChunk), kApplyNT, 0, stream>>>(
-          n_active, d_active, nullptr, d_gtmp, use_gather, d_vd, g->off_out, iteration, d_chg_v,
+          n_active, d_active, nullptr, d_gtmp, use_gather, d_vd, g->off_out, d_deg_out8, iteration, d_chg_v,
           d_chg_e, cbits, d_ctr, d_srcval, cv, d_applied, ApplyDefer());
     if (defer) chg_pending = true;
     tock(kPhApply);
@@ -2910,7 +2932,7 @@ struct Engine final : EngineBase {
     fin.seq = ctr_seq;
     fin.applied = d_applied;
     k_compact_count<<<n_cblocks, kCompactNT, 0, stream>>>(
-        d_bitmap, n_words, g->nV, g->off_in, need_deg, d_blk, d_act_bits,
+        d_bitmap, n_words, g->nV, g->off_in, d_deg_in8, need_deg, d_blk, d_act_bits,
         changed_bits_on() ? d_chg_prev : nullptr, hub, fin);
     hub = HubFold();
     list_valid = false;
@@ -2942,7 +2964,8 @@ struct Engine final : EngineBase {
         ad.sc = d_sc;
         ad.sc_final = d_sc_final;
         k_apply<P, true><<<grid_1d(g->nV, kApplyChunk), kApplyNT, 0, stream>>>(
-            g->nV, nullptr, d_act_bits, d_gtmp, 3, d_vd, g->off_out, iteration + 1, d_chg_v,
+            g->nV, nullptr, d_act_bits, d_gtmp, 3, d_vd, g->off_out, d_deg_out8, iteration + 1,
+            d_chg_v,
             d_chg_e, d_chg_bits, d_ctr, d_srcval, d_gdense, d_applied, ad, fprev);
         tock(kPhApply);
         GR_HIP(hipGetLastError());
