@@ -374,7 +374,8 @@ constexpr int kChgNT = kApplyChunk / 8;
 // iteration or the next one reads the list: nothing is written then.
 [[maybe_unused]] static __global__ __launch_bounds__(kChgNT) void k_chg_write(
     const uint32_t* __restrict__ chg_bits, int nV, const int32_t* __restrict__ off_out,
-    const unsigned long long* __restrict__ cc, const unsigned long long* __restrict__ sc_final,
+    const uint8_t* __restrict__ deg8, const unsigned long long* __restrict__ cc,
+    const unsigned long long* __restrict__ sc_final,
     int n_chunks, int32_t* __restrict__ chg_v, int32_t* __restrict__ chg_e,
     const DevCounters* __restrict__ ctr, int standdown, int mode, long long threshold,
     long long fuse_max) {
@@ -401,14 +402,20 @@ constexpr int kChgNT = kApplyChunk / 8;
     const int v0 = c * kApplyChunk + (int)threadIdx.x * 8;
     const uint32_t word = v0 < nV ? chg_bits[v0 >> 5] : 0u;
     uint32_t bits = (word >> ((threadIdx.x & 3) * 8)) & 0xFFu;
-    int o[9];
+    int dg[8];
     unsigned long long mine = 0ull;
     if (bits) {
+      // one-byte out-degrees of the 8 vertices (v0 < nV, a multiple of 8:
+      // deg8 covers them); the offsets for a degree of kDegSat or more
+      const uint2 d = *reinterpret_cast<const uint2*>(deg8 + v0);
 #pragma unroll
-      for (int j = 0; j < 9; ++j) o[j] = off_out[min(v0 + j, nV)];
+      for (int j = 0; j < 8; ++j) {
+        dg[j] = (int)(((j < 4 ? d.x : d.y) >> (8 * (j & 3))) & 0xFFu);
+        if (((bits >> j) & 1u) && dg[j] == kDegSat) dg[j] = off_out[v0 + j + 1] - off_out[v0 + j];
+      }
 #pragma unroll
       for (int j = 0; j < 8; ++j)
-        if ((bits >> j) & 1u) mine += (1ull << 32) | (unsigned long long)(uint32_t)(o[j + 1] - o[j]);
+        if ((bits >> j) & 1u) mine += (1ull << 32) | (unsigned long long)(uint32_t)dg[j];
     }
     unsigned long long tot;
     const unsigned long long ex = block_excl_scan_add<kChgNT>(mine, s_scan, &tot);
@@ -419,7 +426,7 @@ constexpr int kChgNT = kApplyChunk / 8;
       bits &= bits - 1;
       s_v[pos] = v0 + j;
       s_e[pos] = dpos;
-      dpos += o[j + 1] - o[j];
+      dpos += dg[j];
       ++pos;
     }
     __syncthreads();
@@ -2739,7 +2746,8 @@ struct Engine final : EngineBase {
     chg_cond = standdown;
     const int grid = n_apply_chunks < 8192 ? n_apply_chunks : 8192;
     k_chg_write<<<grid < 1 ? 1 : grid, kChgNT, 0, stream>>>(
-        d_chg_bits, g->nV, g->off_out, d_cc, d_sc_final, n_apply_chunks, d_chg_v, d_chg_e, d_ctr,
+        d_chg_bits, g->nV, g->off_out, d_deg_out8, d_cc, d_sc_final, n_apply_chunks, d_chg_v,
+        d_chg_e, d_ctr,
         standdown ? 1 : 0, mode, thr, fuse_max);
     GR_HIP(hipGetLastError());
     return GR_OK;
