@@ -714,6 +714,9 @@ def run_cc_grid(gr):
     vd = np.arange(n, dtype=np.int32)
     eng = gr.GASEngine(gr.CC)
     eng.bindGraph(gg, vd)
+    from graphreduce_amd import tune
+
+    tuned = tune.apply_tuning(eng, gr.CC, uniform=True)  # chosen on the seed-6 grid
     gr.synchronize()
     t0 = time.perf_counter()
     trips = run_engine(eng, n)
@@ -735,7 +738,8 @@ def run_cc_grid(gr):
                components=int(nc), gathered_edges=st["gathered_edges"],
                gteps=round(st["gathered_edges"] / sec / 1e9, 3),
                us_per_trip=round(sec / max(1, trips) * 1e6, 1),
-               labels_equal_scipy_min_id=ok)
+               labels_equal_scipy_min_id=ok,
+               tuning="graphreduce_amd/tuned.json, uniform (grid seed 6)" if tuned else "defaults")
     gg.close()
     log(f"CC grid: {out}")
     return out

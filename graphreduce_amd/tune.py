@@ -9,9 +9,13 @@ changed in-edges below which a gather pushes, by timing complete runs of the
 reference loop on the device graph.  The default graph is RMAT-24 seed 7, so
 that the benchmark's graph (seed 6) is not the one the settings were chosen
 on.  Results are written as JSON and applied with
-:func:`apply_tuning`.
+:func:`apply_tuning`.  Graphs of uniform degree (grids, road maps) have
+settings of their own under "uniform", chosen with ``--grid`` on the 4096 x
+4096 grid with 1 % of its edges deleted, seed 6 (the benchmark's grid is seed
+5), by timing ``run()``.
 
 Usage:  python -m graphreduce_amd.tune --scale 24 --seed 7 --out graphreduce_amd/tuned.json
+        python -m graphreduce_amd.tune --grid 4096 --grid-seed 6
 """
 from __future__ import annotations
 
@@ -30,6 +34,10 @@ TILES = (4, 8, 16)
 ALPHAS = (1.0, 2.0, 4.0, 8.0, 15.0, 30.0, 60.0)
 FUSE_SHARES = (0.0, 0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.8)  # SSSP and CC only
 PUSH_SHARES = (0.1, 0.2, 0.3, 0.45, 0.6, 0.8)  # SSSP and CC only
+# the uniform-degree sweep (CC on a grid: thousands of trips per run)
+GRID_ALPHAS = (4.0, 15.0, 60.0)
+GRID_FUSE_SHARES = (0.0, 0.05, 0.1, 0.2, 0.4)
+GRID_PUSH_SHARES = (0.1, 0.6)
 
 
 def _loop(eng, n, source=None):
@@ -68,20 +76,40 @@ def _engine(program, g, weights=None):
     return eng, vd
 
 
-def tune_program(program, g, source, weights=None, repeats=2, log=print):
+def _run(eng, n):
+    """The engine's own loop over every vertex (connected_component.cu:75-77)."""
+    eng.resetVertexData()
+    eng.setActive(0, n)
+    eng.setIterationCount(0)
+    return eng.run()
+
+
+def tune_program(program, g, source, weights=None, repeats=2, log=print, uniform=False):
     """Returns {"gather_tile": vt, "alpha": a, ["fuse_share": f,] "ms": best}
-    for one program."""
+    for one program.  uniform: the reduced sweep for graphs of uniform degree,
+    timing run() over every vertex."""
     eng, vd = _engine(program, g)
     if program == gr.SSSP:
         vd[source] = 0
     eng.bindGraph(g, vd, weights)
     src = source if program == gr.BFS else None
-    _loop(eng, g.n_vertices, src)  # warm
+    if uniform:
+        def loop():
+            return _run(eng, g.n_vertices)
+    else:
+        def loop():
+            return _loop(eng, g.n_vertices, src)
+    loop()  # warm
     best = None
     tiles = TILES if program != gr.BFS else (8,)
+    alphas = ALPHAS
     fuses = FUSE_SHARES if program in (gr.SSSP, gr.CC) else (None,)
     pushes = PUSH_SHARES if program in (gr.SSSP, gr.CC) else (None,)
-    for vt, alpha, fuse, push in itertools.product(tiles, ALPHAS, fuses, pushes):
+    if uniform:
+        tiles, alphas = (8,), GRID_ALPHAS
+        if program in (gr.SSSP, gr.CC):
+            fuses, pushes = GRID_FUSE_SHARES, GRID_PUSH_SHARES
+    for vt, alpha, fuse, push in itertools.product(tiles, alphas, fuses, pushes):
         eng.setGatherTile(vt)
         eng.setDirection(0, alpha)
         if fuse is not None:
@@ -91,7 +119,7 @@ def tune_program(program, g, source, weights=None, repeats=2, log=print):
         for _ in range(repeats):
             gr.synchronize()
             t0 = time.perf_counter()
-            _loop(eng, g.n_vertices, src)
+            loop()
             gr.synchronize()
             ts.append(time.perf_counter() - t0)
         ms = min(ts) * 1e3
@@ -106,12 +134,16 @@ def tune_program(program, g, source, weights=None, repeats=2, log=print):
     return best
 
 
-def apply_tuning(eng: "gr.GASEngine", program: int, path: Path | str = DEFAULT_FILE) -> bool:
-    """Applies the tuned parameters of `program` from `path` (if present)."""
+def apply_tuning(eng: "gr.GASEngine", program: int, path: Path | str = DEFAULT_FILE,
+                 uniform: bool = False) -> bool:
+    """Applies the tuned parameters of `program` from `path` (if present);
+    uniform: those chosen for graphs of uniform degree."""
     p = Path(path)
     if not p.exists():
         return False
     data = json.loads(p.read_text())
+    if uniform:
+        data = data.get("uniform", {})
     entry = data.get("programs", {}).get(gr.PROGRAMS[program]["name"])
     if not entry:
         return False
@@ -130,7 +162,13 @@ def main():
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--out", default=str(DEFAULT_FILE))
     ap.add_argument("--programs", default="BFS,SSSP,CC,PageRank")
+    ap.add_argument("--grid", type=int, default=0,
+                    help="tune CC on a grid of this side (1 %% of edges deleted) into 'uniform'")
+    ap.add_argument("--grid-seed", type=int, default=6)
     args = ap.parse_args()
+    if args.grid:
+        tune_uniform(args.grid, args.grid_seed, args.out)
+        return
     g = gr.Graph.rmat(args.scale, 16, seed=args.seed)
     src, _ = g.max_out_degree_vertex()
     w = g.hash_weights_device(seed=args.seed)
@@ -147,6 +185,23 @@ def main():
         print(f"  -> {out['programs'][name]}", flush=True)
     Path(args.out).write_text(json.dumps(out, indent=1))
     print(f"wrote {args.out}")
+
+
+def tune_uniform(side, seed, out_path):
+    """CC on the grid: the settings go under "uniform" of out_path, the rest
+    of the file is kept."""
+    g = gr.Graph.grid(side, 0.01, seed=seed)
+    p = Path(out_path)
+    data = json.loads(p.read_text()) if p.exists() else {}
+    uni = {"graph": f"grid {side} x {side}, 1 % of edges deleted, seed {seed}",
+           "device": gr.device_info()["name"], "programs": {}}
+    print("tuning CC (uniform)", flush=True)
+    uni["programs"]["CC"] = tune_program(gr.CC, g, None, repeats=1, uniform=True)
+    print(f"  -> {uni['programs']['CC']}", flush=True)
+    g.close()
+    data["uniform"] = uni
+    p.write_text(json.dumps(data, indent=1))
+    print(f"wrote {out_path}")
 
 
 if __name__ == "__main__":
