@@ -238,6 +238,15 @@ __global__ __launch_bounds__(kApplyNT) void k_apply(
 #pragma unroll
     for (int m = 0; m < kApplyVT; ++m)
       if (valid_mask & (1u << m)) vs[m] = vd[dvs[m]];
+    // bitmap form: one-byte out-degrees of every entry, loaded with the
+    // vertex values (not after apply: that would add a round trip to memory;
+    // the entries are consecutive vertices, so the bytes share lines); vertex
+    // 0's for the others keeps the loads unconditional
+    uint32_t d8[kApplyVT];
+    if constexpr (kBits) {
+#pragma unroll
+      for (int m = 0; m < kApplyVT; ++m) d8[m] = deg8[(valid_mask & (1u << m)) ? dvs[m] : 0];
+    }
     // use_gather 2 (push gathers of delta programs): the relaxed candidate of
     // each active row is its gather value; read it and reset it to gatherZero
     if (use_gather == 2) {
@@ -262,12 +271,13 @@ __global__ __launch_bounds__(kApplyNT) void k_apply(
         if (!pod_equal(vs[m], old)) chg_mask |= 1u << m;
       }
     }
-    // out-degrees of the entries whose apply returned true, one byte each
-    // (unconditional reads, vertex 0's for the others, keep them batched);
-    // the offsets only for a degree of kDegSat or more
-    uint32_t d8[kApplyVT];
+    // out-degrees of the entries whose apply returned true (list form: only
+    // theirs are loaded, the entries being scattered vertices); the offsets
+    // only for a degree of kDegSat or more
+    if constexpr (!kBits) {
 #pragma unroll
-    for (int m = 0; m < kApplyVT; ++m) d8[m] = deg8[(r_mask & (1u << m)) ? dvs[m] : 0];
+      for (int m = 0; m < kApplyVT; ++m) d8[m] = deg8[(r_mask & (1u << m)) ? dvs[m] : 0];
+    }
 #pragma unroll
     for (int m = 0; m < kApplyVT; ++m) {
       degs[m] = (r_mask & (1u << m)) ? (int)d8[m] : 0;
