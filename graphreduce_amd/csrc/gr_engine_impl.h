@@ -1226,22 +1226,34 @@ static_assert((long long)kApplyGroup * 8 * kCompactRounds * kCompactNT <
   s_word[threadIdx.x] = myword;
   __syncthreads();
   int cnt = 0, deg = 0;
+  uint32_t rbits[kCompactRounds];
+#pragma unroll
   for (int r = 0; r < kCompactRounds; ++r) {
     const int bi = (blockIdx.x * kCompactRounds + r) * kCompactNT + threadIdx.x;
     const uint32_t word = s_word[r * (kCompactNT / 4) + threadIdx.x / 4];
-    const uint32_t bits = (word >> ((bi & 3) * 8)) & 0xFFu;
-    cnt += __popc(bits);
-    if (!need_deg || !bits) continue;
-    // the byte's 8 vertices' one-byte in-degrees, one 8-byte load (a set bit
-    // means bi < 4 * n_words, and deg8 covers every word's vertices); the
+    rbits[r] = (word >> ((bi & 3) * 8)) & 0xFFu;
+    cnt += __popc(rbits[r]);
+  }
+  if (need_deg) {
+    // each round's byte: its 8 vertices' one-byte in-degrees, one 8-byte
+    // load (a set bit means bi < 4 * n_words, and deg8 covers every word's
+    // vertices), all rounds' loads issued before the first is used; the
     // offsets only for a degree of kDegSat or more
-    const uint2 d = *reinterpret_cast<const uint2*>(deg8 + (size_t)bi * 8);
-    const int v0 = bi * 8;
+    uint2 d[kCompactRounds];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      if ((bits >> j) & 1u) {  // bit set => v0 + j < nV
-        const int dj = (int)(((j < 4 ? d.x : d.y) >> (8 * (j & 3))) & 0xFFu);
-        deg += dj == kDegSat ? off_in[v0 + j + 1] - off_in[v0 + j] : dj;
+    for (int r = 0; r < kCompactRounds; ++r) {
+      const int bi = (blockIdx.x * kCompactRounds + r) * kCompactNT + threadIdx.x;
+      d[r] = rbits[r] ? *reinterpret_cast<const uint2*>(deg8 + (size_t)bi * 8) : make_uint2(0u, 0u);
+    }
+#pragma unroll
+    for (int r = 0; r < kCompactRounds; ++r) {
+      const int v0 = ((blockIdx.x * kCompactRounds + r) * kCompactNT + threadIdx.x) * 8;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        if ((rbits[r] >> j) & 1u) {  // bit set => v0 + j < nV
+          const int dj = (int)(((j < 4 ? d[r].x : d[r].y) >> (8 * (j & 3))) & 0xFFu);
+          deg += dj == kDegSat ? off_in[v0 + j + 1] - off_in[v0 + j] : dj;
+        }
       }
     }
   }
