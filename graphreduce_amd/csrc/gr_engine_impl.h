@@ -177,7 +177,10 @@ __global__ __launch_bounds__(kApplyNT) void k_apply(
     uint32_t* __restrict__ chg_bits, DevCounters* __restrict__ ctr,
     source_value_t<P>* __restrict__ srcval, typename P::GatherResult* __restrict__ cand,
     unsigned long long* __restrict__ applied, ApplyDefer defer,
-    const uint32_t* __restrict__ flag_prev = nullptr) {
+    const uint32_t* __restrict__ flag_prev = nullptr, int gate_fab = 0) {
+  // gate_fab: queued behind a blocked gather launched in place of the pull;
+  // it runs only where that gather ran (ctr->fab, the same for every thread)
+  if (gate_fab && ctr->fab == 0) return;
   constexpr int kNW = kApplyNT / kWave;
   constexpr int kWords = kApplyChunk / 32;
   static_assert(kWords <= kApplyNT, "one bitmap word per thread");
@@ -1675,6 +1678,7 @@ struct Engine final : EngineBase {
   bool chg_pending = false;
   bool chg_cond = false;  // written only if the blocked gather did not run
   bool pre_applied = false;  // nextIter queued the next iteration's apply
+  bool gated_apply = false;  // ... gated on the device by ctr->fab (kBlockedMin)
   // source_flag_v programs: the sign bits of d_srcval hold at most the last
   // flag-keeping apply's changed set (src_flags_ok), exactly it (gather_flags:
   // the fused blocked sum may read them instead of the changed bitmap)
@@ -3002,6 +3006,25 @@ struct Engine final : EngineBase {
         pre_applied = true;
       }
     }
+    if constexpr (kBlockedMin) {
+      // The same after the blocked gather that scatterActivate launched in
+      // place of the pull, when the next gather may use its values: the
+      // queued apply runs only if that gather ran on the device (it records
+      // that in ctr->fab); the host learns which below.
+      if (run_mode && fab_launched && delta_ready && P::kHasGather && !streaming) {
+        tick();
+        ApplyDefer ad;
+        ad.cc = d_cc;
+        ad.sc = d_sc;
+        ad.sc_final = d_sc_final;
+        k_apply<P, true><<<grid_1d(g->nV, kApplyChunk), kApplyNT, 0, stream>>>(
+            g->nV, nullptr, d_act_bits, d_gtmp, 3, d_vd, g->off_out, d_deg_out8, iteration + 1,
+            d_chg_v, d_chg_e, d_chg_bits, d_ctr, d_srcval, d_gdense, d_applied, ad, nullptr, 1);
+        tock(kPhApply);
+        GR_HIP(hipGetLastError());
+        gated_apply = true;
+      }
+    }
     GR_TRY(wait_counters(ctr_seq));
     if constexpr (kDelta) {
       const bool fused = fuse_launched && h_ctr->fused;
@@ -3013,6 +3036,10 @@ struct Engine final : EngineBase {
       if (fab_launched && !h_ctr->fab) stats.idle_gather_launches += 1;
       fab_pending = fab_launched && h_ctr->fab && delta_ready;
       fab_launched = false;
+      if (gated_apply) {  // it ran exactly where the next apply is its form
+        pre_applied = fab_pending;
+        gated_apply = false;
+      }
     }
     const unsigned long long c = h_ctr->chg_last;
     stats.changed_vertices += (int64_t)h_ctr->applied_last;
